@@ -1,0 +1,87 @@
+/*
+ * eh_device.h - internal C ABI between the C host (elegant_amd/host) and the
+ * HIP launch layer (elegant_amd/csrc/*.hip).  Not part of the public API.
+ */
+#ifndef ELEGANT_AMD_EH_DEVICE_H
+#define ELEGANT_AMD_EH_DEVICE_H
+
+#include <stdint.h>
+
+#include "eh_program.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ehd_dev ehd_dev;
+
+/* returns 0 or a negative EH_ERR_* code; msg (if not NULL) gets a description */
+int ehd_create(int device, ehd_dev **out, char *msg, int msglen);
+void ehd_destroy(ehd_dev *d);
+
+/* particle state: SoA doubles x, xp, y, yp, s, dp and int32 loss_pass/loss_code */
+int ehd_alloc_state(ehd_dev *d, int64_t n);
+int ehd_put_state(ehd_dev *d, const double *x, const double *xp, const double *y, const double *yp,
+                  const double *s, const double *dp, const int32_t *loss_pass, const int32_t *loss_code, int64_t n);
+int ehd_get_state(ehd_dev *d, double *x, double *xp, double *y, double *yp, double *s, double *dp,
+                  int32_t *loss_pass, int32_t *loss_code, int64_t n);
+int ehd_get_sdp(ehd_dev *d, double *s, double *dp, int32_t *loss_pass, int64_t n);
+int64_t ehd_state_n(const ehd_dev *d);
+
+/* program */
+int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops);
+int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op);
+
+/* run passes [pass0, pass0 + npasses): ops [op_first, nops) in the first pass,
+ * [0, op_last) in the last, all ops in between.  Asynchronous on the
+ * library stream; timing is accumulated with HIP events. */
+int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int32_t op_last);
+
+/* survivors after the last completed work (synchronizes) */
+int ehd_count_alive(ehd_dev *d, int64_t *n_alive);
+int ehd_sync(ehd_dev *d);
+
+/* timing of the tracking kernel since the last reset */
+void ehd_timing_reset(ehd_dev *d);
+int ehd_timing(ehd_dev *d, int64_t *launches, double *ms);
+
+/* launch tuning: particles per workgroup (multiple of 64) */
+void ehd_set_block(ehd_dev *d, int32_t block);
+
+/* ---- collective elements (WAKE / ZLONGIT) ---- */
+typedef struct {
+  int32_t kind;         /* EH_OP_WAKE or EH_OP_ZLONGIT */
+  int32_t op_index;     /* position in the program (loss codes) */
+  int32_t interpolate;
+  int32_t n_bins_fixed; /* WAKE N_BINS (0 = autoscale) */
+  int64_t pass;
+  double Po, dt, factor, macro_charge, ramp;
+  double z_end, ap_xmax, ap_ymax;
+  int32_t flags;        /* EH_F_AP */
+  int32_t nb;           /* ZLONGIT n_bins */
+  double wake_span;     /* WAKE: t[last] - t[0] (long-beam check) */
+  int32_t allow_long_beam;
+  int32_t pad;
+} ehd_collective;
+
+/* WAKE Green function samples; ZLONGIT impedance Z[nb] and twiddles */
+int ehd_put_wake(ehd_dev *d, int32_t slot, const double *W, int64_t npoints);
+int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc, const double *ts, int32_t nb);
+/* one WAKE or ZLONGIT step on all alive particles.  Returns EH_ERR_LONGBEAM if the
+ * bunch is longer than allowed.  hist/volt (optional, may be NULL) receive the
+ * histogram and the scaled voltage; *nb_out/*tmin_out describe them. */
+int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist, double *volt,
+                        int64_t hist_cap, int64_t *nb_out, double *tmin_out);
+int ehd_collective_timing(ehd_dev *d, int64_t *launches, double *ms);
+/* exact min/max of the alive particles' arrival times and their count */
+int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count);
+#define EHD_COLL_NOCHECKS (1 << 30) /* element entry points: no do_tracking checks after the step */
+
+/* deterministic math check: out[i] = f(in[i]) on the device, f = 0 sin, 1 cos, 2 asin, 3 log, 4 tan,
+ * 5 sqrt, 6 division in[i]/in2[i] */
+int ehd_math_probe(ehd_dev *d, int32_t fn, const double *in, const double *in2, double *out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

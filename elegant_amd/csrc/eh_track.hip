@@ -1,0 +1,977 @@
+/*
+ * eh_track.hip - gfx950 tracking kernel: one particle per lane, FP64.
+ *
+ * Each lane loads its particle's (x, x', y, y', s, delta) into registers,
+ * runs a contiguous run of element-program records for one or many passes,
+ * and writes the coordinates back once.  A particle that is lost stops, with
+ * its row frozen exactly as the reference leaves a lost row (s = loss
+ * position, delta = Po(1+delta)) and its pass and element/stage recorded;
+ * the host reproduces the reference's row order from those records.
+ *
+ * Arithmetic follows the reference routines' operation order with no FP
+ * contraction (built with -ffp-contract=off); transcendentals come from
+ * emath.h so the CPU checker can reproduce every bit.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/elegant_hip.h"
+#include "eh_consts.h"
+#include "eh_device.h"
+#include "eh_program.h"
+#include "emath.h"
+#include "ehd_internal.h"
+
+#pragma clang fp contract(off)
+
+#define DEV static __device__ __forceinline__
+
+DEV double sq(double v) { return v * v; }
+
+struct particle {
+  double x, xp, y, yp, s, dp;
+};
+
+/* ---- frozen rows of lost particles ---- */
+
+/* limit_amplitudes loss (limit_amplitudes.c:232-268): extrapolate along the
+ * preceding drift if asked, then s = z + dz, delta = Po(1 + delta) */
+DEV void aperture_loss(particle &p, double xmax, double ymax, int is_out, bool extrap, double z, double Po) {
+  double dz = 0;
+  if (is_out && !(is_out & 4) && extrap) {
+    double dzx = -1.7976931348623157e308, dzy = -1.7976931348623157e308;
+    if ((is_out & 1) && p.xp != 0)
+      dzx = (p.x - (p.xp > 0 ? 1 : -1) * xmax) / p.xp;
+    if ((is_out & 2) && p.yp != 0)
+      dzy = (p.y - (p.yp > 0 ? 1 : -1) * ymax) / p.yp;
+    if (dzx > dzy)
+      dz = -dzx;
+    else
+      dz = -dzy;
+    if (dz == -1.7976931348623157e308)
+      dz = 0;
+  }
+  p.x += dz * p.xp;
+  p.y += dz * p.yp;
+  p.s = z + dz;
+  p.dp = Po * (1 + p.dp);
+}
+
+/* the two checks that follow every element (do_tracking.c:2459-2477, 2628-2637).
+ * Returns 0 if the particle survives, else stage + 1 with the row frozen. */
+DEV int post_element_checks(particle &p, const eh_op &op) {
+  if (op.flags & EH_F_AP) {
+    double xmax = op.ap_xmax, ymax = op.ap_ymax;
+    int is_out = 0;
+    if (xmax > 0 && __builtin_fabs(p.x) > xmax)
+      is_out += 1;
+    if (ymax > 0 && __builtin_fabs(p.y) > ymax)
+      is_out += 2;
+    if (__builtin_isinf(p.x) || __builtin_isinf(p.y) || __builtin_isnan(p.x) || __builtin_isnan(p.y))
+      is_out += 4;
+    if (is_out) {
+      aperture_loss(p, xmax, ymax, is_out, (op.flags & EH_F_EXTRAP) != 0, op.z_end, op.Po);
+      return EH_STAGE_MAXAMP + 1;
+    }
+  }
+  {
+    const double big = 1.7976931348623157e308;
+    int is_out = 0;
+    if (__builtin_fabs(p.x) > big)
+      is_out += 1;
+    if (__builtin_fabs(p.y) > big)
+      is_out += 2;
+    if (__builtin_isinf(p.x) || __builtin_isinf(p.y) || __builtin_isnan(p.x) || __builtin_isnan(p.y))
+      is_out += 4;
+    if (is_out) {
+      aperture_loss(p, big, big, is_out, false, op.z_end, op.Po);
+      return EH_STAGE_NAN + 1;
+    }
+  }
+  return 0;
+}
+
+/* ---- drifts ---- */
+
+/* order-2 drift matrix: the sparse loop of matrix.c:389-429 with the drift
+ * matrix reduces exactly to these sums (R/T zeros only add signed zeros) */
+DEV void drift2(particle &p, double L, double hL) {
+  double s = L;
+  if (p.s != 0)
+    s = s + p.s;
+  if (p.yp != 0)
+    s = s + (hL * p.yp) * p.yp;
+  if (p.xp != 0)
+    s = s + (hL * p.xp) * p.xp;
+  double y = (p.yp != 0) ? L * p.yp : 0.0;
+  if (p.y != 0)
+    y = y + p.y;
+  double x = (p.xp != 0) ? L * p.xp : 0.0;
+  if (p.x != 0)
+    x = x + p.x;
+  p.x = x;
+  p.y = y;
+  p.s = s;
+}
+
+/* order-1 matrix branch (matrix.c:430-455) */
+DEV void drift1(particle &p, double L) {
+  /* sum = C[i] + sum_j R[i][j] ini[j], j = 5..0 */
+  double x = 0.0 + 0.0 * p.dp + 0.0 * p.s + 0.0 * p.yp + 0.0 * p.y + L * p.xp + 1.0 * p.x;
+  double y = 0.0 + 0.0 * p.dp + 0.0 * p.s + L * p.yp + 1.0 * p.y + 0.0 * p.xp + 0.0 * p.x;
+  double s = L + 0.0 * p.dp + 1.0 * p.s + 0.0 * p.yp + 0.0 * p.y + 0.0 * p.xp + 0.0 * p.x;
+  p.x = x;
+  p.y = y;
+  p.s = s;
+}
+
+DEV void exact_drift(particle &p, double L) {
+  p.x += p.xp * L;
+  p.y += p.yp * L;
+  p.s += L * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+}
+
+/* ---- misalignment (malign_mat.c:172-182, tilt_matrices.c:156-188) ---- */
+
+DEV void offset(particle &p, double dx, double dy, double dz) {
+  p.s += dz * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.x = p.x - dx + dz * p.xp;
+  p.y = p.y - dy + dz * p.yp;
+}
+
+DEV void rotate(particle &p, double c, double s) {
+  double x = p.x, xp = p.xp, y = p.y, yp = p.yp;
+  p.x = x * c + y * s;
+  p.y = -x * s + y * c;
+  p.xp = xp * c + yp * s;
+  p.yp = -xp * s + yp * c;
+}
+
+/* ---- multipoles ---- */
+
+DEV bool in_aperture(double x, double y, const eh_op &op) {
+  if (!(op.flags & EH_F_INAP))
+    return true;
+  if (op.flags & EH_F_INTILT) {
+    double x0 = x, y0 = y;
+    x = x0 * op.in_cs0 + y0 * op.in_cs1;
+    y = -x0 * op.in_cs1 + y0 * op.in_cs0;
+  }
+  if ((op.in_xmax > 0 && __builtin_fabs(x) > op.in_xmax) || (op.in_ymax > 0 && __builtin_fabs(y) > op.in_ymax))
+    return false;
+  return true;
+}
+
+/* convertMomentaToSlopes (multipole.h:266-281) */
+DEV bool momenta_to_slopes(double &xp, double &yp, double qx, double qy, double dp, bool expand) {
+  if (expand) {
+    xp = qx / (1 + dp);
+    yp = qy / (1 + dp);
+    return true;
+  }
+  double denom = sq(1 + dp) - sq(qx) - sq(qy);
+  if (denom <= 0)
+    return false;
+  denom = __builtin_sqrt(denom);
+  xp = qx / denom;
+  yp = qy / denom;
+  return true;
+}
+
+/* sum_Fx, sum_Fy of apply_canonical_multipole_kicks (multipole.h:208-241)
+ * for order n, written out: coefficients (-1)^floor(i/2)/(i!(n-i)!) are
+ * 1,1 | 1/2,1,-1/2 | 1/6,1/2,-1/2,-1/6, and products with 1 or sums with 0
+ * in the reference loop are exact. */
+DEV void multipole_fields(int order, double x, double y, double &Fx, double &Fy) {
+  if (order == 1) {
+    Fx = y;
+    Fy = x;
+  } else if (order == 2) {
+    Fx = x * y;
+    Fy = 0.5 * (x * x) + -0.5 * (y * y);
+  } else if (order == 3) {
+    double x2 = x * x, y2 = y * y;
+    Fx = (0.5 * x2) * y + (-1.0 / 6.0) * (y2 * y);
+    Fy = (1.0 / 6.0) * (x2 * x) + (-0.5 * x) * y2;
+  } else { /* order 0 */
+    Fx = 0;
+    Fy = 1;
+  }
+}
+
+DEV void apply_kick(double &qx, double &qy, int order, int skew, double KnL, double x, double y) {
+  double Fx, Fy;
+  multipole_fields(order, x, y, Fx, Fy);
+  if (skew) {
+    double t = Fx;
+    Fx = -Fy;
+    Fy = t;
+  }
+  qx -= KnL * Fy;
+  qy += KnL * Fx;
+}
+
+/* multipole_tracking2 per particle: entry misalignment, the integrator of
+ * integrate_kick_multipole_ordn (multipole.c:1122-1410), exit misalignment.
+ * Returns 0 if alive, EH_STAGE_ELEMENT + 1 if lost (row frozen). */
+DEV int track_multipole(particle &p, const eh_op &op) {
+  const int flags = op.flags;
+  const bool expand = (flags & EH_F_EXPAND) != 0;
+  if (flags & EH_F_OFFSET)
+    offset(p, op.d[EH_D_DX], op.d[EH_D_DY], op.d[EH_D_DZ]);
+  if (flags & EH_F_ROTATE)
+    rotate(p, op.d[EH_D_COS_IN], op.d[EH_D_SIN_IN]);
+
+  /* the row as the integrator receives it stays in p until the end */
+  const double dp = p.dp;
+  double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, qx = 0, qy = 0, dzLoss = 0;
+  int lost = 0; /* 1: aperture (x, y written), 2: slopes (nothing written), 3: final limits */
+  const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
+
+  if (__builtin_isnan(x) || __builtin_isnan(xp) || __builtin_isnan(y) || __builtin_isnan(yp) ||
+      __builtin_fabs(x) > EH_COORD_LIMIT || __builtin_fabs(y) > EH_COORD_LIMIT ||
+      __builtin_fabs(xp) > EH_SLOPE_LIMIT || __builtin_fabs(yp) > EH_SLOPE_LIMIT) {
+    lost = 2;
+    dzLoss = op.d[EH_D_TOTLEN];
+  } else {
+    if (expand) {
+      qx = (1 + dp) * xp;
+      qy = (1 + dp) * yp;
+    } else {
+      double denom = __builtin_sqrt(1 + sq(xp) + sq(yp));
+      qx = (1 + dp) * xp / denom;
+      qy = (1 + dp) * yp / denom;
+    }
+    if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
+      lost = 2;
+      dzLoss = op.d[EH_D_TOTLEN];
+    }
+  }
+  if (!lost) {
+    const int nsub = op.nsub;
+    const bool has_drift = op.d[EH_D_DRIFT] != 0;
+    for (int ik = 0; ik < op.nslices && !lost; ik++) {
+      if (!in_aperture(x + dxa, y + dya, op)) {
+        lost = 1;
+        break;
+      }
+      for (int step = 0; step < nsub; step++) {
+        if (has_drift) {
+          double dsh = op.d[EH_D_DSH + step];
+          x += xp * dsh;
+          y += yp * dsh;
+          if (expand)
+            s += dsh * (1 + (sq(xp) + sq(yp)) / 2);
+          else
+            s += dsh * __builtin_sqrt(1 + sq(xp) + sq(yp));
+          dzLoss += dsh;
+        }
+        if (step == nsub - 1)
+          break; /* kickFrac[last] == 0 */
+        if (op.nkick_orders > 0)
+          apply_kick(qx, qy, op.order0, op.skew0, op.d[EH_D_KICK + step], x, y);
+        if (op.nkick_orders > 1)
+          apply_kick(qx, qy, op.order1, op.skew1, op.d[EH_D_KICK + 8 + step], x, y);
+        if (op.nkick_orders > 2)
+          apply_kick(qx, qy, op.order2, op.skew2, op.d[EH_D_KICK + 16 + step], x, y);
+        if (flags & EH_F_XKICK)
+          apply_kick(qx, qy, 0, 0, op.d[EH_D_XK + step], x, y);
+        if (flags & EH_F_YKICK)
+          apply_kick(qx, qy, 0, 1, op.d[EH_D_YK + step], x, y);
+        if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
+          lost = 2;
+          break;
+        }
+        if (has_drift) {
+          /* rad_coef = 0, isr_coef < 0: the q/(1+dp)*(1+dp) round trip (multipole.c:1321-1351) */
+          qx /= (1 + dp);
+          qy /= (1 + dp);
+          qx *= (1 + dp);
+          qy *= (1 + dp);
+          if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
+            lost = 2;
+            break;
+          }
+        }
+      }
+    }
+    if (!lost) {
+      if (!in_aperture(x + dxa, y + dya, op))
+        lost = 1;
+      else if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+        lost = 2;
+      else {
+        p.x = x;
+        p.xp = xp;
+        p.y = y;
+        p.yp = yp;
+        p.s += s;
+        p.dp = dp;
+        if (__builtin_isnan(x) || __builtin_isnan(xp) || __builtin_isnan(y) || __builtin_isnan(yp) ||
+            __builtin_fabs(x) > EH_COORD_LIMIT || __builtin_fabs(y) > EH_COORD_LIMIT ||
+            __builtin_fabs(xp) > EH_SLOPE_LIMIT || __builtin_fabs(yp) > EH_SLOPE_LIMIT)
+          lost = 3;
+      }
+    }
+  }
+  if (lost) {
+    if (lost == 1) {
+      p.x = x;
+      p.y = y;
+    }
+    /* lost == 2: row as received; lost == 3: row fully written */
+    p.s = op.z_start + dzLoss;
+    p.dp = op.Po * (1 + p.dp);
+  }
+  /* exit misalignment, applied to lost rows too (multipole.c:1103-1104) */
+  if (flags & EH_F_ROTATE)
+    rotate(p, op.d[EH_D_COS_OUT], op.d[EH_D_SIN_OUT]);
+  if (flags & EH_F_OFFSET)
+    offset(p, -op.d[EH_D_DX], -op.d[EH_D_DY], -op.d[EH_D_DZ]);
+  return lost ? EH_STAGE_ELEMENT + 1 : 0;
+}
+
+/* ---- CSBEND ---- */
+
+DEV int track_csbend(particle &p, const eh_op &op) {
+  const int flags = op.flags;
+  const double ct = op.d[EH_D_CT], st = op.d[EH_D_ST];
+  const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA];
+  double x, xp, y, yp, s, dp, dp0;
+  /* entry: misalignment then rotation by the total tilt (csbend.c:1120-1131) */
+  p.s += op.d[EH_D_DZI] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.x = p.x + op.d[EH_D_DXI] + op.d[EH_D_DZI] * p.xp;
+  p.y = p.y + op.d[EH_D_DYI] + op.d[EH_D_DZI] * p.yp;
+  x = p.x * ct + p.y * st;
+  y = -p.x * st + p.y * ct;
+  xp = p.xp * ct + p.yp * st;
+  yp = -p.xp * st + p.yp * ct;
+  s = p.s;
+  dp = dp0 = p.dp;
+  if (flags & EH_F_EDGE1) {
+    double rho = (1 + dp) * rhoa;
+    double delta_xp = op.d[EH_D_TANE1] / rho * x;
+    if ((flags & EH_F_KLIM1) && __builtin_fabs(delta_xp) > op.d[EH_D_KLIM1])
+      delta_xp = (delta_xp > 0 ? 1 : -1) * op.d[EH_D_KLIM1];
+    xp += delta_xp;
+    double te = em_tan(op.d[EH_D_E1] - op.d[EH_D_PSI1] / (1 + dp));
+    yp -= te / rho * y;
+  }
+  /* canonical (csbend.c:1411-1419) */
+  double X = x, QX, Y = y, QY, S = 0, DP = dp;
+  {
+    double f = (1 + DP) / __builtin_sqrt(1 + sq(xp) + sq(yp));
+    QX = xp * f;
+    QY = yp * f;
+  }
+  /* integrate_csbend_ordn (csbend.c:1434-1693), Fx = 0, Fy = 1 */
+  double dist = 0, dz_lost = 0;
+  bool lost = false;
+  const int nsub = op.nsub;
+  for (int i = 0; i < op.nslices && !lost; i++) {
+    if (!in_aperture(X, Y, op)) {
+      lost = true;
+      break;
+    }
+    for (int j = 0; j < nsub; j++) {
+      double dsh = op.d[EH_D_BDSH + j];
+      double f = sq(1 + DP) - sq(QY);
+      if (f <= 0) {
+        lost = true;
+        break;
+      }
+      f = __builtin_sqrt(f);
+      if (__builtin_fabs(QX / f) > 1) {
+        lost = true;
+        break;
+      }
+      double sin_phi = QX / f;
+      double phi = em_asin(sin_phi);
+      double sine, cosi;
+      em_sincos(op.d[EH_D_BDSHR + j] + phi, &sine, &cosi);
+      if (cosi == 0) {
+        lost = true;
+        break;
+      }
+      double tang = sine / cosi;
+      double cos_phi = em_cos(phi);
+      QX = f * sine;
+      double factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
+      Y += QY * factor;
+      dist += factor * (1 + DP);
+      dz_lost += dsh;
+      f = cos_phi / cosi;
+      X = rho0 * (f - 1) + f * X;
+      if (j == nsub - 1)
+        break; /* kickFrac[last] == 0 */
+      double ds = op.d[EH_D_BDS + j];
+      double g = 1 + X / rho0;
+      QX += -ds * g * 1.0 / rhoa;
+      QY += ds * g * 0.0 / rhoa;
+      QX /= (1 + DP);
+      QY /= (1 + DP);
+      QX *= (1 + DP);
+      QY *= (1 + DP);
+    }
+  }
+  if (!lost) {
+    dz_lost = op.d[EH_D_NS];
+    if (!in_aperture(X, Y, op))
+      lost = true;
+    else
+      S += dist;
+  }
+  {
+    /* convertFromDipoleCanonicalCoordinates (csbend.c:1421-1429) */
+    double f = 1 / __builtin_sqrt(sq(1 + DP) - sq(QX) - sq(QY));
+    QX *= f;
+    QY *= f;
+  }
+  if (lost) {
+    /* memcpy(Qf) + convertFromCSBendCoords (csbend.c:1237-1239, 4502-4524) */
+    p.x = X * ct - Y * st;
+    p.y = X * st + Y * ct;
+    p.xp = QX * ct - QY * st;
+    p.yp = QX * st + QY * ct;
+    p.s = op.z_start + dz_lost;
+    p.dp = op.Po * (1 + DP);
+    return EH_STAGE_ELEMENT + 1;
+  }
+  /* velocity rescaling of s (csbend.c:1280-1292) */
+  {
+    double p0 = op.Po * (1 + dp0);
+    double beta0 = p0 / __builtin_sqrt(sq(p0) + 1);
+    double p1 = op.Po * (1 + DP);
+    double beta1 = p1 / __builtin_sqrt(sq(p1) + 1);
+    s = beta1 * s / beta0 + S;
+  }
+  x = X;
+  xp = QX;
+  y = Y;
+  yp = QY;
+  dp = DP;
+  if (flags & EH_F_EDGE2) {
+    double rho = (1 + dp) * rhoa;
+    double delta_xp = op.d[EH_D_TANE2] / rho * x;
+    if ((flags & EH_F_KLIM2) && __builtin_fabs(delta_xp) > op.d[EH_D_KLIM2])
+      delta_xp = (delta_xp > 0 ? 1 : -1) * op.d[EH_D_KLIM2];
+    xp += delta_xp;
+    double te = em_tan(op.d[EH_D_E2] - op.d[EH_D_PSI2] / (1 + dp));
+    yp -= te / rho * y;
+  }
+  p.x = x * ct - y * st + 0.0;
+  p.y = x * st + y * ct + 0.0;
+  p.xp = xp * ct - yp * st + 0.0;
+  p.yp = xp * st + yp * ct + 0.0;
+  p.s = s + 0.0;
+  p.dp = dp;
+  p.x += op.d[EH_D_DXF] + op.d[EH_D_DZF] * p.xp;
+  p.y += op.d[EH_D_DYF] + op.d[EH_D_DZF] * p.yp;
+  p.s += op.d[EH_D_DZF] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  return 0;
+}
+
+/* ---- RFCA ---- */
+
+/* add_to_particle_energy (simple_rfca.c:987-1007) */
+DEV void add_energy(particle &p, double tof, double Po, double dgamma) {
+  double P = Po * (1 + p.dp);
+  double gamma = __builtin_sqrt(P * P + 1);
+  double gamma1 = gamma + dgamma;
+  if (gamma1 <= 1)
+    gamma1 = 1 + 1e-7;
+  double P1 = __builtin_sqrt(gamma1 * gamma1 - 1);
+  p.dp = (P1 - Po) / Po;
+  p.s = tof * EH_C_MKS * P1 / gamma1;
+  double Pz = P / __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  double Pz1 = __builtin_sqrt(Pz * Pz + gamma1 * gamma1 - gamma * gamma);
+  double ratio = Pz / Pz1;
+  p.xp *= ratio;
+  p.yp *= ratio;
+}
+
+/* trackRfCavityWithWakes for one particle, forward tracking, no wakes
+ * (simple_rfca.c:557-984) followed by removeInvalidParticles (:975) */
+DEV int track_rfca(particle &p, const eh_op &op) {
+  const double Po = op.Po;
+  const double volt = op.d[EH_D_VOLT], omega = op.d[EH_D_OMEGA], phase = op.d[EH_D_PHASE];
+  const double length = op.d[EH_D_LENGTH], dtLight = op.d[EH_D_DTLIGHT];
+  const int nKicks = op.nkicks_rf;
+  p.x -= op.d[EH_D_RFDX];
+  p.y -= op.d[EH_D_RFDY];
+  if (!(op.flags & EH_F_MATRIX)) {
+    double inverseF = 0;
+    for (int ik = 0; ik < nKicks; ik++) {
+      if (p.dp != -1) {
+        double dc4 = length ? length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp)) : 0;
+        double P = Po * (1 + p.dp);
+        double gamma = __builtin_sqrt(sq(P) + 1);
+        double beta_i = P / gamma;
+        double t = (p.s + dc4) / (EH_C_MKS * beta_i) - 0.0;
+        double phi = omega * (t - 0 - ik * dtLight) + phase;
+        double dgamma = volt * em_sin(phi) * 1;
+        if (length) {
+          if ((op.flags & EH_F_END1FOCUS) && ik == 0) {
+            inverseF = dgamma / (2 * gamma * length);
+            p.xp -= p.x * inverseF;
+            p.yp -= p.y * inverseF;
+          }
+          p.x += p.xp * length / 2;
+          p.y += p.yp * length / 2;
+          p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+        }
+        add_energy(p, t, Po, dgamma);
+        double gamma1 = gamma + dgamma;
+        if (gamma1 <= 1)
+          p.dp = -1;
+        else
+          inverseF = -dgamma / (2 * gamma1 * length);
+      }
+      if (length) {
+        p.x += p.xp * length / 2;
+        p.y += p.yp * length / 2;
+        p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+        if ((op.flags & EH_F_END2FOCUS) && ik == nKicks - 1) {
+          p.xp -= p.x * inverseF;
+          p.yp -= p.y * inverseF;
+        }
+      }
+    }
+  } else {
+    for (int ik = 0; ik < nKicks; ik++) {
+      double P = Po * (1 + p.dp);
+      double gamma = __builtin_sqrt(sq(P) + 1);
+      double beta_i = P / gamma;
+      double ds1 = length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+      double t = (p.s + ds1) / (EH_C_MKS * beta_i) - 0.0;
+      double sin_phase = em_sin(omega * (t - 0 - ik * dtLight) + phase);
+      double dgamma = volt * 1 * sin_phase;
+      if ((op.flags & EH_F_END1FOCUS) && length && ik == 0) {
+        double inverseF = dgamma / (2 * gamma * length);
+        p.xp -= p.x * inverseF;
+        p.yp -= p.y * inverseF;
+      }
+      double dP = __builtin_sqrt(sq(gamma + dgamma) - 1) - P;
+      double R22 = 1 / (1 + dP / P), R12, R11 = 1, R21 = 0;
+      if (__builtin_fabs(dP / P) > 1e-14)
+        R12 = length * (P / dP * em_log(1 + dP / P));
+      else
+        R12 = length;
+      p.s += ds1;
+      double xx = p.x, xxp = p.xp;
+      p.x = xx * R11 + xxp * R12;
+      p.xp = xx * R21 + xxp * R22;
+      xx = p.y;
+      xxp = p.yp;
+      p.y = xx * R11 + xxp * R12;
+      p.yp = xx * R21 + xxp * R22;
+      p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+      p.dp = (P + dP - Po) / Po;
+      if ((gamma += dgamma) <= 1)
+        p.dp = -1;
+      if ((op.flags & EH_F_END2FOCUS) && length && ik == nKicks - 1) {
+        double inverseF = -dgamma / (2 * gamma * length);
+        p.xp -= p.x * inverseF;
+        p.yp -= p.y * inverseF;
+      }
+      p.s = (P + dP) / gamma * p.s / beta_i;
+    }
+  }
+  p.x += op.d[EH_D_RFDX];
+  p.y += op.d[EH_D_RFDY];
+  /* removeInvalidParticles (limit_amplitudes.c:274-320) */
+  if (__builtin_isnan(p.x) || __builtin_isnan(p.xp) || __builtin_isnan(p.y) || __builtin_isnan(p.yp) ||
+      __builtin_isnan(p.s) || __builtin_isnan(p.dp) || p.dp <= -1) {
+    p.s = op.z_end;
+    p.dp = Po * (1 + p.dp);
+    return EH_STAGE_ELEMENT + 1;
+  }
+  return 0;
+}
+
+DEV int run_op(particle &p, const eh_op &op) {
+  int lost = 0;
+  switch (op.op) {
+  case EH_OP_DRIFT2:
+    drift2(p, op.d[0], op.d[1]);
+    break;
+  case EH_OP_DRIFT1:
+    drift1(p, op.d[0]);
+    break;
+  case EH_OP_EXACT_DRIFT:
+    exact_drift(p, op.d[0]);
+    break;
+  case EH_OP_MULT:
+    lost = track_multipole(p, op);
+    break;
+  case EH_OP_CSBEND:
+    lost = track_csbend(p, op);
+    break;
+  case EH_OP_RFCA:
+    lost = track_rfca(p, op);
+    break;
+  case EH_OP_RFCA_DRIFT:
+    p.x += p.xp * op.d[EH_D_RFLEN_TOTAL];
+    p.y += p.yp * op.d[EH_D_RFLEN_TOTAL];
+    p.s += op.d[EH_D_RFLEN_TOTAL] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+    break;
+  default:
+    break;
+  }
+  if (lost)
+    return lost;
+  return post_element_checks(p, op);
+}
+
+__global__ void __launch_bounds__(256) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
+                                                       int64_t pass0, int32_t npasses, int32_t op_first,
+                                                       int32_t op_last) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n)
+    return;
+  if (st.loss_pass[i] >= 0)
+    return;
+  particle p;
+  p.x = st.x[i];
+  p.xp = st.xp[i];
+  p.y = st.y[i];
+  p.yp = st.yp[i];
+  p.s = st.s[i];
+  p.dp = st.dp[i];
+  int lost = 0, lost_op = 0;
+  int64_t lost_pass = 0;
+  for (int32_t ip = 0; ip < npasses && !lost; ip++) {
+    const int32_t ob = (ip == 0) ? op_first : 0;
+    const int32_t oe = (ip == npasses - 1) ? op_last : nops;
+    for (int32_t io = ob; io < oe; io++) {
+      lost = run_op(p, ops[io]);
+      if (lost) {
+        lost_op = io;
+        lost_pass = pass0 + ip;
+        break;
+      }
+    }
+  }
+  st.x[i] = p.x;
+  st.xp[i] = p.xp;
+  st.y[i] = p.y;
+  st.yp[i] = p.yp;
+  st.s[i] = p.s;
+  st.dp[i] = p.dp;
+  if (lost) {
+    st.loss_pass[i] = (int32_t)lost_pass;
+    st.loss_code[i] = 4 * lost_op + (lost - 1);
+  }
+}
+
+/* ------------------------------------------------------------ host side */
+
+extern "C" {
+
+static int hip_fail(ehd_dev *d, hipError_t e, const char *what) {
+  if (d)
+    snprintf(d->msg, sizeof(d->msg), "%s: %s", what, hipGetErrorString(e));
+  return EH_ERR_HIP;
+}
+
+#define HIPCHK(d, call)                                                                                                \
+  do {                                                                                                                 \
+    hipError_t e_ = (call);                                                                                            \
+    if (e_ != hipSuccess)                                                                                              \
+      return hip_fail((d), e_, #call);                                                                                 \
+  } while (0)
+
+int ehd_create(int device, ehd_dev **out, char *msg, int msglen) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    if (msg && msglen > 0)
+      snprintf(msg, msglen, "no HIP device available (%s)", hipGetErrorString(e));
+    return EH_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= count) {
+    if (msg && msglen > 0)
+      snprintf(msg, msglen, "device %d out of range (%d devices)", device, count);
+    return EH_ERR_INVALID;
+  }
+  ehd_dev *d = (ehd_dev *)calloc(1, sizeof(ehd_dev));
+  if (!d)
+    return EH_ERR_NOMEM;
+  d->device = device;
+  d->block = 256;
+  e = hipSetDevice(device);
+  if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipEventCreate(&d->ev0);
+  if (e == hipSuccess)
+    e = hipEventCreate(&d->ev1);
+  if (e == hipSuccess)
+    e = hipEventCreate(&d->cev0);
+  if (e == hipSuccess)
+    e = hipEventCreate(&d->cev1);
+  if (e != hipSuccess) {
+    if (msg && msglen > 0)
+      snprintf(msg, msglen, "HIP set-up failed: %s", hipGetErrorString(e));
+    free(d);
+    return EH_ERR_HIP;
+  }
+  *out = d;
+  return EH_OK;
+}
+
+void ehd_free_collective(ehd_dev *d);
+
+void ehd_destroy(ehd_dev *d) {
+  if (!d)
+    return;
+  hipSetDevice(d->device);
+  hipStreamSynchronize(d->stream);
+  for (int k = 0; k < 6; k++)
+    if (d->coord[k])
+      hipFree(d->coord[k]);
+  if (d->loss_pass)
+    hipFree(d->loss_pass);
+  if (d->loss_code)
+    hipFree(d->loss_code);
+  if (d->ops)
+    hipFree(d->ops);
+  if (d->counter)
+    hipFree(d->counter);
+  ehd_free_collective(d);
+  hipEventDestroy(d->ev0);
+  hipEventDestroy(d->ev1);
+  hipEventDestroy(d->cev0);
+  hipEventDestroy(d->cev1);
+  hipStreamDestroy(d->stream);
+  free(d);
+}
+
+int ehd_alloc_state(ehd_dev *d, int64_t n) {
+  HIPCHK(d, hipSetDevice(d->device));
+  if (n > d->cap) {
+    for (int k = 0; k < 6; k++) {
+      if (d->coord[k])
+        HIPCHK(d, hipFree(d->coord[k]));
+      d->coord[k] = NULL;
+    }
+    if (d->loss_pass)
+      HIPCHK(d, hipFree(d->loss_pass));
+    if (d->loss_code)
+      HIPCHK(d, hipFree(d->loss_code));
+    d->loss_pass = d->loss_code = NULL;
+    d->cap = 0;
+    for (int k = 0; k < 6; k++)
+      HIPCHK(d, hipMalloc((void **)&d->coord[k], sizeof(double) * (n > 0 ? n : 1)));
+    HIPCHK(d, hipMalloc((void **)&d->loss_pass, sizeof(int32_t) * (n > 0 ? n : 1)));
+    HIPCHK(d, hipMalloc((void **)&d->loss_code, sizeof(int32_t) * (n > 0 ? n : 1)));
+    d->cap = n;
+  }
+  if (!d->counter)
+    HIPCHK(d, hipMalloc((void **)&d->counter, sizeof(unsigned long long) * 8));
+  d->n = n;
+  return EH_OK;
+}
+
+int64_t ehd_state_n(const ehd_dev *d) { return d->n; }
+
+int ehd_put_state(ehd_dev *d, const double *x, const double *xp, const double *y, const double *yp, const double *s,
+                  const double *dp, const int32_t *loss_pass, const int32_t *loss_code, int64_t n) {
+  const double *src[6] = {x, xp, y, yp, s, dp};
+  if (n != d->n)
+    return EH_ERR_STATE;
+  HIPCHK(d, hipSetDevice(d->device));
+  for (int k = 0; k < 6; k++)
+    HIPCHK(d, hipMemcpyAsync(d->coord[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->loss_pass, loss_pass, sizeof(int32_t) * n, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->loss_code, loss_code, sizeof(int32_t) * n, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
+int ehd_get_state(ehd_dev *d, double *x, double *xp, double *y, double *yp, double *s, double *dp,
+                  int32_t *loss_pass, int32_t *loss_code, int64_t n) {
+  double *dst[6] = {x, xp, y, yp, s, dp};
+  if (n != d->n)
+    return EH_ERR_STATE;
+  HIPCHK(d, hipSetDevice(d->device));
+  for (int k = 0; k < 6; k++)
+    if (dst[k])
+      HIPCHK(d, hipMemcpyAsync(dst[k], d->coord[k], sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+  if (loss_pass)
+    HIPCHK(d, hipMemcpyAsync(loss_pass, d->loss_pass, sizeof(int32_t) * n, hipMemcpyDeviceToHost, d->stream));
+  if (loss_code)
+    HIPCHK(d, hipMemcpyAsync(loss_code, d->loss_code, sizeof(int32_t) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
+int ehd_get_sdp(ehd_dev *d, double *s, double *dp, int32_t *loss_pass, int64_t n) {
+  return ehd_get_state(d, NULL, NULL, NULL, NULL, s, dp, loss_pass, NULL, n);
+}
+
+int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops) {
+  HIPCHK(d, hipSetDevice(d->device));
+  if (nops > d->ops_cap) {
+    if (d->ops)
+      HIPCHK(d, hipFree(d->ops));
+    d->ops = NULL;
+    HIPCHK(d, hipMalloc((void **)&d->ops, sizeof(eh_op) * (nops > 0 ? nops : 1)));
+    d->ops_cap = nops;
+  }
+  if (nops > 0)
+    HIPCHK(d, hipMemcpyAsync(d->ops, ops, sizeof(eh_op) * nops, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  d->nops = nops;
+  return EH_OK;
+}
+
+int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op) {
+  if (index < 0 || index >= d->nops)
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  HIPCHK(d, hipMemcpyAsync(d->ops + index, op, sizeof(eh_op), hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
+void ehd_set_block(ehd_dev *d, int32_t block) {
+  if (block >= 64 && block <= 256 && block % 64 == 0)
+    d->block = block;
+}
+
+int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int32_t op_last) {
+  if (npasses <= 0 || d->n == 0)
+    return EH_OK;
+  if (op_first < 0 || op_last > d->nops || op_first > d->nops || (npasses == 1 && op_first > op_last))
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  eh_state st;
+  st.x = d->coord[0];
+  st.xp = d->coord[1];
+  st.y = d->coord[2];
+  st.yp = d->coord[3];
+  st.s = d->coord[4];
+  st.dp = d->coord[5];
+  st.loss_pass = d->loss_pass;
+  st.loss_code = d->loss_code;
+  st.n = d->n;
+  const int block = d->block;
+  const int64_t grid = (d->n + block - 1) / block;
+  HIPCHK(d, hipEventRecord(d->ev0, d->stream));
+  hipLaunchKernelGGL(eh_track_kernel, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops, pass0,
+                     npasses, op_first, op_last);
+  HIPCHK(d, hipGetLastError());
+  HIPCHK(d, hipEventRecord(d->ev1, d->stream));
+  HIPCHK(d, hipEventSynchronize(d->ev1));
+  float ms = 0;
+  HIPCHK(d, hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->track_ms += ms;
+  d->track_launches += 1;
+  return EH_OK;
+}
+
+__global__ void eh_count_alive_kernel(const int32_t *loss_pass, int64_t n, unsigned long long *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int alive = (i < n && loss_pass[i] < 0) ? 1 : 0;
+  unsigned long long b = __ballot(alive);
+  if ((threadIdx.x & 63) == 0 && b)
+    atomicAdd(out, (unsigned long long)__popcll(b));
+}
+
+int ehd_count_alive(ehd_dev *d, int64_t *n_alive) {
+  HIPCHK(d, hipSetDevice(d->device));
+  unsigned long long h = 0;
+  HIPCHK(d, hipMemcpyAsync(d->counter, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
+  if (d->n > 0) {
+    int64_t grid = (d->n + 255) / 256;
+    hipLaunchKernelGGL(eh_count_alive_kernel, dim3((unsigned)grid), dim3(256), 0, d->stream, d->loss_pass, d->n,
+                       d->counter);
+    HIPCHK(d, hipGetLastError());
+  }
+  HIPCHK(d, hipMemcpyAsync(&h, d->counter, sizeof(h), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  *n_alive = (int64_t)h;
+  return EH_OK;
+}
+
+int ehd_sync(ehd_dev *d) {
+  HIPCHK(d, hipSetDevice(d->device));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
+void ehd_timing_reset(ehd_dev *d) {
+  d->track_ms = 0;
+  d->track_launches = 0;
+  d->coll_ms = 0;
+  d->coll_launches = 0;
+}
+
+int ehd_timing(ehd_dev *d, int64_t *launches, double *ms) {
+  *launches = d->track_launches;
+  *ms = d->track_ms;
+  return EH_OK;
+}
+
+const char *ehd_message(const ehd_dev *d) { return d ? d->msg : ""; }
+
+/* ---- deterministic-math probe (used by tests to compare with the CPU) ---- */
+__global__ void eh_math_probe_kernel(int32_t fn, const double *in, const double *in2, double *out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double v = in[i], r;
+  switch (fn) {
+  case 0:
+    r = em_sin(v);
+    break;
+  case 1:
+    r = em_cos(v);
+    break;
+  case 2:
+    r = em_asin(v);
+    break;
+  case 3:
+    r = em_log(v);
+    break;
+  case 4:
+    r = em_tan(v);
+    break;
+  case 5:
+    r = __builtin_sqrt(v);
+    break;
+  default:
+    r = v / in2[i];
+    break;
+  }
+  out[i] = r;
+}
+
+int ehd_math_probe(ehd_dev *d, int32_t fn, const double *in, const double *in2, double *out, int64_t n) {
+  double *din = NULL, *din2 = NULL, *dout = NULL;
+  if (n <= 0)
+    return EH_OK;
+  HIPCHK(d, hipSetDevice(d->device));
+  HIPCHK(d, hipMalloc((void **)&din, sizeof(double) * n));
+  HIPCHK(d, hipMalloc((void **)&din2, sizeof(double) * n));
+  HIPCHK(d, hipMalloc((void **)&dout, sizeof(double) * n));
+  HIPCHK(d, hipMemcpyAsync(din, in, sizeof(double) * n, hipMemcpyHostToDevice, d->stream));
+  if (in2)
+    HIPCHK(d, hipMemcpyAsync(din2, in2, sizeof(double) * n, hipMemcpyHostToDevice, d->stream));
+  else
+    HIPCHK(d, hipMemcpyAsync(din2, in, sizeof(double) * n, hipMemcpyHostToDevice, d->stream));
+  hipLaunchKernelGGL(eh_math_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, fn, din, din2,
+                     dout, n);
+  HIPCHK(d, hipGetLastError());
+  HIPCHK(d, hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  HIPCHK(d, hipFree(din));
+  HIPCHK(d, hipFree(din2));
+  HIPCHK(d, hipFree(dout));
+  return EH_OK;
+}
+
+} /* extern "C" */

@@ -1,0 +1,41 @@
+/*
+ * ehd_internal.h - state of one device context (HIP side only).
+ */
+#ifndef ELEGANT_AMD_EHD_INTERNAL_H
+#define ELEGANT_AMD_EHD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eh_program.h"
+
+#define EHD_MAX_COLL 16
+
+typedef struct {
+  double *W;          /* WAKE Green function */
+  int64_t wpoints;
+  double *Z, *tc, *ts; /* ZLONGIT impedance and FFT twiddles */
+  int32_t nb;
+} ehd_coll_slot;
+
+struct ehd_dev {
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1, cev0, cev1;
+  double *coord[6];
+  int32_t *loss_pass, *loss_code;
+  int64_t n, cap;
+  eh_op *ops;
+  int32_t nops, ops_cap;
+  int32_t block;
+  unsigned long long *counter;
+  double track_ms, coll_ms;
+  int64_t track_launches, coll_launches;
+  /* collective work buffers */
+  ehd_coll_slot slot[EHD_MAX_COLL];
+  void *cwork;       /* reductions, histogram, voltages */
+  int64_t cwork_bytes;
+  char msg[512];
+};
+
+#endif

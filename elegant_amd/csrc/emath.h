@@ -1,0 +1,265 @@
+/*
+ * emath.h - deterministic FP64 elementary functions shared by the gfx950
+ * kernels and the CPU checker build.
+ *
+ * Why this exists: the per-particle integrators call sin, cos, asin, tan and
+ * log (csbend.c:1567-1574, simple_rfca.c:846-879).  The device math library
+ * and glibc differ by up to one ULP on some arguments, so a kernel that calls
+ * the device library can never be compared bit-for-bit against a CPU run.
+ * Every function here is written only with IEEE-754 operations that are
+ * correctly rounded on both x86-64 and CDNA4 (+, -, *, /, sqrt, explicit fma)
+ * and is compiled with -ffp-contract=off on both sides, so the same source
+ * gives the same bits on the host and on the GPU.
+ *
+ * Methods (all public, none taken from a vendor library):
+ *  - argument reduction modulo pi/2: Cody & Waite (1980) with pi/2 split into
+ *    seven 24-bit pieces, accumulated in double-double (Knuth TwoSum), valid
+ *    for |x| < 2^28; larger |x| returns NaN (documented domain limit);
+ *  - sin/cos on [-pi/4, pi/4]: Taylor series to degree 17/16 (coefficients
+ *    are 1/n! rounded to double), with the compensated 1 - z/2 step for cos;
+ *  - asin: Taylor series of asin in x^2 on |x| <= 0.5 and the reflection
+ *    asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above;
+ *  - log: x = 2^e m, log(m) = 2 atanh(f/(2+f)) with f = m - 1.
+ * Constants were derived from exact rational arithmetic (pi from Machin's
+ * formula, ln 2 from sum 1/(k 2^k)) and are written as hex literals.
+ */
+#ifndef ELEGANT_AMD_EMATH_H
+#define ELEGANT_AMD_EMATH_H
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define EM_FN static inline __host__ __device__
+#else
+#define EM_FN static inline
+#endif
+
+#define EM_PIO4 0x1.921fb54442d18p-1
+#define EM_PIO2_HI 0x1.921fb54442d18p+0
+#define EM_PIO2_LO 0x1.1a62633145c07p-54
+#define EM_TWO_OVER_PI 0x1.45f306dc9c883p-1
+#define EM_SHIFT 0x1.8p+52
+#define EM_LN2_HI 0x1.62e42fee00000p-1
+#define EM_LN2_LO 0x1.a39ef35600000p-33
+
+EM_FN uint64_t em_bits(double d) {
+  uint64_t u;
+  __builtin_memcpy(&u, &d, sizeof u);
+  return u;
+}
+
+EM_FN double em_from_bits(uint64_t u) {
+  double d;
+  __builtin_memcpy(&d, &u, sizeof d);
+  return d;
+}
+
+EM_FN double em_fabs(double x) { return em_from_bits(em_bits(x) & 0x7fffffffffffffffULL); }
+
+EM_FN double em_nan(void) { return em_from_bits(0x7ff8000000000000ULL); }
+
+/* a - b as an unevaluated sum s + e (Knuth's TwoSum, no ordering needed). */
+EM_FN void em_two_diff(double a, double b, double *s, double *e) {
+  double t = a - b;
+  double bb = t - a;
+  *e = (a - (t - bb)) - (b + bb);
+  *s = t;
+}
+
+/* x = k*(pi/2) + (rh + rl), |rh| <= ~pi/4.  Returns k (mod 4 is what matters).
+ * Returns -1 with *rh = NaN if x is NaN, infinite or |x| >= 2^28. */
+EM_FN int em_reduce_pio2(double x, double *rh, double *rl) {
+  const double c1 = 0x1.921fb40000000p+0, c2 = 0x1.4442d00000000p-24, c3 = 0x1.8469880000000p-48,
+               c4 = 0x1.8cc5160000000p-72, c5 = 0x1.01b8380000000p-96, c6 = 0x1.a252040000000p-120,
+               c7 = 0x1.3822280000000p-145;
+  double ax = em_fabs(x);
+  if (ax <= EM_PIO4) {
+    *rh = x;
+    *rl = 0.0;
+    return 0;
+  }
+  if (!(ax < 0x1.0p+28)) {
+    *rh = em_nan();
+    *rl = 0.0;
+    return -1;
+  }
+  /* round x*2/pi to the nearest integer (|k| < 2^28, so k*ci is exact) */
+  double kd = (x * EM_TWO_OVER_PI + EM_SHIFT) - EM_SHIFT;
+  double hi = x - kd * c1; /* exact: x and k*c1 agree to within a factor 2 */
+  double lo = 0.0, e, s;
+  em_two_diff(hi, kd * c2, &s, &e);
+  hi = s;
+  lo += e;
+  em_two_diff(hi, kd * c3, &s, &e);
+  hi = s;
+  lo += e;
+  em_two_diff(hi, kd * c4, &s, &e);
+  hi = s;
+  lo += e;
+  lo -= kd * c5;
+  lo -= kd * c6;
+  lo -= kd * c7;
+  s = hi + lo;
+  *rl = lo - (s - hi);
+  *rh = s;
+  return (int)(long long)kd;
+}
+
+/* sin(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
+EM_FN double em_sin_kernel(double x, double y) {
+  const double s1 = -0x1.5555555555555p-3, s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13,
+               s4 = 0x1.71de3a556c734p-19, s5 = -0x1.ae64567f544e4p-26, s6 = 0x1.6124613a86d09p-33,
+               s7 = -0x1.ae7f3e733b81fp-41, s8 = 0x1.952c77030ad4ap-49;
+  double z = x * x;
+  double v = z * x;
+  double r = s2 + z * (s3 + z * (s4 + z * (s5 + z * (s6 + z * (s7 + z * s8)))));
+  return x + ((y - 0.5 * z * y) + v * (s1 + z * r));
+}
+
+/* cos(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
+EM_FN double em_cos_kernel(double x, double y) {
+  const double c2 = 0x1.5555555555555p-5, c3 = -0x1.6c16c16c16c17p-10, c4 = 0x1.a01a01a01a01ap-16,
+               c5 = -0x1.27e4fb7789f5cp-22, c6 = 0x1.1eed8eff8d898p-29, c7 = -0x1.93974a8c07c9dp-37,
+               c8 = 0x1.ae7f3e733b81fp-45;
+  double z = x * x;
+  double r = z * z * (c2 + z * (c3 + z * (c4 + z * (c5 + z * (c6 + z * (c7 + z * c8))))));
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  return w + (((1.0 - w) - hz) + (r - x * y));
+}
+
+EM_FN void em_sincos(double x, double *sp, double *cp) {
+  double rh, rl, s, c;
+  int k;
+  if (x == 0.0) {
+    *sp = x;
+    *cp = 1.0;
+    return;
+  }
+  k = em_reduce_pio2(x, &rh, &rl);
+  if (k < 0 && rh != rh) {
+    *sp = *cp = em_nan();
+    return;
+  }
+  s = em_sin_kernel(rh, rl);
+  c = em_cos_kernel(rh, rl);
+  switch (k & 3) {
+  case 0:
+    *sp = s;
+    *cp = c;
+    break;
+  case 1:
+    *sp = c;
+    *cp = -s;
+    break;
+  case 2:
+    *sp = -s;
+    *cp = -c;
+    break;
+  default:
+    *sp = -c;
+    *cp = s;
+    break;
+  }
+}
+
+EM_FN double em_sin(double x) {
+  double s, c;
+  em_sincos(x, &s, &c);
+  return s;
+}
+
+EM_FN double em_cos(double x) {
+  double s, c;
+  em_sincos(x, &s, &c);
+  return c;
+}
+
+EM_FN double em_tan(double x) {
+  double s, c;
+  em_sincos(x, &s, &c);
+  return s / c;
+}
+
+/* (asin(x) - x)/x as a polynomial in z = x*x; short form for z <= 1/64 */
+EM_FN double em_asin_poly_small(double z) {
+  const double a1 = 0x1.5555555555555p-3, a2 = 0x1.3333333333333p-4, a3 = 0x1.6db6db6db6db7p-5,
+               a4 = 0x1.f1c71c71c71c7p-6, a5 = 0x1.6e8ba2e8ba2e9p-6, a6 = 0x1.1c4ec4ec4ec4fp-6,
+               a7 = 0x1.c99999999999ap-7, a8 = 0x1.7a87878787878p-7, a9 = 0x1.3fde50d79435ep-7;
+  return z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * (a5 + z * (a6 + z * (a7 + z * (a8 + z * a9))))))));
+}
+
+/* same series, long form for z <= 1/4 (25 terms) */
+EM_FN double em_asin_poly_large(double z) {
+  const double a[25] = {0x1.5555555555555p-3, 0x1.3333333333333p-4, 0x1.6db6db6db6db7p-5, 0x1.f1c71c71c71c7p-6,
+                        0x1.6e8ba2e8ba2e9p-6, 0x1.1c4ec4ec4ec4fp-6, 0x1.c99999999999ap-7, 0x1.7a87878787878p-7,
+                        0x1.3fde50d79435ep-7, 0x1.12ef3cf3cf3cfp-7, 0x1.df3bd37a6f4dfp-8, 0x1.a6863d70a3d71p-8,
+                        0x1.782dda12f684cp-8, 0x1.51ba308d3dcb1p-8, 0x1.31683bdef7bdfp-8, 0x1.15ee9d45d1746p-8,
+                        0x1.fcaf8fb6db6dbp-9, 0x1.d3d2a8e0dd67dp-9, 0x1.b026f57b13b14p-9, 0x1.90cb77f60c7cep-9,
+                        0x1.750de64d7d05fp-9, 0x1.5c5f56efaaaabp-9, 0x1.464c0950f7d47p-9, 0x1.3275586c5f2f0p-9,
+                        0x1.208d3570ae5a6p-9};
+  double p = a[24];
+  for (int i = 23; i >= 0; i--)
+    p = a[i] + z * p;
+  return z * p;
+}
+
+EM_FN double em_asin(double x) {
+  double ax = em_fabs(x);
+  if (!(ax <= 1.0))
+    return em_nan();
+  if (ax <= 0.125) {
+    double z = x * x;
+    return x + x * em_asin_poly_small(z);
+  }
+  if (ax <= 0.5) {
+    double z = x * x;
+    return x + x * em_asin_poly_large(z);
+  }
+  /* asin(ax) = pi/2 - 2 asin(sqrt(t)), t = (1 - ax)/2 exact */
+  double t = (1.0 - ax) * 0.5;
+  double s = __builtin_sqrt(t);
+  double corr = s > 0 ? __builtin_fma(-s, s, t) / (s + s) : 0.0; /* sqrt(t) = s + corr to ~2^-105 */
+  double p = em_asin_poly_large(t);
+  double r = EM_PIO2_HI - (2.0 * s + (2.0 * corr + 2.0 * s * p - EM_PIO2_LO));
+  return x < 0 ? -r : r;
+}
+
+EM_FN double em_log(double x) {
+  const double l1 = 0x1.5555555555555p-1, l2 = 0x1.999999999999ap-2, l3 = 0x1.2492492492492p-2,
+               l4 = 0x1.c71c71c71c71cp-3, l5 = 0x1.745d1745d1746p-3, l6 = 0x1.3b13b13b13b14p-3,
+               l7 = 0x1.1111111111111p-3, l8 = 0x1.e1e1e1e1e1e1ep-4, l9 = 0x1.af286bca1af28p-4,
+               l10 = 0x1.8618618618618p-4, l11 = 0x1.642c8590b2164p-4, l12 = 0x1.47ae147ae147bp-4;
+  uint64_t u;
+  int e;
+  double m, f, s, z, r, hfsq, ed;
+  if (x != x)
+    return x;
+  if (x < 0.0)
+    return em_nan();
+  if (x == 0.0)
+    return -em_from_bits(0x7ff0000000000000ULL);
+  if (x == em_from_bits(0x7ff0000000000000ULL))
+    return x;
+  e = 0;
+  if (x < 0x1.0p-1022) { /* subnormal: scale into the normal range */
+    x *= 0x1.0p+54;
+    e = -54;
+  }
+  u = em_bits(x);
+  e += (int)((u >> 52) & 0x7ff) - 1023;
+  m = em_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL); /* [1, 2) */
+  if (m > 0x1.6a09e667f3bcdp+0) {
+    m *= 0.5;
+    e += 1;
+  }
+  f = m - 1.0; /* exact */
+  s = f / (2.0 + f);
+  z = s * s;
+  r = z * (l1 + z * (l2 + z * (l3 + z * (l4 + z * (l5 + z * (l6 + z * (l7 + z * (l8 + z * (l9 + z * (l10 + z * (l11 + z * l12)))))))))));
+  hfsq = 0.5 * f * f;
+  ed = (double)e;
+  return ed * EM_LN2_HI + (f - (hfsq - (s * (hfsq + r) + ed * EM_LN2_LO)));
+}
+
+#endif

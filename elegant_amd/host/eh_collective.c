@@ -1,0 +1,289 @@
+/*
+ * eh_collective.c - host side of the WAKE and ZLONGIT steps: element
+ * set-up that the reference does once per element (wake function spacing,
+ * set_up_wake at wake.c:340-477; impedance table and bin settings,
+ * set_up_zlongit at zlongit.c:559-632 and optimizeBinSettingsForImpedance at
+ * ztransverse.c:672-759), the per-pass scale factors, and the
+ * element-by-element entry points.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "eh_consts.h"
+#include "eh_host.h"
+
+ehd_dev *eh_ctx_dev(eh_ctx *c);
+const eh_program *eh_ctx_prog(eh_ctx *c);
+const eh_element *eh_ctx_elems(eh_ctx *c);
+double eh_ctx_macro_charge(eh_ctx *c, int *seen);
+char *eh_ctx_msg(eh_ctx *c);
+int *eh_ctx_coll_loaded(eh_ctx *c);
+int32_t *eh_ctx_zl_nbins(eh_ctx *c);
+double *eh_ctx_zl_binsize(eh_ctx *c);
+const char *ehd_message(const ehd_dev *d);
+int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops);
+int eh_download(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int64_t *nLeft);
+
+static double sqr(double x) { return x * x; }
+
+static int msgf(eh_ctx *c, int code, const char *m) {
+  snprintf(eh_ctx_msg(c), 512, "%s", m);
+  return code;
+}
+
+/* optimizeBinSettingsForImpedance (ztransverse.c:672-759) */
+static int optimize_bins(double timeSpan, double freq, double Q, double *binSize, long *nBins, long maxBins) {
+  long n_bins = *nBins, maxBins2;
+  double bin_size = *binSize, factor;
+  if (maxBins <= 0)
+    maxBins2 = (long)pow(2, 20);
+  else
+    maxBins2 = (long)pow(2, (long)(log(maxBins) / log(2) + 1));
+  if (1 / (2 * freq * bin_size) < 10)
+    bin_size = 1. / freq / 20;
+  if (2 * timeSpan > bin_size * n_bins) {
+    n_bins = (long)pow(2, (long)(log(2 * timeSpan * 1.05 / bin_size) / log(2) + 1));
+    if (maxBins2 < n_bins)
+      return EH_ERR_LONGBEAM;
+  }
+  if (Q < 1)
+    factor = 200 / (n_bins * bin_size * freq / Q);
+  else
+    factor = 200 / (n_bins * bin_size * freq);
+  if (factor > 1) {
+    if (n_bins * factor > maxBins2) {
+      if ((n_bins * factor) / maxBins2 > 50)
+        return EH_ERR_INVALID;
+      n_bins = maxBins2;
+    } else
+      n_bins = (long)pow(2, (long)(log(n_bins * factor) / log(2) + 1));
+  }
+  *nBins = n_bins;
+  *binSize = bin_size;
+  return EH_OK;
+}
+
+static double ramp_factor(long i_pass, long rampPasses) {
+  if (i_pass >= (rampPasses - 1))
+    return 1;
+  return (i_pass + 1.0) / rampPasses;
+}
+
+static int wake_setup(eh_ctx *c, int slot, const EH_WAKE *w, double *dt, double *span) {
+  double tmin = DBL_MAX, tmax = -DBL_MAX;
+  if (w->smoothing || w->change_p0)
+    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE with SMOOTHING or CHANGE_P0");
+  if (w->wakePoints < 2 || !w->t || !w->W)
+    return msgf(c, EH_ERR_INVALID, "WAKE: too little wake data");
+  for (int64_t i = 0; i < w->wakePoints; i++) {
+    if (w->t[i] < tmin)
+      tmin = w->t[i];
+    if (w->t[i] > tmax)
+      tmax = w->t[i];
+  }
+  if (tmin >= tmax)
+    return msgf(c, EH_ERR_INVALID, "WAKE: zero or negative time span");
+  if (tmin != 0)
+    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE: acausal Green functions (t[0] != 0)");
+  *dt = (tmax - tmin) / (w->wakePoints - 1);
+  *span = w->t[w->wakePoints - 1] - w->t[0];
+  if (!eh_ctx_coll_loaded(c)[slot]) {
+    int rc = ehd_put_wake(eh_ctx_dev(c), slot, w->W, w->wakePoints);
+    if (rc)
+      return msgf(c, rc, ehd_message(eh_ctx_dev(c)));
+    eh_ctx_coll_loaded(c)[slot] = 1;
+  }
+  return EH_OK;
+}
+
+/* set_up_zlongit, broad-band branch (zlongit.c:587-632), done once per element */
+static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po) {
+  ehd_dev *d = eh_ctx_dev(c);
+  double tmin, tmax, bin_size, df, Ra;
+  long n_bins, nfreq;
+  int64_t count;
+  int rc;
+  if (eh_ctx_coll_loaded(c)[slot])
+    return EH_OK;
+  if (!zl->broad_band || zl->area_weight || zl->smoothing || zl->reverseTimeOrder)
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: broad-band resonator without AREA_WEIGHT/SMOOTHING/REVERSE only");
+  if (zl->bin_size <= 0 || (zl->Ra && zl->Rs) || zl->n_bins % 2 != 0)
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: BIN_SIZE, RA/RS or N_BINS invalid");
+  rc = ehd_time_span(d, Po, &tmin, &tmax, &count);
+  if (rc)
+    return msgf(c, rc, ehd_message(d));
+  Ra = zl->Ra ? zl->Ra : 2 * zl->Rs;
+  bin_size = zl->bin_size;
+  n_bins = zl->n_bins;
+  rc = optimize_bins(tmax - tmin, zl->freq, zl->Q, &bin_size, &n_bins, zl->max_n_bins);
+  if (rc)
+    return msgf(c, rc, "ZLONGIT: bin settings cannot cover the bunch");
+  if (n_bins > 8192 || (n_bins & (n_bins - 1)))
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: more than 8192 bins");
+  {
+    double *Z = (double *)malloc(sizeof(double) * n_bins);
+    double *tc = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
+    double *ts = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
+    if (!Z || !tc || !ts) {
+      free(Z);
+      free(tc);
+      free(ts);
+      return EH_ERR_NOMEM;
+    }
+    df = 1 / (n_bins * bin_size) / (zl->freq);
+    nfreq = n_bins / 2 + 1;
+    Z[0] = 0;
+    Z[n_bins - 1] = 0;
+    for (long i = 1; i < nfreq - 1; i++) {
+      double term = zl->Q * (1.0 / (i * df) - i * df);
+      Z[2 * i - 1] = Ra / 2 / (1 + sqr(term));
+      Z[2 * i] = Z[2 * i - 1] * term;
+    }
+    /* FFT twiddles cos/sin(2 pi k / n), the same table the CPU checker builds */
+    for (long k = 0; k <= n_bins / 2; k++) {
+      double a = EH_PIx2 * k / n_bins;
+      tc[k] = cos(a);
+      ts[k] = sin(a);
+    }
+    rc = ehd_put_zlongit(d, slot, Z, tc, ts, (int32_t)n_bins);
+    free(Z);
+    free(tc);
+    free(ts);
+    if (rc)
+      return msgf(c, rc, ehd_message(d));
+  }
+  eh_ctx_zl_nbins(c)[slot] = (int32_t)n_bins;
+  eh_ctx_zl_binsize(c)[slot] = bin_size;
+  eh_ctx_coll_loaded(c)[slot] = 1;
+  return EH_OK;
+}
+
+static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op *op, int op_index, int64_t pass,
+                          double macro, int flags, double *hist, double *volt, int64_t *nb, double *tmin) {
+  ehd_collective cc;
+  int rc;
+  memset(&cc, 0, sizeof(cc));
+  cc.kind = e->type == EH_T_WAKE ? EH_OP_WAKE : EH_OP_ZLONGIT;
+  cc.op_index = op_index;
+  cc.pass = pass;
+  cc.Po = op->Po;
+  cc.z_end = op->z_end;
+  cc.ap_xmax = op->ap_xmax;
+  cc.ap_ymax = op->ap_ymax;
+  cc.flags = (op->flags & EH_F_AP) | flags;
+  if (e->type == EH_T_WAKE) {
+    const EH_WAKE *w = &e->p.wake;
+    double dt, span;
+    rc = wake_setup(c, slot, w, &dt, &span);
+    if (rc)
+      return rc;
+    cc.dt = dt;
+    cc.wake_span = span;
+    cc.interpolate = w->interpolate;
+    cc.n_bins_fixed = w->n_bins;
+    cc.allow_long_beam = w->allowLongBeam;
+    /* factor = macroParticleCharge * particleRelSign * FACTOR * rampFactor (wake.c:258) */
+    cc.factor = macro * EH_REL_SIGN * w->factor * ramp_factor((long)pass, w->rampPasses);
+  } else {
+    const EH_ZLONGIT *zl = &e->p.zlongit;
+    long i_pass = (long)pass - zl->startOnPass;
+    if (i_pass < 0 || zl->factor == 0)
+      return EH_OK;
+    rc = zlongit_setup(c, slot, zl, op->Po);
+    if (rc)
+      return rc;
+    cc.nb = eh_ctx_zl_nbins(c)[slot];
+    cc.dt = eh_ctx_zl_binsize(c)[slot];
+    cc.interpolate = zl->interpolate;
+    cc.allow_long_beam = zl->allowLongBeam;
+    /* factor = macroParticleCharge * particleRelSign / dt * FACTOR * rampFactor (zlongit.c:377) */
+    cc.factor = macro * EH_REL_SIGN / cc.dt * zl->factor * ramp_factor(i_pass, zl->rampPasses);
+  }
+  rc = ehd_collective_step(eh_ctx_dev(c), slot, &cc, hist, volt, 0, nb, tmin);
+  if (rc == EH_ERR_LONGBEAM)
+    return msgf(c, rc, "bunch longer than the wake/impedance window (wake.c:134-140, zlongit.c:219-236)");
+  if (rc)
+    return msgf(c, rc, ehd_message(eh_ctx_dev(c)));
+  return EH_OK;
+}
+
+int eh_collective(eh_ctx *c, int slot, int64_t pass, int64_t n_alive) {
+  const eh_program *p = eh_ctx_prog(c);
+  const eh_element *e = &eh_ctx_elems(c)[p->coll_elem[slot]];
+  const eh_op *op = &p->ops[p->coll_op[slot]];
+  int seen;
+  double macro = eh_ctx_macro_charge(c, &seen);
+  (void)n_alive;
+  if (!seen)
+    return msgf(c, EH_ERR_INVALID, "WAKE/ZLONGIT need a CHARGE element before them");
+  return collective_run(c, slot, e, op, p->coll_op[slot], pass, macro, 0, NULL, NULL, NULL, NULL);
+}
+
+/* last histogram/voltage of a collective element, for kernel-level tests */
+int eh_collective_probe(eh_ctx *c, double **part, int64_t np, const eh_element *e, double Po, int64_t i_pass,
+                        double macro, double *hist, double *volt, int64_t *nb, double *tmin) {
+  eh_element bl[1];
+  eh_program *prog = (eh_program *)eh_ctx_prog(c);
+  int rc;
+  int64_t nLeft;
+  bl[0] = *e;
+  rc = eh_compile(bl, 1, Po, prog, eh_ctx_msg(c), 512);
+  if (rc)
+    return rc;
+  memset(eh_ctx_coll_loaded(c), 0, sizeof(int) * EH_MAX_COLL);
+  rc = eh_upload(c, part, np, 8);
+  if (rc)
+    return rc;
+  rc = collective_run(c, 0, e, &prog->ops[0], 0, i_pass, macro, EHD_COLL_NOCHECKS, hist, volt, nb, tmin);
+  if (rc)
+    return rc;
+  {
+    double **tmp = (double **)malloc(sizeof(double *) * (np > 0 ? np : 1));
+    double *buf = (double *)malloc(sizeof(double) * 8 * (np > 0 ? np : 1));
+    if (!tmp || !buf) {
+      free(tmp);
+      free(buf);
+      return EH_ERR_NOMEM;
+    }
+    for (int64_t i = 0; i < np; i++)
+      tmp[i] = buf + 8 * i;
+    rc = eh_download(c, tmp, np, 8, &nLeft);
+    if (!rc)
+      for (int64_t i = 0; i < np; i++)
+        memcpy(part[i], tmp[i], sizeof(double) * 7);
+    free(tmp);
+    free(buf);
+  }
+  return rc;
+}
+
+int64_t eh_track_through_wake(eh_ctx *c, double **part, int64_t np, const EH_WAKE *wake, double Po, int64_t i_pass,
+                              double macroParticleCharge) {
+  eh_element e;
+  int rc;
+  if (!c || !wake)
+    return EH_ERR_INVALID;
+  if (np <= 0)
+    return np;
+  eh_element_defaults(&e, EH_T_WAKE);
+  e.p.wake = *wake;
+  rc = eh_collective_probe(c, part, np, &e, Po, i_pass, macroParticleCharge, NULL, NULL, NULL, NULL);
+  return rc ? rc : np;
+}
+
+int64_t eh_track_through_zlongit(eh_ctx *c, double **part, int64_t np, const EH_ZLONGIT *zlongit, double Po,
+                                 int64_t i_pass, double macroParticleCharge) {
+  eh_element e;
+  int rc;
+  if (!c || !zlongit)
+    return EH_ERR_INVALID;
+  if (np <= 0)
+    return np;
+  eh_element_defaults(&e, EH_T_ZLONGIT);
+  e.p.zlongit = *zlongit;
+  rc = eh_collective_probe(c, part, np, &e, Po, i_pass, macroParticleCharge, NULL, NULL, NULL, NULL);
+  return rc ? rc : np;
+}

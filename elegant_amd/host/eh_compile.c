@@ -1,0 +1,646 @@
+/*
+ * eh_compile.c - turns a flattened beamline into the kernel's element program.
+ *
+ * Every per-element constant the reference evaluates in its set-up code is
+ * computed here, on the host, with glibc and in the reference's operation
+ * order: the kernel then reproduces only the per-particle arithmetic.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "eh_consts.h"
+#include "eh_host.h"
+
+static double sqr(double x) { return x * x; }
+
+/* BETA = 2^(1/3); integrator fractions (multipole.c:1119-1149, csbend.c:1456-1480) */
+#define EH_BETA 1.25992104989487316477
+static const double driftFrac2[2] = {0.5, 0.5};
+static const double kickFrac2[2] = {1.0, 0.0};
+static const double driftFrac4[4] = {0.5 / (2 - EH_BETA), (1 - EH_BETA) / (2 - EH_BETA) / 2,
+                                     (1 - EH_BETA) / (2 - EH_BETA) / 2, 0.5 / (2 - EH_BETA)};
+static const double kickFrac4[4] = {1. / (2 - EH_BETA), -EH_BETA / (2 - EH_BETA), 1 / (2 - EH_BETA), 0};
+static const double driftFrac6[8] = {0.39225680523878,   0.5100434119184585, -0.47105338540975655,
+                                     0.0687531682525181, 0.0687531682525181, -0.47105338540975655,
+                                     0.5100434119184585, 0.39225680523878};
+static const double kickFrac6[8] = {0.784513610477560,  0.235573213359357, -1.17767998417887,
+                                    1.3151863206839063, -1.17767998417887, 0.235573213359357,
+                                    0.784513610477560,  0};
+
+static int fracs(int order, const double **drift, const double **kick) {
+  switch (order) {
+  case 2:
+    *drift = driftFrac2;
+    *kick = kickFrac2;
+    return 2;
+  case 4:
+    *drift = driftFrac4;
+    *kick = kickFrac4;
+    return 4;
+  case 6:
+    *drift = driftFrac6;
+    *kick = kickFrac6;
+    return 8;
+  default:
+    return 0;
+  }
+}
+
+int eh_element_defaults(eh_element *e, int32_t type) {
+  memset(e, 0, sizeof(*e));
+  e->type = type;
+  switch (type) {
+  case EH_T_MARK:
+  case EH_T_EDRIFT:
+  case EH_T_CHARGE:
+    break;
+  case EH_T_DRIF:
+    e->p.drift.order = 0;
+    break;
+  case EH_T_KQUAD: /* track_data.c:1222-1283 */
+    e->p.kquad.xKickCalibration = e->p.kquad.yKickCalibration = 1;
+    e->p.kquad.nSlices = 1;
+    e->p.kquad.integration_order = 4;
+    break;
+  case EH_T_KSEXT: /* track_data.c:1120-1160 */
+    e->p.ksext.xKickCalibration = e->p.ksext.yKickCalibration = 1;
+    e->p.ksext.nSlices = 1;
+    e->p.ksext.integration_order = 4;
+    break;
+  case EH_T_KOCT: /* track_data.c:1164-1187: N_KICKS = N_SLICES = 4 */
+    e->p.koct.n_kicks = 4;
+    e->p.koct.nSlices = 4;
+    e->p.koct.integration_order = 4;
+    break;
+  case EH_T_CSBEND: /* track_data.c:1893-1980 */
+    e->p.csbend.fint = 0.5;
+    e->p.csbend.fint1 = e->p.csbend.fint2 = -1;
+    e->p.csbend.nSlices = 4;
+    e->p.csbend.etiltSign = 1;
+    e->p.csbend.nonlinear = 1;
+    e->p.csbend.edge1_effects = e->p.csbend.edge2_effects = 1;
+    e->p.csbend.edge_order = 1;
+    e->p.csbend.integration_order = 4;
+    break;
+  case EH_T_RFCA: /* track_data.c:576-597 */
+    e->p.rfca.freq = 500e6;
+    e->p.rfca.tReference = -1;
+    e->p.rfca.fiducialMode = EH_FID_TMEAN;
+    break;
+  case EH_T_WAKE: /* track_data.c:2396-2414 */
+    e->p.wake.factor = 1;
+    e->p.wake.bunchedBeamMode = 1;
+    break;
+  case EH_T_ZLONGIT: /* track_data.c:2251-2283 (IS_LONG defaults from the integer field) */
+    e->p.zlongit.n_bins = 128;
+    e->p.zlongit.factor = 1;
+    e->p.zlongit.bunchedBeamMode = 1;
+    break;
+  case EH_T_MAXAMP: /* track_data.c:990-997 */
+    e->p.maxamp.exponent = 2;
+    break;
+  default:
+    return EH_ERR_INVALID;
+  }
+  return EH_OK;
+}
+
+int64_t eh_struct_size(int32_t type) {
+  switch (type) {
+  case 0:
+    return (int64_t)sizeof(eh_element);
+  case EH_T_DRIF:
+    return (int64_t)sizeof(EH_DRIFT);
+  case EH_T_EDRIFT:
+    return (int64_t)sizeof(EH_EDRIFT);
+  case EH_T_KQUAD:
+    return (int64_t)sizeof(EH_KQUAD);
+  case EH_T_KSEXT:
+    return (int64_t)sizeof(EH_KSEXT);
+  case EH_T_KOCT:
+    return (int64_t)sizeof(EH_KOCT);
+  case EH_T_CSBEND:
+    return (int64_t)sizeof(EH_CSBEND);
+  case EH_T_RFCA:
+    return (int64_t)sizeof(EH_RFCA);
+  case EH_T_WAKE:
+    return (int64_t)sizeof(EH_WAKE);
+  case EH_T_ZLONGIT:
+    return (int64_t)sizeof(EH_ZLONGIT);
+  case EH_T_MAXAMP:
+    return (int64_t)sizeof(EH_MAXAMP);
+  case EH_T_CHARGE:
+    return (int64_t)sizeof(EH_CHARGE);
+  default:
+    return -1;
+  }
+}
+
+double eh_element_length(const eh_element *e) {
+  switch (e->type) {
+  case EH_T_DRIF:
+    return e->p.drift.length;
+  case EH_T_EDRIFT:
+    return e->p.edrift.length;
+  case EH_T_KQUAD:
+    return e->p.kquad.length;
+  case EH_T_KSEXT:
+    return e->p.ksext.length;
+  case EH_T_KOCT:
+    return e->p.koct.length;
+  case EH_T_CSBEND:
+    return e->p.csbend.length;
+  case EH_T_RFCA:
+    return e->p.rfca.length;
+  default:
+    return 0;
+  }
+}
+
+int eh_has_length(int32_t type) {
+  return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT ||
+         type == EH_T_KOCT || type == EH_T_CSBEND || type == EH_T_RFCA;
+}
+
+/* rotateBeamCoordinatesForMisalignment's choice of cos/sin (tilt_matrices.c:161-176) */
+static void rotation_cs(double angle, double *c, double *s) {
+  if (fabs(fabs(angle) - EH_PI) < 1e-12) {
+    *c = -1;
+    *s = 0;
+  } else if (fabs(angle - EH_PIo2) < 1e-12) {
+    *c = 0;
+    *s = 1;
+  } else if (fabs(angle + EH_PIo2) < 1e-12) {
+    *c = 0;
+    *s = -1;
+  } else {
+    *c = cos(angle);
+    *s = sin(angle);
+  }
+}
+
+static int rotation_active(double angle) { return angle && !(fabs(fabs(angle) - EH_PIx2) < 1e-12); }
+
+static void set_in_aperture(eh_op *op, double reverseTilt, const EH_MAXAMP *maxamp) {
+  op->in_cs0 = cos(reverseTilt);
+  op->in_cs1 = sin(reverseTilt);
+  if (reverseTilt)
+    op->flags |= EH_F_INTILT;
+  if (maxamp) {
+    op->in_xmax = maxamp->x_max;
+    op->in_ymax = maxamp->y_max;
+    if (op->in_xmax > 0 || op->in_ymax > 0)
+      op->flags |= EH_F_INAP;
+  }
+}
+
+/* multipole_tracking2 set-up (multipole.c:765-1017) */
+static int compile_multipole(const eh_element *e, double Po, const EH_MAXAMP *maxamp, eh_op *op, char *msg,
+                             int msglen) {
+  double KnL[3] = {0, 0, 0}, drift = 0, tilt = 0, dx = 0, dy = 0, dz = 0, xkick = 0, ykick = 0;
+  long order[3] = {0, 0, 0}, n_kicks = 0, nSlices = 0, integ_order = 4;
+  int skew[3] = {0, 0, 0}, expand = 0;
+  const double *df, *kf;
+  int nsub, k;
+  switch (e->type) {
+  case EH_T_KQUAD: {
+    const EH_KQUAD *q = &e->p.kquad;
+    if (q->synch_rad || q->isr || q->edge1_effects || q->edge2_effects || q->radial || q->lEffective > 0 ||
+        q->malignMethod) {
+      snprintf(msg, msglen, "%s: KQUAD with SYNCH_RAD/ISR/EDGE*_EFFECTS/RADIAL/LEFFECTIVE/MALIGN_METHOD", e->name);
+      return EH_ERR_UNSUPPORTED;
+    }
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 1;
+    if (q->bore)
+      KnL[0] = q->B / q->bore * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+    else
+      KnL[0] = q->k1 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    break;
+  }
+  case EH_T_KSEXT: {
+    const EH_KSEXT *q = &e->p.ksext;
+    if (q->synch_rad || q->isr || q->malignMethod) {
+      snprintf(msg, msglen, "%s: KSEXT with SYNCH_RAD/ISR/MALIGN_METHOD", e->name);
+      return EH_ERR_UNSUPPORTED;
+    }
+    n_kicks = q->n_kicks;
+    nSlices = q->nSlices;
+    expand = q->expandHamiltonian;
+    order[0] = 2;
+    if (q->bore)
+      KnL[0] = 2 * q->B / sqr(q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+    else
+      KnL[0] = q->k2 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    if (q->k1) {
+      KnL[1] = q->k1 * q->length;
+      order[1] = 1;
+    }
+    if (q->j1) {
+      KnL[2] = q->j1 * q->length;
+      order[2] = 1;
+      skew[2] = 1;
+    }
+    break;
+  }
+  case EH_T_KOCT: {
+    const EH_KOCT *q = &e->p.koct;
+    if (q->synch_rad || q->isr || q->malignMethod) {
+      snprintf(msg, msglen, "%s: KOCT with SYNCH_RAD/ISR/MALIGN_METHOD", e->name);
+      return EH_ERR_UNSUPPORTED;
+    }
+    n_kicks = q->n_kicks;
+    nSlices = q->nSlices;
+    expand = q->expandHamiltonian;
+    order[0] = 3;
+    if (q->bore)
+      KnL[0] = 6 * q->B / (q->bore * q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length *
+               (1 + q->fse);
+    else
+      KnL[0] = q->k3 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    integ_order = q->integration_order;
+    break;
+  }
+  default:
+    return EH_ERR_INVALID;
+  }
+  if ((nsub = fracs((int)integ_order, &df, &kf)) == 0) {
+    snprintf(msg, msglen, "%s: INTEGRATION_ORDER must be 2, 4 or 6", e->name);
+    return EH_ERR_INVALID;
+  }
+  if (n_kicks <= 0) {
+    if (nSlices <= 0) {
+      snprintf(msg, msglen, "%s: N_KICKS<=0 and N_SLICES<=0", e->name);
+      return EH_ERR_INVALID;
+    }
+  } else {
+    if (integ_order > 2) {
+      if ((nSlices = (long)ceil(n_kicks / (1.0 * integ_order))) < 1)
+        nSlices = 1;
+    } else
+      nSlices = n_kicks;
+  }
+  op->op = EH_OP_MULT;
+  op->nslices = (int32_t)nSlices;
+  op->nsub = nsub;
+  if (expand)
+    op->flags |= EH_F_EXPAND;
+  /* the kernel applies the nonzero KnL terms in iOrder order */
+  {
+    int slot = 0;
+    for (k = 0; k < 3; k++) {
+      if (!KnL[k])
+        continue;
+      if (slot == 0) {
+        op->order0 = (int32_t)order[k];
+        op->skew0 = skew[k];
+      } else if (slot == 1) {
+        op->order1 = (int32_t)order[k];
+        op->skew1 = skew[k];
+      } else {
+        op->order2 = (int32_t)order[k];
+        op->skew2 = skew[k];
+      }
+      for (int step = 0; step < nsub; step++)
+        op->d[EH_D_KICK + 8 * slot + step] = KnL[k] / nSlices * kf[step];
+      slot++;
+    }
+    op->nkick_orders = slot;
+  }
+  {
+    double driftn = drift / nSlices, xk = xkick / nSlices, yk = ykick / nSlices, tot = 0;
+    op->d[EH_D_DRIFT] = driftn;
+    for (int step = 0; step < nsub; step++) {
+      op->d[EH_D_DSH + step] = driftn * df[step];
+      op->d[EH_D_XK + step] = -xk * kf[step];
+      op->d[EH_D_YK + step] = -yk * kf[step];
+    }
+    /* dzLoss of a particle that completed every substep: the running sum in step order */
+    for (long ik = 0; ik < nSlices; ik++)
+      for (int step = 0; step < nsub; step++) {
+        if (driftn)
+          tot += op->d[EH_D_DSH + step];
+        if (!kf[step])
+          break;
+      }
+    op->d[EH_D_TOTLEN] = tot;
+    if (xk)
+      op->flags |= EH_F_XKICK;
+    if (yk)
+      op->flags |= EH_F_YKICK;
+  }
+  op->d[EH_D_DX] = dx;
+  op->d[EH_D_DY] = dy;
+  op->d[EH_D_DZ] = dz;
+  if (dx || dy || dz)
+    op->flags |= EH_F_OFFSET;
+  if (tilt && rotation_active(tilt)) {
+    op->flags |= EH_F_ROTATE;
+    rotation_cs(tilt, &op->d[EH_D_COS_IN], &op->d[EH_D_SIN_IN]);
+    rotation_cs(-tilt, &op->d[EH_D_COS_OUT], &op->d[EH_D_SIN_OUT]);
+  }
+  set_in_aperture(op, -tilt, maxamp);
+  return EH_OK;
+}
+
+/* track_through_csbend set-up for a pure dipole, edge order 1 (csbend.c:785-1090) */
+static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *op, char *msg, int msglen) {
+  const EH_CSBEND *cb = &e->p.csbend;
+  double angle, e1, e2, tilt, etilt, ttilt, fse, h, rho0, rho_actual, Kg1, Kg2, fint1, fint2, psi1, psi2;
+  const double *df, *kf;
+  int nsub;
+  if (cb->angle == 0) {
+    op->op = EH_OP_EXACT_DRIFT;
+    op->d[0] = cb->length;
+    return EH_OK;
+  }
+  if (cb->k1 || cb->k2 || cb->k3 || cb->k4 || cb->xKick || cb->yKick || cb->etilt || cb->malignMethod ||
+      cb->referenceCorrection || cb->expandHamiltonian || cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 ||
+      cb->h2 || cb->fseQuadrupole) {
+    snprintf(msg, msglen, "%s: CSBEND supports pure dipoles without ETILT, radiation, H1/H2 or reference correction",
+             e->name);
+    return EH_ERR_UNSUPPORTED;
+  }
+  if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1) ||
+      cb->edge_order > 1) {
+    snprintf(msg, msglen, "%s: CSBEND edge effects other than EDGE_ORDER=1, EDGE*_EFFECTS in {0,1}", e->name);
+    return EH_ERR_UNSUPPORTED;
+  }
+  if ((nsub = fracs(cb->integration_order, &df, &kf)) == 0 || cb->nSlices < 1) {
+    snprintf(msg, msglen, "%s: CSBEND INTEGRATION_ORDER or N_SLICES invalid", e->name);
+    return EH_ERR_INVALID;
+  }
+  if (cb->angle < 0) {
+    angle = -cb->angle;
+    e1 = -cb->e1;
+    e2 = -cb->e2;
+    etilt = cb->etilt * cb->etiltSign;
+    tilt = cb->tilt + EH_PI;
+  } else {
+    angle = cb->angle;
+    e1 = cb->e1;
+    e2 = cb->e2;
+    etilt = cb->etilt * cb->etiltSign;
+    tilt = cb->tilt;
+  }
+  rho0 = cb->length / angle;
+  if (fabs(rho0) > 1e6) {
+    snprintf(msg, msglen, "%s: CSBEND with radius > 1e6", e->name);
+    return EH_ERR_UNSUPPORTED;
+  }
+  fse = cb->fse + cb->fseDipole;
+  h = 1 / rho0;
+  if (fabs(fse + 1) < 1e-12)
+    fse = -1 + 1e-12;
+  rho_actual = 1 / ((1 + fse) * h);
+  {
+    double l1 = cb->edge1_kick_limit, l2 = cb->edge2_kick_limit;
+    if (cb->kick_limit_scaling) {
+      l1 *= rho0 / rho_actual;
+      l2 *= rho0 / rho_actual;
+    }
+    op->d[EH_D_KLIM1] = l1;
+    op->d[EH_D_KLIM2] = l2;
+    if (l1 > 0)
+      op->flags |= EH_F_KLIM1;
+    if (l2 > 0)
+      op->flags |= EH_F_KLIM2;
+  }
+  fint1 = cb->fint1 >= 0 ? cb->fint1 : cb->fint;
+  fint2 = cb->fint2 >= 0 ? cb->fint2 : cb->fint;
+  Kg1 = 2 * cb->hgap * fint1 * (rho0 > 0 ? 1 : -1);
+  psi1 = Kg1 / fabs(rho_actual) / cos(e1) * (1 + sqr(sin(e1)));
+  Kg2 = 2 * cb->hgap * fint2 * (rho0 > 0 ? 1 : -1);
+  psi2 = Kg2 / fabs(rho_actual) / cos(e2) * (1 + sqr(sin(e2)));
+  if (cb->length < 0) {
+    psi1 *= -1;
+    psi2 *= -1;
+  }
+  ttilt = tilt + etilt;
+  if (ttilt == 0) {
+    op->d[EH_D_CT] = 1;
+    op->d[EH_D_ST] = 0;
+  } else
+    rotation_cs(ttilt, &op->d[EH_D_CT], &op->d[EH_D_ST]);
+  op->op = EH_OP_CSBEND;
+  op->nslices = cb->nSlices;
+  op->nsub = nsub;
+  op->d[EH_D_RHO0] = rho0;
+  op->d[EH_D_RHOA] = rho_actual;
+  op->d[EH_D_TANE1] = tan(e1);
+  op->d[EH_D_TANE2] = tan(e2);
+  op->d[EH_D_E1] = e1;
+  op->d[EH_D_E2] = e2;
+  op->d[EH_D_PSI1] = psi1;
+  op->d[EH_D_PSI2] = psi2;
+  if (cb->edge1_effects == 1)
+    op->flags |= EH_F_EDGE1;
+  if (cb->edge2_effects == 1)
+    op->flags |= EH_F_EDGE2;
+  op->d[EH_D_DXI] = -cb->dx;
+  op->d[EH_D_DZI] = cb->dz;
+  op->d[EH_D_DYI] = -cb->dy;
+  op->d[EH_D_DXF] = cb->dx * cos(cb->angle) + cb->dz * sin(cb->angle);
+  op->d[EH_D_DZF] = cb->dx * sin(cb->angle) - cb->dz * cos(cb->angle);
+  op->d[EH_D_DYF] = cb->dy;
+  {
+    double s = cb->length / cb->nSlices;
+    op->d[EH_D_SSLICE] = s;
+    op->d[EH_D_NS] = cb->nSlices * s;
+    for (int j = 0; j < nsub; j++) {
+      double dsh = s * df[j];
+      op->d[EH_D_BDSH + j] = dsh;
+      op->d[EH_D_BDSHR + j] = dsh / rho0;
+      op->d[EH_D_BDS + j] = s * kf[j];
+      op->d[EH_D_BKF + j] = kf[j];
+    }
+  }
+  set_in_aperture(op, -tilt, maxamp);
+  return EH_OK;
+}
+
+/* trackRfCavityWithWakes set-up without wakes (simple_rfca.c:378-478); the
+ * phase is completed by eh_rfca_set_phase once the fiducial is known */
+static int compile_rfca(const eh_element *e, eh_op *op, char *msg, int msglen) {
+  const EH_RFCA *rf = &e->p.rfca;
+  double length = rf->length, volt, omega;
+  long nKicks;
+  int matrixMethod = 0;
+  if (rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase || rf->bodyFocusModel || rf->standingWave ||
+      rf->Q) {
+    snprintf(msg, msglen, "%s: RFCA with CHANGE_T/CHANGE_P0/LINEARIZE/LOCK_PHASE/BODY_FOCUS_MODEL/STANDING_WAVE/Q",
+             e->name);
+    return EH_ERR_UNSUPPORTED;
+  }
+  if (rf->fiducialMode != EH_FID_TMEAN && rf->fiducialMode != EH_FID_LIGHT) {
+    snprintf(msg, msglen, "%s: RFCA fiducial mode must be tmean or light", e->name);
+    return EH_ERR_UNSUPPORTED;
+  }
+  if (rf->volt == 0) {
+    op->op = EH_OP_RFCA_DRIFT;
+    op->d[EH_D_RFLEN_TOTAL] = rf->length;
+    return EH_OK;
+  }
+  omega = EH_PIx2 * rf->freq;
+  volt = rf->volt / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+  nKicks = length ? rf->nKicks : 1;
+  if (nKicks <= 0) {
+    matrixMethod = 1;
+    nKicks = 1;
+  }
+  length /= nKicks;
+  volt /= nKicks;
+  op->op = EH_OP_RFCA;
+  op->nkicks_rf = (int32_t)nKicks;
+  if (matrixMethod)
+    op->flags |= EH_F_MATRIX;
+  if (rf->end1Focus)
+    op->flags |= EH_F_END1FOCUS;
+  if (rf->end2Focus)
+    op->flags |= EH_F_END2FOCUS;
+  op->d[EH_D_VOLT] = volt;
+  op->d[EH_D_OMEGA] = omega;
+  op->d[EH_D_LENGTH] = length;
+  op->d[EH_D_DTLIGHT] = length / EH_C_MKS;
+  op->d[EH_D_RFDX] = rf->dx;
+  op->d[EH_D_RFDY] = rf->dy;
+  op->d[EH_D_PHASE] = NAN; /* set by eh_rfca_set_phase */
+  return EH_OK;
+}
+
+/* phase = phase_fiducial + PHASE*pi/180 (simple_rfca.c:466-478) */
+void eh_rfca_set_phase(eh_op *op, const EH_RFCA *rf, double phase_fiducial) {
+  double phase = phase_fiducial;
+  phase += rf->phase * EH_PI / 180.0;
+  op->d[EH_D_PHASE] = phase;
+}
+
+/* t0 from a TMEAN/LIGHT fiducialization (simple_rfca.c:412-471) -> phase_fiducial */
+double eh_rfca_phase_fiducial(const EH_RFCA *rf, double t0) { return -(EH_PIx2 * rf->freq) * t0; }
+
+int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *prog, char *msg, int msglen) {
+  double z = 0, last_z;
+  double x_max = 0, y_max = 0;
+  const EH_MAXAMP *maxamp = NULL;
+  int64_t i;
+  prog->nops = 0;
+  prog->n_rfca = 0;
+  prog->n_coll = 0;
+  prog->Po = Po;
+  for (i = 0; i < n_elems; i++) {
+    const eh_element *e = &bl[i];
+    eh_op op;
+    int rc = EH_OK, keep = 1;
+    memset(&op, 0, sizeof(op));
+    last_z = z;
+    if (eh_has_length(e->type))
+      z += eh_element_length(e);
+    op.elem = (int32_t)i;
+    op.z_start = last_z;
+    op.z_end = z;
+    op.Po = Po;
+    switch (e->type) {
+    case EH_T_MARK:
+    case EH_T_CHARGE:
+      keep = 0; /* NO_APERTURE; the NaN cull after them cannot fire (every element ends with it) */
+      break;
+    case EH_T_MAXAMP:
+      if (e->p.maxamp.elliptical || e->p.maxamp.openSide) {
+        snprintf(msg, msglen, "%s: only rectangular MAXAMP without OPEN_SIDE is supported", e->name);
+        return EH_ERR_UNSUPPORTED;
+      }
+      maxamp = &e->p.maxamp;
+      x_max = maxamp->x_max;
+      y_max = maxamp->y_max;
+      op.op = EH_OP_APERTURE;
+      break;
+    case EH_T_DRIF: {
+      int order = e->p.drift.order ? e->p.drift.order : 2;
+      op.op = order >= 2 ? EH_OP_DRIFT2 : EH_OP_DRIFT1;
+      op.d[0] = e->p.drift.length;
+      op.d[1] = e->p.drift.length / 2;
+      op.flags |= EH_F_EXTRAP;
+      break;
+    }
+    case EH_T_EDRIFT:
+      op.op = EH_OP_EXACT_DRIFT;
+      op.d[0] = e->p.edrift.length;
+      break;
+    case EH_T_KQUAD:
+    case EH_T_KSEXT:
+    case EH_T_KOCT:
+      rc = compile_multipole(e, Po, maxamp, &op, msg, msglen);
+      break;
+    case EH_T_CSBEND:
+      rc = compile_csbend(e, maxamp, &op, msg, msglen);
+      break;
+    case EH_T_RFCA:
+      rc = compile_rfca(e, &op, msg, msglen);
+      if (rc == EH_OK && op.op == EH_OP_RFCA) {
+        if (prog->n_rfca >= EH_MAX_RFCA) {
+          snprintf(msg, msglen, "too many RFCA elements");
+          return EH_ERR_UNSUPPORTED;
+        }
+        prog->rfca_op[prog->n_rfca] = (int32_t)prog->nops;
+        prog->rfca_elem[prog->n_rfca] = (int32_t)i;
+        prog->n_rfca++;
+      }
+      break;
+    case EH_T_WAKE:
+    case EH_T_ZLONGIT:
+      if (prog->n_coll >= EH_MAX_COLL) {
+        snprintf(msg, msglen, "too many WAKE/ZLONGIT elements");
+        return EH_ERR_UNSUPPORTED;
+      }
+      op.op = e->type == EH_T_WAKE ? EH_OP_WAKE : EH_OP_ZLONGIT;
+      prog->coll_op[prog->n_coll] = (int32_t)prog->nops;
+      prog->coll_elem[prog->n_coll] = (int32_t)i;
+      op.nkicks_rf = prog->n_coll; /* collective slot */
+      prog->n_coll++;
+      break;
+    default:
+      snprintf(msg, msglen, "element %lld (%s): unsupported type %d", (long long)i, e->name, e->type);
+      return EH_ERR_UNSUPPORTED;
+    }
+    if (rc)
+      return rc;
+    if (!keep)
+      continue;
+    if (x_max || y_max) {
+      op.flags |= EH_F_AP;
+      op.ap_xmax = x_max;
+      op.ap_ymax = y_max;
+    }
+    if (prog->nops >= EH_MAX_OPS) {
+      snprintf(msg, msglen, "beamline longer than %d elements", EH_MAX_OPS);
+      return EH_ERR_UNSUPPORTED;
+    }
+    prog->ops[prog->nops++] = op;
+  }
+  return EH_OK;
+}

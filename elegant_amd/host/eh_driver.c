@@ -1,0 +1,900 @@
+/*
+ * eh_driver.c - the public C ABI: context, upload/download, the
+ * do_tracking-shaped pass loop, reference row order, and the
+ * element-by-element entry points.
+ *
+ * The device keeps particles in fixed slots with a per-slot loss record
+ * (pass, element, stage).  elegant instead moves each element's lost rows to
+ * the tail of the array (swap-to-tail, e.g. multipole.c:1065-1080).  The
+ * order that produces is a deterministic function of which rows were lost
+ * at each element, so it is rebuilt here, on the host, whenever a caller or
+ * an order-dependent reduction (the RFCA fiducial mean, simple_rfca.c:179-210)
+ * needs it.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "eh_consts.h"
+#include "eh_host.h"
+
+const char *ehd_message(const ehd_dev *d);
+
+struct eh_ctx {
+  ehd_dev *dev;
+  char msg[512];
+  eh_program *prog;
+  eh_element *elems;
+  int64_t n_elems;
+  double Po;
+  int loaded;
+  /* particles */
+  int64_t n;
+  double *ids;
+  int64_t *order, *pos, *scratch;
+  int64_t n_alive_ref;
+  uint8_t *applied;
+  int32_t *h_pass, *h_code;
+  double *h_s, *h_dp;
+  /* per-program state (reference element-struct state) */
+  int fid_seen[EH_MAX_RFCA];
+  double phase_fid[EH_MAX_RFCA];
+  double macro_charge;
+  int charge_seen;
+  int coll_loaded[EH_MAX_COLL];
+  int32_t zl_nbins[EH_MAX_COLL];
+  double zl_binsize[EH_MAX_COLL];
+  int64_t pass_offset0; /* first pass of the run (CHARGE acts on it) */
+  int64_t n_original;
+  int32_t chunk_passes;
+};
+
+static int fail(eh_ctx *c, int code, const char *m) {
+  if (c)
+    snprintf(c->msg, sizeof(c->msg), "%s", m);
+  return code;
+}
+
+static int devfail(eh_ctx *c, int code) {
+  if (c && code)
+    snprintf(c->msg, sizeof(c->msg), "%s", ehd_message(c->dev));
+  return code;
+}
+
+const char *eh_strerror(int code) {
+  switch (code) {
+  case EH_OK:
+    return "ok";
+  case EH_ERR_INVALID:
+    return "invalid argument or element parameter";
+  case EH_ERR_UNSUPPORTED:
+    return "element feature outside the implemented subset";
+  case EH_ERR_HIP:
+    return "HIP runtime failure";
+  case EH_ERR_NOMEM:
+    return "out of memory";
+  case EH_ERR_STATE:
+    return "call out of order";
+  case EH_ERR_NODEVICE:
+    return "no GPU device";
+  case EH_ERR_LONGBEAM:
+    return "bunch longer than the wake or impedance window";
+  case EH_ERR_FIDUCIAL:
+    return "no particle available for RF fiducialization";
+  default:
+    return "unknown error";
+  }
+}
+
+const char *eh_last_error(const eh_ctx *c) { return c ? c->msg : ""; }
+
+int eh_ctx_create(int device, eh_ctx **out) {
+  eh_ctx *c;
+  int rc;
+  if (!out)
+    return EH_ERR_INVALID;
+  *out = NULL;
+  c = (eh_ctx *)calloc(1, sizeof(eh_ctx));
+  if (!c)
+    return EH_ERR_NOMEM;
+  c->prog = (eh_program *)calloc(1, sizeof(eh_program));
+  if (!c->prog) {
+    free(c);
+    return EH_ERR_NOMEM;
+  }
+  rc = ehd_create(device, &c->dev, c->msg, sizeof(c->msg));
+  if (rc) {
+    free(c->prog);
+    free(c);
+    return rc;
+  }
+  c->chunk_passes = 64;
+  *out = c;
+  return EH_OK;
+}
+
+static void free_particles(eh_ctx *c) {
+  free(c->ids);
+  free(c->order);
+  free(c->pos);
+  free(c->scratch);
+  free(c->applied);
+  free(c->h_pass);
+  free(c->h_code);
+  free(c->h_s);
+  free(c->h_dp);
+  c->ids = NULL;
+  c->order = c->pos = c->scratch = NULL;
+  c->applied = NULL;
+  c->h_pass = c->h_code = NULL;
+  c->h_s = c->h_dp = NULL;
+  c->n = 0;
+}
+
+int eh_ctx_destroy(eh_ctx *c) {
+  if (!c)
+    return EH_OK;
+  free_particles(c);
+  free(c->elems);
+  free(c->prog);
+  ehd_destroy(c->dev);
+  free(c);
+  return EH_OK;
+}
+
+void eh_set_chunk_passes(eh_ctx *c, int32_t passes) {
+  if (c && passes > 0)
+    c->chunk_passes = passes;
+}
+
+void eh_set_block(eh_ctx *c, int32_t block) {
+  if (c)
+    ehd_set_block(c->dev, block);
+}
+
+/* ------------------------------------------------------------ program */
+
+int eh_load_program(eh_ctx *c, const eh_element *elems, int64_t n_elems, double Po) {
+  int rc;
+  if (!c || (!elems && n_elems > 0) || n_elems < 0 || !(Po > 0))
+    return fail(c, EH_ERR_INVALID, "eh_load_program: bad arguments");
+  c->msg[0] = 0;
+  rc = eh_compile(elems, n_elems, Po, c->prog, c->msg, sizeof(c->msg));
+  if (rc)
+    return rc;
+  free(c->elems);
+  c->elems = (eh_element *)malloc(sizeof(eh_element) * (n_elems > 0 ? n_elems : 1));
+  if (!c->elems)
+    return EH_ERR_NOMEM;
+  memcpy(c->elems, elems, sizeof(eh_element) * n_elems);
+  c->n_elems = n_elems;
+  c->Po = Po;
+  memset(c->fid_seen, 0, sizeof(c->fid_seen));
+  memset(c->coll_loaded, 0, sizeof(c->coll_loaded));
+  c->charge_seen = 0;
+  c->macro_charge = 0;
+  /* RFCA with a given T_REFERENCE or LIGHT fiducial needs no particles (simple_rfca.c:429-431) */
+  for (int k = 0; k < c->prog->n_rfca; k++) {
+    const EH_RFCA *rf = &elems[c->prog->rfca_elem[k]].p.rfca;
+    eh_op *op = &c->prog->ops[c->prog->rfca_op[k]];
+    double t0;
+    if (rf->tReference != -1 || rf->fiducialMode == EH_FID_LIGHT) {
+      if (rf->tReference != -1)
+        t0 = rf->tReference;
+      else {
+        double length = rf->length, zEnd = op->z_end;
+        long nKicks = length ? rf->nKicks : 1;
+        if (nKicks <= 0)
+          nKicks = 1;
+        length /= nKicks;
+        t0 = ((zEnd - rf->length) + length / 2) / EH_C_MKS;
+      }
+      c->phase_fid[k] = eh_rfca_phase_fiducial(rf, t0);
+      c->fid_seen[k] = 1;
+      eh_rfca_set_phase(op, rf, c->phase_fid[k]);
+    }
+  }
+  rc = ehd_put_ops(c->dev, c->prog->ops, c->prog->nops);
+  if (rc)
+    return devfail(c, rc);
+  c->loaded = 1;
+  return EH_OK;
+}
+
+/* ------------------------------------------------------- particle state */
+
+static int alloc_particles(eh_ctx *c, int64_t n) {
+  free_particles(c);
+  c->ids = (double *)malloc(sizeof(double) * (n + 1));
+  c->order = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  c->pos = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  c->scratch = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  c->applied = (uint8_t *)calloc(n + 1, 1);
+  c->h_pass = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+  c->h_code = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+  c->h_s = (double *)malloc(sizeof(double) * (n + 1));
+  c->h_dp = (double *)malloc(sizeof(double) * (n + 1));
+  if (!c->ids || !c->order || !c->pos || !c->scratch || !c->applied || !c->h_pass || !c->h_code || !c->h_s ||
+      !c->h_dp) {
+    free_particles(c);
+    return EH_ERR_NOMEM;
+  }
+  c->n = n;
+  for (int64_t i = 0; i < n; i++) {
+    c->order[i] = i;
+    c->pos[i] = i;
+  }
+  c->n_alive_ref = n;
+  return EH_OK;
+}
+
+/* AoS rows -> SoA, with do_tracking's initial NaN/Inf cull (do_tracking.c:468-473):
+ * such rows are recorded as lost on pass 0 before the first element. */
+int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) {
+  int rc;
+  double *soa;
+  if (!c || (!coord && n > 0) || n < 0 || nprops < 7)
+    return fail(c, EH_ERR_INVALID, "eh_upload: bad arguments");
+  if (n > 0x7fffffff)
+    return fail(c, EH_ERR_INVALID, "eh_upload: more than 2^31-1 particles");
+  rc = alloc_particles(c, n);
+  if (rc)
+    return rc;
+  soa = (double *)malloc(sizeof(double) * 6 * (n + 1));
+  if (!soa)
+    return EH_ERR_NOMEM;
+  {
+    int64_t nlost = 0;
+    for (int64_t i = 0; i < n; i++) {
+      double *r = coord[i];
+      for (int k = 0; k < 6; k++)
+        soa[k * n + i] = r[k];
+      c->ids[i] = r[6];
+      c->h_pass[i] = -1;
+      c->h_code[i] = 0;
+      if (isinf(r[0]) || isinf(r[2]) || isnan(r[0]) || isnan(r[2])) {
+        /* limit_amplitudes(DBL_MAX) at z = 0: s = 0, delta = Po(1+delta) */
+        soa[4 * n + i] = 0.0;
+        soa[5 * n + i] = c->Po * (1 + r[5]);
+        c->h_pass[i] = 0;
+        c->h_code[i] = -1;
+        nlost++;
+      }
+    }
+    (void)nlost;
+  }
+  rc = ehd_alloc_state(c->dev, n);
+  if (!rc)
+    rc = ehd_put_state(c->dev, soa, soa + n, soa + 2 * n, soa + 3 * n, soa + 4 * n, soa + 5 * n, c->h_pass, c->h_code,
+                       n);
+  free(soa);
+  if (rc)
+    return devfail(c, rc);
+  return EH_OK;
+}
+
+/* ----------------------------------------------------- reference order */
+
+/* Swap-to-tail compaction of one element call's losses
+ * (multipole.c:1046-1081, limit_amplitudes.c:217-268):
+ *   for (ip = 0; ip <= itop; ip++) if lost(ip) { swap(ip, itop); itop--; ip--; }
+ * Survivors above the new nLeft fill the holes from the top down; the cost
+ * is O(k log k) for k lost rows, not O(n). */
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void eh_compact_event(int64_t *order, int64_t *pos, int64_t n_alive, const int64_t *lost_slots, int64_t k,
+                      int64_t *lp) {
+  int64_t itop = n_alive - 1, li = 0, hi = k - 1;
+  for (int64_t j = 0; j < k; j++)
+    lp[j] = pos[lost_slots[j]];
+  qsort(lp, (size_t)k, sizeof(int64_t), cmp_i64);
+  while (li <= hi) {
+    int64_t ip = lp[li];
+    for (;;) {
+      int64_t a = order[ip], b = order[itop];
+      order[ip] = b;
+      order[itop] = a;
+      pos[b] = ip;
+      pos[a] = itop;
+      if (lp[hi] == itop) {
+        /* the row brought down from the top is lost as well: examine it again */
+        itop--;
+        hi--;
+        if (li > hi || ip > itop)
+          break;
+      } else {
+        itop--;
+        li++;
+        break;
+      }
+    }
+  }
+}
+
+typedef struct {
+  int64_t pass;
+  int32_t code;
+  int64_t slot;
+} loss_ev;
+
+static int cmp_ev(const void *a, const void *b) {
+  const loss_ev *x = (const loss_ev *)a, *y = (const loss_ev *)b;
+  if (x->pass != y->pass)
+    return x->pass < y->pass ? -1 : 1;
+  if (x->code != y->code)
+    return x->code < y->code ? -1 : 1;
+  return 0;
+}
+
+/* Apply, in order, every loss event not yet applied whose (pass, code) is
+ * strictly before (pass_lim, code_lim).  h_pass/h_code must be current. */
+static int replay_until(eh_ctx *c, int64_t pass_lim, int32_t code_lim) {
+  int64_t m = 0;
+  loss_ev *ev;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] >= 0 && !c->applied[i])
+      m++;
+  if (!m)
+    return EH_OK;
+  ev = (loss_ev *)malloc(sizeof(loss_ev) * m);
+  if (!ev)
+    return EH_ERR_NOMEM;
+  m = 0;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] >= 0 && !c->applied[i]) {
+      if (c->h_pass[i] < pass_lim || (c->h_pass[i] == pass_lim && c->h_code[i] < code_lim)) {
+        ev[m].pass = c->h_pass[i];
+        ev[m].code = c->h_code[i];
+        ev[m].slot = i;
+        m++;
+      }
+    }
+  qsort(ev, (size_t)m, sizeof(loss_ev), cmp_ev);
+  {
+    int64_t a = 0;
+    while (a < m) {
+      int64_t b = a;
+      while (b < m && ev[b].pass == ev[a].pass && ev[b].code == ev[a].code)
+        b++;
+      for (int64_t j = a; j < b; j++)
+        c->scratch[j - a] = ev[j].slot;
+      {
+        /* compaction needs the slot list and a position buffer */
+        int64_t k = b - a;
+        int64_t *lp = (int64_t *)malloc(sizeof(int64_t) * (k > 0 ? k : 1));
+        if (!lp) {
+          free(ev);
+          return EH_ERR_NOMEM;
+        }
+        eh_compact_event(c->order, c->pos, c->n_alive_ref, c->scratch, k, lp);
+        free(lp);
+        c->n_alive_ref -= k;
+      }
+      for (int64_t j = a; j < b; j++)
+        c->applied[ev[j].slot] = 1;
+      a = b;
+    }
+  }
+  free(ev);
+  return EH_OK;
+}
+
+static int refresh_losses(eh_ctx *c) {
+  int rc = ehd_get_state(c->dev, NULL, NULL, NULL, NULL, NULL, NULL, c->h_pass, c->h_code, c->n);
+  return rc ? devfail(c, rc) : EH_OK;
+}
+
+/* ------------------------------------------------------------ tracking */
+
+/* RFCA TMEAN fiducial on the first pass: KahanPlus mean of arrival times over
+ * the particles in reference row order (simple_rfca.c:179-210, kahan.c:27-36) */
+static double kahan_plus(double oldSum, double b, double *error) {
+  volatile double sum, Y;
+  Y = b + *error;
+  sum = oldSum + Y;
+  *error = Y - (sum - oldSum);
+  return sum;
+}
+
+static int rfca_fiducial(eh_ctx *c, int k, int64_t pass) {
+  const int32_t opi = c->prog->rfca_op[k];
+  eh_op *op = &c->prog->ops[opi];
+  const EH_RFCA *rf = &c->elems[c->prog->rfca_elem[k]].p.rfca;
+  double length = rf->length, sOffset, tsum = 0, error = 0, t0;
+  long nKicks = length ? rf->nKicks : 1;
+  int64_t nsum = 0;
+  int rc;
+  if (nKicks <= 0)
+    nKicks = 1;
+  length /= nKicks;
+  sOffset = length / 2;
+  rc = ehd_get_state(c->dev, NULL, NULL, NULL, NULL, c->h_s, c->h_dp, c->h_pass, c->h_code, c->n);
+  if (rc)
+    return devfail(c, rc);
+  rc = replay_until(c, pass, 4 * opi);
+  if (rc)
+    return rc;
+  for (int64_t j = 0; j < c->n_alive_ref; j++) {
+    int64_t i = c->order[j];
+    double p = c->Po * (1 + c->h_dp[i]);
+    double beta = p / sqrt(p * p + 1);
+    tsum = kahan_plus(tsum, (c->h_s[i] + sOffset) / (EH_C_MKS * beta), &error);
+    nsum++;
+  }
+  if (nsum <= 0)
+    return fail(c, EH_ERR_FIDUCIAL, "RFCA: no particle available for fiducialization");
+  t0 = tsum / nsum;
+  c->phase_fid[k] = eh_rfca_phase_fiducial(rf, t0);
+  c->fid_seen[k] = 1;
+  eh_rfca_set_phase(op, rf, c->phase_fid[k]);
+  rc = ehd_patch_op(c->dev, opi, op);
+  return rc ? devfail(c, rc) : EH_OK;
+}
+
+int eh_collective(eh_ctx *c, int slot, int64_t pass, int64_t n_alive);
+
+/* index of the first op >= from that needs the host on this pass, or nops */
+static int32_t next_barrier(const eh_ctx *c, int32_t from) {
+  const eh_program *p = c->prog;
+  int32_t best = p->nops;
+  for (int k = 0; k < p->n_rfca; k++)
+    if (!c->fid_seen[k] && p->rfca_op[k] >= from && p->rfca_op[k] < best)
+      best = p->rfca_op[k];
+  for (int k = 0; k < p->n_coll; k++)
+    if (p->coll_op[k] >= from && p->coll_op[k] < best)
+      best = p->coll_op[k];
+  return best;
+}
+
+static void charge_elements(eh_ctx *c, int64_t pass) {
+  for (int64_t i = 0; i < c->n_elems; i++) {
+    const eh_element *e = &c->elems[i];
+    if (e->type != EH_T_CHARGE)
+      continue;
+    /* CHARGE (do_tracking.c:1324-1353) */
+    if (pass == c->pass_offset0) {
+      c->macro_charge = 0;
+      if (c->n_original)
+        c->macro_charge = e->p.charge.charge / c->n_original;
+      c->charge_seen = 1;
+    }
+    if (e->p.charge.chargePerParticle)
+      c->macro_charge = e->p.charge.chargePerParticle;
+  }
+}
+
+int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_stats *stats) {
+  int rc;
+  int64_t n_alive0 = 0;
+  if (!c || !P_central || npasses < 0)
+    return fail(c, EH_ERR_INVALID, "eh_track: bad arguments");
+  if (!c->loaded)
+    return fail(c, EH_ERR_STATE, "eh_track: no program loaded");
+  if (*P_central != c->Po)
+    return fail(c, EH_ERR_INVALID, "eh_track: P_central differs from the program's Po");
+  if (c->n == 0) {
+    if (stats)
+      memset(stats, 0, sizeof(*stats));
+    return EH_OK;
+  }
+  ehd_timing_reset(c->dev);
+  rc = refresh_losses(c);
+  if (rc)
+    return rc;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] < 0 || c->h_pass[i] >= pass0)
+      n_alive0++;
+  {
+    int64_t pass = pass0, end = pass0 + npasses;
+    while (pass < end) {
+      int32_t b = next_barrier(c, 0);
+      charge_elements(c, pass);
+      if (b >= c->prog->nops) {
+        /* no host step in this pass: run many passes in one launch */
+        int64_t chunk = end - pass;
+        if (chunk > c->chunk_passes)
+          chunk = c->chunk_passes;
+        rc = ehd_track(c->dev, pass, (int32_t)chunk, 0, c->prog->nops);
+        if (rc)
+          return devfail(c, rc);
+        pass += chunk;
+        continue;
+      }
+      /* segmented pass */
+      {
+        int32_t cur = 0;
+        while (cur < c->prog->nops) {
+          b = next_barrier(c, cur);
+          if (b > cur) {
+            rc = ehd_track(c->dev, pass, 1, cur, b);
+            if (rc)
+              return devfail(c, rc);
+          }
+          if (b >= c->prog->nops)
+            break;
+          {
+            const eh_op *op = &c->prog->ops[b];
+            if (op->op == EH_OP_RFCA) {
+              int k;
+              for (k = 0; k < c->prog->n_rfca; k++)
+                if (c->prog->rfca_op[k] == b)
+                  break;
+              rc = rfca_fiducial(c, k, pass);
+              if (rc)
+                return rc;
+              cur = b; /* the RFCA itself runs in the next launch */
+              if (next_barrier(c, cur) == cur) {
+                rc = ehd_track(c->dev, pass, 1, cur, cur + 1);
+                if (rc)
+                  return devfail(c, rc);
+                cur = cur + 1;
+              }
+            } else {
+              rc = eh_collective(c, op->nkicks_rf, pass, 0);
+              if (rc)
+                return rc;
+              cur = b + 1;
+            }
+          }
+        }
+        pass++;
+      }
+    }
+  }
+  rc = refresh_losses(c);
+  if (rc)
+    return rc;
+  if (stats) {
+    int64_t effort = 0, left = 0;
+    for (int64_t i = 0; i < c->n; i++) {
+      int64_t lp = c->h_pass[i];
+      if (lp >= 0 && lp < pass0)
+        continue;
+      if (lp < 0) {
+        effort += npasses;
+        left++;
+      } else
+        effort += lp - pass0;
+    }
+    memset(stats, 0, sizeof(*stats));
+    stats->effort = effort;
+    stats->n_left = left;
+    stats->passes_done = npasses;
+    ehd_timing(c->dev, &stats->kernel_launches, &stats->track_ms);
+    ehd_collective_timing(c->dev, &left, &stats->collective_ms);
+  }
+  (void)n_alive0;
+  return EH_OK;
+}
+
+int eh_last_kernel_timing(const eh_ctx *c, int64_t *launches, double *total_ms) {
+  if (!c || !launches || !total_ms)
+    return EH_ERR_INVALID;
+  return ehd_timing(c->dev, launches, total_ms);
+}
+
+/* rows in reference order: survivors, then lost rows (tail order of the
+ * compactions), column 7 = pass of loss (-1 for survivors) */
+int eh_download(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int64_t *nLeft) {
+  double *soa;
+  int rc;
+  if (!c || (!coord && n > 0) || nprops < 8 || n != c->n)
+    return fail(c, EH_ERR_INVALID, "eh_download: bad arguments");
+  soa = (double *)malloc(sizeof(double) * 6 * (n + 1));
+  if (!soa)
+    return EH_ERR_NOMEM;
+  rc = ehd_get_state(c->dev, soa, soa + n, soa + 2 * n, soa + 3 * n, soa + 4 * n, soa + 5 * n, c->h_pass, c->h_code,
+                     n);
+  if (rc) {
+    free(soa);
+    return devfail(c, rc);
+  }
+  rc = replay_until(c, INT64_MAX, INT32_MAX);
+  if (rc) {
+    free(soa);
+    return rc;
+  }
+  for (int64_t j = 0; j < n; j++) {
+    int64_t i = c->order[j];
+    double *r = coord[j];
+    for (int k = 0; k < 6; k++)
+      r[k] = soa[k * n + i];
+    r[6] = c->ids[i];
+    r[7] = c->h_pass[i] >= 0 ? (double)c->h_pass[i] : -1.0;
+  }
+  if (nLeft)
+    *nLeft = c->n_alive_ref;
+  free(soa);
+  return EH_OK;
+}
+
+int64_t eh_do_tracking(eh_ctx *c, double **coord, int64_t nOriginal, int64_t *effort, const eh_element *beamline,
+                       int64_t n_elems, double *P_central, int64_t n_passes, int64_t passOffset) {
+  eh_stats st;
+  int64_t nLeft = 0;
+  int rc;
+  if (!c || !P_central)
+    return EH_ERR_INVALID;
+  rc = eh_load_program(c, beamline, n_elems, *P_central);
+  if (rc)
+    return rc;
+  rc = eh_upload(c, coord, nOriginal, 8);
+  if (rc)
+    return rc;
+  c->pass_offset0 = passOffset;
+  c->n_original = nOriginal;
+  rc = eh_track(c, passOffset, n_passes, P_central, &st);
+  if (rc)
+    return rc;
+  rc = eh_download(c, coord, nOriginal, 8, &nLeft);
+  if (rc)
+    return rc;
+  if (effort)
+    *effort += st.effort;
+  return nLeft;
+}
+
+/* ------------------------------------------------ element entry points */
+
+/* run a one-element program on host rows and compact them like the
+ * reference element routine (lost rows to the tail) */
+static int64_t run_single(eh_ctx *c, double **rows, int64_t n, const eh_element *e, double Po, double z_start,
+                          const EH_MAXAMP *maxamp, int64_t pass) {
+  eh_element bl[2];
+  int nb = 0, rc;
+  int64_t nLeft = 0;
+  eh_stats st;
+  if (n <= 0)
+    return 0;
+  if (maxamp) {
+    eh_element_defaults(&bl[nb], EH_T_MAXAMP);
+    bl[nb].p.maxamp = *maxamp;
+    nb++;
+  }
+  bl[nb++] = *e;
+  rc = eh_compile(bl, nb, Po, c->prog, c->msg, sizeof(c->msg));
+  if (rc)
+    return rc;
+  /* keep only the element itself, without the checks do_tracking adds after it */
+  c->prog->ops[0] = c->prog->ops[c->prog->nops - 1];
+  c->prog->nops = 1;
+  c->prog->ops[0].flags &= ~EH_F_AP;
+  c->prog->ops[0].flags |= 0;
+  c->prog->ops[0].z_start = z_start;
+  c->prog->ops[0].z_end = z_start + eh_element_length(e);
+  c->prog->ops[0].op = c->prog->ops[0].op;
+  c->Po = Po;
+  c->loaded = 1;
+  rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+  if (rc)
+    return devfail(c, rc);
+  rc = eh_upload(c, rows, n, 8);
+  if (rc)
+    return rc;
+  rc = ehd_track(c->dev, pass, 1, 0, 1);
+  if (rc)
+    return devfail(c, rc);
+  {
+    double **tmp = (double **)malloc(sizeof(double *) * n);
+    double *buf = (double *)malloc(sizeof(double) * 8 * n);
+    if (!tmp || !buf) {
+      free(tmp);
+      free(buf);
+      return EH_ERR_NOMEM;
+    }
+    for (int64_t i = 0; i < n; i++)
+      tmp[i] = buf + 8 * i;
+    rc = eh_download(c, tmp, n, 8, &nLeft);
+    if (!rc)
+      for (int64_t i = 0; i < n; i++) {
+        int ncopy = 7;
+        memcpy(rows[i], tmp[i], sizeof(double) * ncopy);
+      }
+    free(tmp);
+    free(buf);
+    if (rc)
+      return rc;
+  }
+  (void)st;
+  return nLeft;
+}
+
+int64_t eh_multipole_tracking2(eh_ctx *c, double **particle, int64_t n_part, const eh_element *elem, double Po,
+                               double z_start, const EH_MAXAMP *maxamp) {
+  if (!c || !elem || (elem->type != EH_T_KQUAD && elem->type != EH_T_KSEXT && elem->type != EH_T_KOCT))
+    return fail(c, EH_ERR_INVALID, "eh_multipole_tracking2: element must be KQUAD, KSEXT or KOCT");
+  return run_single(c, particle, n_part, elem, Po, z_start, maxamp, 0);
+}
+
+int64_t eh_track_through_csbend(eh_ctx *c, double **part, int64_t n_part, const EH_CSBEND *csbend, double Po,
+                                double z_start, const EH_MAXAMP *maxamp) {
+  eh_element e;
+  if (!c || !csbend)
+    return EH_ERR_INVALID;
+  eh_element_defaults(&e, EH_T_CSBEND);
+  e.p.csbend = *csbend;
+  return run_single(c, part, n_part, &e, Po, z_start, maxamp, 0);
+}
+
+int64_t eh_track_particles_drift(eh_ctx *c, double **part, int64_t n_part, const EH_DRIFT *drift) {
+  eh_element e;
+  if (!c || !drift)
+    return EH_ERR_INVALID;
+  eh_element_defaults(&e, EH_T_DRIF);
+  e.p.drift = *drift;
+  return run_single(c, part, n_part, &e, 1.0, 0.0, NULL, 0);
+}
+
+int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA *rfca, double *P_central,
+                            double zEnd, double *phase_fiducial) {
+  eh_element e;
+  int64_t r;
+  if (!c || !rfca || !P_central || !phase_fiducial)
+    return EH_ERR_INVALID;
+  if (np <= 0)
+    return np;
+  eh_element_defaults(&e, EH_T_RFCA);
+  e.p.rfca = *rfca;
+  if (isnan(*phase_fiducial)) {
+    if (rfca->tReference != -1 || rfca->fiducialMode == EH_FID_LIGHT) {
+      double length = rfca->length, t0;
+      long nKicks = length ? rfca->nKicks : 1;
+      if (nKicks <= 0)
+        nKicks = 1;
+      length /= nKicks;
+      t0 = rfca->tReference != -1 ? rfca->tReference : ((zEnd - rfca->length) + length / 2) / EH_C_MKS;
+      *phase_fiducial = eh_rfca_phase_fiducial(rfca, t0);
+    } else {
+      double length = rfca->length, tsum = 0, error = 0;
+      long nKicks = length ? rfca->nKicks : 1;
+      if (nKicks <= 0)
+        nKicks = 1;
+      length /= nKicks;
+      for (int64_t i = 0; i < np; i++) {
+        double p = *P_central * (1 + part[i][5]);
+        tsum = kahan_plus(tsum, (part[i][4] + length / 2) / (EH_C_MKS * (p / sqrt(p * p + 1))), &error);
+      }
+      *phase_fiducial = eh_rfca_phase_fiducial(rfca, tsum / np);
+    }
+  }
+  {
+    /* compile the cavity with the known phase */
+    eh_element bl[1];
+    int rc;
+    int64_t nLeft = 0;
+    bl[0] = e;
+    rc = eh_compile(bl, 1, *P_central, c->prog, c->msg, sizeof(c->msg));
+    if (rc)
+      return rc;
+    if (c->prog->ops[0].op == EH_OP_RFCA)
+      eh_rfca_set_phase(&c->prog->ops[0], rfca, *phase_fiducial);
+    c->prog->ops[0].flags &= ~EH_F_AP;
+    c->prog->ops[0].z_end = zEnd;
+    c->Po = *P_central;
+    c->loaded = 1;
+    rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+    if (rc)
+      return devfail(c, rc);
+    rc = eh_upload(c, part, np, 8);
+    if (rc)
+      return rc;
+    rc = ehd_track(c->dev, 0, 1, 0, 1);
+    if (rc)
+      return devfail(c, rc);
+    {
+      double **tmp = (double **)malloc(sizeof(double *) * np);
+      double *buf = (double *)malloc(sizeof(double) * 8 * np);
+      if (!tmp || !buf) {
+        free(tmp);
+        free(buf);
+        return EH_ERR_NOMEM;
+      }
+      for (int64_t i = 0; i < np; i++)
+        tmp[i] = buf + 8 * i;
+      rc = eh_download(c, tmp, np, 8, &nLeft);
+      if (!rc)
+        for (int64_t i = 0; i < np; i++)
+          memcpy(part[i], tmp[i], sizeof(double) * 7);
+      free(tmp);
+      free(buf);
+      if (rc)
+        return rc;
+    }
+    r = nLeft;
+  }
+  return r;
+}
+
+int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
+                            int32_t extrapolate_z) {
+  /* the aperture check of do_tracking.c:2463-2467 after a zero-length MARK-like step */
+  eh_element e;
+  if (!c)
+    return EH_ERR_INVALID;
+  if (np <= 0 || (xmax < 0 && ymax < 0))
+    return np;
+  eh_element_defaults(&e, EH_T_DRIF);
+  e.p.drift.length = 0;
+  {
+    eh_element bl[2];
+    int rc;
+    int64_t nLeft = 0;
+    eh_element_defaults(&bl[0], EH_T_MAXAMP);
+    bl[0].p.maxamp.x_max = xmax;
+    bl[0].p.maxamp.y_max = ymax;
+    bl[1] = e;
+    rc = eh_compile(bl, 2, Po, c->prog, c->msg, sizeof(c->msg));
+    if (rc)
+      return rc;
+    c->prog->ops[0] = c->prog->ops[1];
+    c->prog->ops[0].op = EH_OP_NOP;
+    c->prog->ops[0].z_end = z;
+    if (!extrapolate_z)
+      c->prog->ops[0].flags &= ~EH_F_EXTRAP;
+    c->prog->nops = 1;
+    c->Po = Po;
+    c->loaded = 1;
+    rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+    if (rc)
+      return devfail(c, rc);
+    rc = eh_upload(c, coord, np, 8);
+    if (rc)
+      return rc;
+    rc = ehd_track(c->dev, 0, 1, 0, 1);
+    if (rc)
+      return devfail(c, rc);
+    {
+      double **tmp = (double **)malloc(sizeof(double *) * np);
+      double *buf = (double *)malloc(sizeof(double) * 8 * np);
+      if (!tmp || !buf) {
+        free(tmp);
+        free(buf);
+        return EH_ERR_NOMEM;
+      }
+      for (int64_t i = 0; i < np; i++)
+        tmp[i] = buf + 8 * i;
+      rc = eh_download(c, tmp, np, 8, &nLeft);
+      if (!rc)
+        for (int64_t i = 0; i < np; i++)
+          memcpy(coord[i], tmp[i], sizeof(double) * 7);
+      free(tmp);
+      free(buf);
+      if (rc)
+        return rc;
+    }
+    return nLeft;
+  }
+}
+
+/* debugging/measurement helpers (not in the public header) */
+int eh_get_loss_records(eh_ctx *c, int32_t *pass, int32_t *code) {
+  int rc = refresh_losses(c);
+  if (rc)
+    return rc;
+  memcpy(pass, c->h_pass, sizeof(int32_t) * c->n);
+  memcpy(code, c->h_code, sizeof(int32_t) * c->n);
+  return EH_OK;
+}
+
+int eh_math_probe(eh_ctx *c, int32_t fn, const double *in, const double *in2, double *out, int64_t n) {
+  int rc = ehd_math_probe(c->dev, fn, in, in2, out, n);
+  return rc ? devfail(c, rc) : EH_OK;
+}
+
+/* accessors used by the collective step (eh_collective.c) */
+ehd_dev *eh_ctx_dev(eh_ctx *c) { return c->dev; }
+const eh_program *eh_ctx_prog(eh_ctx *c) { return c->prog; }
+const eh_element *eh_ctx_elems(eh_ctx *c) { return c->elems; }
+double eh_ctx_macro_charge(eh_ctx *c, int *seen) {
+  *seen = c->charge_seen;
+  return c->macro_charge;
+}
+char *eh_ctx_msg(eh_ctx *c) { return c->msg; }
+int *eh_ctx_coll_loaded(eh_ctx *c) { return c->coll_loaded; }
+int32_t *eh_ctx_zl_nbins(eh_ctx *c) { return c->zl_nbins; }
+double *eh_ctx_zl_binsize(eh_ctx *c) { return c->zl_binsize; }

@@ -1,0 +1,235 @@
+"""Loads libelegant_hip.so (built in-tree by elegant_amd/build.py) and wraps
+its C ABI.  There is no fallback: if the library is missing, or no GPU is
+present when a context is created, the call raises."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as A
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libelegant_hip.so")
+_lib = None
+
+P = C.c_void_p
+PP = C.POINTER(C.POINTER(C.c_double))
+PE = C.POINTER(A.eh_element)
+PD = C.POINTER(C.c_double)
+
+
+class EHError(RuntimeError):
+    def __init__(self, code, message=""):
+        self.code = code
+        super().__init__("elegant_hip error %d (%s)%s" % (code, ERRNAMES.get(code, "?"),
+                                                           (": " + message) if message else ""))
+
+
+ERRNAMES = {A.EH_ERR_INVALID: "invalid", A.EH_ERR_UNSUPPORTED: "unsupported", A.EH_ERR_HIP: "hip",
+            A.EH_ERR_NOMEM: "nomem", A.EH_ERR_STATE: "state", A.EH_ERR_NODEVICE: "nodevice",
+            A.EH_ERR_LONGBEAM: "longbeam", A.EH_ERR_FIDUCIAL: "fiducial"}
+
+# every symbol include/elegant_hip.h declares: (name, restype, argtypes)
+SIGNATURES = [
+    ("eh_element_defaults", C.c_int, [PE, C.c_int32]),
+    ("eh_struct_size", C.c_int64, [C.c_int32]),
+    ("eh_ctx_create", C.c_int, [C.c_int, C.POINTER(P)]),
+    ("eh_ctx_destroy", C.c_int, [P]),
+    ("eh_strerror", C.c_char_p, [C.c_int]),
+    ("eh_last_error", C.c_char_p, [P]),
+    ("eh_load_program", C.c_int, [P, PE, C.c_int64, C.c_double]),
+    ("eh_upload", C.c_int, [P, PP, C.c_int64, C.c_int32]),
+    ("eh_track", C.c_int, [P, C.c_int64, C.c_int64, PD, C.POINTER(A.eh_stats)]),
+    ("eh_download", C.c_int, [P, PP, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
+    ("eh_do_tracking", C.c_int64, [P, PP, C.c_int64, C.POINTER(C.c_int64), PE, C.c_int64, PD, C.c_int64,
+                                   C.c_int64]),
+    ("eh_multipole_tracking2", C.c_int64, [P, PP, C.c_int64, PE, C.c_double, C.c_double,
+                                           C.POINTER(A.EH_MAXAMP)]),
+    ("eh_track_through_csbend", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_CSBEND), C.c_double, C.c_double,
+                                            C.POINTER(A.EH_MAXAMP)]),
+    ("eh_simple_rf_cavity", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_RFCA), PD, C.c_double, PD]),
+    ("eh_track_particles_drift", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_DRIFT)]),
+    ("eh_limit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                        C.c_int32]),
+    ("eh_track_through_wake", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_WAKE), C.c_double, C.c_int64,
+                                          C.c_double]),
+    ("eh_track_through_zlongit", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_ZLONGIT), C.c_double,
+                                             C.c_int64, C.c_double]),
+    ("eh_last_kernel_timing", C.c_int, [P, C.POINTER(C.c_int64), PD]),
+]
+# helpers outside the public header (measurement and tests)
+EXTRA = [
+    ("eh_get_loss_records", C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("eh_math_probe", C.c_int, [P, C.c_int32, PD, PD, PD, C.c_int64]),
+    ("eh_set_chunk_passes", None, [P, C.c_int32]),
+    ("eh_set_block", None, [P, C.c_int32]),
+    ("eh_collective_probe", C.c_int, [P, PP, C.c_int64, PE, C.c_double, C.c_int64, C.c_double, PD, PD,
+                                      C.POINTER(C.c_int64), PD]),
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libelegant_hip.so is not built: run `python -m elegant_amd.build` "
+                               "(or __graft_entry__.build())")
+        _lib = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES + EXTRA:
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def check(ctx, rc):
+    if rc < 0:
+        msg = ""
+        if ctx:
+            m = lib().eh_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise EHError(int(rc), msg or lib().eh_strerror(int(rc)).decode())
+    return rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(PD)
+
+
+class Tracker:
+    """One device context: a loaded beamline and a particle population.
+
+    Particle arrays are float64 (n, 8): x, x', y, y', s, delta, particleID,
+    pass of loss (track.h:81-101)."""
+
+    def __init__(self, device=0):
+        self._ctx = P()
+        check(None, lib().eh_ctx_create(device, C.byref(self._ctx)))
+        self._beamline = None
+        self.Po = None
+
+    def close(self):
+        if self._ctx:
+            lib().eh_ctx_destroy(self._ctx)
+            self._ctx = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, beamline, Po):
+        self._beamline = beamline
+        self.Po = float(Po)
+        check(self._ctx, lib().eh_load_program(self._ctx, beamline.pointer(), len(beamline), self.Po))
+
+    def upload(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        ptr, keep = A.rows_pointer(rows)
+        check(self._ctx, lib().eh_upload(self._ctx, ptr, rows.shape[0], rows.shape[1]))
+        self.n = rows.shape[0]
+
+    def track(self, pass0, npasses):
+        st = A.eh_stats()
+        pc = C.c_double(self.Po)
+        check(self._ctx, lib().eh_track(self._ctx, pass0, npasses, C.byref(pc), C.byref(st)))
+        return st
+
+    def download(self):
+        out = np.zeros((self.n, 8), dtype=np.float64)
+        ptr, keep = A.rows_pointer(out)
+        nleft = C.c_int64()
+        check(self._ctx, lib().eh_download(self._ctx, ptr, self.n, 8, C.byref(nleft)))
+        return out, int(nleft.value)
+
+    def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0):
+        """do_tracking on host rows (n, >=8), in place; returns (nLeft, effort)."""
+        if rows.dtype != np.float64 or not rows.flags.c_contiguous or rows.shape[1] < 8:
+            raise ValueError("rows must be C-contiguous float64 (n, >=8)")
+        ptr, keep = A.rows_pointer(rows)
+        eff = C.c_int64(0)
+        pc = C.c_double(Po)
+        self._beamline = beamline
+        self.Po = float(Po)
+        r = lib().eh_do_tracking(self._ctx, ptr, rows.shape[0], C.byref(eff), beamline.pointer(), len(beamline),
+                                 C.byref(pc), n_passes, pass_offset)
+        check(self._ctx, r)
+        self.n = rows.shape[0]
+        return int(r), int(eff.value)
+
+    def loss_records(self):
+        p = np.zeros(self.n, dtype=np.int32)
+        c = np.zeros(self.n, dtype=np.int32)
+        check(self._ctx, lib().eh_get_loss_records(self._ctx, p.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   c.ctypes.data_as(C.POINTER(C.c_int32))))
+        return p, c
+
+    def kernel_timing(self):
+        n = C.c_int64()
+        ms = C.c_double()
+        check(self._ctx, lib().eh_last_kernel_timing(self._ctx, C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+    def set_chunk_passes(self, k):
+        lib().eh_set_chunk_passes(self._ctx, int(k))
+
+    def set_block(self, b):
+        lib().eh_set_block(self._ctx, int(b))
+
+    def math_probe(self, fn, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = x if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        out = np.empty_like(x)
+        check(self._ctx, lib().eh_math_probe(self._ctx, fn, _dptr(x), _dptr(y), _dptr(out), x.size))
+        return out
+
+    def collective_probe(self, rows, elem, Po, i_pass, macro, cap=1 << 17):
+        """one WAKE/ZLONGIT step on rows (in place); returns (hist, volt, tmin)"""
+        ptr, keep = A.rows_pointer(rows)
+        h = np.zeros(cap)
+        v = np.zeros(cap)
+        nb = C.c_int64()
+        tmin = C.c_double()
+        check(self._ctx, lib().eh_collective_probe(self._ctx, ptr, rows.shape[0], C.byref(elem), Po, i_pass, macro,
+                                                   _dptr(h), _dptr(v), C.byref(nb), C.byref(tmin)))
+        return h[:nb.value], v[:nb.value], tmin.value
+
+    # element-by-element entry points (rows: float64 (n, 8), mutated in place)
+    def multipole_tracking2(self, rows, elem, Po, z_start, maxamp=None):
+        ptr, keep = A.rows_pointer(rows)
+        mp = C.byref(maxamp) if maxamp is not None else None
+        return check(self._ctx, lib().eh_multipole_tracking2(self._ctx, ptr, rows.shape[0], C.byref(elem), Po,
+                                                             z_start, mp))
+
+    def track_through_csbend(self, rows, csbend, Po, z_start, maxamp=None):
+        ptr, keep = A.rows_pointer(rows)
+        mp = C.byref(maxamp) if maxamp is not None else None
+        return check(self._ctx, lib().eh_track_through_csbend(self._ctx, ptr, rows.shape[0], C.byref(csbend), Po,
+                                                              z_start, mp))
+
+    def simple_rf_cavity(self, rows, rfca, Po, zEnd, phase_fiducial=float("nan")):
+        ptr, keep = A.rows_pointer(rows)
+        pc = C.c_double(Po)
+        ph = C.c_double(phase_fiducial)
+        r = check(self._ctx, lib().eh_simple_rf_cavity(self._ctx, ptr, rows.shape[0], C.byref(rfca), C.byref(pc),
+                                                       zEnd, C.byref(ph)))
+        return r, ph.value
+
+    def track_particles_drift(self, rows, drift):
+        ptr, keep = A.rows_pointer(rows)
+        return check(self._ctx, lib().eh_track_particles_drift(self._ctx, ptr, rows.shape[0], C.byref(drift)))
+
+    def limit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z):
+        ptr, keep = A.rows_pointer(rows)
+        return check(self._ctx, lib().eh_limit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], z, Po,
+                                                          extrapolate_z))
+
+    def track_through_wake(self, rows, wake, Po, i_pass, macro_charge):
+        ptr, keep = A.rows_pointer(rows)
+        return check(self._ctx, lib().eh_track_through_wake(self._ctx, ptr, rows.shape[0], C.byref(wake), Po,
+                                                            i_pass, macro_charge))
+
+    def track_through_zlongit(self, rows, zl, Po, i_pass, macro_charge):
+        ptr, keep = A.rows_pointer(rows)
+        return check(self._ctx, lib().eh_track_through_zlongit(self._ctx, ptr, rows.shape[0], C.byref(zl), Po,
+                                                               i_pass, macro_charge))

@@ -1,0 +1,262 @@
+/*
+ * elegant_hip.h - C ABI of the gfx950 6D tracking library (libelegant_hip.so).
+ *
+ * The library replaces the per-particle inner loops of elegant's tracking
+ * core.  The reference seam is the top-of-routine hook each element routine
+ * carries for an accelerator ("if (getElementOnGpu()) { ...; return nLeft; }",
+ * multipole.c:731-742, csbend.c:700-712, simple_rfca.c:267-289, wake.c:47-59,
+ * limit_amplitudes.c:200-213, matrix.c:306-321) and the lifecycle calls made
+ * by do_tracking (gpuBaseInit at do_tracking.c:401-404, gpuBaseDealloc at
+ * do_tracking.c:3104-3111).
+ *
+ * Conventions kept from the reference:
+ *  - particles are rows of a caller-owned double **coord; columns 0..5 are
+ *    (x, x', y, y', s, delta), column 6 the particle ID, column 7 the pass of
+ *    loss (track.h:81-101); rows are mutated in place;
+ *  - element routines return the survivor count nLeft; lost rows are moved to
+ *    [nLeft, n) by elegant's swap-to-tail compaction, with s replaced by the
+ *    loss position and delta by the absolute momentum Po(1+delta)
+ *    (multipole.c:1065-1080, limit_amplitudes.c:232-268);
+ *  - element parameters use elegant's names and defaults (track_data.c).
+ * Differences: errors are negative return codes (never exit()), there is no
+ * global mutable state, and one context per device/stream may be used from
+ * one host thread.
+ *
+ * All functions return EH_OK (0) or a negative EH_ERR_* code unless stated.
+ */
+#ifndef ELEGANT_HIP_H
+#define ELEGANT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EH_OK 0
+#define EH_ERR_INVALID (-1)     /* bad argument or element parameter */
+#define EH_ERR_UNSUPPORTED (-2) /* element feature outside the implemented subset */
+#define EH_ERR_HIP (-3)         /* HIP runtime failure */
+#define EH_ERR_NOMEM (-4)
+#define EH_ERR_STATE (-5)       /* call out of order (e.g. track before upload) */
+#define EH_ERR_NODEVICE (-6)
+#define EH_ERR_LONGBEAM (-7)    /* bunch longer than wake/impedance window (wake.c:134-140, zlongit.c:219-236) */
+#define EH_ERR_FIDUCIAL (-8)    /* no particle available for RF fiducialization (simple_rfca.c:213-214) */
+
+/* element type codes, named after elegant's entity names (track_data.c:72-173) */
+#define EH_T_MARK 1
+#define EH_T_DRIF 2
+#define EH_T_EDRIFT 3
+#define EH_T_KQUAD 4
+#define EH_T_KSEXT 5
+#define EH_T_KOCT 6
+#define EH_T_CSBEND 7
+#define EH_T_RFCA 8
+#define EH_T_WAKE 9
+#define EH_T_ZLONGIT 10
+#define EH_T_MAXAMP 11
+#define EH_T_CHARGE 12
+
+/* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
+ * (compute_matrices.c:1101-1104, matrix.c:389-429) */
+typedef struct {
+  double length;
+  int32_t order; /* 0 = run default (2); 1 or 2 */
+  int32_t pad;
+} EH_DRIFT;
+
+/* EDRIFT: exact drift (csbend.c:3767-3790) */
+typedef struct {
+  double length;
+} EH_EDRIFT;
+
+/* KQUAD (track.h:2151-2175; defaults track_data.c:1222-1283) */
+typedef struct {
+  double length, k1, tilt, dx, dy, dz, fse;
+  double xkick, ykick, xKickCalibration, yKickCalibration;
+  double bore, B, lEffective;
+  int32_t n_kicks, nSlices, integration_order, expandHamiltonian;
+  int32_t synch_rad, isr, edge1_effects, edge2_effects;
+  int32_t radial, malignMethod;
+} EH_KQUAD;
+
+/* KSEXT (track.h:2096-2113; defaults track_data.c:1120-1160) */
+typedef struct {
+  double length, k2, k1, j1, tilt, dx, dy, dz, fse;
+  double xkick, ykick, xKickCalibration, yKickCalibration;
+  double bore, B;
+  int32_t n_kicks, nSlices, integration_order, expandHamiltonian;
+  int32_t synch_rad, isr, malignMethod, pad;
+} EH_KSEXT;
+
+/* KOCT (track.h:2118-2129; defaults track_data.c:1164-1187) */
+typedef struct {
+  double length, k3, tilt, dx, dy, dz, fse, bore, B;
+  int32_t n_kicks, nSlices, integration_order, expandHamiltonian;
+  int32_t synch_rad, isr, malignMethod, pad;
+} EH_KOCT;
+
+/* CSBEND (track.h:2409-2448; defaults track_data.c:1893-1980) */
+typedef struct {
+  double length, angle, k1, k2, k3, k4;
+  double e1, e2, tilt, h1, h2, hgap, fint, fint1, fint2;
+  double dx, dy, dz, xKick, yKick;
+  double fse, fseDipole, fseQuadrupole, etilt;
+  double edge1_kick_limit, edge2_kick_limit;
+  int32_t nSlices, etiltSign, nonlinear, synch_rad;
+  int32_t edge1_effects, edge2_effects, edge_order, integration_order;
+  int32_t expandHamiltonian, isr, malignMethod, referenceCorrection;
+  int32_t kick_limit_scaling, fseCorrection;
+} EH_CSBEND;
+
+/* RFCA (track.h:1443-1457; defaults track_data.c:576-597) */
+#define EH_FID_TMEAN 0
+#define EH_FID_LIGHT 1
+#define EH_FID_FIRST 2
+#define EH_FID_PMAX 3
+typedef struct {
+  double length, volt, phase, freq, Q, dx, dy, tReference;
+  int32_t nKicks, change_p0, change_t, end1Focus;
+  int32_t end2Focus, standingWave, linearize, lockPhase;
+  int32_t fiducialMode, bodyFocusModel; /* bodyFocusModel: 0 = none */
+} EH_RFCA;
+
+/* WAKE (track.h:2908-2924; defaults track_data.c:2396-2414).  The Green
+ * function W(t) (V/C) is given as wakePoints samples t[], W[] starting at
+ * t = 0 with uniform spacing (set_up_wake, wake.c:340-477). */
+typedef struct {
+  const double *t;
+  const double *W;
+  int64_t wakePoints;
+  double factor;
+  int32_t n_bins, interpolate, rampPasses, allowLongBeam;
+  int32_t smoothing, change_p0, bunchedBeamMode, pad;
+} EH_WAKE;
+
+/* ZLONGIT (track.h:2836-2865; defaults track_data.c:2251-2283), broad-band
+ * resonator form only (zlongit.c:587-632) */
+typedef struct {
+  double Ra, Rs, Q, freq, bin_size, factor;
+  int32_t broad_band, n_bins, max_n_bins, interpolate;
+  int32_t area_weight, startOnPass, rampPasses, reverseTimeOrder;
+  int32_t smoothing, allowLongBeam, bunchedBeamMode, pad;
+} EH_ZLONGIT;
+
+/* MAXAMP (track.h:1955-1961; defaults track_data.c:990-997) */
+typedef struct {
+  double x_max, y_max;
+  int32_t elliptical, exponent, yExponent, openSide; /* openSide: 0 = none */
+} EH_MAXAMP;
+
+/* CHARGE (track.h:3220-3226) */
+typedef struct {
+  double charge, chargePerParticle;
+} EH_CHARGE;
+
+/* One beamline element; mirrors ELEMENT_LIST's (type, p_elem) pair
+ * (track.h:308-344) with the parameters held by value. */
+typedef struct {
+  int32_t type;
+  int32_t pad;
+  char name[32];
+  union {
+    EH_DRIFT drift;
+    EH_EDRIFT edrift;
+    EH_KQUAD kquad;
+    EH_KSEXT ksext;
+    EH_KOCT koct;
+    EH_CSBEND csbend;
+    EH_RFCA rfca;
+    EH_WAKE wake;
+    EH_ZLONGIT zlongit;
+    EH_MAXAMP maxamp;
+    EH_CHARGE charge;
+  } p;
+} eh_element;
+
+/* Fill an element with elegant's defaults for its type (track_data.c tables). */
+int eh_element_defaults(eh_element *elem, int32_t type);
+
+/* sizeof of eh_element (type 0) or of the parameter struct of an EH_T_*
+ * type, so foreign-function bindings can check their layouts. */
+int64_t eh_struct_size(int32_t type);
+
+/* Run statistics returned by eh_track / eh_do_tracking. */
+typedef struct {
+  int64_t effort;          /* sum over passes of survivors (do_tracking.c:2758-2765) */
+  int64_t n_left;          /* survivors after the last pass */
+  int64_t passes_done;
+  int64_t kernel_launches;
+  double track_ms;         /* device time of the tracking kernels (HIP events) */
+  double collective_ms;    /* device time of WAKE/ZLONGIT steps */
+} eh_stats;
+
+typedef struct eh_ctx eh_ctx;
+
+/* ---- lifecycle (replaces gpuBaseInit / gpuBaseDealloc) ---- */
+int eh_ctx_create(int device, eh_ctx **out);
+int eh_ctx_destroy(eh_ctx *ctx);
+const char *eh_strerror(int code);
+/* Message describing the last error on this context (empty if none). */
+const char *eh_last_error(const eh_ctx *ctx);
+
+/* ---- lattice: flattened beamline, all defaults resolved by the caller ---- */
+int eh_load_program(eh_ctx *ctx, const eh_element *elems, int64_t n_elems, double Po);
+
+/* ---- particle state ---- */
+/* Copy AoS rows (>= 7 columns: x, x', y, y', s, delta, ID) to device SoA. */
+int eh_upload(eh_ctx *ctx, double **coord, int64_t n, int32_t nprops);
+/* Track passes [pass0, pass0+npasses) through the loaded program (the pass
+ * and element loops of do_tracking.c:488,816).  *P_central is read (and
+ * would be updated by CHANGE_P0 elements, which are not supported). */
+int eh_track(eh_ctx *ctx, int64_t pass0, int64_t npasses, double *P_central, eh_stats *stats);
+/* Write the particles back in elegant's row order: survivors first in the
+ * order swap-to-tail compaction leaves them, then lost rows with s = loss
+ * position, delta = Po(1+delta) and column 7 = pass of loss.  *nLeft gets
+ * the survivor count.  nprops >= 8. */
+int eh_download(eh_ctx *ctx, double **coord, int64_t n, int32_t nprops, int64_t *nLeft);
+
+/* do_tracking (do_tracking.c:144-3241) for a flattened beamline: upload,
+ * track n_passes starting at passOffset, download.  Returns nLeft >= 0 or
+ * a negative error code.  *effort accumulates survivors per pass. */
+int64_t eh_do_tracking(eh_ctx *ctx, double **coord, int64_t nOriginal, int64_t *effort,
+                       const eh_element *beamline, int64_t n_elems, double *P_central,
+                       int64_t n_passes, int64_t passOffset);
+
+/* ---- element-by-element entry points (host-row API of the reference) ----
+ * Each uploads the rows, runs one element on the GPU and writes the rows
+ * back with the reference's compaction.  The aperture is the MAXAMP in force
+ * (NULL for none).  They return nLeft >= 0 or a negative error code. */
+/* multipole_tracking2 (multipole.c:693-1117) for KQUAD/KSEXT/KOCT */
+int64_t eh_multipole_tracking2(eh_ctx *ctx, double **particle, int64_t n_part, const eh_element *elem,
+                               double Po, double z_start, const EH_MAXAMP *maxamp);
+/* track_through_csbend (csbend.c:665-1409) */
+int64_t eh_track_through_csbend(eh_ctx *ctx, double **part, int64_t n_part, const EH_CSBEND *csbend,
+                                double Po, double z_start, const EH_MAXAMP *maxamp);
+/* simple_rf_cavity (simple_rfca.c:264-293); *phase_fiducial carries the
+ * element's fiducial phase between calls (NaN = not yet set, as
+ * rfca->fiducial_seen == 0) */
+int64_t eh_simple_rf_cavity(eh_ctx *ctx, double **part, int64_t np, const EH_RFCA *rfca, double *P_central,
+                            double zEnd, double *phase_fiducial);
+/* track_particles with the order-2 drift matrix (matrix.c:294-468) */
+int64_t eh_track_particles_drift(eh_ctx *ctx, double **part, int64_t n_part, const EH_DRIFT *drift);
+/* limit_amplitudes (limit_amplitudes.c:193-272) */
+int64_t eh_limit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np, double z,
+                            double Po, int32_t extrapolate_z);
+/* track_through_wake (wake.c:23-303); macroParticleCharge as set by CHARGE */
+int64_t eh_track_through_wake(eh_ctx *ctx, double **part, int64_t np, const EH_WAKE *wake, double Po,
+                              int64_t i_pass, double macroParticleCharge);
+/* track_through_zlongit (zlongit.c:56-557) */
+int64_t eh_track_through_zlongit(eh_ctx *ctx, double **part, int64_t np, const EH_ZLONGIT *zlongit, double Po,
+                                 int64_t i_pass, double macroParticleCharge);
+
+/* ---- device information for measurement ---- */
+/* Device time of the last eh_track call's dominant kernel: number of
+ * launches and summed milliseconds (HIP events on the library's stream). */
+int eh_last_kernel_timing(const eh_ctx *ctx, int64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

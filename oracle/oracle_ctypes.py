@@ -1,0 +1,152 @@
+"""ctypes wrapper of the CPU checker (TEST INFRASTRUCTURE ONLY).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+never by the product package."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from elegant_amd import _abi as A
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+PP = C.POINTER(C.POINTER(C.c_double))
+PE = C.POINTER(A.eh_element)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(mode="glibc"):
+    if mode not in _libs:
+        path = os.path.join(HERE, "liboracle_%s.so" % mode)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_ctx_new.restype = C.c_void_p
+        L.or_ctx_free.argtypes = [C.c_void_p]
+        L.or_math_mode.restype = C.c_int
+        L.or_ctx_effort.restype = C.c_int64
+        L.or_ctx_effort.argtypes = [C.c_void_p]
+        L.or_math_eval.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.c_long]
+        L.or_do_tracking.restype = C.c_long
+        L.or_do_tracking.argtypes = [C.c_void_p, PP, C.c_long, PE, C.c_long, C.POINTER(C.c_double), C.c_long,
+                                     C.c_long]
+        L.or_multipole_tracking2.restype = C.c_long
+        L.or_multipole_tracking2.argtypes = [PP, C.c_long, PE, C.c_double, C.c_double, C.POINTER(A.EH_MAXAMP)]
+        L.or_track_through_csbend.restype = C.c_long
+        L.or_track_through_csbend.argtypes = [PP, C.c_long, C.POINTER(A.EH_CSBEND), C.c_double, C.c_double,
+                                              C.POINTER(A.EH_MAXAMP)]
+        L.or_track_particles_drift.argtypes = [PP, C.c_long, C.c_double, C.c_int]
+        L.or_exact_drift.argtypes = [PP, C.c_long, C.c_double]
+        L.or_limit_amplitudes.restype = C.c_long
+        L.or_limit_amplitudes.argtypes = [PP, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long]
+        L.or_add_to_particle_energy.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
+        L.or_kahan_plus.restype = C.c_double
+        L.or_kahan_plus.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
+        L.or_simple_rf_cavity.restype = C.c_long
+        L.or_simple_rf_cavity.argtypes = [C.c_void_p, C.c_long, PP, C.c_long, C.POINTER(A.EH_RFCA),
+                                          C.POINTER(C.c_double), C.c_double]
+        L.or_track_through_wake.restype = C.c_long
+        L.or_track_through_wake.argtypes = [C.c_void_p, PP, C.c_long, C.POINTER(A.EH_WAKE), C.c_double, C.c_long]
+        L.or_track_through_zlongit.restype = C.c_long
+        L.or_track_through_zlongit.argtypes = [C.c_void_p, C.c_long, PP, C.c_long, C.POINTER(A.EH_ZLONGIT),
+                                               C.c_double, C.c_long]
+        PD = C.POINTER(C.c_double)
+        L.or_last_hist.restype = C.c_int64
+        L.or_last_hist.argtypes = [C.c_void_p, PD, PD, C.c_int64, PD, PD]
+        L.or_compute_average.restype = C.c_double
+        L.or_compute_average.argtypes = [PD, C.c_long]
+        L.or_set_macro_charge.argtypes = [C.c_void_p, C.c_double]
+        _libs[mode] = L
+    return _libs[mode]
+
+
+class Oracle:
+    """or_ctx plus its library (math mode 'glibc' or 'emath')."""
+
+    def __init__(self, mode="glibc"):
+        self.L = lib(mode)
+        self.ctx = self.L.or_ctx_new()
+
+    def __del__(self):
+        try:
+            self.L.or_ctx_free(self.ctx)
+        except Exception:
+            pass
+
+    def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0):
+        """rows: float64 (n, >=8) C-contiguous, modified in place.  Returns nLeft."""
+        ptr, keep = A.rows_pointer(rows)
+        pc = C.c_double(Po)
+        r = self.L.or_do_tracking(self.ctx, ptr, rows.shape[0], beamline.pointer(), len(beamline), C.byref(pc),
+                                  n_passes, pass_offset)
+        if r < 0:
+            raise RuntimeError("oracle do_tracking failed: %d" % r)
+        return int(r)
+
+    def last_hist(self):
+        PD = C.POINTER(C.c_double)
+        nb = self.L.or_last_hist(self.ctx, None, None, 0, None, None)
+        h = np.zeros(max(nb, 1))
+        v = np.zeros(max(nb, 1))
+        tmin = C.c_double()
+        dt = C.c_double()
+        self.L.or_last_hist(self.ctx, h.ctypes.data_as(PD), v.ctypes.data_as(PD), nb, C.byref(tmin), C.byref(dt))
+        return h[:nb], v[:nb], tmin.value, dt.value
+
+    def set_macro_charge(self, q):
+        self.L.or_set_macro_charge(self.ctx, q)
+
+    def wake(self, rows, wake, Po, i_pass):
+        ptr, keep = A.rows_pointer(rows)
+        r = self.L.or_track_through_wake(self.ctx, ptr, rows.shape[0], C.byref(wake), Po, i_pass)
+        if r < 0:
+            raise RuntimeError("oracle wake failed: %d" % r)
+        return r
+
+    def zlongit(self, rows, zl, Po, i_pass, elem_index=0):
+        ptr, keep = A.rows_pointer(rows)
+        r = self.L.or_track_through_zlongit(self.ctx, elem_index, ptr, rows.shape[0], C.byref(zl), Po, i_pass)
+        if r < 0:
+            raise RuntimeError("oracle zlongit failed: %d" % r)
+        return r
+
+    def effort(self):
+        return int(self.L.or_ctx_effort(self.ctx))
+
+
+def track(rows, beamline, Po, n_passes, mode="glibc"):
+    """Convenience: copy rows to (n, 8), run the oracle, return (rows, nLeft)."""
+    r = np.zeros((rows.shape[0], 8), dtype=np.float64)
+    r[:, :rows.shape[1]] = rows[:, :8]
+    o = Oracle(mode)
+    n = o.do_tracking(r, beamline, Po, n_passes)
+    return r, n
+
+
+def math_eval(fn, x, y=None, mode="emath"):
+    """Per-particle math of the given oracle build on an array."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = x if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(x)
+    pd = C.POINTER(C.c_double)
+    lib(mode).or_math_eval(fn, x.ctypes.data_as(pd), y.ctypes.data_as(pd), out.ctypes.data_as(pd), x.size)
+    return out
+
+
+def real_fft(x, inverse=False, mode="glibc"):
+    d = np.array(x, dtype=np.float64)
+    lib(mode).or_real_fft(d.ctypes.data_as(C.POINTER(C.c_double)), d.size, 1 if inverse else 0)
+    return d
+
+
+def compute_average(v, mode="glibc"):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib(mode).or_compute_average(v.ctypes.data_as(C.POINTER(C.c_double)), v.size)

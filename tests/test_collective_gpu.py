@@ -1,0 +1,65 @@
+"""WAKE and ZLONGIT on the GPU vs the CPU checker: integer histograms,
+voltages and kicked coordinates bit-identical, then a full ring with both
+collective elements over several passes."""
+import numpy as np
+import pytest
+
+from elegant_amd import elements as E
+from elegant_amd import lattices as Lt
+from oracle import oracle_ctypes as O
+from tests.helpers import assert_rows_identical
+from tests.test_oracle import PO, wake_bunch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_wake_step_bit_exact(tracker):
+    n = 300000
+    rows = wake_bunch(n, 20e-12, seed=12)
+    table = Lt.resonator_wake()
+    w = E.WAKE(table)
+    q = 5e-9 / n
+    g = rows.copy()
+    h, v, tmin = tracker.collective_probe(g, w, PO, 0, q)
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    c = rows.copy()
+    o.wake(c, w.p.wake, PO, 0)
+    hc, vc, tminc, dt = o.last_hist()
+    assert tmin == tminc
+    assert np.array_equal(h, hc) and h.sum() == n
+    assert np.array_equal(v, vc)
+    assert_rows_identical(g, c, "wake kick")
+
+
+def test_zlongit_step_bit_exact(tracker):
+    n = 300000
+    rows = wake_bunch(n, 20e-12, seed=13)
+    zl = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=8192, BIN_SIZE=2e-12)
+    q = 5e-9 / n
+    g = rows.copy()
+    h, v, tmin = tracker.collective_probe(g, zl, PO, 0, q)
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    c = rows.copy()
+    o.zlongit(c, zl.p.zlongit, PO, 0)
+    hc, vc, tminc, dt = o.last_hist()
+    assert h.size == 8192 and tmin == tminc
+    assert np.array_equal(h, hc) and h.sum() == n
+    assert np.array_equal(v, vc)
+    assert_rows_identical(g, c, "zlongit kick")
+
+
+def test_ring_with_wakes_bit_exact_vs_emath_checker(tracker):
+    bl, Po = Lt.storage_ring_wake()
+    rows = Lt.matched_beam(bl, 3000, 1e-9, 1e-11, 1e-3, 20e-12, 31, Po)
+    rows[:300, 0] *= 30  # halo for losses
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 3)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 3)
+    print("ring+wakes: survivors", ng, "effort", eg)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 10
+    assert_rows_identical(g, c, "ring with wakes")

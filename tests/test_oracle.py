@@ -1,0 +1,279 @@
+"""The CPU checker against closed forms, invariants and brute force.
+
+Each test pins one restated routine with something a plausible restatement
+mistake would violate: textbook maps, exact arithmetic, numpy brute force,
+or the physics a convention fixes."""
+import ctypes as C
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from elegant_amd import _abi as A
+from elegant_amd import elements as E
+from elegant_amd import lattices as Lt
+from oracle import oracle_ctypes as O
+
+PO = Lt.momentum(6000.0)
+
+
+def rows_of(pts):
+    r = np.zeros((len(pts), 8))
+    r[:, :6] = np.asarray(pts, dtype=np.float64)
+    r[:, 6] = np.arange(1, len(pts) + 1)
+    r[:, 7] = -1
+    return r
+
+
+def track(elems, rows, passes=1, mode="glibc", Po=PO):
+    bl = E.Beamline(elems)
+    o = O.Oracle(mode)
+    r = rows.copy()
+    n = o.do_tracking(r, bl, Po, passes)
+    return r, n, o
+
+
+def test_drift_matrix_second_order():
+    """DRIF is the order-2 matrix: x += L x', s += L + L/2 (x'^2 + y'^2)."""
+    L = 1.7
+    pts = [[1e-3, 2e-3, -3e-4, 5e-4, 0.1, 1e-3], [0, 0, 0, 0, 0, 0], [-2e-3, -1e-3, 4e-4, 0, 0, -2e-3]]
+    r, n, _ = track([E.DRIF(L=L)], rows_of(pts))
+    for p, q in zip(pts, r):
+        assert q[0] == L * p[1] + p[0] if p[1] else q[0] == p[0]
+        exact_s = Fraction(p[4]) + Fraction(L) + Fraction(L / 2) * (Fraction(p[1]) ** 2 + Fraction(p[3]) ** 2)
+        assert abs(float(exact_s) - q[4]) <= 2e-16 * max(1, abs(q[4]))
+        assert q[1] == p[1] and q[3] == p[3] and q[5] == p[5]
+
+
+def test_thick_quadrupole_linear_limit():
+    """KQUAD with 4th-order integration converges to the cos/sin/cosh/sinh map
+    with the Yoshida O(h^4) error; halving the slice length cuts the error ~16x."""
+    k, L = 1.2, 0.5
+    x0 = 1e-6
+    errs = []
+    for nk in (4, 8, 16):
+        r, n, _ = track([E.KQUAD(L=L, K1=k, N_KICKS=nk)], rows_of([[x0, 0, x0, 0, 0, 0]]))
+        s = math.sqrt(k)
+        ex = x0 * math.cos(s * L)
+        ey = x0 * math.cosh(s * L)
+        errs.append((abs(r[0, 0] - ex), abs(r[0, 2] - ey)))
+    assert errs[0][0] < 3e-3 * x0 and errs[0][1] < 3e-3 * x0
+    assert 10 < errs[0][0] / errs[1][0] < 22 and 10 < errs[1][1] / errs[2][1] < 22
+
+
+def test_multipole_kick_coefficients():
+    """Thin-lens limit: sextupole kick dx' = -K2 L (x^2 - y^2)/2, dy' = K2 L x y;
+    octupole dx' = -K3 L (x^3 - 3 x y^2)/6 (coefficients of multipole.h:137-165)."""
+    L, K2, K3 = 1e-4, 40.0, 900.0
+    x, y = 2e-3, -1e-3
+    r, n, _ = track([E.KSEXT(L=L, K2=K2)], rows_of([[x, 0, y, 0, 0, 0]]))
+    assert abs(r[0, 1] - (-K2 * L * (x * x - y * y) / 2)) < 1e-6 * abs(K2 * L * x * x)
+    assert abs(r[0, 3] - (K2 * L * x * y)) < 1e-6 * abs(K2 * L * x * y)
+    r, n, _ = track([E.KOCT(L=L, K3=K3)], rows_of([[x, 0, y, 0, 0, 0]]))
+    assert abs(r[0, 1] - (-K3 * L * (x ** 3 - 3 * x * y * y) / 6)) < 1e-6 * abs(K3 * L * x ** 3)
+    assert abs(r[0, 3] - (K3 * L * (3 * x * x * y - y ** 3) / 6)) < 1e-6 * abs(K3 * L * x ** 3)
+
+
+def jacobian(elems, p0, h=1e-9, mode="glibc"):
+    J = np.zeros((6, 6))
+    for j in range(6):
+        a = np.array(p0, dtype=float)
+        b = np.array(p0, dtype=float)
+        a[j] += h
+        b[j] -= h
+        ra, _, _ = track(elems, rows_of([a]), mode=mode)
+        rb, _, _ = track(elems, rows_of([b]), mode=mode)
+        J[:, j] = (ra[0, :6] - rb[0, :6]) / (2 * h)
+    return J
+
+
+def canonical(p):
+    # (x, px, y, py, -s?, delta): use slopes -> momenta qx = (1+d) x'/sqrt(1+x'^2+y'^2)
+    x, xp, y, yp, s, d = p
+    den = math.sqrt(1 + xp * xp + yp * yp)
+    return np.array([x, (1 + d) * xp / den, y, (1 + d) * yp / den])
+
+
+@pytest.mark.parametrize("elem", [
+    lambda: E.KQUAD(L=0.5, K1=1.3, N_KICKS=8),
+    lambda: E.KSEXT(L=0.2, K2=40.0, N_KICKS=4),
+    lambda: E.KOCT(L=0.2, K3=500.0),
+    lambda: E.CSBEND(L=1.0, ANGLE=2 * math.pi / 80, N_KICKS=16, E1=0.02, E2=0.03),
+])
+def test_transverse_map_is_symplectic(elem):
+    """The 4x4 transverse Jacobian in canonical variables (fixed delta)
+    satisfies J^T S J = S (multipoles and the exact-sector bend body; the
+    edge kicks of edge order 1 are linear and symplectic too)."""
+    p0 = [1.5e-3, 2e-4, -8e-4, 1e-4, 0.0, 1e-3]
+    h = 1e-8
+    J = np.zeros((4, 4))
+    base = canonical(p0)
+    for j in range(4):
+        pa = list(p0)
+        pb = list(p0)
+        # perturb canonical coordinate j, convert back to slopes
+        def to_slopes(q):
+            x, px, y, py = q
+            d = p0[5]
+            pz = math.sqrt((1 + d) ** 2 - px * px - py * py)
+            return [x, px / pz, y, py / pz, 0.0, d]
+        qa = base.copy()
+        qb = base.copy()
+        qa[j] += h
+        qb[j] -= h
+        ra, _, _ = track([elem()], rows_of([to_slopes(qa)]))
+        rb, _, _ = track([elem()], rows_of([to_slopes(qb)]))
+        J[:, j] = (canonical(ra[0, :6]) - canonical(rb[0, :6])) / (2 * h)
+    S = np.array([[0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 0, 1], [0, 0, -1, 0]], dtype=float)
+    assert np.abs(J.T @ S @ J - S).max() < 5e-6
+
+
+def test_bend_reference_orbit_closes():
+    """An on-momentum particle on axis stays on axis through a sector bend
+    (no edge angles), to integrator accuracy."""
+    r, n, _ = track([E.CSBEND(L=1.0, ANGLE=0.1, N_KICKS=16)], rows_of([[0, 0, 0, 0, 0, 0]]))
+    assert abs(r[0, 0]) < 1e-10 and abs(r[0, 1]) < 1e-10
+    assert abs(r[0, 4] - 1.0) < 1e-9
+
+
+def test_add_to_particle_energy_conserves_transverse_momentum():
+    L = O.lib("glibc")
+    p = (C.c_double * 8)(1e-3, 2e-4, -1e-3, 3e-4, 0.5, 1e-3, 1, -1)
+    P0 = PO * (1 + p[5])
+    px0 = P0 * p[1] / math.sqrt(1 + p[1] ** 2 + p[3] ** 2)
+    g0 = math.sqrt(P0 * P0 + 1)
+    dg = 37.5
+    L.or_add_to_particle_energy(p, 1.2e-9, PO, dg)
+    P1 = PO * (1 + p[5])
+    px1 = P1 * p[1] / math.sqrt(1 + p[1] ** 2 + p[3] ** 2)
+    assert abs(math.sqrt(P1 * P1 + 1) - (g0 + dg)) < 1e-9
+    assert abs(px1 - px0) < 1e-12 * abs(px0)
+
+
+def test_rfca_zero_length_energy_gain():
+    """RFCA with L=0: delta' = (sqrt((gamma + dgamma)^2 - 1) - Po)/Po with
+    dgamma = V sin(phase) / (me c^2) for a particle at the fiducial time."""
+    V, phase = 5e6, 30.0
+    rf = E.RFCA(L=0.0, VOLT=V, PHASE=phase, FREQ=352e6)
+    r, n, _ = track([rf], rows_of([[0, 0, 0, 0, 0, 0]]))
+    g = math.sqrt(PO * PO + 1) + V / (1e6 * Lt.ME_MEV) * math.sin(math.radians(phase))
+    assert abs(r[0, 5] - (math.sqrt(g * g - 1) - PO) / PO) < 1e-14
+
+
+def test_kahan_plus_against_exact_sum():
+    L = O.lib("glibc")
+    rng = np.random.default_rng(4)
+    v = (rng.uniform(1, 2, 20000) * 1e-6).tolist()
+    s, err = 0.0, C.c_double(0.0)
+    for x in v:
+        s = L.or_kahan_plus(s, x, C.byref(err))
+    exact = sum(Fraction(x) for x in v)
+    naive = 0.0
+    for x in v:
+        naive += x
+    assert abs(Fraction(s) - exact) <= abs(Fraction(naive) - exact)
+    assert abs(float(Fraction(s) - exact)) <= np.spacing(s)
+
+
+def test_fixed_point_mean_is_exact_and_order_free():
+    rng = np.random.default_rng(8)
+    v = 1e-6 + rng.normal(0, 1e-11, 10001)
+    m = O.compute_average(v)
+    exact = sum(Fraction(x) for x in v) / len(v)
+    assert abs(Fraction(m) - exact) <= Fraction(np.spacing(m)) / 2 + Fraction(np.spacing(m)) / 2
+    assert O.compute_average(v[::-1]) == m
+    assert O.compute_average(rng.permutation(v)) == m
+    assert O.compute_average(np.array([-1.5, 2.5, 3.0])) == float(Fraction(4) / 3)
+
+
+def test_real_fft_matches_dft():
+    rng = np.random.default_rng(1)
+    for n in (16, 32, 1024):
+        x = rng.normal(size=n)
+        X = O.real_fft(x)
+        ref = np.fft.rfft(x) / n
+        packed = np.empty(n)
+        packed[0] = ref[0].real
+        packed[1:n - 1:2] = ref[1:n // 2].real
+        packed[2:n - 1:2] = ref[1:n // 2].imag
+        packed[n - 1] = ref[n // 2].real
+        assert np.abs(X - packed).max() < 1e-14 * np.abs(packed).max()
+        back = O.real_fft(X, inverse=True)
+        assert np.abs(back - x).max() < 1e-12
+    # direct O(n^2) DFT for n = 16
+    n = 16
+    x = rng.normal(size=n)
+    j = np.arange(n)
+    X = O.real_fft(x)
+    for k in range(1, n // 2):
+        ref = sum(x[jj] * np.exp(-2j * np.pi * jj * k / n) for jj in range(n)) / n
+        assert abs(X[2 * k - 1] - ref.real) < 1e-15 and abs(X[2 * k] - ref.imag) < 1e-15
+
+
+def wake_bunch(n, sigma_t, seed=3):
+    rng = Lt.Rng(seed)
+    beta = PO / math.sqrt(PO * PO + 1)
+    z = rng.normal(2 * n).reshape(2, n)
+    r = np.zeros((n, 8))
+    r[:, 4] = 0.3 + 2.99792458e8 * beta * sigma_t * z[0]
+    r[:, 5] = 1e-3 * z[1]
+    r[:, 6] = np.arange(1, n + 1)
+    r[:, 7] = -1
+    return r
+
+
+def test_wake_histogram_and_convolution_brute_force():
+    table = Lt.resonator_wake(points=4000)
+    w = E.WAKE(table)
+    rows = wake_bunch(20000, 20e-12)
+    o = O.Oracle()
+    o.set_macro_charge(5e-9 / 20000)
+    before = rows.copy()
+    o.wake(rows, w.p.wake, PO, 0)
+    hist, volt, tmin, dt = o.last_hist()
+    P = PO * (before[:, 5] + 1)
+    t = before[:, 4] * np.sqrt(P * P + 1) / (2.99792458e8 * P)
+    ib = ((t - tmin) / dt + 0.5).astype(np.int64)
+    assert hist.sum() == rows.shape[0]
+    assert np.array_equal(np.bincount(ib, minlength=hist.size)[:hist.size], hist)
+    conv = np.convolve(hist, table.W[:hist.size])[:hist.size] * (5e-9 / 20000)
+    assert np.abs(conv - volt).max() < 1e-10 * np.abs(volt).max()
+
+
+def test_wake_and_zlongit_agree_for_the_same_resonator():
+    """The time-domain WAKE with the resonator's Green function and the
+    frequency-domain ZLONGIT with its impedance give the same energy kicks
+    (pins the FFT normalization and sign conventions)."""
+    n = 200000
+    rows = wake_bunch(n, 20e-12, seed=9)
+    q = 5e-9 / n
+    ow = O.Oracle()
+    ow.set_macro_charge(q)
+    rw = rows.copy()
+    table = Lt.resonator_wake(dt=0.2e-12, points=5000)
+    ow.wake(rw, E.WAKE(table).p.wake, PO, 0)
+    oz = O.Oracle()
+    oz.set_macro_charge(q)
+    rz = rows.copy()
+    zl = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=8192, BIN_SIZE=0.2e-12)
+    oz.zlongit(rz, zl.p.zlongit, PO, 0)
+    dw = rw[:, 5] - rows[:, 5]
+    dz = rz[:, 5] - rows[:, 5]
+    assert np.corrcoef(dw, dz)[0, 1] > 0.99
+    assert abs(dw.mean() - dz.mean()) < 0.02 * abs(dw.mean())
+    assert dw.mean() < 0  # the wake decelerates the bunch
+
+
+def test_glibc_and_emath_checkers_diverge_like_the_gpu():
+    """The CPU checker built with glibc and with the shared deterministic math
+    differ after one turn of the ring by the same ~1e-10 the GPU shows: the
+    bend's X = rho0 (f - 1) + f X amplifies last-bit cos differences by rho0/x."""
+    bl, Po, rows = Lt.storage_ring_workload(n=200, seed=9)
+    a = rows.copy()
+    b = rows.copy()
+    O.Oracle("glibc").do_tracking(a, bl, Po, 1)
+    O.Oracle("emath").do_tracking(b, bl, Po, 1)
+    rel = np.abs(a[:, 0] - b[:, 0]) / np.maximum(np.abs(b[:, 0]), 1e-3)
+    assert rel.max() < 5e-10
+    assert rel.max() > 0
