@@ -53,7 +53,7 @@ def test_zlongit_step_bit_exact(tracker):
 def test_ring_with_wakes_bit_exact_vs_emath_checker(tracker):
     bl, Po = Lt.storage_ring_wake()
     rows = Lt.matched_beam(bl, 3000, 1e-9, 1e-11, 1e-3, 20e-12, 31, Po)
-    rows[:300, 0] *= 30  # halo for losses
+    rows[:300, 0] += 0.0149  # a halo at the edge of the 15 mm aperture: losses within a few passes
     g = rows.copy()
     ng, eg = tracker.do_tracking(g, bl, Po, 3)
     c = rows.copy()
