@@ -131,6 +131,11 @@ class eh_stats(C.Structure):
                 ("track_ms", D), ("collective_ms", D)]
 
 
+class eh_batch_tuning(C.Structure):
+    _fields_ = [("particles_per_lane", I), ("block", I), ("passes_per_launch", I), ("use_graph", I),
+                ("sample_ms", D)]
+
+
 PARAM_STRUCT = {
     EH_T_DRIF: ("drift", EH_DRIFT), EH_T_EDRIFT: ("edrift", EH_EDRIFT), EH_T_KQUAD: ("kquad", EH_KQUAD),
     EH_T_KSEXT: ("ksext", EH_KSEXT), EH_T_KOCT: ("koct", EH_KOCT), EH_T_CSBEND: ("csbend", EH_CSBEND),

@@ -13,8 +13,8 @@ HOST = os.path.join(HERE, "host")
 OUT = os.path.join(HERE, "libelegant_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["eh_track.hip", "eh_collective.hip"]
-C_SOURCES = ["eh_compile.c", "eh_driver.c", "eh_collective.c"]
+HIP_SOURCES = ["eh_track.hip", "eh_collective.hip", "eh_mlp.hip"]
+C_SOURCES = ["eh_compile.c", "eh_driver.c", "eh_collective.c", "eh_batch.c", "eh_da.c", "eh_learn.c"]
 
 # FP64 parity needs IEEE division/sqrt and no contraction on both sides.
 HIPCC_FLAGS = [

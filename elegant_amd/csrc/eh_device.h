@@ -48,6 +48,15 @@ int ehd_timing(ehd_dev *d, int64_t *launches, double *ms);
 /* launch tuning: particles per workgroup (multiple of 64) */
 void ehd_set_block(ehd_dev *d, int32_t block);
 
+/* ---- batched variants ---- */
+/* vdesc/chunks describe variants and work items; ops are all programs concatenated */
+int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk *chunks, int32_t nchunks);
+/* passes [pass0, pass0+npasses) for every variant, npasses split into launches of
+ * passes_per_launch; ppl particles per lane (1 or 2); graph = capture and replay */
+int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,
+                    int32_t block, int32_t use_graph, double *ms, int64_t *launches);
+int ehd_state_snapshot(ehd_dev *d, int restore);
+
 /* ---- collective elements (WAKE / ZLONGIT) ---- */
 typedef struct {
   int32_t kind;         /* EH_OP_WAKE or EH_OP_ZLONGIT */

@@ -121,6 +121,18 @@ typedef struct {
 #define EH_D_HALFLEN 7    /* length / 2 */
 #define EH_D_RFLEN_TOTAL 8 /* element length (VOLT=0 drift) */
 
+/* batched variants: one element program and one particle range per variant,
+ * processed as chunks pulled from a queue by a persistent kernel */
+typedef struct {
+  int32_t op_offset, nops;
+  int64_t first, n;
+} eh_vdesc;
+
+typedef struct {
+  int32_t variant, count;
+  int64_t first;
+} eh_chunk;
+
 /* SoA particle state in HBM.  A particle is alive while loss_pass < 0. */
 typedef struct {
   double *x, *xp, *y, *yp, *s, *dp;

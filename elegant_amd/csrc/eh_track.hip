@@ -666,6 +666,93 @@ __global__ void __launch_bounds__(256) eh_track_kernel(eh_state st, const eh_op 
   }
 }
 
+/* Persistent batch kernel: each workgroup pulls chunks (a variant and a run
+ * of its particles) from a queue until it is empty; every lane tracks PPL
+ * particles of the chunk through the variant's program.  The chunk queue is
+ * ordered by variant so that workgroups resident at the same time share a few
+ * programs in L2. */
+template <int PPL>
+__global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op *__restrict__ ops,
+                                                       const eh_vdesc *__restrict__ vars,
+                                                       const eh_chunk *__restrict__ chunks, int32_t nchunks,
+                                                       int64_t pass0, int32_t npasses, unsigned int *queue) {
+  __shared__ int s_chunk;
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_chunk = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int c = s_chunk;
+    __syncthreads();
+    if (c >= nchunks)
+      return;
+    const eh_chunk ch = chunks[c];
+    const eh_vdesc vd = vars[ch.variant];
+    const eh_op *vops = ops + vd.op_offset;
+    const int32_t nops = vd.nops;
+    particle p[PPL];
+    int lost[PPL], lost_op[PPL];
+    int64_t lost_pass[PPL], idx[PPL];
+    bool live[PPL];
+    for (int k = 0; k < PPL; k++) {
+      const int64_t local = threadIdx.x + (int64_t)k * blockDim.x;
+      idx[k] = ch.first + local;
+      live[k] = local < ch.count && st.loss_pass[idx[k]] < 0;
+      lost[k] = 0;
+      lost_op[k] = 0;
+      lost_pass[k] = 0;
+      if (live[k]) {
+        p[k].x = st.x[idx[k]];
+        p[k].xp = st.xp[idx[k]];
+        p[k].y = st.y[idx[k]];
+        p[k].yp = st.yp[idx[k]];
+        p[k].s = st.s[idx[k]];
+        p[k].dp = st.dp[idx[k]];
+      }
+    }
+    for (int32_t ip = 0; ip < npasses; ip++) {
+      bool any = false;
+      for (int k = 0; k < PPL; k++)
+        any |= live[k];
+      if (!any)
+        break;
+      for (int32_t io = 0; io < nops; io++) {
+        const eh_op &op = vops[io];
+        for (int k = 0; k < PPL; k++) {
+          if (!live[k])
+            continue;
+          int l = run_op(p[k], op);
+          if (l) {
+            live[k] = false;
+            lost[k] = l;
+            lost_op[k] = io;
+            lost_pass[k] = pass0 + ip;
+          }
+        }
+      }
+    }
+    for (int k = 0; k < PPL; k++) {
+      const int64_t local = threadIdx.x + (int64_t)k * blockDim.x;
+      if (local >= ch.count)
+        continue;
+      if (live[k] || lost[k]) {
+        const int64_t i = idx[k];
+        st.x[i] = p[k].x;
+        st.xp[i] = p[k].xp;
+        st.y[i] = p[k].y;
+        st.yp[i] = p[k].yp;
+        st.s[i] = p[k].s;
+        st.dp[i] = p[k].dp;
+        if (lost[k]) {
+          st.loss_pass[i] = (int32_t)lost_pass[k];
+          st.loss_code[i] = 4 * lost_op[k] + (lost[k] - 1);
+        }
+      }
+    }
+  }
+}
+
+__global__ void eh_queue_reset(unsigned int *queue) { *queue = 0u; }
+
 /* ------------------------------------------------------------ host side */
 
 extern "C" {
@@ -740,6 +827,19 @@ void ehd_destroy(ehd_dev *d) {
     hipFree(d->ops);
   if (d->counter)
     hipFree(d->counter);
+  if (d->vdesc)
+    hipFree(d->vdesc);
+  if (d->chunks)
+    hipFree(d->chunks);
+  if (d->queue)
+    hipFree(d->queue);
+  for (int k = 0; k < 6; k++)
+    if (d->snap[k])
+      hipFree(d->snap[k]);
+  if (d->snap_lp)
+    hipFree(d->snap_lp);
+  if (d->snap_lc)
+    hipFree(d->snap_lc);
   ehd_free_collective(d);
   hipEventDestroy(d->ev0);
   hipEventDestroy(d->ev1);
@@ -917,6 +1017,146 @@ int ehd_timing(ehd_dev *d, int64_t *launches, double *ms) {
 }
 
 const char *ehd_message(const ehd_dev *d) { return d ? d->msg : ""; }
+
+int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk *chunks, int32_t nchunks) {
+  HIPCHK(d, hipSetDevice(d->device));
+  if (nvar > d->var_cap) {
+    if (d->vdesc)
+      HIPCHK(d, hipFree(d->vdesc));
+    d->vdesc = NULL;
+    HIPCHK(d, hipMalloc((void **)&d->vdesc, sizeof(eh_vdesc) * nvar));
+    d->var_cap = nvar;
+  }
+  if (nchunks > d->chunk_cap) {
+    if (d->chunks)
+      HIPCHK(d, hipFree(d->chunks));
+    d->chunks = NULL;
+    HIPCHK(d, hipMalloc((void **)&d->chunks, sizeof(eh_chunk) * nchunks));
+    d->chunk_cap = nchunks;
+  }
+  if (!d->queue)
+    HIPCHK(d, hipMalloc((void **)&d->queue, 256));
+  if (!d->num_cu) {
+    int cu = 0;
+    HIPCHK(d, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
+    d->num_cu = cu > 0 ? cu : 256;
+  }
+  HIPCHK(d, hipMemcpyAsync(d->vdesc, v, sizeof(eh_vdesc) * nvar, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->chunks, chunks, sizeof(eh_chunk) * nchunks, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  d->nvar = nvar;
+  d->nchunks = nchunks;
+  return EH_OK;
+}
+
+static int batch_launch(ehd_dev *d, eh_state st, int64_t pass0, int32_t npasses, int32_t ppl, int32_t block,
+                        int grid) {
+  hipLaunchKernelGGL(eh_queue_reset, dim3(1), dim3(1), 0, d->stream, d->queue);
+  if (ppl == 2)
+    hipLaunchKernelGGL(eh_batch_kernel<2>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
+                       d->nchunks, pass0, npasses, d->queue);
+  else
+    hipLaunchKernelGGL(eh_batch_kernel<1>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
+                       d->nchunks, pass0, npasses, d->queue);
+  return EH_OK;
+}
+
+int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,
+                    int32_t block, int32_t use_graph, double *ms_out, int64_t *launches_out) {
+  if (npasses <= 0 || d->nchunks <= 0) {
+    *ms_out = 0;
+    *launches_out = 0;
+    return EH_OK;
+  }
+  if ((ppl != 1 && ppl != 2) || block < 64 || block > 256 || block % 64 || passes_per_launch < 1)
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  eh_state st;
+  st.x = d->coord[0];
+  st.xp = d->coord[1];
+  st.y = d->coord[2];
+  st.yp = d->coord[3];
+  st.s = d->coord[4];
+  st.dp = d->coord[5];
+  st.loss_pass = d->loss_pass;
+  st.loss_code = d->loss_code;
+  st.n = d->n;
+  int per_cu = 0;
+  if (ppl == 2)
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<2>, block, 0));
+  else
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<1>, block, 0));
+  if (per_cu < 1)
+    per_cu = 1;
+  int grid = d->num_cu * per_cu;
+  if (grid > d->nchunks)
+    grid = d->nchunks;
+  int64_t nl = 0;
+  hipGraph_t graph = NULL;
+  hipGraphExec_t exec = NULL;
+  if (use_graph)
+    HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+  else
+    HIPCHK(d, hipEventRecord(d->ev0, d->stream));
+  for (int32_t done = 0; done < npasses; done += passes_per_launch) {
+    int32_t k = npasses - done < passes_per_launch ? npasses - done : passes_per_launch;
+    batch_launch(d, st, pass0 + done, k, ppl, block, grid);
+    nl++;
+  }
+  if (use_graph) {
+    HIPCHK(d, hipStreamEndCapture(d->stream, &graph));
+    HIPCHK(d, hipGraphInstantiate(&exec, graph, NULL, NULL, 0));
+    HIPCHK(d, hipEventRecord(d->ev0, d->stream));
+    HIPCHK(d, hipGraphLaunch(exec, d->stream));
+  }
+  HIPCHK(d, hipGetLastError());
+  HIPCHK(d, hipEventRecord(d->ev1, d->stream));
+  HIPCHK(d, hipEventSynchronize(d->ev1));
+  float ms = 0;
+  HIPCHK(d, hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  if (exec)
+    HIPCHK(d, hipGraphExecDestroy(exec));
+  if (graph)
+    HIPCHK(d, hipGraphDestroy(graph));
+  d->track_ms += ms;
+  d->track_launches += nl;
+  *ms_out = ms;
+  *launches_out = nl;
+  return EH_OK;
+}
+
+/* copy of the particle state on the device (autotuning trials restore it) */
+int ehd_state_snapshot(ehd_dev *d, int restore) {
+  HIPCHK(d, hipSetDevice(d->device));
+  if (!restore && d->n > d->snap_cap) {
+    for (int k = 0; k < 6; k++) {
+      if (d->snap[k])
+        HIPCHK(d, hipFree(d->snap[k]));
+      d->snap[k] = NULL;
+    }
+    if (d->snap_lp)
+      HIPCHK(d, hipFree(d->snap_lp));
+    if (d->snap_lc)
+      HIPCHK(d, hipFree(d->snap_lc));
+    d->snap_lp = d->snap_lc = NULL;
+    for (int k = 0; k < 6; k++)
+      HIPCHK(d, hipMalloc((void **)&d->snap[k], sizeof(double) * d->n));
+    HIPCHK(d, hipMalloc((void **)&d->snap_lp, sizeof(int32_t) * d->n));
+    HIPCHK(d, hipMalloc((void **)&d->snap_lc, sizeof(int32_t) * d->n));
+    d->snap_cap = d->n;
+  }
+  if (restore && d->snap_cap < d->n)
+    return EH_ERR_STATE;
+  for (int k = 0; k < 6; k++)
+    HIPCHK(d, hipMemcpyAsync(restore ? d->coord[k] : d->snap[k], restore ? d->snap[k] : d->coord[k],
+                             sizeof(double) * d->n, hipMemcpyDeviceToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(restore ? d->loss_pass : d->snap_lp, restore ? d->snap_lp : d->loss_pass,
+                           sizeof(int32_t) * d->n, hipMemcpyDeviceToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(restore ? d->loss_code : d->snap_lc, restore ? d->snap_lc : d->loss_code,
+                           sizeof(int32_t) * d->n, hipMemcpyDeviceToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
 
 /* ---- deterministic-math probe (used by tests to compare with the CPU) ---- */
 __global__ void eh_math_probe_kernel(int32_t fn, const double *in, const double *in2, double *out, int64_t n) {

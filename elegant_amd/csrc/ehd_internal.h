@@ -35,6 +35,15 @@ struct ehd_dev {
   ehd_coll_slot slot[EHD_MAX_COLL];
   void *cwork;       /* reductions, histogram, voltages */
   int64_t cwork_bytes;
+  /* batched variants */
+  eh_vdesc *vdesc;
+  eh_chunk *chunks;
+  int32_t nvar, nchunks, chunk_cap, var_cap;
+  unsigned int *queue;
+  double *snap[6];
+  int32_t *snap_lp, *snap_lc;
+  int64_t snap_cap;
+  int num_cu;
   char msg[512];
 };
 

@@ -1,0 +1,456 @@
+/*
+ * eh_batch.c - many lattice variants (error seeds) with their own particle
+ * sets tracked in one persistent, graph-captured launch sequence.
+ *
+ * The reference runs each variant (and, for dynamic-aperture scans, each
+ * particle) through its own do_tracking call (aperture_search.c:1080-1082).
+ * Here every variant's element program and particles are uploaded together;
+ * the persistent kernel pulls (variant, particle run) chunks from a queue, so
+ * variants of very different sizes keep all compute units busy.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "eh_host.h"
+
+ehd_dev *eh_ctx_dev(eh_ctx *c);
+char *eh_ctx_msg(eh_ctx *c);
+const char *ehd_message(const ehd_dev *d);
+
+struct eh_batch {
+  eh_ctx *ctx;
+  eh_program *prog; /* scratch for compiling one variant */
+  eh_op *ops;
+  int64_t nops, ops_cap;
+  eh_vdesc *var;
+  int32_t nvar, var_cap;
+  double *soa[6];
+  int32_t *lp, *lc;
+  double *ids;
+  int64_t n, cap;
+  int uploaded;
+  eh_batch_tuning tune;
+};
+
+static int bmsg(eh_batch *b, int code, const char *m) {
+  snprintf(eh_ctx_msg(b->ctx), 512, "%s", m);
+  return code;
+}
+
+int eh_batch_create(eh_ctx *ctx, eh_batch **out) {
+  eh_batch *b;
+  if (!ctx || !out)
+    return EH_ERR_INVALID;
+  b = (eh_batch *)calloc(1, sizeof(eh_batch));
+  if (!b)
+    return EH_ERR_NOMEM;
+  b->prog = (eh_program *)calloc(1, sizeof(eh_program));
+  if (!b->prog) {
+    free(b);
+    return EH_ERR_NOMEM;
+  }
+  b->ctx = ctx;
+  b->tune.particles_per_lane = 1;
+  b->tune.block = 256;
+  b->tune.passes_per_launch = 64;
+  b->tune.use_graph = 1;
+  *out = b;
+  return EH_OK;
+}
+
+int eh_batch_destroy(eh_batch *b) {
+  if (!b)
+    return EH_OK;
+  for (int k = 0; k < 6; k++)
+    free(b->soa[k]);
+  free(b->lp);
+  free(b->lc);
+  free(b->ids);
+  free(b->ops);
+  free(b->var);
+  free(b->prog);
+  free(b);
+  return EH_OK;
+}
+
+static int grow(void **p, int64_t *cap, int64_t need, size_t elem) {
+  if (need <= *cap)
+    return EH_OK;
+  int64_t nc = *cap ? *cap : 1024;
+  while (nc < need)
+    nc *= 2;
+  void *q = realloc(*p, elem * nc);
+  if (!q)
+    return EH_ERR_NOMEM;
+  *p = q;
+  *cap = nc;
+  return EH_OK;
+}
+
+int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord, int64_t n,
+                         int32_t nprops) {
+  int rc;
+  if (!b || !elems || n_elems <= 0 || (!coord && n > 0) || n < 0 || nprops < 7 || !(Po > 0))
+    return EH_ERR_INVALID;
+  rc = eh_compile(elems, n_elems, Po, b->prog, eh_ctx_msg(b->ctx), 512);
+  if (rc)
+    return rc;
+  if (b->prog->n_coll)
+    return bmsg(b, EH_ERR_UNSUPPORTED, "batched variants cannot contain WAKE/ZLONGIT");
+  /* RF cavities need a fiducial known without a reduction over particles */
+  for (int k = 0; k < b->prog->n_rfca; k++) {
+    const EH_RFCA *rf = &elems[b->prog->rfca_elem[k]].p.rfca;
+    eh_op *op = &b->prog->ops[b->prog->rfca_op[k]];
+    double t0;
+    if (rf->tReference != -1)
+      t0 = rf->tReference;
+    else if (rf->fiducialMode == EH_FID_LIGHT) {
+      double length = rf->length;
+      long nKicks = length ? rf->nKicks : 1;
+      if (nKicks <= 0)
+        nKicks = 1;
+      length /= nKicks;
+      t0 = ((op->z_end - rf->length) + length / 2) / 2.99792458e8;
+    } else
+      return bmsg(b, EH_ERR_UNSUPPORTED, "batched RFCA needs T_REFERENCE or FIDUCIAL=light");
+    eh_rfca_set_phase(op, rf, eh_rfca_phase_fiducial(rf, t0));
+  }
+  {
+    int64_t ocap = b->ops_cap, vcap = b->var_cap;
+    if (grow((void **)&b->ops, &ocap, b->nops + b->prog->nops, sizeof(eh_op)) ||
+        grow((void **)&b->var, &vcap, b->nvar + 1, sizeof(eh_vdesc)))
+      return EH_ERR_NOMEM;
+    b->ops_cap = ocap;
+    b->var_cap = (int32_t)vcap;
+  }
+  if (b->n + n > b->cap) {
+    int64_t nc = b->cap ? b->cap : 4096;
+    while (nc < b->n + n)
+      nc *= 2;
+    for (int k = 0; k < 6; k++) {
+      double *q = (double *)realloc(b->soa[k], sizeof(double) * nc);
+      if (!q)
+        return EH_ERR_NOMEM;
+      b->soa[k] = q;
+    }
+    int32_t *a = (int32_t *)realloc(b->lp, sizeof(int32_t) * nc);
+    if (!a)
+      return EH_ERR_NOMEM;
+    b->lp = a;
+    a = (int32_t *)realloc(b->lc, sizeof(int32_t) * nc);
+    if (!a)
+      return EH_ERR_NOMEM;
+    b->lc = a;
+    double *q = (double *)realloc(b->ids, sizeof(double) * nc);
+    if (!q)
+      return EH_ERR_NOMEM;
+    b->ids = q;
+    b->cap = nc;
+  }
+  memcpy(b->ops + b->nops, b->prog->ops, sizeof(eh_op) * b->prog->nops);
+  b->var[b->nvar].op_offset = (int32_t)b->nops;
+  b->var[b->nvar].nops = b->prog->nops;
+  b->var[b->nvar].first = b->n;
+  b->var[b->nvar].n = n;
+  for (int64_t i = 0; i < n; i++) {
+    const double *r = coord[i];
+    for (int k = 0; k < 6; k++)
+      b->soa[k][b->n + i] = r[k];
+    b->ids[b->n + i] = r[6];
+    b->lp[b->n + i] = -1;
+    b->lc[b->n + i] = 0;
+    if (isinf(r[0]) || isinf(r[2]) || isnan(r[0]) || isnan(r[2])) {
+      b->soa[4][b->n + i] = 0.0;
+      b->soa[5][b->n + i] = Po * (1 + r[5]);
+      b->lp[b->n + i] = 0;
+      b->lc[b->n + i] = -1;
+    }
+  }
+  b->nops += b->prog->nops;
+  b->n += n;
+  b->uploaded = 0;
+  return b->nvar++;
+}
+
+static int upload(eh_batch *b) {
+  ehd_dev *d = eh_ctx_dev(b->ctx);
+  int rc;
+  int64_t nchunks = 0;
+  eh_chunk *ch;
+  const int64_t csize = (int64_t)b->tune.block * b->tune.particles_per_lane;
+  if (b->nops > 0x7fffffff)
+    return bmsg(b, EH_ERR_UNSUPPORTED, "batch programs too long");
+  rc = ehd_alloc_state(d, b->n);
+  if (!rc)
+    rc = ehd_put_state(d, b->soa[0], b->soa[1], b->soa[2], b->soa[3], b->soa[4], b->soa[5], b->lp, b->lc, b->n);
+  if (!rc)
+    rc = ehd_put_ops(d, b->ops, (int32_t)b->nops);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  for (int v = 0; v < b->nvar; v++)
+    nchunks += (b->var[v].n + csize - 1) / csize;
+  ch = (eh_chunk *)malloc(sizeof(eh_chunk) * (nchunks > 0 ? nchunks : 1));
+  if (!ch)
+    return EH_ERR_NOMEM;
+  nchunks = 0;
+  for (int v = 0; v < b->nvar; v++)
+    for (int64_t f = 0; f < b->var[v].n; f += csize) {
+      ch[nchunks].variant = v;
+      ch[nchunks].first = b->var[v].first + f;
+      ch[nchunks].count = (int32_t)(b->var[v].n - f < csize ? b->var[v].n - f : csize);
+      nchunks++;
+    }
+  rc = ehd_batch_setup(d, b->var, b->nvar, ch, (int32_t)nchunks);
+  free(ch);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  b->uploaded = 1;
+  return EH_OK;
+}
+
+int eh_batch_set_tuning(eh_batch *b, const eh_batch_tuning *t) {
+  if (!b || !t || (t->particles_per_lane != 1 && t->particles_per_lane != 2) || t->block < 64 || t->block > 256 ||
+      t->block % 64 || t->passes_per_launch < 1)
+    return EH_ERR_INVALID;
+  if (t->block * t->particles_per_lane != b->tune.block * b->tune.particles_per_lane && b->uploaded) {
+    /* the chunk size changed: rebuild the queue from the current device state */
+    int rc;
+    ehd_dev *d = eh_ctx_dev(b->ctx);
+    rc = ehd_get_state(d, b->soa[0], b->soa[1], b->soa[2], b->soa[3], b->soa[4], b->soa[5], b->lp, b->lc, b->n);
+    if (rc)
+      return bmsg(b, rc, ehd_message(d));
+    b->tune = *t;
+    return upload(b);
+  }
+  b->tune = *t;
+  return EH_OK;
+}
+
+int eh_batch_get_tuning(const eh_batch *b, eh_batch_tuning *t) {
+  if (!b || !t)
+    return EH_ERR_INVALID;
+  *t = b->tune;
+  return EH_OK;
+}
+
+int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats) {
+  ehd_dev *d;
+  double ms = 0;
+  int64_t nl = 0;
+  int rc;
+  if (!b || npasses < 0 || npasses > 0x7fffffff)
+    return EH_ERR_INVALID;
+  d = eh_ctx_dev(b->ctx);
+  if (!b->uploaded) {
+    rc = upload(b);
+    if (rc)
+      return rc;
+  }
+  ehd_timing_reset(d);
+  rc = ehd_batch_track(d, pass0, (int32_t)npasses, b->tune.passes_per_launch, b->tune.particles_per_lane,
+                       b->tune.block, b->tune.use_graph, &ms, &nl);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  if (stats) {
+    int64_t effort = 0, left = 0;
+    rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);
+    if (rc)
+      return bmsg(b, rc, ehd_message(d));
+    for (int64_t i = 0; i < b->n; i++) {
+      int64_t lp = b->lp[i];
+      if (lp >= 0 && lp < pass0)
+        continue;
+      if (lp < 0) {
+        effort += npasses;
+        left++;
+      } else
+        effort += lp - pass0;
+    }
+    memset(stats, 0, sizeof(*stats));
+    stats->effort = effort;
+    stats->n_left = left;
+    stats->passes_done = npasses;
+    stats->kernel_launches = nl;
+    stats->track_ms = ms;
+  }
+  return EH_OK;
+}
+
+/* Time candidate (particles per lane, workgroup size, passes per launch) on a
+ * short sample of the loaded batch, restoring the state after each trial, and
+ * keep the fastest. */
+int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_out) {
+  static const int ppl[2] = {1, 2};
+  static const int blocks[3] = {64, 128, 256};
+  ehd_dev *d;
+  eh_batch_tuning best, t;
+  double best_ms = 1e300;
+  int rc;
+  if (!b || sample_passes < 1)
+    return EH_ERR_INVALID;
+  d = eh_ctx_dev(b->ctx);
+  if (!b->uploaded) {
+    rc = upload(b);
+    if (rc)
+      return rc;
+  }
+  rc = ehd_state_snapshot(d, 0);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  best = b->tune;
+  for (int a = 0; a < 2; a++)
+    for (int c = 0; c < 3; c++) {
+      double ms = 0;
+      int64_t nl = 0;
+      t = b->tune;
+      t.particles_per_lane = ppl[a];
+      t.block = blocks[c];
+      t.passes_per_launch = (int32_t)sample_passes;
+      rc = eh_batch_set_tuning(b, &t);
+      if (!rc)
+        rc = ehd_state_snapshot(d, 1);
+      if (!rc)
+        rc = ehd_batch_track(d, 0, (int32_t)sample_passes, t.passes_per_launch, t.particles_per_lane, t.block, 0,
+                             &ms, &nl);
+      if (rc)
+        return bmsg(b, rc, ehd_message(d));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  /* passes per launch: long enough to amortize the launch and the queue tail,
+   * short enough to keep launches under ~1 s */
+  {
+    double per_pass = best_ms / sample_passes;
+    int32_t ppass = per_pass > 0 ? (int32_t)(1000.0 / per_pass) : 1024;
+    if (ppass < 1)
+      ppass = 1;
+    if (ppass > 1024)
+      ppass = 1024;
+    best.passes_per_launch = ppass;
+  }
+  best.use_graph = 1;
+  best.sample_ms = best_ms;
+  rc = eh_batch_set_tuning(b, &best);
+  if (!rc)
+    rc = ehd_state_snapshot(d, 1);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  ehd_timing_reset(d);
+  if (best_out)
+    *best_out = best;
+  return EH_OK;
+}
+
+int eh_batch_loss_passes(eh_batch *b, int32_t variant, int32_t *loss_pass, int64_t n) {
+  ehd_dev *d;
+  int rc;
+  if (!b || variant < 0 || variant >= b->nvar || !loss_pass || n != b->var[variant].n)
+    return EH_ERR_INVALID;
+  d = eh_ctx_dev(b->ctx);
+  if (!b->uploaded)
+    memcpy(loss_pass, b->lp + b->var[variant].first, sizeof(int32_t) * n);
+  else {
+    rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);
+    if (rc)
+      return bmsg(b, rc, ehd_message(d));
+    memcpy(loss_pass, b->lp + b->var[variant].first, sizeof(int32_t) * n);
+  }
+  return EH_OK;
+}
+
+typedef struct {
+  int64_t pass;
+  int32_t code;
+  int64_t slot;
+} bev;
+
+static int cmp_bev(const void *a, const void *b) {
+  const bev *x = (const bev *)a, *y = (const bev *)b;
+  if (x->pass != y->pass)
+    return x->pass < y->pass ? -1 : 1;
+  if (x->code != y->code)
+    return x->code < y->code ? -1 : 1;
+  return 0;
+}
+
+/* one variant's rows in elegant's order (survivors, then lost rows) */
+int eh_batch_download(eh_batch *b, int32_t variant, double **coord, int64_t n, int32_t nprops, int64_t *nLeft) {
+  ehd_dev *d;
+  int rc;
+  int64_t first, m = 0, alive;
+  int64_t *order, *pos, *lp;
+  bev *ev;
+  if (!b || variant < 0 || variant >= b->nvar || nprops < 8 || n != b->var[variant].n)
+    return EH_ERR_INVALID;
+  d = eh_ctx_dev(b->ctx);
+  if (b->uploaded) {
+    rc = ehd_get_state(d, b->soa[0], b->soa[1], b->soa[2], b->soa[3], b->soa[4], b->soa[5], b->lp, b->lc, b->n);
+    if (rc)
+      return bmsg(b, rc, ehd_message(d));
+  }
+  first = b->var[variant].first;
+  order = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  pos = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  lp = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  ev = (bev *)malloc(sizeof(bev) * (n + 1));
+  if (!order || !pos || !lp || !ev) {
+    free(order);
+    free(pos);
+    free(lp);
+    free(ev);
+    return EH_ERR_NOMEM;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    order[i] = pos[i] = i;
+    if (b->lp[first + i] >= 0) {
+      ev[m].pass = b->lp[first + i];
+      ev[m].code = b->lc[first + i];
+      ev[m].slot = i;
+      m++;
+    }
+  }
+  qsort(ev, (size_t)m, sizeof(bev), cmp_bev);
+  alive = n;
+  for (int64_t a = 0; a < m;) {
+    int64_t e = a;
+    while (e < m && ev[e].pass == ev[a].pass && ev[e].code == ev[a].code)
+      e++;
+    {
+      int64_t *slots = (int64_t *)malloc(sizeof(int64_t) * (e - a));
+      if (!slots) {
+        free(order);
+        free(pos);
+        free(lp);
+        free(ev);
+        return EH_ERR_NOMEM;
+      }
+      for (int64_t j = a; j < e; j++)
+        slots[j - a] = ev[j].slot;
+      eh_compact_event(order, pos, alive, slots, e - a, lp);
+      free(slots);
+    }
+    alive -= e - a;
+    a = e;
+  }
+  for (int64_t j = 0; j < n; j++) {
+    int64_t i = first + order[j];
+    for (int k = 0; k < 6; k++)
+      coord[j][k] = b->soa[k][i];
+    coord[j][6] = b->ids[i];
+    coord[j][7] = b->lp[i] >= 0 ? (double)b->lp[i] : -1.0;
+  }
+  if (nLeft)
+    *nLeft = alive;
+  free(order);
+  free(pos);
+  free(lp);
+  free(ev);
+  return EH_OK;
+}
+
+int32_t eh_batch_variants(const eh_batch *b) { return b ? b->nvar : 0; }

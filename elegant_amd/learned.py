@@ -1,0 +1,272 @@
+"""Learned dynamic-aperture boundary proposals.
+
+A perceptron (19 -> 256 -> 256 -> 256 -> 1, trained and evaluated by the
+library's own GPU kernels) maps (x0, y0, delta) and a variant's 16 error
+summaries to a survival probability.  Its proposed boundary on each ray is
+then confirmed by FP64 tracking of two points straddling it; rays where the
+proposal is wrong fall back to bisection.  The FP64 tracker is the judge of
+every boundary; the network only decides where to look.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import da
+from . import native
+
+PF = C.POINTER(C.c_float)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+    def __init__(self, tracker, n_in=19, hidden=256, seed=1):
+        self.tracker = tracker
+        self._m = C.c_void_p()
+        native.check(tracker._ctx, native.lib().eh_mlp_create(tracker._ctx, n_in, hidden, seed, C.byref(self._m)))
+        self.n_in = n_in
+        self.hidden = hidden
+
+    def close(self):
+        if self._m:
+            native.lib().eh_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def num_params(self):
+        return int(native.lib().eh_mlp_num_params(self._m))
+
+    def params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        native.check(self.tracker._ctx, native.lib().eh_mlp_get_params(self._m, p.ctypes.data_as(PF)))
+        return p
+
+    def set_params(self, p):
+        p = _f(p)
+        native.check(self.tracker._ctx, native.lib().eh_mlp_set_params(self._m, p.ctypes.data_as(PF)))
+
+    def gradient(self, X, y):
+        X = _f(X)
+        y = _f(y)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = C.c_float()
+        native.check(self.tracker._ctx, native.lib().eh_mlp_gradient(self._m, X.ctypes.data_as(PF),
+                                                                     y.ctypes.data_as(PF), X.shape[0],
+                                                                     g.ctypes.data_as(PF), C.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, X, y, epochs=8, batch=4096, lr=1e-3, seed=7):
+        X = _f(X)
+        y = _f(y)
+        losses = np.zeros(epochs, dtype=np.float32)
+        native.check(self.tracker._ctx, native.lib().eh_mlp_train(self._m, X.ctypes.data_as(PF), y.ctypes.data_as(PF),
+                                                                  X.shape[0], epochs, batch, lr, seed,
+                                                                  losses.ctypes.data_as(PF)))
+        return losses
+
+    def predict(self, X):
+        X = _f(X)
+        p = np.zeros(X.shape[0], dtype=np.float32)
+        native.check(self.tracker._ctx, native.lib().eh_mlp_predict(self._m, X.ctypes.data_as(PF), X.shape[0],
+                                                                    p.ctypes.data_as(PF)))
+        return p
+
+
+def unpack(params, n_in=19, hidden=256):
+    """views (W1, b1, W2, b2, W3, b3, W4, b4) of the flat parameter array"""
+    dims = [n_in, hidden, hidden, hidden, 1]
+    out = []
+    off = 0
+    for l in range(4):
+        w = params[off:off + dims[l] * dims[l + 1]].reshape(dims[l + 1], dims[l])
+        off += w.size
+        b = params[off:off + dims[l + 1]]
+        off += b.size
+        out += [w, b]
+    return out
+
+
+def grid_features(rays, deltas, nx, feats):
+    """network inputs for every grid point of one variant, ordered
+    (offset, ray, index): x0/xmax, y0/ymax, delta/0.03, then the variant's
+    16 normalized error summaries"""
+    frac = np.arange(nx) / (nx - 1)
+    nd = len(deltas)
+    X = np.empty((nd, rays.lines, nx, 3 + len(feats)), dtype=np.float32)
+    X[..., 0] = frac[None, None, :] * rays.dxf[None, :, None]
+    X[..., 1] = frac[None, None, :] * rays.dyf[None, :, None]
+    X[..., 2] = (np.asarray(deltas, dtype=float) / 0.03)[:, None, None]
+    X[..., 3:] = np.asarray(feats, dtype=np.float32)
+    return X.reshape(-1, 3 + len(feats))
+
+
+def training_set(vs, grid, rays, deltas):
+    """(X, y): every tracked grid point of the training variants with the
+    tracker's own verdict (1 = survived all passes)"""
+    Xs = [grid_features(rays, deltas, v.nx, v.features) for v in vs]
+    ys = [s.astype(np.float32).ravel() for s in grid["survived"]]
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def propose(mlp, vs, rays, deltas):
+    """guessed limit index per variant (offsets x rays): the end of the first
+    run of points with survival probability >= 0.5 going outward"""
+    out = []
+    for v in vs:
+        p = mlp.predict(grid_features(rays, deltas, v.nx, v.features)).reshape(len(deltas), rays.lines, v.nx)
+        bad = p < 0.5
+        out.append(np.where(bad.any(axis=2), np.argmax(bad, axis=2), v.nx) - 1)
+    return out
+
+
+def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, max_rounds=64):
+    """Find, on every ray of every (variant, offset), a grid index lo that
+    survives with lo + 1 lost (or lo = nx - 1 surviving, or lo = -1 with the
+    origin lost), assuming survival is monotone along the ray.  Each round
+    tracks one point per unresolved ray, all rays of all variants in one
+    batch.  Without a guess the point is the interval midpoint (bisection:
+    the interval splitting of aperture_search.c:1027-1050 with
+    split_fraction = 0.5, here on grid indices).  With a guess
+    the first round tracks guess and guess + 1; a ray whose guess is refuted
+    gallops away from it with doubling steps until the interval closes
+    around the boundary, then bisects.  Returns (limits per variant, effort,
+    rounds, wall seconds, fraction of rays settled by the first round)."""
+    nd, nl = len(deltas), rays.lines
+    nx = np.concatenate([np.full(nd * nl, v.nx) for v in vs])
+    R = nx.size
+    vidx = np.repeat(np.arange(len(vs)), nd * nl)
+    didx = np.tile(np.repeat(np.arange(nd), nl), len(vs))
+    lidx = np.tile(np.arange(nl), nd * len(vs))
+    lo = np.full(R, -1)
+    hi = nx.copy()
+    g = None if guess is None else np.clip(np.concatenate([gv.ravel() for gv in guess]), -1, nx - 1)
+    step = np.ones(R, dtype=np.int64)
+    effort = 0
+    wall = 0.0
+    rounds = 0
+    settled_first = None
+    dl = np.asarray(deltas, dtype=float)
+    while rounds < max_rounds:
+        active = hi - lo > 1
+        if not active.any():
+            break
+        if rounds == 0 and g is not None:
+            tests = [np.where(active & (g >= 0), g, -1), np.where(active & (g + 1 < nx), g + 1, -1)]
+        else:
+            t = (lo + hi) // 2
+            if g is not None:
+                up = lo > g   # the boundary lies above the guess
+                dn = hi <= g  # ... or below it
+                step = np.where(up | dn, step * 2, step)
+                t = np.where(up & (lo + step < hi), lo + step, t)
+                t = np.where(dn & (hi - step > lo), hi - step, t)
+            tests = [np.where(active, t, -1)]
+        sets, where = [], []
+        for k, v in enumerate(vs):
+            sel = slice(k * nd * nl, (k + 1) * nd * nl)
+            rows, idx = [], []
+            for t in tests:
+                m = np.nonzero(t[sel] >= 0)[0] + k * nd * nl
+                fr = t[m] / (nx[m] - 1)
+                rows.append(da._rows(fr * rays.xmax * rays.dxf[lidx[m]], fr * rays.ymax * rays.dyf[lidx[m]],
+                                     dl[didx[m]], 1))
+                idx.append((m, t[m]))
+            sets.append(np.concatenate(rows))
+            where.append(idx)
+        keep = [k for k in range(len(vs)) if sets[k].shape[0]]
+        from .studies import track_sets
+        lps, e, w, _ = track_sets(tracker, [vs[k] for k in keep], [sets[k] for k in keep], passes, tune)
+        effort += e
+        wall += w
+        rounds += 1
+        for j, k in enumerate(keep):
+            off = 0
+            for m, t in where[k]:
+                s = lps[j][off:off + m.size] < 0
+                off += m.size
+                hi[m] = np.where(~s, np.minimum(hi[m], t), hi[m])
+                lo[m] = np.where(s & (t < hi[m]), np.maximum(lo[m], t), lo[m])
+        # a survivor above a lost point says nothing about the first loss below it
+        bad = lo >= hi
+        if bad.any():
+            lo[bad] = -1
+        if settled_first is None:
+            settled_first = float(np.mean(hi - lo <= 1))
+    limits = [lo[k * nd * nl:(k + 1) * nd * nl].reshape(nd, nl) for k in range(len(vs))]
+    return limits, effort, rounds, wall, settled_first
+
+
+def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, lr=1e-3, tune=None, log=None,
+          rays=None, deltas=None):
+    """Train on the survival verdicts of the training variants' full grids,
+    then find the aperture of the evaluation variants three ways: the plain
+    grid (the reference answer), bisection alone, and the network's guess
+    confirmed by tracking with galloping/bisection fallback.  Returns a
+    report of particle-turns, wall time and area agreement."""
+    import time
+    from . import studies
+    rays = rays or da.Rays(studies.RAYS)
+    deltas = studies.DELTAS if deltas is None else deltas
+    say = log or (lambda *a: None)
+    tv = studies.variants(train_seeds)
+    ev = studies.variants(eval_seeds)
+    t0 = time.perf_counter()
+    gtrain = studies.plain_grid(tracker, tv, passes, rays, deltas, tune)
+    say("training grids tracked: %d particle-turns in %.1f s" % (gtrain["effort"], gtrain["wall"]))
+    X, y = training_set(tv, gtrain, rays, deltas)
+    mlp = MLP(tracker, n_in=X.shape[1])
+    t1 = time.perf_counter()
+    losses = mlp.train(X, y, epochs=epochs, batch=min(batch, X.shape[0]), lr=lr)
+    t_train = time.perf_counter() - t1
+    say("trained on %d samples: loss %.4f -> %.4f in %.1f s" % (X.shape[0], losses[0], losses[-1], t_train))
+    t_label = time.perf_counter() - t0
+
+    gref = studies.plain_grid(tracker, ev, passes, rays, deltas, tune)
+    say("evaluation grids: %d particle-turns in %.1f s" % (gref["effort"], gref["wall"]))
+    t2 = time.perf_counter()
+    guess = propose(mlp, ev, rays, deltas)
+    t_prop = time.perf_counter() - t2
+    lim_l, eff_l, rounds_l, wall_l, first_l = index_search(tracker, ev, passes, rays, deltas, guess, tune)
+    say("learned search: %d particle-turns, %d rounds, %.1f s" % (eff_l, rounds_l, wall_l))
+    lim_b, eff_b, rounds_b, wall_b, _ = index_search(tracker, ev, passes, rays, deltas, None, tune)
+    say("bisection search: %d particle-turns, %d rounds, %.1f s" % (eff_b, rounds_b, wall_b))
+    mlp.close()
+
+    def compare(lims):
+        same = np.mean([np.mean(a == b) for a, b in zip(lims, gref["limits"])])
+        errs = []
+        for v, lim, aref in zip(ev, lims, gref["areas"]):
+            a = studies.areas(rays, v.nx, lim)
+            for x, r in zip(a, aref):
+                if r > 0:
+                    errs.append(abs(x - r) / r)
+                elif x > 0:
+                    errs.append(1.0)
+        errs = np.array(errs) if errs else np.zeros(1)
+        return {"rays_identical": float(same), "area_rel_err_mean": float(errs.mean()),
+                "area_rel_err_max": float(errs.max())}
+
+    guess_hit = np.mean([np.mean(a == b) for a, b in zip(guess, gref["limits"])])
+    maps = len(ev) * len(deltas)
+    rep = {
+        "variants_train": len(tv), "variants_eval": len(ev), "passes": passes,
+        "grid_points_eval": int(sum(len(deltas) * rays.lines * v.nx for v in ev)),
+        "training": {"samples": int(X.shape[0]), "effort": gtrain["effort"], "track_s": gtrain["wall"],
+                     "train_s": t_train, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                     "total_s": t_label},
+        "plain_grid": {"effort_batched": gref["effort"], "effort_sequential": gref["effort_sequential"],
+                       "wall_s": gref["wall"], "maps_per_hour": maps / gref["wall"] * 3600},
+        "bisection": dict(effort=eff_b, rounds=rounds_b, wall_s=wall_b, maps_per_hour=maps / wall_b * 3600,
+                          **compare(lim_b)),
+        "learned": dict(effort=eff_l, rounds=rounds_l, wall_s=wall_l + t_prop, propose_s=t_prop,
+                        maps_per_hour=maps / (wall_l + t_prop) * 3600, guess_exact=float(guess_hit),
+                        settled_first_round=first_l, **compare(lim_l)),
+    }
+    return rep

@@ -15,6 +15,7 @@ P = C.c_void_p
 PP = C.POINTER(C.POINTER(C.c_double))
 PE = C.POINTER(A.eh_element)
 PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
 
 
 class EHError(RuntimeError):
@@ -55,6 +56,25 @@ SIGNATURES = [
     ("eh_track_through_zlongit", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_ZLONGIT), C.c_double,
                                              C.c_int64, C.c_double]),
     ("eh_last_kernel_timing", C.c_int, [P, C.POINTER(C.c_int64), PD]),
+    ("eh_batch_create", C.c_int, [P, C.POINTER(P)]),
+    ("eh_batch_destroy", C.c_int, [P]),
+    ("eh_batch_add_variant", C.c_int, [P, PE, C.c_int64, C.c_double, PP, C.c_int64, C.c_int32]),
+    ("eh_batch_autotune", C.c_int, [P, C.c_int64, C.POINTER(A.eh_batch_tuning)]),
+    ("eh_batch_set_tuning", C.c_int, [P, C.POINTER(A.eh_batch_tuning)]),
+    ("eh_batch_get_tuning", C.c_int, [P, C.POINTER(A.eh_batch_tuning)]),
+    ("eh_batch_track", C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(A.eh_stats)]),
+    ("eh_batch_download", C.c_int, [P, C.c_int32, PP, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
+    ("eh_batch_loss_passes", C.c_int, [P, C.c_int32, C.POINTER(C.c_int32), C.c_int64]),
+    ("eh_batch_variants", C.c_int32, [P]),
+    ("eh_da_trim_area", C.c_double, [C.c_int64, PD, PD, PD, PD, C.c_int32]),
+    ("eh_mlp_create", C.c_int, [P, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]),
+    ("eh_mlp_destroy", C.c_int, [P]),
+    ("eh_mlp_num_params", C.c_int64, [P]),
+    ("eh_mlp_get_params", C.c_int, [P, PF]),
+    ("eh_mlp_set_params", C.c_int, [P, PF]),
+    ("eh_mlp_gradient", C.c_int, [P, PF, PF, C.c_int32, PF, PF]),
+    ("eh_mlp_train", C.c_int, [P, PF, PF, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, PF]),
+    ("eh_mlp_predict", C.c_int, [P, PF, C.c_int64, PF]),
 ]
 # helpers outside the public header (measurement and tests)
 EXTRA = [
@@ -93,6 +113,81 @@ def check(ctx, rc):
 
 def _dptr(a):
     return a.ctypes.data_as(PD)
+
+
+class Batch:
+    """Many lattice variants, each with its own particles, tracked together
+    (eh_batch_* of the C ABI).  Keeps a reference to its Tracker's context."""
+
+    def __init__(self, tracker):
+        self.tracker = tracker
+        self._b = P()
+        check(tracker._ctx, lib().eh_batch_create(tracker._ctx, C.byref(self._b)))
+        self.sizes = []
+        self._keep = []
+
+    def close(self):
+        if self._b:
+            lib().eh_batch_destroy(self._b)
+            self._b = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_variant(self, beamline, Po, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        ptr, keep = A.rows_pointer(rows)
+        self._keep.append(beamline)
+        v = check(self.tracker._ctx, lib().eh_batch_add_variant(self._b, beamline.pointer(), len(beamline), Po, ptr,
+                                                                 rows.shape[0], rows.shape[1]))
+        self.sizes.append(rows.shape[0])
+        return v
+
+    def autotune(self, sample_passes=4):
+        t = A.eh_batch_tuning()
+        check(self.tracker._ctx, lib().eh_batch_autotune(self._b, sample_passes, C.byref(t)))
+        return t
+
+    def set_tuning(self, ppl, block, passes_per_launch, use_graph=1):
+        t = A.eh_batch_tuning(ppl, block, passes_per_launch, use_graph, 0.0)
+        check(self.tracker._ctx, lib().eh_batch_set_tuning(self._b, C.byref(t)))
+
+    def tuning(self):
+        t = A.eh_batch_tuning()
+        check(self.tracker._ctx, lib().eh_batch_get_tuning(self._b, C.byref(t)))
+        return t
+
+    def track(self, pass0, npasses):
+        st = A.eh_stats()
+        check(self.tracker._ctx, lib().eh_batch_track(self._b, pass0, npasses, C.byref(st)))
+        return st
+
+    def download(self, v):
+        n = self.sizes[v]
+        out = np.zeros((n, 8))
+        ptr, keep = A.rows_pointer(out)
+        nl = C.c_int64()
+        check(self.tracker._ctx, lib().eh_batch_download(self._b, v, ptr, n, 8, C.byref(nl)))
+        return out, int(nl.value)
+
+    def loss_passes(self, v):
+        n = self.sizes[v]
+        lp = np.zeros(n, dtype=np.int32)
+        check(self.tracker._ctx, lib().eh_batch_loss_passes(self._b, v, lp.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return lp
+
+
+def da_trim_area(x, y, dxf, dyf, full_plane=0):
+    """trimApertureSearchResult on copies of the limits; returns (area, x, y)."""
+    x = np.array(x, dtype=np.float64)
+    y = np.array(y, dtype=np.float64)
+    dxf = np.ascontiguousarray(dxf, dtype=np.float64)
+    dyf = np.ascontiguousarray(dyf, dtype=np.float64)
+    a = lib().eh_da_trim_area(x.size, _dptr(x), _dptr(y), _dptr(dxf), _dptr(dyf), full_plane)
+    return a, x, y
 
 
 class Tracker:

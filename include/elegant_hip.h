@@ -250,6 +250,64 @@ int64_t eh_track_through_wake(eh_ctx *ctx, double **part, int64_t np, const EH_W
 int64_t eh_track_through_zlongit(eh_ctx *ctx, double **part, int64_t np, const EH_ZLONGIT *zlongit, double Po,
                                  int64_t i_pass, double macroParticleCharge);
 
+/* ---- batched lattice variants (error-seed studies, dynamic-aperture grids) ----
+ * No counterpart call in the reference, which runs each variant and each
+ * aperture-scan particle through its own do_tracking call
+ * (aperture_search.c:1080-1082, error.c seeds via vary_beamline).  Every
+ * variant has its own beamline and particles; all are tracked together by a
+ * persistent kernel over a queue of (variant, particle run) work items,
+ * launched as a captured HIP graph.  Variants may not contain WAKE/ZLONGIT,
+ * and RFCA elements need T_REFERENCE or FIDUCIAL=light. */
+typedef struct eh_batch eh_batch;
+typedef struct {
+  int32_t particles_per_lane; /* 1 or 2 */
+  int32_t block;              /* workgroup size: 64, 128 or 256 */
+  int32_t passes_per_launch;  /* passes fused in one kernel launch */
+  int32_t use_graph;          /* capture the launch sequence in a HIP graph */
+  double sample_ms;           /* autotune: time of the best sample */
+} eh_batch_tuning;
+
+int eh_batch_create(eh_ctx *ctx, eh_batch **out);
+int eh_batch_destroy(eh_batch *b);
+/* returns the new variant's index (>= 0) or a negative error code */
+int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord, int64_t n,
+                         int32_t nprops);
+/* time the candidate tunings on sample_passes of the loaded batch (state restored) and keep the fastest */
+int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best);
+int eh_batch_set_tuning(eh_batch *b, const eh_batch_tuning *t);
+int eh_batch_get_tuning(const eh_batch *b, eh_batch_tuning *t);
+int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats);
+/* one variant's rows in elegant's order, column 7 = pass of loss */
+int eh_batch_download(eh_batch *b, int32_t variant, double **coord, int64_t n, int32_t nprops, int64_t *nLeft);
+/* pass of loss of each particle of a variant, in the order it was added (-1 = survived) */
+int eh_batch_loss_passes(eh_batch *b, int32_t variant, int32_t *loss_pass, int64_t n);
+int32_t eh_batch_variants(const eh_batch *b);
+
+/* ---- dynamic-aperture surrogate network ----
+ * No counterpart in the reference.  A 4-layer perceptron n_in -> hidden ->
+ * hidden -> hidden -> 1 (ReLU, FP32) giving the probability that a test
+ * particle survives; trained and evaluated on the GPU by hand-written
+ * kernels.  Parameters are the flat float array [W1 (hidden x n_in), b1,
+ * W2, b2, W3, b3, W4 (1 x hidden), b4], row-major. */
+typedef struct eh_mlp eh_mlp;
+int eh_mlp_create(eh_ctx *ctx, int32_t n_in, int32_t hidden, uint64_t seed, eh_mlp **out);
+int eh_mlp_destroy(eh_mlp *m);
+int64_t eh_mlp_num_params(const eh_mlp *m);
+int eh_mlp_get_params(eh_mlp *m, float *params);
+int eh_mlp_set_params(eh_mlp *m, const float *params);
+/* gradient of the mean binary cross-entropy over n samples (X row-major n x n_in, y in {0,1}) */
+int eh_mlp_gradient(eh_mlp *m, const float *X, const float *y, int32_t n, float *grad, float *loss);
+/* Adam, minibatches of `batch`, `epochs` passes over shuffled data; losses[epochs] optional */
+int eh_mlp_train(eh_mlp *m, const float *X, const float *y, int64_t n, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float *losses);
+int eh_mlp_predict(eh_mlp *m, const float *X, int64_t n, float *prob);
+
+/* trimApertureSearchResult (trimApertureSearchResult.c:105-117): clip islands
+ * from per-ray limits (modified in place) and return the trapezoid-rule area;
+ * -1 for fewer than two lines. */
+double eh_da_trim_area(int64_t lines, double *xLimit, double *yLimit, const double *dxFactor, const double *dyFactor,
+                       int32_t fullPlane);
+
 /* ---- device information for measurement ---- */
 /* Device time of the last eh_track call's dominant kernel: number of
  * launches and summed milliseconds (HIP events on the library's stream). */

@@ -277,3 +277,99 @@ def test_glibc_and_emath_checkers_diverge_like_the_gpu():
     rel = np.abs(a[:, 0] - b[:, 0]) / np.maximum(np.abs(b[:, 0]), 1e-3)
     assert rel.max() < 5e-10
     assert rel.max() > 0
+
+
+def jacobian6(elems, h=1e-7, mode="glibc"):
+    """central-difference 6x6 Jacobian at the origin in (x, x', y, y', s, delta)"""
+    J = np.zeros((6, 6))
+    for j in range(6):
+        a = np.zeros(6)
+        b = np.zeros(6)
+        a[j] = h
+        b[j] = -h
+        ra, na, _ = track(elems, rows_of([a]), mode=mode)
+        rb, nb, _ = track(elems, rows_of([b]), mode=mode)
+        assert na == nb == 1
+        J[:, j] = (ra[0, :6] - rb[0, :6]) / (2 * h)
+    return J
+
+
+def test_sector_bend_linear_map():
+    """Pure-dipole CSBEND without edges: the textbook sector-bend matrix
+    (x block cos/rho sin, dispersion rho(1-cos), sin; path length
+    ds = sin(t) x + rho (1 - cos t) x'; y block a drift)."""
+    L, th = 1.0, 0.1
+    rho = L / th
+    J = jacobian6([E.CSBEND(L=L, ANGLE=th, N_KICKS=16, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)])
+    c, s = math.cos(th), math.sin(th)
+    want = {(0, 0): c, (0, 1): rho * s, (1, 0): -s / rho, (1, 1): c, (0, 5): rho * (1 - c), (1, 5): s,
+            (2, 2): 1.0, (2, 3): L, (3, 3): 1.0, (4, 0): s, (4, 1): rho * (1 - c)}
+    for (i, j), v in want.items():
+        assert abs(J[i, j] - v) < 1e-6 * max(1.0, abs(v)), (i, j, J[i, j], v)
+    assert abs(J[0, 2]) < 1e-9 and abs(J[2, 0]) < 1e-9
+
+
+def test_rectangular_bend_edge_focusing():
+    """Edge angles E1 = E2 = angle/2 with first-order edges (no gap): the map
+    is Edge * Sector * Edge with horizontal edge kick +tan(e)/rho x and
+    vertical -tan(e)/rho y."""
+    L, th = 1.0, 0.1
+    rho = L / th
+    e = th / 2
+    J = jacobian6([E.CSBEND(L=L, ANGLE=th, N_KICKS=16, E1=e, E2=e)])
+    c, s = math.cos(th), math.sin(th)
+    Ex = np.array([[1, 0], [math.tan(e) / rho, 1]])
+    Ey = np.array([[1, 0], [-math.tan(e) / rho, 1]])
+    Mx = Ex @ np.array([[c, rho * s], [-s / rho, c]]) @ Ex
+    My = Ey @ np.array([[1, L], [0, 1]]) @ Ey
+    assert np.abs(J[0:2, 0:2] - Mx).max() < 1e-6 * rho
+    assert np.abs(J[2:4, 2:4] - My).max() < 1e-6 * rho
+    # the rectangular bend focuses vertically: |trace| < 2 would not hold for a drift with these kicks
+    assert J[3, 2] < 0
+
+
+@pytest.mark.parametrize("ctor,key,k,tilt", [
+    (E.KQUAD, "K1", 1.2, math.pi / 2),
+    (E.KSEXT, "K2", 40.0, math.pi / 3),
+    (E.KOCT, "K3", 800.0, math.pi / 4),
+])
+def test_roll_by_symmetry_angle_flips_the_multipole(ctor, key, k, tilt):
+    """A 2n-pole rolled by pi/n is the same magnet with the opposite sign
+    (pins the roll direction and the rotation in/out of the element frame)."""
+    pts = [[1.1e-3, 2e-4, -7e-4, -1e-4, 0, 5e-4], [-2e-3, 0, 1e-3, 3e-4, 0, -1e-3]]
+    ra, _, _ = track([ctor(L=0.3, TILT=tilt, N_KICKS=4, **{key: k})], rows_of(pts))
+    rb, _, _ = track([ctor(L=0.3, N_KICKS=4, **{key: -k})], rows_of(pts))
+    assert np.abs(ra[:, :4] - rb[:, :4]).max() < 1e-15
+    assert np.abs(ra[:, 4] - rb[:, 4]).max() < 1e-14
+
+
+def test_offset_quadrupole_is_a_translation():
+    """DX/DY move the magnet: a particle on the displaced axis feels no
+    field, and any orbit is the centred magnet's orbit shifted by (DX, DY)."""
+    dx, dy = 1e-3, -5e-4
+    q = dict(L=0.5, K1=1.4, N_KICKS=8)
+    r, _, _ = track([E.KQUAD(DX=dx, DY=dy, **q)], rows_of([[dx, 0, dy, 0, 0, 0]]))
+    assert r[0, 1] == 0 and r[0, 3] == 0
+    assert abs(r[0, 0] - dx) < 1e-18 and abs(r[0, 2] - dy) < 1e-18
+    pts = [[2e-4, 1e-4, 3e-4, -2e-4, 0, 1e-3]]
+    shifted = [[p[0] + dx, p[1], p[2] + dy, p[3], p[4], p[5]] for p in pts]
+    ra, _, _ = track([E.KQUAD(DX=dx, DY=dy, **q)], rows_of(shifted))
+    rb, _, _ = track([E.KQUAD(**q)], rows_of(pts))
+    assert abs((ra[0, 0] - dx) - rb[0, 0]) < 1e-17 and abs((ra[0, 2] - dy) - rb[0, 2]) < 1e-17
+    assert abs(ra[0, 1] - rb[0, 1]) < 1e-16 and abs(ra[0, 3] - rb[0, 3]) < 1e-16
+
+
+def test_drift_aperture_loss_point_is_extrapolated():
+    """A particle leaving a drift beyond X_MAX is lost where its straight
+    line crossed the aperture (limit_amplitudes.c:232-268); a particle
+    exactly on the aperture survives (the test is strict)."""
+    els = [E.MAXAMP(X_MAX=0.02, Y_MAX=0.01), E.DRIF(L=2.0)]
+    r, n, _ = track(els, rows_of([[0.02, 0, 0, 0, 0, 0], [0.019, 0.001, 0, 0, 0, 2e-3],
+                                  [0, 0, 0.009, 0.002, 0, 0]]))
+    assert n == 1 and r[0, 0] == 0.02
+    lost = {int(row[6]): row for row in r[1:]}
+    a = lost[2]
+    assert abs(a[4] - 1.0) < 1e-12 and abs(a[0] - 0.02) < 1e-15
+    assert a[5] == PO * (1 + 2e-3)
+    b = lost[3]
+    assert abs(b[4] - 0.5) < 1e-12 and abs(b[2] - 0.01) < 1e-15
