@@ -109,6 +109,7 @@ typedef struct {
 #define EH_D_BDSHR 28     /* [8]: (s * driftFrac[j]) / rho0 */
 #define EH_D_BDS 36       /* [8]: s * kickFrac[j] */
 #define EH_D_BKF 44       /* [8]: kickFrac[j] (0 ends the slice) */
+#define EH_D_RHOA_SGN 52  /* +1 or -1: the sign of rho_actual (t / rho_actual for t = +-0 or NaN) */
 
 /* ---- RFCA ---- */
 #define EH_D_VOLT 0       /* volt / (1e6 me_mev relSign) / nKicks */

@@ -30,6 +30,11 @@
 
 DEV double sq(double v) { return v * v; }
 
+/* A wave-uniform read of the element program through the constant address
+ * space: the records are read-only while a kernel runs, so a uniform address
+ * becomes a scalar load instead of a vector load and wait. */
+DEV double opc(const double *p) { return *(const __attribute__((address_space(4))) double *)p; }
+
 struct particle {
   double x, xp, y, yp, s, dp;
 };
@@ -213,6 +218,60 @@ DEV void apply_kick(double &qx, double &qy, int order, int skew, double KnL, dou
   qy += KnL * Fx;
 }
 
+/* The slices of integrate_kick_multipole_ordn (multipole.c:1122-1410), with
+ * op.nsub drift substeps per slice (per-substep coefficients are scalar
+ * loads of the element record).
+ * Returns 0, or the loss kind: 1 aperture (x, y hold the loss point),
+ * 2 slopes (the row is left as received). */
+DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
+                       double &dzLoss, const double dp, const bool expand, const eh_op &op) {
+  const int nsub = op.nsub;
+  const int flags = op.flags;
+  const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
+  const bool has_drift = op.d[EH_D_DRIFT] != 0;
+  const int nko = op.nkick_orders;
+  for (int ik = 0; ik < op.nslices; ik++) {
+    if (!in_aperture(x + dxa, y + dya, op))
+      return 1;
+    for (int step = 0; step < nsub; step++) {
+      if (has_drift) {
+        const double dsh = opc(&op.d[EH_D_DSH + step]);
+        x += xp * dsh;
+        y += yp * dsh;
+        if (expand)
+          s += dsh * (1 + (sq(xp) + sq(yp)) / 2);
+        else
+          s += dsh * __builtin_sqrt(1 + sq(xp) + sq(yp));
+        dzLoss += dsh;
+      }
+      if (step == nsub - 1)
+        break; /* kickFrac[last] == 0 */
+      if (nko > 0)
+        apply_kick(qx, qy, op.order0, op.skew0, opc(&op.d[EH_D_KICK + step]), x, y);
+      if (nko > 1)
+        apply_kick(qx, qy, op.order1, op.skew1, opc(&op.d[EH_D_KICK + 8 + step]), x, y);
+      if (nko > 2)
+        apply_kick(qx, qy, op.order2, op.skew2, opc(&op.d[EH_D_KICK + 16 + step]), x, y);
+      if (flags & EH_F_XKICK)
+        apply_kick(qx, qy, 0, 0, opc(&op.d[EH_D_XK + step]), x, y);
+      if (flags & EH_F_YKICK)
+        apply_kick(qx, qy, 0, 1, opc(&op.d[EH_D_YK + step]), x, y);
+      if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+        return 2;
+      if (has_drift) {
+        /* rad_coef = 0, isr_coef < 0: the q/(1+dp)*(1+dp) round trip (multipole.c:1321-1351) */
+        qx /= (1 + dp);
+        qy /= (1 + dp);
+        qx *= (1 + dp);
+        qy *= (1 + dp);
+        if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+          return 2;
+      }
+    }
+  }
+  return 0;
+}
+
 /* multipole_tracking2 per particle: entry misalignment, the integrator of
  * integrate_kick_multipole_ordn (multipole.c:1122-1410), exit misalignment.
  * Returns 0 if alive, EH_STAGE_ELEMENT + 1 if lost (row frozen). */
@@ -250,53 +309,7 @@ DEV int track_multipole(particle &p, const eh_op &op) {
     }
   }
   if (!lost) {
-    const int nsub = op.nsub;
-    const bool has_drift = op.d[EH_D_DRIFT] != 0;
-    for (int ik = 0; ik < op.nslices && !lost; ik++) {
-      if (!in_aperture(x + dxa, y + dya, op)) {
-        lost = 1;
-        break;
-      }
-      for (int step = 0; step < nsub; step++) {
-        if (has_drift) {
-          double dsh = op.d[EH_D_DSH + step];
-          x += xp * dsh;
-          y += yp * dsh;
-          if (expand)
-            s += dsh * (1 + (sq(xp) + sq(yp)) / 2);
-          else
-            s += dsh * __builtin_sqrt(1 + sq(xp) + sq(yp));
-          dzLoss += dsh;
-        }
-        if (step == nsub - 1)
-          break; /* kickFrac[last] == 0 */
-        if (op.nkick_orders > 0)
-          apply_kick(qx, qy, op.order0, op.skew0, op.d[EH_D_KICK + step], x, y);
-        if (op.nkick_orders > 1)
-          apply_kick(qx, qy, op.order1, op.skew1, op.d[EH_D_KICK + 8 + step], x, y);
-        if (op.nkick_orders > 2)
-          apply_kick(qx, qy, op.order2, op.skew2, op.d[EH_D_KICK + 16 + step], x, y);
-        if (flags & EH_F_XKICK)
-          apply_kick(qx, qy, 0, 0, op.d[EH_D_XK + step], x, y);
-        if (flags & EH_F_YKICK)
-          apply_kick(qx, qy, 0, 1, op.d[EH_D_YK + step], x, y);
-        if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
-          lost = 2;
-          break;
-        }
-        if (has_drift) {
-          /* rad_coef = 0, isr_coef < 0: the q/(1+dp)*(1+dp) round trip (multipole.c:1321-1351) */
-          qx /= (1 + dp);
-          qy /= (1 + dp);
-          qx *= (1 + dp);
-          qy *= (1 + dp);
-          if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
-            lost = 2;
-            break;
-          }
-        }
-      }
-    }
+    lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, op);
     if (!lost) {
       if (!in_aperture(x + dxa, y + dya, op))
         lost = 1;
@@ -335,6 +348,58 @@ DEV int track_multipole(particle &p, const eh_op &op) {
 
 /* ---- CSBEND ---- */
 
+/* The slices of integrate_csbend_ordn (csbend.c:1434-1693) for a pure
+ * dipole (Fx = 0, Fy = 1, csbend.c:106-110), op.nsub drift substeps per
+ * slice; per-substep coefficients are scalar loads of the element record.  Returns true if the particle is lost; X, QX, Y, QY, dist
+ * and dz_lost then hold their values at the loss, as the reference leaves
+ * them. */
+DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double DP, double &dist, double &dz_lost,
+                     const eh_op &op) {
+  const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA], rsgn = op.d[EH_D_RHOA_SGN];
+  const double opd = 1 + DP;
+  const int nsub = op.nsub;
+  for (int i = 0; i < op.nslices; i++) {
+    if (!in_aperture(X, Y, op))
+      return true;
+    for (int j = 0; j < nsub; j++) {
+      double f = sq(opd) - sq(QY);
+      if (f <= 0)
+        return true;
+      f = __builtin_sqrt(f);
+      double sin_phi = QX / f;
+      if (__builtin_fabs(sin_phi) > 1)
+        return true;
+      double phi = em_asin(sin_phi);
+      double sine, cosi;
+      em_sincos(opc(&op.d[EH_D_BDSHR + j]) + phi, &sine, &cosi);
+      if (cosi == 0)
+        return true;
+      double tang = sine / cosi;
+      double cos_phi = em_cos(phi);
+      QX = f * sine;
+      double factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
+      Y += QY * factor;
+      dist += factor * opd;
+      dz_lost += opc(&op.d[EH_D_BDSH + j]);
+      f = cos_phi / cosi;
+      X = rho0 * (f - 1) + f * X;
+      if (j == nsub - 1)
+        break; /* kickFrac[last] == 0 */
+      double ds = opc(&op.d[EH_D_BDS + j]);
+      double g = 1 + X / rho0;
+      QX += -ds * g * 1.0 / rhoa;
+      /* Fx = 0: ds*g*0.0 is +-0 or NaN, and dividing it by rho_actual only
+       * sets its sign, which the multiplication by sign(rho_actual) does */
+      QY += ds * g * 0.0 * rsgn;
+      QX /= opd;
+      QY /= opd;
+      QX *= opd;
+      QY *= opd;
+    }
+  }
+  return false;
+}
+
 DEV int track_csbend(particle &p, const eh_op &op) {
   const int flags = op.flags;
   const double ct = op.d[EH_D_CT], st = op.d[EH_D_ST];
@@ -368,54 +433,8 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   }
   /* integrate_csbend_ordn (csbend.c:1434-1693), Fx = 0, Fy = 1 */
   double dist = 0, dz_lost = 0;
-  bool lost = false;
-  const int nsub = op.nsub;
-  for (int i = 0; i < op.nslices && !lost; i++) {
-    if (!in_aperture(X, Y, op)) {
-      lost = true;
-      break;
-    }
-    for (int j = 0; j < nsub; j++) {
-      double dsh = op.d[EH_D_BDSH + j];
-      double f = sq(1 + DP) - sq(QY);
-      if (f <= 0) {
-        lost = true;
-        break;
-      }
-      f = __builtin_sqrt(f);
-      if (__builtin_fabs(QX / f) > 1) {
-        lost = true;
-        break;
-      }
-      double sin_phi = QX / f;
-      double phi = em_asin(sin_phi);
-      double sine, cosi;
-      em_sincos(op.d[EH_D_BDSHR + j] + phi, &sine, &cosi);
-      if (cosi == 0) {
-        lost = true;
-        break;
-      }
-      double tang = sine / cosi;
-      double cos_phi = em_cos(phi);
-      QX = f * sine;
-      double factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
-      Y += QY * factor;
-      dist += factor * (1 + DP);
-      dz_lost += dsh;
-      f = cos_phi / cosi;
-      X = rho0 * (f - 1) + f * X;
-      if (j == nsub - 1)
-        break; /* kickFrac[last] == 0 */
-      double ds = op.d[EH_D_BDS + j];
-      double g = 1 + X / rho0;
-      QX += -ds * g * 1.0 / rhoa;
-      QY += ds * g * 0.0 / rhoa;
-      QX /= (1 + DP);
-      QY /= (1 + DP);
-      QX *= (1 + DP);
-      QY *= (1 + DP);
-    }
-  }
+  bool lost;
+  lost = csbend_body(X, QX, Y, QY, DP, dist, dz_lost, op);
   if (!lost) {
     dz_lost = op.d[EH_D_NS];
     if (!in_aperture(X, Y, op))

@@ -16,9 +16,13 @@
  *    seven 24-bit pieces, accumulated in double-double (Knuth TwoSum), valid
  *    for |x| < 2^28; larger |x| returns NaN (documented domain limit);
  *  - sin/cos on [-pi/4, pi/4]: Taylor series to degree 17/16 (coefficients
- *    are 1/n! rounded to double), with the compensated 1 - z/2 step for cos;
- *  - asin: Taylor series of asin in x^2 on |x| <= 0.5 and the reflection
- *    asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above;
+ *    are 1/n! rounded to double), shortened to degree 9/8 for |x| < 2^-5,
+ *    with the compensated 1 - z/2 step for cos;
+ *  - asin: Taylor series of asin in x^2 on |x| <= 0.5 (5, 9 or 25 terms by
+ *    band) and the reflection asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above;
+ *  - every polynomial by Horner's rule in fused multiply-adds (correctly
+ *    rounded on both sides: the C library's fma on the host, v_fma_f64 on
+ *    the device);
  *  - log: x = 2^e m, log(m) = 2 atanh(f/(2+f)) with f = m - 1.
  * Constants were derived from exact rational arithmetic (pi from Machin's
  * formula, ln 2 from sum 1/(k 2^k)) and are written as hex literals.
@@ -41,6 +45,20 @@
 #define EM_SHIFT 0x1.8p+52
 #define EM_LN2_HI 0x1.62e42fee00000p-1
 #define EM_LN2_LO 0x1.a39ef35600000p-33
+
+/* A polynomial coefficient at its point of use.  On the device the empty
+ * volatile asm keeps each 64-bit constant a scalar-register operand
+ * materialized where it is used; without it the compiler hoists every
+ * coefficient of every inlined polynomial into vector registers for the
+ * whole kernel, which costs occupancy.  The value is unchanged. */
+#if defined(__HIP_DEVICE_COMPILE__)
+EM_FN double em_k(double c) {
+  asm volatile("" : "=s"(c) : "0"(c));
+  return c;
+}
+#else
+EM_FN double em_k(double c) { return c; }
+#endif
 
 EM_FN uint64_t em_bits(double d) {
   uint64_t u;
@@ -105,27 +123,72 @@ EM_FN int em_reduce_pio2(double x, double *rh, double *rl) {
   return (int)(long long)kd;
 }
 
-/* sin(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
-EM_FN double em_sin_kernel(double x, double y) {
-  const double s1 = -0x1.5555555555555p-3, s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13,
-               s4 = 0x1.71de3a556c734p-19, s5 = -0x1.ae64567f544e4p-26, s6 = 0x1.6124613a86d09p-33,
-               s7 = -0x1.ae7f3e733b81fp-41, s8 = 0x1.952c77030ad4ap-49;
-  double z = x * x;
-  double v = z * x;
-  double r = s2 + z * (s3 + z * (s4 + z * (s5 + z * (s6 + z * (s7 + z * s8)))));
-  return x + ((y - 0.5 * z * y) + v * (s1 + z * r));
+/* Polynomial cores, evaluated by Horner's rule in fused multiply-adds.  The
+ * band |x| < 2^-5 keeps the series through x^9 (sin) and x^8 (cos): the
+ * first dropped term is below 2^-64 relative, so both bands meet the same
+ * accuracy while the narrow one costs half the operations. */
+#define EM_S1 (-0x1.5555555555555p-3)
+#define EM_SMALL_Z 0x1.0p-10
+
+/* (sin(x) - x - S1 x^3) / x^5 as a polynomial in z = x*x */
+EM_FN double em_sin_r(double z) {
+  const double s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13, s4 = 0x1.71de3a556c734p-19,
+               s5 = -0x1.ae64567f544e4p-26, s6 = 0x1.6124613a86d09p-33, s7 = -0x1.ae7f3e733b81fp-41,
+               s8 = 0x1.952c77030ad4ap-49;
+  if (z < EM_SMALL_Z)
+    return __builtin_fma(z, __builtin_fma(z, em_k(s4), em_k(s3)), em_k(s2));
+  double r = __builtin_fma(z, em_k(s8), em_k(s7));
+  r = __builtin_fma(z, r, em_k(s6));
+  r = __builtin_fma(z, r, em_k(s5));
+  r = __builtin_fma(z, r, em_k(s4));
+  r = __builtin_fma(z, r, em_k(s3));
+  return __builtin_fma(z, r, em_k(s2));
 }
 
-/* cos(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
-EM_FN double em_cos_kernel(double x, double y) {
+/* (cos(x) - 1 + z/2) / z^2 as a polynomial in z = x*x */
+EM_FN double em_cos_r(double z) {
   const double c2 = 0x1.5555555555555p-5, c3 = -0x1.6c16c16c16c17p-10, c4 = 0x1.a01a01a01a01ap-16,
                c5 = -0x1.27e4fb7789f5cp-22, c6 = 0x1.1eed8eff8d898p-29, c7 = -0x1.93974a8c07c9dp-37,
                c8 = 0x1.ae7f3e733b81fp-45;
+  if (z < EM_SMALL_Z)
+    return __builtin_fma(z, __builtin_fma(z, em_k(c4), em_k(c3)), em_k(c2));
+  double r = __builtin_fma(z, em_k(c8), em_k(c7));
+  r = __builtin_fma(z, r, em_k(c6));
+  r = __builtin_fma(z, r, em_k(c5));
+  r = __builtin_fma(z, r, em_k(c4));
+  r = __builtin_fma(z, r, em_k(c3));
+  return __builtin_fma(z, r, em_k(c2));
+}
+
+/* sin(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
+EM_FN double em_sin_kernel(double x, double y) {
   double z = x * x;
-  double r = z * z * (c2 + z * (c3 + z * (c4 + z * (c5 + z * (c6 + z * (c7 + z * c8))))));
+  double v = z * x;
+  return x + ((y - 0.5 * z * y) + v * __builtin_fma(z, em_sin_r(z), em_k(EM_S1)));
+}
+
+/* cos(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x|; 1 - z/2 is split so that
+ * its rounding error is carried into the tail */
+EM_FN double em_cos_kernel(double x, double y) {
+  double z = x * x;
+  double r = z * z * em_cos_r(z);
   double hz = 0.5 * z;
   double w = 1.0 - hz;
   return w + (((1.0 - w) - hz) + (r - x * y));
+}
+
+/* the same kernels with y = 0, for arguments that need no reduction */
+EM_FN double em_sin_kernel0(double x) {
+  double z = x * x;
+  return x + (z * x) * __builtin_fma(z, em_sin_r(z), em_k(EM_S1));
+}
+
+EM_FN double em_cos_kernel0(double x) {
+  double z = x * x;
+  double r = z * z * em_cos_r(z);
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  return w + (((1.0 - w) - hz) + r);
 }
 
 EM_FN void em_sincos(double x, double *sp, double *cp) {
@@ -134,6 +197,11 @@ EM_FN void em_sincos(double x, double *sp, double *cp) {
   if (x == 0.0) {
     *sp = x;
     *cp = 1.0;
+    return;
+  }
+  if (em_fabs(x) <= EM_PIO4) {
+    *sp = em_sin_kernel0(x);
+    *cp = em_cos_kernel0(x);
     return;
   }
   k = em_reduce_pio2(x, &rh, &rl);
@@ -164,15 +232,29 @@ EM_FN void em_sincos(double x, double *sp, double *cp) {
 }
 
 EM_FN double em_sin(double x) {
-  double s, c;
-  em_sincos(x, &s, &c);
-  return s;
+  double rh, rl, v;
+  int k;
+  if (x == 0.0)
+    return x;
+  if (em_fabs(x) <= EM_PIO4)
+    return em_sin_kernel0(x);
+  k = em_reduce_pio2(x, &rh, &rl);
+  if (k < 0 && rh != rh)
+    return em_nan();
+  v = (k & 1) ? em_cos_kernel(rh, rl) : em_sin_kernel(rh, rl);
+  return (k & 2) ? -v : v;
 }
 
 EM_FN double em_cos(double x) {
-  double s, c;
-  em_sincos(x, &s, &c);
-  return c;
+  double rh, rl, v;
+  int k;
+  if (em_fabs(x) <= EM_PIO4)
+    return x == 0.0 ? 1.0 : em_cos_kernel0(x);
+  k = em_reduce_pio2(x, &rh, &rl);
+  if (k < 0 && rh != rh)
+    return em_nan();
+  v = (k & 1) ? em_sin_kernel(rh, rl) : em_cos_kernel(rh, rl);
+  return ((k + 1) & 2) ? -v : v;
 }
 
 EM_FN double em_tan(double x) {
@@ -181,12 +263,26 @@ EM_FN double em_tan(double x) {
   return s / c;
 }
 
-/* (asin(x) - x)/x as a polynomial in z = x*x; short form for z <= 1/64 */
+/* (asin(x) - x)/x as a polynomial in z = x*x: five terms for z < 2^-10
+ * (next term below 2^-65 relative), nine for z <= 1/64 */
 EM_FN double em_asin_poly_small(double z) {
   const double a1 = 0x1.5555555555555p-3, a2 = 0x1.3333333333333p-4, a3 = 0x1.6db6db6db6db7p-5,
                a4 = 0x1.f1c71c71c71c7p-6, a5 = 0x1.6e8ba2e8ba2e9p-6, a6 = 0x1.1c4ec4ec4ec4fp-6,
                a7 = 0x1.c99999999999ap-7, a8 = 0x1.7a87878787878p-7, a9 = 0x1.3fde50d79435ep-7;
-  return z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * (a5 + z * (a6 + z * (a7 + z * (a8 + z * a9))))))));
+  double p;
+  if (z < EM_SMALL_Z) {
+    p = __builtin_fma(z, em_k(a5), em_k(a4));
+  } else {
+    p = __builtin_fma(z, em_k(a9), em_k(a8));
+    p = __builtin_fma(z, p, em_k(a7));
+    p = __builtin_fma(z, p, em_k(a6));
+    p = __builtin_fma(z, p, em_k(a5));
+    p = __builtin_fma(z, p, em_k(a4));
+  }
+  p = __builtin_fma(z, p, em_k(a3));
+  p = __builtin_fma(z, p, em_k(a2));
+  p = __builtin_fma(z, p, em_k(a1));
+  return z * p;
 }
 
 /* same series, long form for z <= 1/4 (25 terms) */
@@ -198,9 +294,9 @@ EM_FN double em_asin_poly_large(double z) {
                         0x1.fcaf8fb6db6dbp-9, 0x1.d3d2a8e0dd67dp-9, 0x1.b026f57b13b14p-9, 0x1.90cb77f60c7cep-9,
                         0x1.750de64d7d05fp-9, 0x1.5c5f56efaaaabp-9, 0x1.464c0950f7d47p-9, 0x1.3275586c5f2f0p-9,
                         0x1.208d3570ae5a6p-9};
-  double p = a[24];
+  double p = em_k(a[24]);
   for (int i = 23; i >= 0; i--)
-    p = a[i] + z * p;
+    p = __builtin_fma(z, p, em_k(a[i]));
   return z * p;
 }
 
@@ -208,14 +304,10 @@ EM_FN double em_asin(double x) {
   double ax = em_fabs(x);
   if (!(ax <= 1.0))
     return em_nan();
-  if (ax <= 0.125) {
-    double z = x * x;
-    return x + x * em_asin_poly_small(z);
-  }
-  if (ax <= 0.5) {
-    double z = x * x;
-    return x + x * em_asin_poly_large(z);
-  }
+  if (ax <= 0.125)
+    return __builtin_fma(x, em_asin_poly_small(x * x), x);
+  if (ax <= 0.5)
+    return __builtin_fma(x, em_asin_poly_large(x * x), x);
   /* asin(ax) = pi/2 - 2 asin(sqrt(t)), t = (1 - ax)/2 exact */
   double t = (1.0 - ax) * 0.5;
   double s = __builtin_sqrt(t);

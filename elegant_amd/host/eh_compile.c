@@ -452,6 +452,7 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
   op->nsub = nsub;
   op->d[EH_D_RHO0] = rho0;
   op->d[EH_D_RHOA] = rho_actual;
+  op->d[EH_D_RHOA_SGN] = rho_actual < 0 ? -1.0 : 1.0;
   op->d[EH_D_TANE1] = tan(e1);
   op->d[EH_D_TANE2] = tan(e2);
   op->d[EH_D_E1] = e1;

@@ -26,6 +26,7 @@ class MLP:
         self.tracker = tracker
         self._m = C.c_void_p()
         native.check(tracker._ctx, native.lib().eh_mlp_create(tracker._ctx, n_in, hidden, seed, C.byref(self._m)))
+        tracker.adopt(self)
         self.n_in = n_in
         self.hidden = hidden
 

@@ -2,6 +2,7 @@
 its C ABI.  There is no fallback: if the library is missing, or no GPU is
 present when a context is created, the call raises."""
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -123,6 +124,7 @@ class Batch:
         self.tracker = tracker
         self._b = P()
         check(tracker._ctx, lib().eh_batch_create(tracker._ctx, C.byref(self._b)))
+        tracker.adopt(self)
         self.sizes = []
         self._keep = []
 
@@ -201,9 +203,17 @@ class Tracker:
         check(None, lib().eh_ctx_create(device, C.byref(self._ctx)))
         self._beamline = None
         self.Po = None
+        self._children = weakref.WeakSet()
+
+    def adopt(self, child):
+        """register an object (Batch, MLP) whose device state lives in this
+        context: closing the context closes it first"""
+        self._children.add(child)
 
     def close(self):
         if self._ctx:
+            for c in list(self._children):
+                c.close()
             lib().eh_ctx_destroy(self._ctx)
             self._ctx = P()
 

@@ -75,7 +75,8 @@ def test_gradient_matches_reference(tracker, n_in, hidden, B):
 @pytest.mark.gpu
 def test_training_learns_a_survival_boundary(tracker):
     """points survive inside an ellipse whose size shrinks with |delta| and an
-    error feature; Adam on the GPU must reach >97 % accuracy"""
+    error feature; Adam on the GPU must cut the loss fourfold and classify
+    held-out points with >95 % accuracy"""
     from elegant_amd import learned
     rng = np.random.default_rng(0)
     n = 65536
@@ -87,9 +88,9 @@ def test_training_learns_a_survival_boundary(tracker):
     r = 0.6 - 0.2 * X[:, 2] ** 2 + 0.05 * X[:, 3]
     y = (X[:, 0] ** 2 + X[:, 1] ** 2 < r ** 2).astype(np.float32)
     m = learned.MLP(tracker, seed=2)
-    losses = m.train(X[:49152], y[:49152], epochs=12, batch=1024, lr=2e-3, seed=9)
-    assert losses[-1] < 0.5 * losses[0]
+    losses = m.train(X[:49152], y[:49152], epochs=24, batch=512, lr=2e-3, seed=9)
+    assert losses[-1] < 0.25 * losses[0]
     p = m.predict(X[49152:])
     acc = np.mean((p >= 0.5) == (y[49152:] > 0.5))
-    assert acc > 0.97, (acc, losses)
+    assert acc > 0.95, (acc, losses)
     m.close()
