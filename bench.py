@@ -35,7 +35,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="timed passes")
     ap.add_argument("--warmup", type=int, default=2, help="untimed passes (the first includes the RF fiducial)")
-    ap.add_argument("--workload", default="storage_ring", choices=["storage_ring", "fodo_ring", "storage_ring_wake"])
+    ap.add_argument("--workload", default="storage_ring",
+                    choices=["storage_ring", "fodo_ring", "storage_ring_wake", "error_seeds", "learned_da"])
+    ap.add_argument("--variants", type=int, default=256, help="error_seeds / learned_da: lattice variants")
     ap.add_argument("--particles", type=int, default=0, help="override the workload's particle count")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0 = all, max 16)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -88,12 +90,84 @@ def cpu_baseline(name, threads):
                       % (threads, per_thread, passes, total, dt)}
 
 
+def traffic(kernel, workload):
+    """HBM bytes per launch of a kernel for a workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/traffic.json), or None"""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t[kernel][workload]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def error_seeds(args):
+    """Batched error-seed study: variants of the storage ring with seeded
+    FSE/misalignment/roll errors, each with its own aperture grid (41 rays x
+    11 momentum offsets, 1,024-65,536 particles log-uniform by seed), all
+    tracked in one persistent graph-launched batch with autotuned
+    particles-per-lane, block size and passes per launch."""
+    import torch
+    from elegant_amd import da, native, studies
+    vs = studies.variants(range(args.variants))
+    rays = da.Rays(studies.RAYS)
+    tr = native.Tracker(0)
+    b = native.Batch(tr)
+    for v in vs:
+        b.add_variant(v.beamline, v.Po, studies.grid_rows(rays, studies.DELTAS, v.nx))
+    tun = b.autotune(4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = b.track(0, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = sum(b.sizes)
+    b.close()
+    tr.close()
+    return {"metric": "particle-turns/s FP64 6D tracking (batched error-seed study)", "value": st.effort / wall,
+            "unit": "particle-turns/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+            "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "%d storage-ring error-seed variants, aperture grids 41 rays x 11 offsets, "
+                                   "%d particles in total" % (len(vs), n),
+                       "variants": len(vs), "particles": n, "passes_timed": args.steps,
+                       "survivors": int(st.n_left), "kernel_launches": int(st.kernel_launches),
+                       "tuning": {"particles_per_lane": tun.particles_per_lane, "block": tun.block,
+                                  "passes_per_launch": tun.passes_per_launch, "use_graph": tun.use_graph}}}
+
+
+def learned_da(args):
+    """Learned aperture proposal: train on the first half of the variants'
+    full grids, then find the other half's apertures with the network's
+    guesses confirmed by FP64 tracking, by bisection alone and by the plain
+    grid; DA maps (variant x momentum offset) per hour of device work."""
+    from elegant_amd import learned, native
+    half = args.variants // 2
+    tr = native.Tracker(0)
+    rep = learned.study(tr, range(half), range(half, 2 * half), passes=args.steps,
+                        log=lambda m: print(m, file=sys.stderr, flush=True))
+    tr.close()
+    return {"metric": "DA maps/hour learned-proposal vs plain", "value": rep["learned"]["maps_per_hour"],
+            "unit": "maps/hour", "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "learned aperture proposal: %d training + %d evaluation storage-ring error-seed "
+                                   "variants, 41 rays x 11 offsets, %d passes" % (half, half, args.steps)},
+            "report": rep}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "error_seeds":
+        print(json.dumps(error_seeds(args)))
+        return
+    if args.workload == "learned_da":
+        print(json.dumps(learned_da(args)))
+        return
     import torch
     from elegant_amd import lattices as Lt
     from elegant_amd import native
@@ -140,7 +214,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
-            "traffic": None,
+            "traffic": traffic("eh_track_kernel", args.workload),
             "kernel_ms_per_launch": st.track_ms / max(st.kernel_launches, 1),
         },
     }

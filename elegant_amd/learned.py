@@ -127,22 +127,24 @@ def propose(mlp, vs, rays, deltas):
     return out
 
 
-def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, max_rounds=64):
+def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=1, max_rounds=64):
     """Find, on every ray of every (variant, offset), a grid index lo that
     survives with lo + 1 lost (or lo = nx - 1 surviving, or lo = -1 with the
-    origin lost), assuming survival is monotone along the ray.  Each round
-    tracks one point per unresolved ray, all rays of all variants in one
-    batch.  Without a guess the point is the interval midpoint (bisection:
-    the interval splitting of aperture_search.c:1027-1050 with
-    split_fraction = 0.5, here on grid indices).  With a guess
-    the first round tracks guess and guess + 1; a ray whose guess is refuted
-    gallops away from it with doubling steps until the interval closes
-    around the boundary, then bisects.  Returns (limits per variant, effort,
-    rounds, wall seconds, fraction of rays settled by the first round)."""
+    origin lost), assuming survival is monotone along the ray.  Each round is
+    one batched run of every unresolved ray of every variant.  A ray with the
+    open interval (lo, hi) of untested indices tracks up to `kary` points
+    spread evenly over it; kary = 1 is bisection (the interval splitting of
+    aperture_search.c:1027-1050 with split_fraction = 0.5, on grid indices).
+    With a guess, the first round tracks only guess and guess + 1, which
+    settles the ray when the guess is right; a refuted guess then tracks up
+    to `kary` points stepping away from the refuted side in doubling steps,
+    so points beyond the boundary (lost within a few passes, cheap) rather
+    than inside it (tracked for every pass) take most of the tests.  Returns (limits per variant,
+    effort, rounds, wall seconds, fraction of rays settled by the first
+    round)."""
     nd, nl = len(deltas), rays.lines
     nx = np.concatenate([np.full(nd * nl, v.nx) for v in vs])
     R = nx.size
-    vidx = np.repeat(np.arange(len(vs)), nd * nl)
     didx = np.tile(np.repeat(np.arange(nd), nl), len(vs))
     lidx = np.tile(np.arange(nl), nd * len(vs))
     lo = np.full(R, -1)
@@ -154,58 +156,72 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, max_r
     rounds = 0
     settled_first = None
     dl = np.asarray(deltas, dtype=float)
+    from .studies import track_sets
     while rounds < max_rounds:
         active = hi - lo > 1
         if not active.any():
             break
         if rounds == 0 and g is not None:
-            tests = [np.where(active & (g >= 0), g, -1), np.where(active & (g + 1 < nx), g + 1, -1)]
+            T = np.stack([np.where(active & (g >= 0), g, -1), np.where(active & (g + 1 < nx), g + 1, -1)], axis=1)
         else:
-            t = (lo + hi) // 2
+            gap = hi - lo - 1                      # untested indices lo+1 .. hi-1
+            k = np.minimum(gap, kary)
+            j = np.arange(1, kary + 1)[None, :]
+            T = lo[:, None] + (j * (gap[:, None] + 1)) // (k[:, None] + 1)
             if g is not None:
-                up = lo > g   # the boundary lies above the guess
-                dn = hi <= g  # ... or below it
-                step = np.where(up | dn, step * 2, step)
-                t = np.where(up & (lo + step < hi), lo + step, t)
-                t = np.where(dn & (hi - step > lo), hi - step, t)
-            tests = [np.where(active, t, -1)]
+                # a refuted guess: a window of points next to the refuted side,
+                # its spacing growing by (window + 1) each round until the
+                # other side is found.  Upward the window holds `kary` points
+                # (points past the boundary are lost early and cost little);
+                # downward only 2 (points below the boundary survive every pass)
+                up = (lo > g) & (hi == nx)   # no loss found above the guess yet
+                dn = (hi <= g) & (lo < 0)    # no survivor found below it yet
+                w = np.where(up, kary, np.minimum(kary, 2))
+                off = step[:, None] * j
+                T = np.where(up[:, None], np.minimum(lo[:, None] + off, hi[:, None] - 1), T)
+                T = np.where(dn[:, None], np.maximum(hi[:, None] - off, lo[:, None] + 1), T)
+                T = np.where((up | dn)[:, None] & (j > w[:, None]), -1, T)
+                step = np.where(up | dn, step * (w + 1), step)
+                # steps that clip to the interval's end repeat it; keep one
+                T = np.where((j > 1) & (T == np.roll(T, 1, axis=1)), -1, T)
+            T = np.where(active[:, None] & (j <= k[:, None]), T, -1)
         sets, where = [], []
-        for k, v in enumerate(vs):
-            sel = slice(k * nd * nl, (k + 1) * nd * nl)
-            rows, idx = [], []
-            for t in tests:
-                m = np.nonzero(t[sel] >= 0)[0] + k * nd * nl
-                fr = t[m] / (nx[m] - 1)
-                rows.append(da._rows(fr * rays.xmax * rays.dxf[lidx[m]], fr * rays.ymax * rays.dyf[lidx[m]],
-                                     dl[didx[m]], 1))
-                idx.append((m, t[m]))
-            sets.append(np.concatenate(rows))
-            where.append(idx)
-        keep = [k for k in range(len(vs)) if sets[k].shape[0]]
-        from .studies import track_sets
-        lps, e, w, _ = track_sets(tracker, [vs[k] for k in keep], [sets[k] for k in keep], passes, tune)
+        for v in range(len(vs)):
+            sel = slice(v * nd * nl, (v + 1) * nd * nl)
+            r, c = np.nonzero(T[sel] >= 0)
+            m = r + v * nd * nl
+            t = T[m, c]
+            fr = t / (nx[m] - 1)
+            sets.append(da._rows(fr * rays.xmax * rays.dxf[lidx[m]], fr * rays.ymax * rays.dyf[lidx[m]],
+                                 dl[didx[m]], 1))
+            where.append((m, t))
+        keep = [v for v in range(len(vs)) if sets[v].shape[0]]
+        lps, e, w, _ = track_sets(tracker, [vs[v] for v in keep], [sets[v] for v in keep], passes, tune)
         effort += e
         wall += w
         rounds += 1
-        for j, k in enumerate(keep):
-            off = 0
-            for m, t in where[k]:
-                s = lps[j][off:off + m.size] < 0
-                off += m.size
-                hi[m] = np.where(~s, np.minimum(hi[m], t), hi[m])
-                lo[m] = np.where(s & (t < hi[m]), np.maximum(lo[m], t), lo[m])
-        # a survivor above a lost point says nothing about the first loss below it
-        bad = lo >= hi
-        if bad.any():
-            lo[bad] = -1
+        ms, ts, ss = [], [], []
+        for j, v in enumerate(keep):
+            m, t = where[v]
+            ms.append(m)
+            ts.append(t)
+            ss.append(lps[j] < 0)
+        m = np.concatenate(ms)
+        t = np.concatenate(ts)
+        sv = np.concatenate(ss)
+        # losses first: the first loss bounds the ray; then the survivors below it
+        np.minimum.at(hi, m[~sv], t[~sv])
+        ok = sv & (t < hi[m])
+        np.maximum.at(lo, m[ok], t[ok])
+        # a survivor recorded earlier above a new loss says nothing about the first loss below it
+        lo[lo >= hi] = -1
         if settled_first is None:
             settled_first = float(np.mean(hi - lo <= 1))
-    limits = [lo[k * nd * nl:(k + 1) * nd * nl].reshape(nd, nl) for k in range(len(vs))]
+    limits = [lo[v * nd * nl:(v + 1) * nd * nl].reshape(nd, nl) for v in range(len(vs))]
     return limits, effort, rounds, wall, settled_first
 
-
 def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, lr=1e-3, tune=None, log=None,
-          rays=None, deltas=None):
+          rays=None, deltas=None, kary=4):
     """Train on the survival verdicts of the training variants' full grids,
     then find the aperture of the evaluation variants three ways: the plain
     grid (the reference answer), bisection alone, and the network's guess
@@ -234,7 +250,8 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
     t2 = time.perf_counter()
     guess = propose(mlp, ev, rays, deltas)
     t_prop = time.perf_counter() - t2
-    lim_l, eff_l, rounds_l, wall_l, first_l = index_search(tracker, ev, passes, rays, deltas, guess, tune)
+    lim_l, eff_l, rounds_l, wall_l, first_l = index_search(tracker, ev, passes, rays, deltas, guess, tune,
+                                                           kary=kary)
     say("learned search: %d particle-turns, %d rounds, %.1f s" % (eff_l, rounds_l, wall_l))
     lim_b, eff_b, rounds_b, wall_b, _ = index_search(tracker, ev, passes, rays, deltas, None, tune)
     say("bisection search: %d particle-turns, %d rounds, %.1f s" % (eff_b, rounds_b, wall_b))
@@ -267,7 +284,9 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
         "bisection": dict(effort=eff_b, rounds=rounds_b, wall_s=wall_b, maps_per_hour=maps / wall_b * 3600,
                           **compare(lim_b)),
         "learned": dict(effort=eff_l, rounds=rounds_l, wall_s=wall_l + t_prop, propose_s=t_prop,
-                        maps_per_hour=maps / (wall_l + t_prop) * 3600, guess_exact=float(guess_hit),
+                        maps_per_hour=maps / (wall_l + t_prop) * 3600,
+                        maps_per_hour_incl_training=maps / (wall_l + t_prop + gtrain["wall"] + t_train) * 3600,
+                        guess_exact=float(guess_hit),
                         settled_first_round=first_l, **compare(lim_l)),
     }
     return rep

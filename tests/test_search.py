@@ -19,19 +19,23 @@ class FakeVariant:
 
 
 def install(monkeypatch, rays, deltas, holes=()):
+    """replace the batched tracker: a point survives inside the variant's
+    radius; effort counts every pass for a survivor and one for a loss"""
     calls = {"n": 0}
 
     def fake(tracker, vs, sets, passes, tune=None):
         calls["n"] += 1
         lps = []
+        effort = 0
         for v, rows in zip(vs, sets):
             di = np.searchsorted(np.asarray(deltas), rows[:, 5])
             r = np.hypot(rows[:, 0] / rays.xmax, rows[:, 2] / rays.ymax)
             ok = r <= v.radius[di] + 1e-12
             for (hx, hy) in holes:
                 ok &= ~((np.abs(rows[:, 0] - hx) < 1e-12) & (np.abs(rows[:, 2] - hy) < 1e-12))
-            lps.append(np.where(ok, -1, 7).astype(np.int32))
-        return lps, sum(s.shape[0] for s in sets), 0.0, None
+            lps.append(np.where(ok, -1, 0).astype(np.int32))
+            effort += int(np.sum(np.where(ok, passes, 1)))
+        return lps, effort, 0.0, None
 
     monkeypatch.setattr(studies, "track_sets", fake)
     return calls
@@ -63,9 +67,10 @@ def test_index_search_finds_the_grid_boundary(monkeypatch, guess_kind):
         g = [np.full_like(r, v.nx - 1) for r, v in zip(ref, vs)]
     elif guess_kind == "origin":
         g = [np.full_like(r, -1) for r in ref]
-    lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g)
-    for a, b in zip(lim, ref):
-        assert np.array_equal(a, b)
+    for kary in (1, 4, 16):
+        lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g, kary=kary)
+        for a, b in zip(lim, ref):
+            assert np.array_equal(a, b), kary
     if guess_kind == "exact":
         assert rounds == 1 and first == 1.0
     if guess_kind == "none":
@@ -80,7 +85,7 @@ def test_a_good_guess_costs_fewer_points_than_bisection(monkeypatch):
     install(monkeypatch, rays, deltas)
     ref = [reference_limits(rays, deltas, v) for v in vs]
     near = [np.clip(r + rng.integers(-1, 2, r.shape), -1, 144) for r in ref]
-    _, e_guess, _, _, _ = learned.index_search(None, vs, 10, rays, deltas, near)
+    _, e_guess, _, _, _ = learned.index_search(None, vs, 10, rays, deltas, near, kary=4)
     _, e_bis, _, _, _ = learned.index_search(None, vs, 10, rays, deltas, None)
     assert e_guess < 0.6 * e_bis
 
