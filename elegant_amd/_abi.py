@@ -136,6 +136,10 @@ class eh_batch_tuning(C.Structure):
                 ("sample_ms", D)]
 
 
+class eh_rng(C.Structure):
+    _fields_ = [("seed", C.c_int64 * 4), ("have_saved", I), ("pad", I), ("saved", D)]
+
+
 PARAM_STRUCT = {
     EH_T_DRIF: ("drift", EH_DRIFT), EH_T_EDRIFT: ("edrift", EH_EDRIFT), EH_T_KQUAD: ("kquad", EH_KQUAD),
     EH_T_KSEXT: ("ksext", EH_KSEXT), EH_T_KOCT: ("koct", EH_KOCT), EH_T_CSBEND: ("csbend", EH_CSBEND),

@@ -76,6 +76,15 @@ SIGNATURES = [
     ("eh_mlp_gradient", C.c_int, [P, PF, PF, C.c_int32, PF, PF]),
     ("eh_mlp_train", C.c_int, [P, PF, PF, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, PF]),
     ("eh_mlp_predict", C.c_int, [P, PF, C.c_int64, PF]),
+    ("eh_rng_seed", C.c_int, [C.POINTER(A.eh_rng), C.c_int64]),
+    ("eh_rng_uniform", C.c_double, [C.POINTER(A.eh_rng)]),
+    ("eh_gauss_rn", C.c_double, [C.POINTER(A.eh_rng)]),
+    ("eh_gauss_rn_lim", C.c_double, [C.POINTER(A.eh_rng), C.c_double, C.c_double, C.c_double]),
+    ("eh_perturbation", C.c_double, [C.POINTER(A.eh_rng), C.c_double, C.c_double, C.c_int32]),
+    ("eh_gaussian_distribution", C.c_int, [C.POINTER(A.eh_rng), PP, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                           C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    ("eh_transform_from_normalized", None, [PP, C.c_int64, C.c_int32, C.c_double, C.c_double]),
+    ("eh_gaussian_bunch", C.c_int, [C.POINTER(A.eh_rng), PP, C.c_int64, C.c_int32, PD, PD, PD, C.c_int64]),
 ]
 # helpers outside the public header (measurement and tests)
 EXTRA = [

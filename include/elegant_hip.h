@@ -308,6 +308,36 @@ int eh_mlp_predict(eh_mlp *m, const float *X, int64_t n, float *prob);
 double eh_da_trim_area(int64_t lines, double *xLimit, double *yLimit, const double *dxFactor, const double *dyFactor,
                        int32_t fullPlane);
 
+/* ---- seeded input generators (host only) ----
+ * The uniform stream is DLARAN as elegant ships it (dlaran_OAG,
+ * drand_oag.c:118-180; seeding as random_1_elegant, drand_oag.c:94-116,
+ * without the toolkit's seed-bit permutation, which the reference does not
+ * ship).  Gaussian deviates use the polar method (the toolkit's gauss_rn is
+ * not shipped either), so only the uniform stream matches elegant's. */
+typedef struct {
+  int64_t seed[4];    /* 12-bit limbs of the 48-bit state, most significant first */
+  int32_t have_saved; /* a second Gaussian deviate is pending */
+  int32_t pad;
+  double saved;
+} eh_rng;
+int eh_rng_seed(eh_rng *r, int64_t iseed);
+double eh_rng_uniform(eh_rng *r);
+double eh_gauss_rn(eh_rng *r);
+/* gauss_rn_lim: mean + sigma z, z redrawn while |z| > limit (limit <= 0: no cut) */
+double eh_gauss_rn_lim(eh_rng *r, double mean, double sigma, double limit);
+/* perturbation() (error.c:496-534): error_type 0 uniform, 1 Gaussian with cutoff, 2 plus-or-minus */
+double eh_perturbation(eh_rng *r, double amplitude, double cutoff, int32_t error_type);
+/* gaussian_distribution() (generate_bunch.c:557-658, no Halton sequences) into columns offset, offset+1 */
+int eh_gaussian_distribution(eh_rng *r, double **particle, int64_t n, int32_t offset, double s1, double s2,
+                             int32_t symmetrize, double limit, double limit_invar, double beta, int32_t halo);
+/* transform_from_normalized_coordinates (generate_bunch.c:1194-1212) */
+void eh_transform_from_normalized(double **particle, int64_t n, int32_t offset, double beta, double alpha);
+/* Gaussian bunch: x_plane/y_plane = {emit, beta, alpha, cutoff}, longit =
+ * {sigma_s, sigma_dp, alpha_z, cutoff} (generate_bunch.c:138-160, 202-218,
+ * 315-360); IDs from first_id in column 6, column 7 = -1 if nprops > 7 */
+int eh_gaussian_bunch(eh_rng *r, double **particle, int64_t n, int32_t nprops, const double *x_plane,
+                      const double *y_plane, const double *longit, int64_t first_id);
+
 /* ---- device information for measurement ---- */
 /* Device time of the last eh_track call's dominant kernel: number of
  * launches and summed milliseconds (HIP events on the library's stream). */
