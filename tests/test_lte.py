@@ -1,0 +1,91 @@
+"""The .lte subset parser: an example deck the reference ships
+(tests/golden/parTrack.lte), the language features one at a time, and a
+storage ring written out and read back tracking identically on the CPU
+checker."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from elegant_amd import _abi as A
+from elegant_amd import lattices as Lt
+from elegant_amd import lte
+from oracle import oracle_ctypes as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_example_deck_structure():
+    lat = lte.load(os.path.join(GOLDEN, "parTrack.lte"))
+    assert lat.elements["L1A"] == ("DRIF", {"L": 0.2})
+    assert lat.elements["Q1"][0] == "QUAD" and lat.elements["Q1"][1]["K1"] == 1.786022448154
+    assert lat.elements["SDH"] == ("KSEXT", {"L": 0.1, "K2": 5.95873739969822, "N_KICKS": 4})
+    b1 = lat.elements["B1"]
+    assert b1[0] == "CSBEND" and b1[1]["E1"] == -0.445 and b1[1]["N_KICKS"] == 10 and b1[1]["K3"] == 100
+    assert lat.elements["W1"][1]["FILENAME"] == "%s.w1"
+    seq = lat.expand("PAR")
+    # mal, rc, 2 x (quadr2 = 23 elements, sf = 2, -quadr2 = 23), ap, w1
+    assert len(seq) == 100
+    names = [n for n, _ in seq]
+    assert names[:4] == ["MAL", "RC", "L1A", "L1B"] and names[-2:] == ["AP", "W1"]
+    half = names[2:50]
+    assert half[23:25] == ["SFH", "SFH"]
+    assert half[25:] == half[:23][::-1]          # -quadr2 is quadr2 reversed
+    assert all(r for _, r in seq[2 + 25:2 + 48]) and not any(r for _, r in seq[2:2 + 25])
+    assert lat.expand() == seq                   # the last line defined is the default
+
+
+def test_unsupported_types_fail_loudly_unless_ignored_or_transmuted():
+    lat = lte.load(os.path.join(GOLDEN, "parTrack.lte"))
+    with pytest.raises(lte.LatticeError, match="not on the tracked path"):
+        lat.beamline("PAR")
+    bl = lat.beamline("PAR", transmute={"QUAD": "KQUAD", "SEXT": "KSEXT"},
+                      ignore=("MALIGN", "RECIRC", "RCOL", "WATCH"))
+    assert len(bl) == 96
+    assert bl.count(A.EH_T_KQUAD) == 16 and bl.count(A.EH_T_CSBEND) == 8
+
+
+def test_language_features():
+    text = """
+    ! comment line
+    d1: drift, l="1 2 /"          ! RPN: 0.5
+    Q: kquad, l=0.3, k1=1.2, &
+       n_kicks=8                  ! continuation
+    qd: q, k1=-1.1                ! copy of q with an override
+    b: csbend, l=1, angle="pi 40 /", e1=0.01, e2=0.03, edge1_effects=1, edge2_effects=0
+    m: marker
+    cell: line=(d1, q, 2*(d1, qd), -b, m)
+    ring: line=(3*cell, -cell)
+    use, ring
+    """
+    lat = lte.Lattice().parse(text)
+    assert lat.elements["D1"] == ("DRIF", {"L": 0.5})
+    assert lat.elements["QD"] == ("KQUAD", {"L": 0.3, "K1": -1.1, "N_KICKS": 8})
+    assert abs(lat.elements["B"][1]["ANGLE"] - math.pi / 40) < 1e-16
+    cell = ["D1", "Q", "D1", "QD", "D1", "QD", "B", "M"]
+    seq = lat.expand()
+    assert [n for n, _ in seq] == cell * 3 + cell[::-1]
+    bl = lat.beamline()
+    bends = [e.p.csbend for e in bl.elements if e.type == A.EH_T_CSBEND]
+    # "-b" inside cell: edges swapped; reversing the whole cell swaps them back
+    assert (bends[0].e1, bends[0].e2, bends[0].edge1_effects, bends[0].edge2_effects) == (0.03, 0.01, 0, 1)
+    assert (bends[3].e1, bends[3].e2, bends[3].edge1_effects, bends[3].edge2_effects) == (0.01, 0.03, 1, 0)
+    with pytest.raises(lte.LatticeError, match="ambiguous"):
+        lte.entity_type("K")
+    assert lte.entity_type("KQUADRUPOLE") == "KQUAD" and lte.entity_type("RFCA") == "RFCA"
+
+
+def test_storage_ring_round_trip_tracks_identically():
+    bl, Po, rows = Lt.storage_ring_workload(n=64, seed=11)
+    text = lte.dumps(bl)
+    bl2 = lte.Lattice().parse(text).beamline()
+    assert len(bl2) == len(bl)
+    for a, b in zip(bl.elements, bl2.elements):
+        assert a.type == b.type and bytes(a.p) == bytes(b.p)
+    r1 = rows.copy()
+    r2 = rows.copy()
+    n1 = O.Oracle("emath").do_tracking(r1, bl, Po, 2)
+    n2 = O.Oracle("emath").do_tracking(r2, bl2, Po, 2)
+    assert n1 == n2
+    assert np.array_equal(r1, r2)
