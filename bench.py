@@ -134,7 +134,8 @@ def error_seeds(args):
                        "variants": len(vs), "particles": n, "passes_timed": args.steps,
                        "survivors": int(st.n_left), "kernel_launches": int(st.kernel_launches),
                        "tuning": {"particles_per_lane": tun.particles_per_lane, "block": tun.block,
-                                  "passes_per_launch": tun.passes_per_launch, "use_graph": tun.use_graph}}}
+                                  "passes_per_launch": tun.passes_per_launch, "use_graph": tun.use_graph,
+                                  "compact": tun.compact}}}
 
 
 def learned_da(args):
@@ -214,7 +215,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
-            "traffic": traffic("eh_track_kernel", args.workload),
+            "traffic": traffic("eh_track_kernel", "%s_%d" % (args.workload, n)),
             "kernel_ms_per_launch": st.track_ms / max(st.kernel_launches, 1),
         },
     }

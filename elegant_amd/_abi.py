@@ -133,7 +133,7 @@ class eh_stats(C.Structure):
 
 class eh_batch_tuning(C.Structure):
     _fields_ = [("particles_per_lane", I), ("block", I), ("passes_per_launch", I), ("use_graph", I),
-                ("sample_ms", D)]
+                ("compact", I), ("pad", I), ("sample_ms", D)]
 
 
 class eh_rng(C.Structure):

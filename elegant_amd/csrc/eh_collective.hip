@@ -56,17 +56,117 @@ static __host__ __device__ double key_value(unsigned long long k) {
   return d;
 }
 
-/* red[0] = min key, red[1] = max key, red[2] = alive count */
+/* The step's decisions live in device memory, so the whole step is queued
+ * without waiting for the host: the reductions land in red[], the planning
+ * kernels turn them into the bin layout, and every later kernel reads it. */
+typedef struct {
+  unsigned long long red[4]; /* min key, max key, alive count, unused */
+  double tmin, tmax, tmean;
+  int32_t nb;                /* bins of this step; <= 0: nothing to bin or kick */
+  int32_t e;                 /* fixed-point exponent of the mean */
+  int32_t err;               /* EH_ERR_LONGBEAM when the bunch does not fit */
+  int32_t pad[3];
+} coll_plan;
+
+/* start of a step: reset the reductions and clear the histogram (a kernel,
+ * not a memset, so the step can be captured in a graph) */
+__global__ void __launch_bounds__(CB) coll_reset(coll_plan *plan, unsigned int *hist, int32_t nclear) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    plan->red[0] = ~0ULL;
+    plan->red[1] = 0;
+    plan->red[2] = 0;
+    plan->red[3] = 0;
+    plan->nb = 0;
+    plan->err = 0;
+  }
+  for (int k = blockIdx.x * CB + threadIdx.x; k < nclear; k += gridDim.x * CB)
+    hist[k] = 0;
+}
+
+/* after the min/max pass: the decisions of wake.c:120-182 and
+ * zlongit.c:195-236 that do not need the mean */
+__global__ void __launch_bounds__(CB) coll_plan_minmax(coll_plan *plan, const unsigned long long *part,
+                                                       int64_t nparts, int32_t kind, int32_t n_fixed, int32_t zl_nb,
+                                                       double dt, double wake_span, int32_t allow_long,
+                                                       int32_t nb_cap) {
+  __shared__ unsigned long long smin[CB], smax[CB], scnt[CB];
+  unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
+  for (int64_t k = threadIdx.x; k < nparts; k += CB) {
+    unsigned long long a = part[3 * k], b = part[3 * k + 1];
+    kmin = a < kmin ? a : kmin;
+    kmax = b > kmax ? b : kmax;
+    cnt += part[3 * k + 2];
+  }
+  smin[threadIdx.x] = kmin;
+  smax[threadIdx.x] = kmax;
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int w = CB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      unsigned long long a = smin[threadIdx.x + w], b = smax[threadIdx.x + w];
+      smin[threadIdx.x] = a < smin[threadIdx.x] ? a : smin[threadIdx.x];
+      smax[threadIdx.x] = b > smax[threadIdx.x] ? b : smax[threadIdx.x];
+      scnt[threadIdx.x] += scnt[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0)
+    return;
+  coll_plan *p = plan;
+  p->red[0] = smin[0];
+  p->red[1] = smax[0];
+  p->red[2] = scnt[0];
+  if (kind < 0)
+    return; /* the time span only */
+  if (p->red[2] == 0) {
+    p->nb = 0;
+    return;
+  }
+  double tmin = key_value(p->red[0]), tmax = key_value(p->red[1]);
+  p->tmin = tmin;
+  p->tmax = tmax;
+  if (kind == EH_OP_ZLONGIT || n_fixed) {
+    double am = __builtin_fabs(tmin) > __builtin_fabs(tmax) ? __builtin_fabs(tmin) : __builtin_fabs(tmax);
+    p->e = eh_fsum_exponent(am);
+  }
+  if (kind == EH_OP_WAKE) {
+    if ((tmax - tmin) > wake_span && !allow_long) {
+      p->err = EH_ERR_LONGBEAM;
+      return;
+    }
+    if (n_fixed) {
+      p->nb = n_fixed; /* tmin set from the mean */
+    } else {
+      int32_t nb = (int32_t)(int64_t)((tmax - tmin) / dt + 3);
+      p->tmin = tmin - dt;
+      p->nb = nb;
+    }
+  } else {
+    p->nb = zl_nb;
+    if ((tmax - tmin) * 2 > zl_nb * dt && !allow_long) {
+      p->err = EH_ERR_LONGBEAM;
+      return;
+    }
+  }
+  if (p->nb > nb_cap)
+    p->err = EH_ERR_LONGBEAM;
+}
+
+/* per-workgroup (min key, max key, alive count) triples in red[3 k ..];
+ * grid-stride, so there are at most a thousand of them for the planning
+ * kernel to combine (no global atomics) */
 __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, const double *__restrict__ dp,
                                                   const int32_t *__restrict__ lp, int64_t n, double Po,
                                                   unsigned long long *red) {
   __shared__ unsigned long long smin[CB / 64], smax[CB / 64], scnt[CB / 64];
-  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
-  if (i < n && lp[i] < 0) {
-    unsigned long long k = order_key(arrival_time(s[i], dp[i], Po));
-    kmin = kmax = k;
-    cnt = 1;
+  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
+    if (lp[i] < 0) {
+      unsigned long long k = order_key(arrival_time(s[i], dp[i], Po));
+      kmin = k < kmin ? k : kmin;
+      kmax = k > kmax ? k : kmax;
+      cnt++;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     unsigned long long a = __shfl_xor(kmin, off), b = __shfl_xor(kmax, off), c = __shfl_xor(cnt, off);
@@ -87,24 +187,25 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
       kmax = smax[k] > kmax ? smax[k] : kmax;
       cnt += scnt[k];
     }
-    if (cnt) {
-      atomicMin(&red[0], kmin);
-      atomicMax(&red[1], kmax);
-      atomicAdd(&red[2], cnt);
-    }
+    red[3 * blockIdx.x] = kmin;
+    red[3 * blockIdx.x + 1] = kmax;
+    red[3 * blockIdx.x + 2] = cnt;
   }
 }
 
 /* per-workgroup fixed-point partial sums of the arrival times */
 __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, const double *__restrict__ dp,
-                                                const int32_t *__restrict__ lp, int64_t n, double Po, int32_t e,
-                                                eh_fsum *partial) {
+                                                const int32_t *__restrict__ lp, int64_t n, double Po,
+                                                const coll_plan *plan, eh_fsum *partial) {
   __shared__ eh_fsum sh[CB];
-  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (plan->nb <= 0 || plan->err)
+    return;
+  const int32_t e = plan->e;
   eh_fsum a;
   eh_fsum_init(&a, e);
-  if (i < n && lp[i] < 0)
-    eh_fsum_add(&a, arrival_time(s[i], dp[i], Po));
+  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB)
+    if (lp[i] < 0)
+      eh_fsum_add(&a, arrival_time(s[i], dp[i], Po));
   sh[threadIdx.x] = a;
   __syncthreads();
   for (int w = CB / 2; w > 0; w >>= 1) {
@@ -116,9 +217,13 @@ __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, co
     partial[blockIdx.x] = sh[0];
 }
 
-__global__ void __launch_bounds__(CB) fsum_final(eh_fsum *partial, int64_t nparts, int32_t e, double count,
-                                                 double *mean_out) {
+/* the mean from the partials, then tmin = tmean - dt nb / 2 (wake.c:165-168
+ * for fixed N_BINS, zlongit.c:202-206) */
+__global__ void __launch_bounds__(CB) fsum_final(eh_fsum *partial, int64_t nparts, coll_plan *plan, double dt) {
   __shared__ eh_fsum sh[CB];
+  if (plan->nb <= 0 || plan->err)
+    return;
+  const int32_t e = plan->e;
   eh_fsum a;
   eh_fsum_init(&a, e);
   for (int64_t k = threadIdx.x; k < nparts; k += CB)
@@ -130,8 +235,11 @@ __global__ void __launch_bounds__(CB) fsum_final(eh_fsum *partial, int64_t npart
       eh_fsum_merge(&sh[threadIdx.x], &sh[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    mean_out[0] = eh_fsum_to_double(&sh[0]) / count;
+  if (threadIdx.x == 0) {
+    double tmean = eh_fsum_to_double(&sh[0]) / (double)plan->red[2];
+    plan->tmean = tmean;
+    plan->tmin = tmean - dt * plan->nb / 2.0;
+  }
 }
 
 /* bin index of wake.c:507 (round) or zlongit.c:267 (truncate) */
@@ -141,47 +249,101 @@ DEV int64_t bin_of(double t, double tmin, double dt, int round_half) {
 }
 
 __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, const double *__restrict__ dp,
-                                                const int32_t *__restrict__ lp, int64_t n, double Po, double tmin,
-                                                double dt, int32_t nb, int32_t round_half, unsigned int *hist) {
+                                                const int32_t *__restrict__ lp, int64_t n, double Po,
+                                                const coll_plan *plan, double dt, int32_t round_half,
+                                                int32_t lds_bins, unsigned int *hist) {
   extern __shared__ unsigned int lh[];
-  for (int k = threadIdx.x; k < nb; k += CB)
-    lh[k] = 0;
-  __syncthreads();
+  const int32_t nb = plan->nb;
+  const double tmin = plan->tmin;
+  if (nb <= 0 || plan->err)
+    return;
+  /* counts in LDS, one copy per wave when they fit (a bunch puts most
+   * particles in a few central bins, so fewer lanes share an address), one
+   * per workgroup when only that fits, else straight to memory */
+  const int copies = nb * (CB / 64) <= lds_bins ? CB / 64 : (nb <= lds_bins ? 1 : 0);
+  unsigned int *h = copies ? lh + (copies > 1 ? (threadIdx.x >> 6) * nb : 0) : hist;
+  if (copies) {
+    for (int k = threadIdx.x; k < nb * copies; k += CB)
+      lh[k] = 0;
+    __syncthreads();
+  }
   for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
     if (lp[i] >= 0)
       continue;
     int64_t ib = bin_of(arrival_time(s[i], dp[i], Po), tmin, dt, round_half);
     if (ib >= 0 && ib <= nb - 1)
-      atomicAdd(&lh[ib], 1u);
+      atomicAdd(&h[ib], 1u);
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < nb; k += CB)
-    if (lh[k])
-      atomicAdd(&hist[k], lh[k]);
+  if (copies) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += CB) {
+      unsigned int c = 0;
+      for (int w = 0; w < copies; w++)
+        c += lh[w * nb + k];
+      if (c)
+        atomicAdd(&hist[k], c);
+    }
+  }
 }
 
-/* convolveArrays with di2 = 0 (wake.c:479-495): V[ib] = sum_{j=0..ib} I[j] W[ib-j],
- * summed in increasing j; zero counts are skipped (adding 0*W = +-0 to a
- * sum that starts at +0 leaves it unchanged), then V *= factor (wake.c:258-260) */
+/* convolveArrays with di2 = 0 (wake.c:479-495): V[ib] = sum_{j} I[j] W[ib-j]
+ * over j = max(0, ib - wpoints + 1) .. ib, summed in increasing j, then
+ * V *= factor (wake.c:258-260).  One lane per output bin;
+ * the counts and the window of W a workgroup needs are staged through LDS
+ * in tiles of CONV_T counts, which keeps every lane's sum in the reference
+ * order. */
+#define CONV_T 1024
 __global__ void __launch_bounds__(CB) wake_convolve(const unsigned int *__restrict__ hist,
-                                                    const double *__restrict__ W, int64_t wpoints, int32_t nb,
-                                                    double factor, double *volt) {
-  int ib = blockIdx.x * CB + threadIdx.x;
-  if (ib >= nb)
+                                                    const double *__restrict__ W, int64_t wpoints,
+                                                    const coll_plan *plan, double factor, double *volt) {
+  __shared__ unsigned int hs[CONV_T];
+  __shared__ double ws[CONV_T + CB];
+  const int32_t nb = plan->nb;
+  const int b0 = blockIdx.x * CB;
+  if (plan->err || nb <= 0 || b0 > nb)
     return;
-  int64_t ib1 = 0, ib2 = ib;
-  if (ib2 >= wpoints) {
-    int64_t di = ib2 - wpoints + 1;
-    ib1 += di;
-    ib2 -= di;
-  }
+  const int ib = b0 + (int)threadIdx.x;
+  const int wp = (int)wpoints;
+  const int last = (b0 + CB - 1 < nb - 1) ? b0 + CB - 1 : nb - 1;
+  const int first = b0 - wp + 1 > 0 ? b0 - wp + 1 : 0;
   double v = 0;
-  for (; ib1 < nb && ib2 >= 0; ib1++, ib2--) {
-    unsigned int c = hist[ib1];
-    if (c)
-      v += (double)c * W[ib2];
+  for (int t0 = first; t0 <= last; t0 += CONV_T) {
+    const int t1 = (t0 + CONV_T - 1 < last) ? t0 + CONV_T - 1 : last;
+    const int w_lo = b0 - t1; /* W index of ws[0] */
+    for (int k = threadIdx.x; k <= t1 - t0; k += CB)
+      hs[k] = hist[t0 + k];
+    for (int k = threadIdx.x; k < t1 - t0 + CB; k += CB) {
+      int w = w_lo + k;
+      ws[k] = (w >= 0 && w < wp) ? W[w] : 0.0;
+    }
+    __syncthreads();
+    if (ib < nb) {
+      int lo = ib - wp + 1 > t0 ? ib - wp + 1 : t0;
+      int hi = ib < t1 ? ib : t1;
+      int j = lo;
+      /* eight terms per step: the loads are independent, the additions stay
+       * in order; a zero count adds +-0, which leaves the sum (never -0)
+       * unchanged, exactly as the reference's unconditional loop */
+      for (; j + 7 <= hi; j += 8) {
+        double c[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          c[u] = (double)hs[j + u - t0];
+          w[u] = ws[ib - (j + u) - w_lo];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          v += c[u] * w[u];
+      }
+      for (; j <= hi; j++)
+        v += (double)hs[j - t0] * ws[ib - j - w_lo];
+    }
+    __syncthreads();
   }
-  volt[ib] = v * factor;
+  if (ib < nb)
+    volt[ib] = v * factor;
+  else if (ib == nb)
+    volt[nb] = 0; /* the slot past the last bin, read by interpolation */
 }
 
 /* packed real FFT, V = Z I factor, inverse FFT (zlongit.c:373-394), for
@@ -192,8 +354,10 @@ __global__ void __launch_bounds__(CB) wake_convolve(const unsigned int *__restri
 __global__ void __launch_bounds__(FFT_T) zlongit_fft(const unsigned int *__restrict__ hist,
                                                      const double *__restrict__ Z, const double *__restrict__ tc,
                                                      const double *__restrict__ ts, int32_t nb, int32_t bits,
-                                                     double factor, double *volt) {
+                                                     double factor, const coll_plan *plan, double *volt) {
   extern __shared__ double sm[];
+  if (plan->nb <= 0 || plan->err)
+    return;
   const int m = nb / 2;
   double *zr = sm, *zi = sm + m;
   const int tid = threadIdx.x;
@@ -322,12 +486,14 @@ DEV void add_energy(double &xp, double &yp, double &s, double &dp, double tof, d
 /* applyLongitudinalWakeKicks (wake.c:305-329) / the ZLONGIT kick loop
  * (zlongit.c:454-486), then the post-element checks of do_tracking */
 __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, double *yp, double *s, double *dp,
-                                           int32_t *lp, int32_t *lc, int64_t n, double Po, double tmin, double dt,
-                                           int32_t nb, const double *__restrict__ volt, int32_t kind,
+                                           int32_t *lp, int32_t *lc, int64_t n, double Po, const coll_plan *plan,
+                                           double dt, const double *__restrict__ volt, int32_t kind,
                                            int32_t interpolate, double z_end, double ap_x, double ap_y,
                                            int32_t ap_on, int64_t pass, int32_t code_base) {
   int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (i >= n || lp[i] >= 0)
+  const int32_t nb = plan->nb;
+  const double tmin = plan->tmin;
+  if (nb <= 0 || plan->err || i >= n || lp[i] >= 0)
     return;
   const double mc2 = 1e6 * EH_ME_MEV * EH_REL_SIGN;
   double S = s[i], D = dp[i];
@@ -475,10 +641,13 @@ int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc,
   return EH_OK;
 }
 
-/* work area: [0,64) reductions, then histogram (uint32 x cap), voltage
- * (double x (cap+1)), fixed-sum partials */
+/* work area: [0,128) the step plan, then the histogram (uint32 x (cap+8)),
+ * the voltage (double x (cap+8)), the fixed-sum partials */
+#define PLAN_BYTES 128
+static int64_t hist_bytes(int64_t nb_cap) { return ((4 * (nb_cap + 8) + 7) / 8) * 8; }
+
 static int ensure_work(ehd_dev *d, int64_t nb_cap, int64_t nparts) {
-  int64_t need = 64 + 8 * (nb_cap + 8) + 8 * (nb_cap + 8) + (int64_t)sizeof(eh_fsum) * (nparts + 1) + 256;
+  int64_t need = PLAN_BYTES + hist_bytes(nb_cap) + 8 * (nb_cap + 8) + (int64_t)sizeof(eh_fsum) * (nparts + 1) + 256;
   if (need <= d->cwork_bytes)
     return EH_OK;
   if (d->cwork)
@@ -490,6 +659,18 @@ static int ensure_work(ehd_dev *d, int64_t nb_cap, int64_t nparts) {
   return EH_OK;
 }
 
+/* workgroups of the grid-stride reductions: enough to fill the device, few
+ * enough that the per-workgroup atomics and partials stay cheap */
+static int64_t reduce_grid(int64_t n) {
+  int64_t g = (n + CB - 1) / CB;
+  if (g > 1024)
+    g = 1024;
+  return g < 1 ? 1 : g;
+}
+
+/* One WAKE or ZLONGIT step, queued as one sequence of kernels with no host
+ * round trip: reset, min/max, plan, [mean], histogram, convolution or FFT,
+ * kick.  The host waits once at the end for the plan (error, bin layout). */
 int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist_out, double *volt_out,
                         int64_t hist_cap, int64_t *nb_out, double *tmin_out) {
   (void)hist_cap;
@@ -498,97 +679,72 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
   ehd_coll_slot *sl = &d->slot[slot];
   const int64_t n = d->n;
   const int64_t grid = (n + CB - 1) / CB;
+  const int64_t rgrid = reduce_grid(n);
   CHK(hipSetDevice(d->device));
-  int64_t nb_cap = c->kind == EH_OP_WAKE ? sl->wpoints + 8 : sl->nb;
-  int rc = ensure_work(d, nb_cap, grid);
+  const int64_t nb_max = c->kind == EH_OP_WAKE ? sl->wpoints : sl->nb; /* most bins this element can use */
+  const int64_t nb_cap = nb_max + 8;
+  int rc = ensure_work(d, nb_cap, rgrid);
   if (rc)
     return rc;
-  unsigned long long *red = (unsigned long long *)d->cwork;
-  unsigned int *hist = (unsigned int *)((char *)d->cwork + 64);
-  double *volt = (double *)((char *)d->cwork + 64 + 8 * (nb_cap + 8));
+  coll_plan *plan = (coll_plan *)d->cwork;
+  unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
+  double *volt = (double *)((char *)hist + hist_bytes(nb_cap));
   eh_fsum *parts = (eh_fsum *)((char *)volt + 8 * (nb_cap + 8));
-  double *mean_dev = (double *)((char *)d->cwork + 32);
-  unsigned long long init[4] = {~0ULL, 0ULL, 0ULL, 0ULL};
-  unsigned long long hred[4];
+  const int32_t kind = c->kind;
+  const int32_t fixed = kind == EH_OP_WAKE ? c->n_bins_fixed : 0;
+  const int32_t nclear = (int32_t)(nb_cap + 8);
   CHK(hipEventRecord(d->cev0, d->stream));
-  CHK(hipMemcpyAsync(red, init, sizeof(init), hipMemcpyHostToDevice, d->stream));
-  if (n > 0)
-    hipLaunchKernelGGL(time_minmax, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                       d->loss_pass, n, c->Po, red);
-  CHK(hipGetLastError());
-  CHK(hipMemcpyAsync(hred, red, sizeof(hred), hipMemcpyDeviceToHost, d->stream));
-  CHK(hipStreamSynchronize(d->stream));
-  int64_t count = (int64_t)hred[2];
-  if (count == 0) {
-    if (nb_out)
-      *nb_out = 0;
-    return EH_OK;
+  hipLaunchKernelGGL(coll_reset, dim3((unsigned)((nclear + CB - 1) / CB)), dim3(CB), 0, d->stream, plan, hist, nclear);
+  hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
+                     d->loss_pass, n, c->Po, (unsigned long long *)parts);
+  hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)parts, rgrid,
+                     kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max);
+  if (kind == EH_OP_ZLONGIT || fixed) {
+    hipLaunchKernelGGL(time_fsum, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
+                       d->loss_pass, n, c->Po, plan, parts);
+    hipLaunchKernelGGL(fsum_final, dim3(1), dim3(CB), 0, d->stream, parts, rgrid, plan, c->dt);
   }
-  double tmin = key_value(hred[0]), tmax = key_value(hred[1]);
-  double dt = c->dt, tmean = 0;
-  int32_t nb;
-  if (c->kind == EH_OP_ZLONGIT || c->n_bins_fixed) {
-    double am = __builtin_fabs(tmin) > __builtin_fabs(tmax) ? __builtin_fabs(tmin) : __builtin_fabs(tmax);
-    int32_t e = eh_fsum_exponent(am);
-    hipLaunchKernelGGL(time_fsum, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                       d->loss_pass, n, c->Po, e, parts);
-    CHK(hipGetLastError());
-    hipLaunchKernelGGL(fsum_final, dim3(1), dim3(CB), 0, d->stream, parts, grid, e, (double)count, mean_dev);
-    CHK(hipGetLastError());
-    CHK(hipMemcpyAsync(&tmean, mean_dev, sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    CHK(hipStreamSynchronize(d->stream));
-  }
-  if (c->kind == EH_OP_WAKE) {
-    if ((tmax - tmin) > c->wake_span && !c->allow_long_beam)
-      return EH_ERR_LONGBEAM;
-    if (c->n_bins_fixed) {
-      nb = c->n_bins_fixed;
-      tmin = tmean - dt * nb / 2.0;
-    } else {
-      nb = (int32_t)(int64_t)((tmax - tmin) / dt + 3);
-      tmin -= dt;
-    }
-    if (nb <= 0)
-      return EH_OK;
-    if (nb > nb_cap - 8)
-      return EH_ERR_LONGBEAM;
-  } else {
-    nb = c->nb;
-    if ((tmax - tmin) * 2 > nb * dt && !c->allow_long_beam)
-      return EH_ERR_LONGBEAM;
-    tmin = tmean - dt * nb / 2.0;
-  }
-  CHK(hipMemsetAsync(hist, 0, sizeof(unsigned int) * (nb + 8), d->stream));
   {
     int hgrid = (int)((n + CB * 8 - 1) / (CB * 8));
     if (hgrid < 1)
       hgrid = 1;
     if (hgrid > 2048)
       hgrid = 2048;
-    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * nb, d->stream, d->coord[4],
-                       d->coord[5], d->loss_pass, n, c->Po, tmin, dt, nb, c->kind == EH_OP_WAKE ? 1 : 0, hist);
-    CHK(hipGetLastError());
+    /* room for one copy of the counts per wave, at most 48 KB of LDS */
+    const int32_t lds_bins = nb_max * (CB / 64) < 12288 ? (int32_t)(nb_max * (CB / 64)) : 12288;
+    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, d->coord[4],
+                       d->coord[5], d->loss_pass, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins,
+                       hist);
   }
-  if (c->kind == EH_OP_WAKE) {
-    hipLaunchKernelGGL(wake_convolve, dim3((nb + CB - 1) / CB), dim3(CB), 0, d->stream, hist, sl->W, sl->wpoints,
-                       nb, c->factor, volt);
-    CHK(hipGetLastError());
-    CHK(hipMemsetAsync(volt + nb, 0, sizeof(double), d->stream));
+  if (kind == EH_OP_WAKE) {
+    hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CB - 1) / CB)), dim3(CB), 0, d->stream, hist,
+                       sl->W, sl->wpoints, plan, c->factor, volt);
   } else {
     int bits = 0;
-    while ((1 << bits) < nb / 2)
+    while ((1 << bits) < c->nb / 2)
       bits++;
-    hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), sizeof(double) * nb, d->stream, hist, sl->Z, sl->tc,
-                       sl->ts, nb, bits, c->factor, volt);
-    CHK(hipGetLastError());
+    hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), sizeof(double) * c->nb, d->stream, hist, sl->Z, sl->tc,
+                       sl->ts, c->nb, bits, c->factor, plan, volt);
   }
-  hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[0], d->coord[1], d->coord[2],
-                     d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po, tmin, dt, nb, volt,
-                     c->kind, c->interpolate, c->z_end, c->ap_xmax, c->ap_ymax,
-                     (c->flags & EHD_COLL_NOCHECKS) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0), c->pass, 4 * c->op_index);
+  if (n > 0)
+    hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[0], d->coord[1], d->coord[2],
+                       d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po, plan, c->dt, volt,
+                       kind, c->interpolate, c->z_end, c->ap_xmax, c->ap_ymax,
+                       (c->flags & EHD_COLL_NOCHECKS) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0), c->pass,
+                       4 * c->op_index);
   CHK(hipGetLastError());
   CHK(hipEventRecord(d->cev1, d->stream));
-  if (hist_out || volt_out) {
+  coll_plan hp;
+  CHK(hipMemcpyAsync(&hp, plan, sizeof(hp), hipMemcpyDeviceToHost, d->stream));
+  CHK(hipStreamSynchronize(d->stream));
+  float ms = 0;
+  CHK(hipEventElapsedTime(&ms, d->cev0, d->cev1));
+  d->coll_ms += ms;
+  d->coll_launches += 1;
+  if (hp.err)
+    return hp.err;
+  const int32_t nb = hp.red[2] ? hp.nb : 0;
+  if (nb > 0 && (hist_out || volt_out)) {
     unsigned int *h = (unsigned int *)malloc(sizeof(unsigned int) * nb);
     if (!h)
       return EH_ERR_NOMEM;
@@ -601,37 +757,35 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
         hist_out[k] = (double)h[k];
     free(h);
   }
-  CHK(hipEventSynchronize(d->cev1));
-  float ms = 0;
-  CHK(hipEventElapsedTime(&ms, d->cev0, d->cev1));
-  d->coll_ms += ms;
-  d->coll_launches += 1;
   if (nb_out)
     *nb_out = nb;
   if (tmin_out)
-    *tmin_out = tmin;
+    *tmin_out = hp.tmin;
   return EH_OK;
 }
 
 int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count) {
   const int64_t n = d->n;
-  const int64_t grid = (n + CB - 1) / CB;
+  const int64_t rgrid = reduce_grid(n);
   CHK(hipSetDevice(d->device));
-  int rc = ensure_work(d, 64, grid);
+  int rc = ensure_work(d, 64, rgrid);
   if (rc)
     return rc;
-  unsigned long long *red = (unsigned long long *)d->cwork;
-  unsigned long long init[4] = {~0ULL, 0ULL, 0ULL, 0ULL}, hred[4];
-  CHK(hipMemcpyAsync(red, init, sizeof(init), hipMemcpyHostToDevice, d->stream));
-  if (n > 0)
-    hipLaunchKernelGGL(time_minmax, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                       d->loss_pass, n, Po, red);
+  coll_plan *plan = (coll_plan *)d->cwork;
+  unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
+  unsigned long long *mm = (unsigned long long *)((char *)hist + hist_bytes(64) + 8 * (64 + 8));
+  hipLaunchKernelGGL(coll_reset, dim3(1), dim3(CB), 0, d->stream, plan, hist, 0);
+  hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
+                     d->loss_pass, n, Po, mm);
+  hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)mm, rgrid,
+                     -1, 0, 0, 1.0, 0.0, 0, 0);
   CHK(hipGetLastError());
-  CHK(hipMemcpyAsync(hred, red, sizeof(hred), hipMemcpyDeviceToHost, d->stream));
+  coll_plan hp;
+  CHK(hipMemcpyAsync(&hp, plan, sizeof(hp), hipMemcpyDeviceToHost, d->stream));
   CHK(hipStreamSynchronize(d->stream));
-  *count = (int64_t)hred[2];
-  *tmin = key_value(hred[0]);
-  *tmax = key_value(hred[1]);
+  *count = (int64_t)hp.red[2];
+  *tmin = key_value(hp.red[0]);
+  *tmax = key_value(hp.red[1]);
   return EH_OK;
 }
 

@@ -51,6 +51,9 @@ void ehd_set_block(ehd_dev *d, int32_t block);
 /* ---- batched variants ---- */
 /* vdesc/chunks describe variants and work items; ops are all programs concatenated */
 int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk *chunks, int32_t nchunks);
+/* replace the work queue by chunks over a list of particle indices (the
+ * surviving particles, grouped by variant); chunk.first indexes perm */
+int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_chunk *chunks, int32_t nchunks);
 /* passes [pass0, pass0+npasses) for every variant, npasses split into launches of
  * passes_per_launch; ppl particles per lane (1 or 2); graph = capture and replay */
 int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,

@@ -694,7 +694,8 @@ template <int PPL>
 __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op *__restrict__ ops,
                                                        const eh_vdesc *__restrict__ vars,
                                                        const eh_chunk *__restrict__ chunks, int32_t nchunks,
-                                                       int64_t pass0, int32_t npasses, unsigned int *queue) {
+                                                       const int32_t *__restrict__ perm, int64_t pass0,
+                                                       int32_t npasses, unsigned int *queue) {
   __shared__ int s_chunk;
   for (;;) {
     if (threadIdx.x == 0)
@@ -714,8 +715,9 @@ __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op 
     bool live[PPL];
     for (int k = 0; k < PPL; k++) {
       const int64_t local = threadIdx.x + (int64_t)k * blockDim.x;
-      idx[k] = ch.first + local;
-      live[k] = local < ch.count && st.loss_pass[idx[k]] < 0;
+      const bool in = local < ch.count;
+      idx[k] = perm ? (in ? (int64_t)perm[ch.first + local] : 0) : ch.first + local;
+      live[k] = in && st.loss_pass[idx[k]] < 0;
       lost[k] = 0;
       lost_op[k] = 0;
       lost_pass[k] = 0;
@@ -850,6 +852,8 @@ void ehd_destroy(ehd_dev *d) {
     hipFree(d->vdesc);
   if (d->chunks)
     hipFree(d->chunks);
+  if (d->perm)
+    hipFree(d->perm);
   if (d->queue)
     hipFree(d->queue);
   for (int k = 0; k < 6; k++)
@@ -1065,6 +1069,29 @@ int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk 
   HIPCHK(d, hipStreamSynchronize(d->stream));
   d->nvar = nvar;
   d->nchunks = nchunks;
+  d->use_perm = 0;
+  return EH_OK;
+}
+
+int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_chunk *chunks, int32_t nchunks) {
+  HIPCHK(d, hipSetDevice(d->device));
+  if (nperm > d->perm_cap) {
+    if (d->perm)
+      HIPCHK(d, hipFree(d->perm));
+    d->perm = NULL;
+    d->perm_cap = 0;
+    HIPCHK(d, hipMalloc((void **)&d->perm, sizeof(int32_t) * (nperm > 0 ? nperm : 1)));
+    d->perm_cap = nperm > 0 ? nperm : 1;
+  }
+  if (nchunks > d->chunk_cap)
+    return EH_ERR_INVALID; /* compaction never adds chunks */
+  if (nperm > 0)
+    HIPCHK(d, hipMemcpyAsync(d->perm, perm, sizeof(int32_t) * nperm, hipMemcpyHostToDevice, d->stream));
+  if (nchunks > 0)
+    HIPCHK(d, hipMemcpyAsync(d->chunks, chunks, sizeof(eh_chunk) * nchunks, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  d->nchunks = nchunks;
+  d->use_perm = 1;
   return EH_OK;
 }
 
@@ -1073,10 +1100,10 @@ static int batch_launch(ehd_dev *d, eh_state st, int64_t pass0, int32_t npasses,
   hipLaunchKernelGGL(eh_queue_reset, dim3(1), dim3(1), 0, d->stream, d->queue);
   if (ppl == 2)
     hipLaunchKernelGGL(eh_batch_kernel<2>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
-                       d->nchunks, pass0, npasses, d->queue);
+                       d->nchunks, d->use_perm ? d->perm : (const int32_t *)NULL, pass0, npasses, d->queue);
   else
     hipLaunchKernelGGL(eh_batch_kernel<1>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
-                       d->nchunks, pass0, npasses, d->queue);
+                       d->nchunks, d->use_perm ? d->perm : (const int32_t *)NULL, pass0, npasses, d->queue);
   return EH_OK;
 }
 

@@ -40,6 +40,9 @@ struct ehd_dev {
   eh_chunk *chunks;
   int32_t nvar, nchunks, chunk_cap, var_cap;
   unsigned int *queue;
+  int32_t *perm;     /* compacted particle indices the chunks refer to (NULL: identity) */
+  int64_t perm_cap;
+  int use_perm;
   double *snap[6];
   int32_t *snap_lp, *snap_lc;
   int64_t snap_cap;

@@ -32,6 +32,7 @@ struct eh_batch {
   double *ids;
   int64_t n, cap;
   int uploaded;
+  int64_t scheduled; /* particles in the current work queue */
   eh_batch_tuning tune;
 };
 
@@ -175,14 +176,59 @@ int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, 
   return b->nvar++;
 }
 
+/* The work queue: chunks of block x particles-per-lane particles, never
+ * straddling variants.  Full: every particle of every variant in order.
+ * Alive only (b->lp current): the surviving particles of each variant, listed
+ * in an index array the chunks refer to, so the lanes of a wave stay busy
+ * after most of a grid has been lost. */
+static int build_queue(eh_batch *b, int alive_only) {
+  ehd_dev *d = eh_ctx_dev(b->ctx);
+  const int64_t csize = (int64_t)b->tune.block * b->tune.particles_per_lane;
+  int64_t nchunks = 0, nperm = 0;
+  int32_t *perm = NULL;
+  eh_chunk *ch;
+  int rc;
+  if (alive_only) {
+    perm = (int32_t *)malloc(sizeof(int32_t) * (b->n > 0 ? b->n : 1));
+    if (!perm)
+      return EH_ERR_NOMEM;
+  }
+  ch = (eh_chunk *)malloc(sizeof(eh_chunk) * (b->n / csize + b->nvar + 1));
+  if (!ch) {
+    free(perm);
+    return EH_ERR_NOMEM;
+  }
+  for (int v = 0; v < b->nvar; v++) {
+    const int64_t first = b->var[v].first, n = b->var[v].n;
+    int64_t start = alive_only ? nperm : first, count = n;
+    if (alive_only) {
+      for (int64_t i = first; i < first + n; i++)
+        if (b->lp[i] < 0)
+          perm[nperm++] = (int32_t)i;
+      count = nperm - start;
+    }
+    for (int64_t f = 0; f < count; f += csize) {
+      ch[nchunks].variant = v;
+      ch[nchunks].first = start + f;
+      ch[nchunks].count = (int32_t)(count - f < csize ? count - f : csize);
+      nchunks++;
+    }
+  }
+  rc = alive_only ? ehd_batch_compact(d, perm, nperm, ch, (int32_t)nchunks)
+                  : ehd_batch_setup(d, b->var, b->nvar, ch, (int32_t)nchunks);
+  free(ch);
+  free(perm);
+  if (rc)
+    return bmsg(b, rc, ehd_message(d));
+  b->scheduled = alive_only ? nperm : b->n;
+  return EH_OK;
+}
+
 static int upload(eh_batch *b) {
   ehd_dev *d = eh_ctx_dev(b->ctx);
   int rc;
-  int64_t nchunks = 0;
-  eh_chunk *ch;
-  const int64_t csize = (int64_t)b->tune.block * b->tune.particles_per_lane;
-  if (b->nops > 0x7fffffff)
-    return bmsg(b, EH_ERR_UNSUPPORTED, "batch programs too long");
+  if (b->nops > 0x7fffffff || b->n > 0x7fffffff)
+    return bmsg(b, EH_ERR_UNSUPPORTED, "batch programs or particle count too large");
   rc = ehd_alloc_state(d, b->n);
   if (!rc)
     rc = ehd_put_state(d, b->soa[0], b->soa[1], b->soa[2], b->soa[3], b->soa[4], b->soa[5], b->lp, b->lc, b->n);
@@ -190,23 +236,9 @@ static int upload(eh_batch *b) {
     rc = ehd_put_ops(d, b->ops, (int32_t)b->nops);
   if (rc)
     return bmsg(b, rc, ehd_message(d));
-  for (int v = 0; v < b->nvar; v++)
-    nchunks += (b->var[v].n + csize - 1) / csize;
-  ch = (eh_chunk *)malloc(sizeof(eh_chunk) * (nchunks > 0 ? nchunks : 1));
-  if (!ch)
-    return EH_ERR_NOMEM;
-  nchunks = 0;
-  for (int v = 0; v < b->nvar; v++)
-    for (int64_t f = 0; f < b->var[v].n; f += csize) {
-      ch[nchunks].variant = v;
-      ch[nchunks].first = b->var[v].first + f;
-      ch[nchunks].count = (int32_t)(b->var[v].n - f < csize ? b->var[v].n - f : csize);
-      nchunks++;
-    }
-  rc = ehd_batch_setup(d, b->var, b->nvar, ch, (int32_t)nchunks);
-  free(ch);
+  rc = build_queue(b, 0);
   if (rc)
-    return bmsg(b, rc, ehd_message(d));
+    return rc;
   b->uploaded = 1;
   return EH_OK;
 }
@@ -250,10 +282,47 @@ int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats)
       return rc;
   }
   ehd_timing_reset(d);
-  rc = ehd_batch_track(d, pass0, (int32_t)npasses, b->tune.passes_per_launch, b->tune.particles_per_lane,
-                       b->tune.block, b->tune.use_graph, &ms, &nl);
-  if (rc)
-    return bmsg(b, rc, ehd_message(d));
+  if (!b->tune.compact) {
+    rc = ehd_batch_track(d, pass0, (int32_t)npasses, b->tune.passes_per_launch, b->tune.particles_per_lane,
+                         b->tune.block, b->tune.use_graph, &ms, &nl);
+    if (rc)
+      return bmsg(b, rc, ehd_message(d));
+  } else {
+    /* launches of 1, 2, 4 ... passes up to passes_per_launch, so the queue can
+     * shed the many particles a grid loses in its first passes; after each
+     * launch the queue is rebuilt from the survivors once a quarter of the
+     * scheduled lanes are idle */
+    int64_t done = 0, step = 1;
+    while (done < npasses) {
+      double ms1 = 0;
+      int64_t nl1 = 0, alive = 0;
+      int64_t k = step < b->tune.passes_per_launch ? step : b->tune.passes_per_launch;
+      if (k > npasses - done)
+        k = npasses - done;
+      rc = ehd_batch_track(d, pass0 + done, (int32_t)k, (int32_t)k, b->tune.particles_per_lane, b->tune.block, 0,
+                           &ms1, &nl1);
+      if (rc)
+        return bmsg(b, rc, ehd_message(d));
+      ms += ms1;
+      nl += nl1;
+      done += k;
+      step *= 2;
+      if (done >= npasses)
+        break;
+      rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);
+      if (rc)
+        return bmsg(b, rc, ehd_message(d));
+      for (int64_t i = 0; i < b->n; i++)
+        alive += b->lp[i] < 0;
+      if (alive == 0)
+        break;
+      if (alive * 4 < b->scheduled * 3) {
+        rc = build_queue(b, 1);
+        if (rc)
+          return rc;
+      }
+    }
+  }
   if (stats) {
     int64_t effort = 0, left = 0;
     rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);
@@ -296,7 +365,8 @@ int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_
     rc = upload(b);
     if (rc)
       return rc;
-  }
+  } else if ((rc = build_queue(b, 0)))
+    return rc;
   rc = ehd_state_snapshot(d, 0);
   if (rc)
     return bmsg(b, rc, ehd_message(d));
@@ -334,12 +404,15 @@ int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_
     best.passes_per_launch = ppass;
   }
   best.use_graph = 1;
+  best.compact = 1;
   best.sample_ms = best_ms;
   rc = eh_batch_set_tuning(b, &best);
   if (!rc)
     rc = ehd_state_snapshot(d, 1);
   if (rc)
     return bmsg(b, rc, ehd_message(d));
+  if ((rc = build_queue(b, 0)))
+    return rc;
   ehd_timing_reset(d);
   if (best_out)
     *best_out = best;

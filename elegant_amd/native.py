@@ -162,8 +162,8 @@ class Batch:
         check(self.tracker._ctx, lib().eh_batch_autotune(self._b, sample_passes, C.byref(t)))
         return t
 
-    def set_tuning(self, ppl, block, passes_per_launch, use_graph=1):
-        t = A.eh_batch_tuning(ppl, block, passes_per_launch, use_graph, 0.0)
+    def set_tuning(self, ppl, block, passes_per_launch, use_graph=1, compact=0):
+        t = A.eh_batch_tuning(ppl, block, passes_per_launch, use_graph, compact, 0, 0.0)
         check(self.tracker._ctx, lib().eh_batch_set_tuning(self._b, C.byref(t)))
 
     def tuning(self):

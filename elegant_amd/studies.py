@@ -59,10 +59,11 @@ def grid_limits(surv, loss_pass, nd, lines, nx, passes):
     (-1 when the origin is lost), and the particle-turns a sequential
     outward search (aperture_search.c:1019-1120 with no splits) spends to
     find it: every point up to and including the first lost one, each for
-    the passes it survives (a lost point's pass of loss counted)."""
+    the passes it completes, as elegant's effort counts survivors at the end
+    of each pass (do_tracking.c:2758-2765)."""
     s = surv.reshape(nd, lines, nx)
     lp = loss_pass.reshape(nd, lines, nx)
-    cost = np.where(s, passes, lp + 1)
+    cost = np.where(s, passes, lp)
     first = np.where(s.all(axis=2), nx, np.argmax(~s, axis=2))
     upto = np.arange(nx)[None, None, :] <= first[:, :, None]
     return first - 1, int(np.sum(np.where(upto, cost, 0)))

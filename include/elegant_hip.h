@@ -264,6 +264,10 @@ typedef struct {
   int32_t block;              /* workgroup size: 64, 128 or 256 */
   int32_t passes_per_launch;  /* passes fused in one kernel launch */
   int32_t use_graph;          /* capture the launch sequence in a HIP graph */
+  int32_t compact;            /* between launches, rebuild the work queue from the surviving particles when
+                                 fewer than 3/4 of the scheduled lanes are alive (launches are then not
+                                 captured in a graph) */
+  int32_t pad;
   double sample_ms;           /* autotune: time of the best sample */
 } eh_batch_tuning;
 

@@ -13,6 +13,21 @@
  *    (multipole.c:1103-1104).
  * Dead computations that only feed sigmaDelta2 (NULL on the do_tracking
  * path, do_tracking.c:2011) are left out; they change no coordinate.
+ *
+ * What pins each routine (tests/test_oracle.py and friends): closed-form
+ * maps (drift, thick quadrupole, thin multipoles, sector bend and edges,
+ * RF energy gain, drift-aperture loss point), invariants (symplecticity,
+ * roll and offset symmetries, conservation in add_to_particle_energy) and
+ * brute force (Kahan and the fixed-point mean against exact rationals, the
+ * FFT against numpy and a direct DFT, histogram and convolution against
+ * numpy, WAKE against ZLONGIT for one resonator).
+ * Parity unpinned: the SDDS-toolkit routines the reference calls but does
+ * not ship (realFFT's packing and normalisation, compute_average's
+ * summation, find_min_max) are restated from their published behaviour and
+ * checked against mathematics, not against the toolkit's bits; the physical
+ * constants (CODATA 2018, eh_consts.h) are this build's choice; and the
+ * bit-level operation order inside each restated routine rests on review of
+ * the cited lines, since the reference cannot be compiled here.
  */
 #include "oracle.h"
 

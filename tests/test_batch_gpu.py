@@ -25,20 +25,24 @@ def variants():
     return out
 
 
-@pytest.mark.parametrize("tuning", [(1, 256, 64, 1), (2, 64, 1, 0), (2, 128, 2, 1)])
+@pytest.mark.parametrize("tuning", [(1, 256, 64, 1, 0), (2, 64, 1, 0, 0), (2, 128, 2, 1, 0), (1, 64, 4, 1, 1),
+                                    (2, 64, 2, 0, 1)])
 def test_batch_bit_exact_per_variant(tracker, tuning):
+    """the last two tunings rebuild the queue from the survivors between
+    launches (1, 2, ... passes per launch)"""
     vs = variants()
     b = native.Batch(tracker)
     for bl, Po, rows in vs:
         b.add_variant(bl, Po, rows)
     b.set_tuning(*tuning)
-    st = b.track(0, 3)
+    passes = 6 if tuning[4] else 3
+    st = b.track(0, passes)
     total_lost = 0
     for k, (bl, Po, rows) in enumerate(vs):
         g, ng = b.download(k)
         c = rows.copy()
         o = O.Oracle("emath")
-        nc = o.do_tracking(c, bl, Po, 3)
+        nc = o.do_tracking(c, bl, Po, passes)
         assert ng == nc
         total_lost += rows.shape[0] - ng
         assert_rows_identical(g, c, "variant %d" % k)

@@ -95,8 +95,8 @@ def test_grid_limits_and_sequential_cost():
     lp = np.where(surv, -1, np.array([[0, 0, 3, 0], [5, 0, 0, 0], [0, 0, 0, 0]]))
     lim, cost = studies.grid_limits(surv, lp, 1, 3, 4, 10)
     assert lim.tolist() == [[1, -1, 3]]
-    # ray 0: 10 + 10 + (3+1); ray 1: (5+1); ray 2: 4 x 10
-    assert cost == 24 + 6 + 40
+    # passes completed: ray 0: 10 + 10 + 3; ray 1: 5; ray 2: 4 x 10
+    assert cost == 23 + 5 + 40
 
 
 def test_non_monotone_ray_keeps_a_confirmed_boundary(monkeypatch):
