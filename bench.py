@@ -85,9 +85,15 @@ def cpu_baseline(name, threads):
         t.join()
     dt = time.perf_counter() - t0
     total = sum(efforts)
+    model = "?"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "?")
+    except OSError:
+        pass
     return {"value": total / dt, "unit": "particle-turns/s", "cores": threads, "kind": "port",
-            "sample": "%d threads x %d particles x %d passes of the same workload (%d particle-turns, %.1f s)"
-                      % (threads, per_thread, passes, total, dt)}
+            "sample": "%d threads x %d particles x %d passes of the same workload (%d particle-turns, %.1f s) "
+                      "on %s, nproc %d" % (threads, per_thread, passes, total, dt, model, os.cpu_count() or 0)}
 
 
 def traffic(kernel, workload):
