@@ -116,18 +116,28 @@ def training_set(vs, grid, rays, deltas):
     return np.concatenate(Xs), np.concatenate(ys)
 
 
-def propose(mlp, vs, rays, deltas):
+def propose(mlp, vs, rays, deltas, confidence=False):
     """guessed limit index per variant (offsets x rays): the end of the first
-    run of points with survival probability >= 0.5 going outward"""
-    out = []
+    run of points with survival probability >= 0.5 going outward.  With
+    confidence, also a mask of the guesses the network is unsure of: the
+    probability does not step from >= 0.8 at the guess to <= 0.2 just past
+    it."""
+    out, unsure = [], []
     for v in vs:
         p = mlp.predict(grid_features(rays, deltas, v.nx, v.features)).reshape(len(deltas), rays.lines, v.nx)
         bad = p < 0.5
-        out.append(np.where(bad.any(axis=2), np.argmax(bad, axis=2), v.nx) - 1)
-    return out
+        g = np.where(bad.any(axis=2), np.argmax(bad, axis=2), v.nx) - 1
+        out.append(g)
+        if confidence:
+            pin = np.take_along_axis(p, np.maximum(g, 0)[..., None], axis=2)[..., 0]
+            pout = np.take_along_axis(p, np.minimum(g + 1, v.nx - 1)[..., None], axis=2)[..., 0]
+            sure = ((g < 0) | (pin >= 0.8)) & ((g + 1 >= v.nx) | (pout <= 0.2))
+            unsure.append(~sure)
+    return (out, unsure) if confidence else out
 
 
-def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=1, max_rounds=64):
+def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=1, max_rounds=64, ladder=(),
+                 ladder_rays=None):
     """Find, on every ray of every (variant, offset), a grid index lo that
     survives with lo + 1 lost (or lo = nx - 1 surviving, or lo = -1 with the
     origin lost), assuming survival is monotone along the ray.  Each round is
@@ -139,9 +149,13 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
     settles the ray when the guess is right; a refuted guess then tracks up
     to `kary` points stepping away from the refuted side in doubling steps,
     so points beyond the boundary (lost within a few passes, cheap) rather
-    than inside it (tracked for every pass) take most of the tests.  Returns (limits per variant,
-    effort, rounds, wall seconds, fraction of rays settled by the first
-    round)."""
+    than inside it (tracked for every pass) take most of the tests.  `ladder`
+    adds points at these offsets from the guess to the first round: they
+    bracket a wrong guess at once, trading particle-turns for rounds (each
+    round costs the latency of `passes` passes however few particles it
+    holds); `ladder_rays` (per variant, offsets x rays) limits it to the
+    rays whose guess is uncertain.  Returns (limits per variant, effort, rounds, wall seconds,
+    fraction of rays settled by the first round)."""
     nd, nl = len(deltas), rays.lines
     nx = np.concatenate([np.full(nd * nl, v.nx) for v in vs])
     R = nx.size
@@ -150,6 +164,8 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
     lo = np.full(R, -1)
     hi = nx.copy()
     g = None if guess is None else np.clip(np.concatenate([gv.ravel() for gv in guess]), -1, nx - 1)
+    # rays that get the ladder (all of them unless a mask of unsure guesses is given)
+    lm = np.ones(R, dtype=bool) if ladder_rays is None else np.concatenate([m.ravel() for m in ladder_rays])
     step = np.ones(R, dtype=np.int64)
     effort = 0
     wall = 0.0
@@ -162,7 +178,13 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
         if not active.any():
             break
         if rounds == 0 and g is not None:
-            T = np.stack([np.where(active & (g >= 0), g, -1), np.where(active & (g + 1 < nx), g + 1, -1)], axis=1)
+            offs = (0, 1) + tuple(ladder)
+            T = np.stack([np.where(active & (g + o >= 0) & (g + o < nx) & (lm | (k < 2)), g + o, -1)
+                          for k, o in enumerate(offs)], axis=1)
+            # a ladder point that clips onto another test is dropped
+            for a in range(1, len(offs)):
+                for b2 in range(a):
+                    T[:, a] = np.where(T[:, a] == T[:, b2], -1, T[:, a])
         else:
             gap = hi - lo - 1                      # untested indices lo+1 .. hi-1
             k = np.minimum(gap, kary)
@@ -221,7 +243,7 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
     return limits, effort, rounds, wall, settled_first
 
 def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, lr=1e-3, tune=None, log=None,
-          rays=None, deltas=None, kary=4):
+          rays=None, deltas=None, kary=4, ladder=(-12, -4, 4, 12)):
     """Train on the survival verdicts of the training variants' full grids,
     then find the aperture of the evaluation variants three ways: the plain
     grid (the reference answer), bisection alone, and the network's guess
@@ -248,11 +270,14 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
     gref = studies.plain_grid(tracker, ev, passes, rays, deltas, tune)
     say("evaluation grids: %d particle-turns in %.1f s" % (gref["effort"], gref["wall"]))
     t2 = time.perf_counter()
-    guess = propose(mlp, ev, rays, deltas)
+    guess, unsure = propose(mlp, ev, rays, deltas, confidence=True)
     t_prop = time.perf_counter() - t2
     lim_l, eff_l, rounds_l, wall_l, first_l = index_search(tracker, ev, passes, rays, deltas, guess, tune,
                                                            kary=kary)
     say("learned search: %d particle-turns, %d rounds, %.1f s" % (eff_l, rounds_l, wall_l))
+    lim_f, eff_f, rounds_f, wall_f, first_f = index_search(tracker, ev, passes, rays, deltas, guess, tune,
+                                                           kary=16, ladder=ladder, ladder_rays=unsure)
+    say("learned search, ladder %s: %d particle-turns, %d rounds, %.1f s" % (ladder, eff_f, rounds_f, wall_f))
     lim_b, eff_b, rounds_b, wall_b, _ = index_search(tracker, ev, passes, rays, deltas, None, tune)
     say("bisection search: %d particle-turns, %d rounds, %.1f s" % (eff_b, rounds_b, wall_b))
     mlp.close()
@@ -288,5 +313,11 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
                         maps_per_hour_incl_training=maps / (wall_l + t_prop + gtrain["wall"] + t_train) * 3600,
                         guess_exact=float(guess_hit),
                         settled_first_round=first_l, **compare(lim_l)),
+        "learned_ladder": dict(ladder=list(ladder), unsure_rays=float(np.mean(np.concatenate(
+                                   [u.ravel() for u in unsure]))), effort=eff_f, rounds=rounds_f,
+                               wall_s=wall_f + t_prop,
+                               maps_per_hour=maps / (wall_f + t_prop) * 3600,
+                               maps_per_hour_incl_training=maps / (wall_f + t_prop + gtrain["wall"] + t_train) * 3600,
+                               settled_first_round=first_f, **compare(lim_f)),
     }
     return rep

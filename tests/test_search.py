@@ -67,10 +67,10 @@ def test_index_search_finds_the_grid_boundary(monkeypatch, guess_kind):
         g = [np.full_like(r, v.nx - 1) for r, v in zip(ref, vs)]
     elif guess_kind == "origin":
         g = [np.full_like(r, -1) for r in ref]
-    for kary in (1, 4, 16):
-        lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g, kary=kary)
+    for kary, ladder in ((1, ()), (4, ()), (16, ()), (16, (-12, -4, 4, 12)), (2, (-1, 3))):
+        lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g, kary=kary, ladder=ladder)
         for a, b in zip(lim, ref):
-            assert np.array_equal(a, b), kary
+            assert np.array_equal(a, b), (kary, ladder)
     if guess_kind == "exact":
         assert rounds == 1 and first == 1.0
     if guess_kind == "none":
