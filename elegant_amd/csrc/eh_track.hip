@@ -644,7 +644,11 @@ DEV int run_op(particle &p, const eh_op &op) {
   return post_element_checks(p, op);
 }
 
-__global__ void __launch_bounds__(256) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
+/* Six waves per SIMD (at most 80 VGPRs): the extra wave hides more of the
+ * dependency and scalar-load latency; the registers the compiler then has
+ * to spill belong to rarely taken paths (measured: the storage ring runs
+ * 2.5 % faster than at five waves). */
+__global__ void __launch_bounds__(256, 6) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
                                                        int64_t pass0, int32_t npasses, int32_t op_first,
                                                        int32_t op_last) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

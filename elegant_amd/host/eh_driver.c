@@ -427,7 +427,8 @@ static int rfca_fiducial(eh_ctx *c, int k, int64_t pass) {
     nsum++;
   }
   if (nsum <= 0)
-    return fail(c, EH_ERR_FIDUCIAL, "RFCA: no particle available for fiducialization");
+    return EH_OK; /* no particle left: do_tracking's element loop stops before the
+                     cavity (do_tracking.c:816), so the phase stays unset */
   t0 = tsum / nsum;
   c->phase_fid[k] = eh_rfca_phase_fiducial(rf, t0);
   c->fid_seen[k] = 1;

@@ -112,7 +112,8 @@ def test_drift_and_limit_amplitudes(tracker):
 def test_wake_and_zlongit_entry_points(tracker):
     table = Lt.resonator_wake(points=4000)
     w = E.WAKE(table)
-    zl = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=8192, BIN_SIZE=0.2e-12)
+    # 8,192 bins of 2 ps: the bin optimiser keeps this layout for the resonator
+    zl = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=8192, BIN_SIZE=2e-12)
     r = beam(20000, 5, halo=0.0)
     r[:, 4] = 0.2 + r[:, 4] * 2
     q = 2e-9 / r.shape[0]
