@@ -49,6 +49,7 @@
 #define EH_F_END2FOCUS (1 << 16)
 #define EH_F_MATRIX (1 << 17)  /* RFCA matrix method */
 #define EH_F_HASLEN (1 << 18)  /* nonzero length (drift branches) */
+#define EH_F_RCP (1 << 19)     /* CSBEND: rho0 and rho_actual within em_recip_ok's range (emath.h) */
 
 /* loss stages recorded with the element index (loss_code = 4*op_index + stage) */
 #define EH_STAGE_ELEMENT 0  /* inside the element routine's own compaction */
@@ -110,6 +111,8 @@ typedef struct {
 #define EH_D_BDS 36       /* [8]: s * kickFrac[j] */
 #define EH_D_BKF 44       /* [8]: kickFrac[j] (0 ends the slice) */
 #define EH_D_RHOA_SGN 52  /* +1 or -1: the sign of rho_actual (t / rho_actual for t = +-0 or NaN) */
+#define EH_D_RRHO0 53     /* 1 / rho0 rounded to nearest: the quotients by rho0 use em_div */
+#define EH_D_RRHOA 54     /* 1 / rho_actual rounded to nearest */
 
 /* ---- RFCA ---- */
 #define EH_D_VOLT 0       /* volt / (1e6 me_mev relSign) / nKicks */

@@ -135,13 +135,13 @@ DEV void drift1(particle &p, double L) {
 DEV void exact_drift(particle &p, double L) {
   p.x += p.xp * L;
   p.y += p.yp * L;
-  p.s += L * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.s += L * em_sqrt(1 + sq(p.xp) + sq(p.yp));
 }
 
 /* ---- misalignment (malign_mat.c:172-182, tilt_matrices.c:156-188) ---- */
 
 DEV void offset(particle &p, double dx, double dy, double dz) {
-  p.s += dz * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.s += dz * em_sqrt(1 + sq(p.xp) + sq(p.yp));
   p.x = p.x - dx + dz * p.xp;
   p.y = p.y - dy + dz * p.yp;
 }
@@ -169,19 +169,37 @@ DEV bool in_aperture(double x, double y, const eh_op &op) {
   return true;
 }
 
+/* 1 + delta and its reciprocal for the IEEE quotients of emath.h, fixed for
+ * the whole element */
+struct opd_recip {
+  double v, r;
+  int ok;
+};
+
+DEV opd_recip make_opd(double dp) {
+  opd_recip o;
+  o.v = 1 + dp;
+  o.r = em_recip(o.v);
+  o.ok = em_recip_ok(o.v);
+  return o;
+}
+
 /* convertMomentaToSlopes (multipole.h:266-281) */
-DEV bool momenta_to_slopes(double &xp, double &yp, double qx, double qy, double dp, bool expand) {
+DEV bool momenta_to_slopes(double &xp, double &yp, double qx, double qy, double dp, bool expand,
+                           const opd_recip &o) {
   if (expand) {
-    xp = qx / (1 + dp);
-    yp = qy / (1 + dp);
+    xp = em_div(qx, o.v, o.r, o.ok);
+    yp = em_div(qy, o.v, o.r, o.ok);
     return true;
   }
   double denom = sq(1 + dp) - sq(qx) - sq(qy);
   if (denom <= 0)
     return false;
-  denom = __builtin_sqrt(denom);
-  xp = qx / denom;
-  yp = qy / denom;
+  denom = em_sqrt(denom);
+  const double r = em_recip(denom);
+  const int ok = em_recip_ok(denom);
+  xp = em_div(qx, denom, r, ok);
+  yp = em_div(qy, denom, r, ok);
   return true;
 }
 
@@ -224,7 +242,7 @@ DEV void apply_kick(double &qx, double &qy, int order, int skew, double KnL, dou
  * Returns 0, or the loss kind: 1 aperture (x, y hold the loss point),
  * 2 slopes (the row is left as received). */
 DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
-                       double &dzLoss, const double dp, const bool expand, const eh_op &op) {
+                       double &dzLoss, const double dp, const bool expand, const opd_recip &o, const eh_op &op) {
   const int nsub = op.nsub;
   const int flags = op.flags;
   const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
@@ -241,7 +259,7 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
         if (expand)
           s += dsh * (1 + (sq(xp) + sq(yp)) / 2);
         else
-          s += dsh * __builtin_sqrt(1 + sq(xp) + sq(yp));
+          s += dsh * em_sqrt(1 + sq(xp) + sq(yp));
         dzLoss += dsh;
       }
       if (step == nsub - 1)
@@ -256,15 +274,15 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
         apply_kick(qx, qy, 0, 0, opc(&op.d[EH_D_XK + step]), x, y);
       if (flags & EH_F_YKICK)
         apply_kick(qx, qy, 0, 1, opc(&op.d[EH_D_YK + step]), x, y);
-      if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+      if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o))
         return 2;
       if (has_drift) {
         /* rad_coef = 0, isr_coef < 0: the q/(1+dp)*(1+dp) round trip (multipole.c:1321-1351) */
-        qx /= (1 + dp);
-        qy /= (1 + dp);
+        qx = em_div(qx, o.v, o.r, o.ok);
+        qy = em_div(qy, o.v, o.r, o.ok);
         qx *= (1 + dp);
         qy *= (1 + dp);
-        if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+        if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o))
           return 2;
       }
     }
@@ -285,6 +303,7 @@ DEV int track_multipole(particle &p, const eh_op &op) {
 
   /* the row as the integrator receives it stays in p until the end */
   const double dp = p.dp;
+  const opd_recip o = make_opd(dp);
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, qx = 0, qy = 0, dzLoss = 0;
   int lost = 0; /* 1: aperture (x, y written), 2: slopes (nothing written), 3: final limits */
   const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
@@ -299,21 +318,21 @@ DEV int track_multipole(particle &p, const eh_op &op) {
       qx = (1 + dp) * xp;
       qy = (1 + dp) * yp;
     } else {
-      double denom = __builtin_sqrt(1 + sq(xp) + sq(yp));
+      double denom = em_sqrt(1 + sq(xp) + sq(yp));
       qx = (1 + dp) * xp / denom;
       qy = (1 + dp) * yp / denom;
     }
-    if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand)) {
+    if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o)) {
       lost = 2;
       dzLoss = op.d[EH_D_TOTLEN];
     }
   }
   if (!lost) {
-    lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, op);
+    lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, o, op);
     if (!lost) {
       if (!in_aperture(x + dxa, y + dya, op))
         lost = 1;
-      else if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand))
+      else if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o))
         lost = 2;
       else {
         p.x = x;
@@ -358,6 +377,11 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
   const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA], rsgn = op.d[EH_D_RHOA_SGN];
   const double opd = 1 + DP;
   const int nsub = op.nsub;
+  /* reciprocals shared by the divisions below (IEEE quotients, emath.h);
+   * those of rho0 and rho_actual are rounded on the host */
+  const double r_opd = em_recip(opd);
+  const int ok_opd = em_recip_ok(opd);
+  const int ok_rho = (op.flags & EH_F_RCP) != 0;
   for (int i = 0; i < op.nslices; i++) {
     if (!in_aperture(X, Y, op))
       return true;
@@ -365,8 +389,10 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
       double f = sq(opd) - sq(QY);
       if (f <= 0)
         return true;
-      f = __builtin_sqrt(f);
-      double sin_phi = QX / f;
+      f = em_sqrt(f);
+      const double r_f = em_recip(f);
+      const int ok_f = em_recip_ok(f);
+      double sin_phi = em_div(QX, f, r_f, ok_f);
       if (__builtin_fabs(sin_phi) > 1)
         return true;
       double phi = em_asin(sin_phi);
@@ -374,25 +400,27 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
       em_sincos(opc(&op.d[EH_D_BDSHR + j]) + phi, &sine, &cosi);
       if (cosi == 0)
         return true;
-      double tang = sine / cosi;
+      const double r_c = em_recip(cosi);
+      const int ok_c = em_recip_ok(cosi);
+      double tang = em_div(sine, cosi, r_c, ok_c);
       double cos_phi = em_cos(phi);
       QX = f * sine;
-      double factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
+      double factor = em_div((rho0 + X) * cos_phi, f, r_f, ok_f) * (tang - sin_phi / cos_phi);
       Y += QY * factor;
       dist += factor * opd;
       dz_lost += opc(&op.d[EH_D_BDSH + j]);
-      f = cos_phi / cosi;
+      f = em_div(cos_phi, cosi, r_c, ok_c);
       X = rho0 * (f - 1) + f * X;
       if (j == nsub - 1)
         break; /* kickFrac[last] == 0 */
       double ds = opc(&op.d[EH_D_BDS + j]);
-      double g = 1 + X / rho0;
-      QX += -ds * g * 1.0 / rhoa;
+      double g = 1 + em_div(X, rho0, opc(&op.d[EH_D_RRHO0]), ok_rho);
+      QX += em_div(-ds * g * 1.0, rhoa, opc(&op.d[EH_D_RRHOA]), ok_rho);
       /* Fx = 0: ds*g*0.0 is +-0 or NaN, and dividing it by rho_actual only
        * sets its sign, which the multiplication by sign(rho_actual) does */
       QY += ds * g * 0.0 * rsgn;
-      QX /= opd;
-      QY /= opd;
+      QX = em_div(QX, opd, r_opd, ok_opd);
+      QY = em_div(QY, opd, r_opd, ok_opd);
       QX *= opd;
       QY *= opd;
     }
@@ -406,7 +434,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA];
   double x, xp, y, yp, s, dp, dp0;
   /* entry: misalignment then rotation by the total tilt (csbend.c:1120-1131) */
-  p.s += op.d[EH_D_DZI] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.s += op.d[EH_D_DZI] * em_sqrt(1 + sq(p.xp) + sq(p.yp));
   p.x = p.x + op.d[EH_D_DXI] + op.d[EH_D_DZI] * p.xp;
   p.y = p.y + op.d[EH_D_DYI] + op.d[EH_D_DZI] * p.yp;
   x = p.x * ct + p.y * st;
@@ -427,7 +455,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   /* canonical (csbend.c:1411-1419) */
   double X = x, QX, Y = y, QY, S = 0, DP = dp;
   {
-    double f = (1 + DP) / __builtin_sqrt(1 + sq(xp) + sq(yp));
+    double f = (1 + DP) / em_sqrt(1 + sq(xp) + sq(yp));
     QX = xp * f;
     QY = yp * f;
   }
@@ -444,7 +472,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   }
   {
     /* convertFromDipoleCanonicalCoordinates (csbend.c:1421-1429) */
-    double f = 1 / __builtin_sqrt(sq(1 + DP) - sq(QX) - sq(QY));
+    double f = 1 / em_sqrt(sq(1 + DP) - sq(QX) - sq(QY));
     QX *= f;
     QY *= f;
   }
@@ -461,9 +489,9 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   /* velocity rescaling of s (csbend.c:1280-1292) */
   {
     double p0 = op.Po * (1 + dp0);
-    double beta0 = p0 / __builtin_sqrt(sq(p0) + 1);
+    double beta0 = p0 / em_sqrt(sq(p0) + 1);
     double p1 = op.Po * (1 + DP);
-    double beta1 = p1 / __builtin_sqrt(sq(p1) + 1);
+    double beta1 = p1 / em_sqrt(sq(p1) + 1);
     s = beta1 * s / beta0 + S;
   }
   x = X;
@@ -488,7 +516,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   p.dp = dp;
   p.x += op.d[EH_D_DXF] + op.d[EH_D_DZF] * p.xp;
   p.y += op.d[EH_D_DYF] + op.d[EH_D_DZF] * p.yp;
-  p.s += op.d[EH_D_DZF] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.s += op.d[EH_D_DZF] * em_sqrt(1 + sq(p.xp) + sq(p.yp));
   return 0;
 }
 
@@ -497,15 +525,15 @@ DEV int track_csbend(particle &p, const eh_op &op) {
 /* add_to_particle_energy (simple_rfca.c:987-1007) */
 DEV void add_energy(particle &p, double tof, double Po, double dgamma) {
   double P = Po * (1 + p.dp);
-  double gamma = __builtin_sqrt(P * P + 1);
+  double gamma = em_sqrt(P * P + 1);
   double gamma1 = gamma + dgamma;
   if (gamma1 <= 1)
     gamma1 = 1 + 1e-7;
-  double P1 = __builtin_sqrt(gamma1 * gamma1 - 1);
+  double P1 = em_sqrt(gamma1 * gamma1 - 1);
   p.dp = (P1 - Po) / Po;
   p.s = tof * EH_C_MKS * P1 / gamma1;
-  double Pz = P / __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
-  double Pz1 = __builtin_sqrt(Pz * Pz + gamma1 * gamma1 - gamma * gamma);
+  double Pz = P / em_sqrt(1 + sq(p.xp) + sq(p.yp));
+  double Pz1 = em_sqrt(Pz * Pz + gamma1 * gamma1 - gamma * gamma);
   double ratio = Pz / Pz1;
   p.xp *= ratio;
   p.yp *= ratio;
@@ -524,9 +552,9 @@ DEV int track_rfca(particle &p, const eh_op &op) {
     double inverseF = 0;
     for (int ik = 0; ik < nKicks; ik++) {
       if (p.dp != -1) {
-        double dc4 = length ? length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp)) : 0;
+        double dc4 = length ? length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp)) : 0;
         double P = Po * (1 + p.dp);
-        double gamma = __builtin_sqrt(sq(P) + 1);
+        double gamma = em_sqrt(sq(P) + 1);
         double beta_i = P / gamma;
         double t = (p.s + dc4) / (EH_C_MKS * beta_i) - 0.0;
         double phi = omega * (t - 0 - ik * dtLight) + phase;
@@ -539,7 +567,7 @@ DEV int track_rfca(particle &p, const eh_op &op) {
           }
           p.x += p.xp * length / 2;
           p.y += p.yp * length / 2;
-          p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+          p.s += length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp));
         }
         add_energy(p, t, Po, dgamma);
         double gamma1 = gamma + dgamma;
@@ -551,7 +579,7 @@ DEV int track_rfca(particle &p, const eh_op &op) {
       if (length) {
         p.x += p.xp * length / 2;
         p.y += p.yp * length / 2;
-        p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+        p.s += length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp));
         if ((op.flags & EH_F_END2FOCUS) && ik == nKicks - 1) {
           p.xp -= p.x * inverseF;
           p.yp -= p.y * inverseF;
@@ -561,9 +589,9 @@ DEV int track_rfca(particle &p, const eh_op &op) {
   } else {
     for (int ik = 0; ik < nKicks; ik++) {
       double P = Po * (1 + p.dp);
-      double gamma = __builtin_sqrt(sq(P) + 1);
+      double gamma = em_sqrt(sq(P) + 1);
       double beta_i = P / gamma;
-      double ds1 = length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+      double ds1 = length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp));
       double t = (p.s + ds1) / (EH_C_MKS * beta_i) - 0.0;
       double sin_phase = em_sin(omega * (t - 0 - ik * dtLight) + phase);
       double dgamma = volt * 1 * sin_phase;
@@ -572,7 +600,7 @@ DEV int track_rfca(particle &p, const eh_op &op) {
         p.xp -= p.x * inverseF;
         p.yp -= p.y * inverseF;
       }
-      double dP = __builtin_sqrt(sq(gamma + dgamma) - 1) - P;
+      double dP = em_sqrt(sq(gamma + dgamma) - 1) - P;
       double R22 = 1 / (1 + dP / P), R12, R11 = 1, R21 = 0;
       if (__builtin_fabs(dP / P) > 1e-14)
         R12 = length * (P / dP * em_log(1 + dP / P));
@@ -586,7 +614,7 @@ DEV int track_rfca(particle &p, const eh_op &op) {
       xxp = p.yp;
       p.y = xx * R11 + xxp * R12;
       p.yp = xx * R21 + xxp * R22;
-      p.s += length / 2 * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+      p.s += length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp));
       p.dp = (P + dP - Po) / Po;
       if ((gamma += dgamma) <= 1)
         p.dp = -1;
@@ -634,7 +662,7 @@ DEV int run_op(particle &p, const eh_op &op) {
   case EH_OP_RFCA_DRIFT:
     p.x += p.xp * op.d[EH_D_RFLEN_TOTAL];
     p.y += p.yp * op.d[EH_D_RFLEN_TOTAL];
-    p.s += op.d[EH_D_RFLEN_TOTAL] * __builtin_sqrt(1 + sq(p.xp) + sq(p.yp));
+    p.s += op.d[EH_D_RFLEN_TOTAL] * em_sqrt(1 + sq(p.xp) + sq(p.yp));
     break;
   default:
     break;
@@ -1232,6 +1260,15 @@ __global__ void eh_math_probe_kernel(int32_t fn, const double *in, const double 
     break;
   case 5:
     r = __builtin_sqrt(v);
+    break;
+  case 7: /* the unscaled square root the tracking kernels use */
+    r = em_sqrt(v);
+    break;
+  case 8: /* the shared-reciprocal quotient the tracking kernels use */
+    r = em_div(v, in2[i], em_recip(in2[i]), em_recip_ok(in2[i]));
+    break;
+  case 9: /* the same quotient from the rounded reciprocal (bend radii) */
+    r = em_div(v, in2[i], 1.0 / in2[i], em_recip_ok(in2[i]));
     break;
   default:
     r = v / in2[i];

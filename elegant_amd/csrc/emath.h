@@ -354,4 +354,74 @@ EM_FN double em_log(double x) {
   return ed * EM_LN2_HI + (f - (hfsq - (s * (hfsq + r) + ed * EM_LN2_LO)));
 }
 
+/* ---- division and square root without range scaling ----
+ *
+ * The device's correctly rounded FP64 division a/b is
+ *   y = rcp(b); e = fma(-b, y, 1); y = fma(y, e, y); e = fma(-b, y, 1);
+ *   y = fma(y, e, y); q = a*y; r = fma(-b, q, a); q = fma(r, y, q)
+ * (v_rcp_f64, Newton steps, one residual correction), run on operands that
+ * v_div_scale_f64 multiplies by a power of two near the ends of the exponent
+ * range, followed by v_div_fixup_f64 for zeros, infinities and NaN.  For
+ * 2^-64 <= |b| <= 2^64 and 2^-600 <= |a| <= 2^600 no step overflows or
+ * underflows, so the scaling cannot change a rounding and the unscaled
+ * sequence gives the same bits.  Its reciprocal part depends on b alone:
+ * em_recip(b) computes it once for several divisions by the same b,
+ * em_recip_ok(b) says whether b is in range, and em_div(a, b, y, ok) runs
+ * the quotient part when a is in range too and the full division otherwise
+ * (which also keeps the sign of a zero quotient and every special value).
+ * The square root likewise: the device's sequence is v_rsq_f64 and Newton-
+ * Goldschmidt steps on an argument scaled by 2^256 below 2^-767, then a
+ * select for +-0 and +inf; em_sqrt skips the scaling and the select for
+ * 2^-700 <= x <= 2^1000.  On the host these are the C operators, which give
+ * the same correctly rounded bits. */
+EM_FN int em_recip_ok(double b) {
+  double ab = em_fabs(b);
+  return ab >= 0x1.0p-64 && ab <= 0x1.0p+64;
+}
+
+EM_FN double em_recip(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-b, y, 1.0);
+  return __builtin_fma(y, e, y);
+#else
+  return 1.0 / b;
+#endif
+}
+
+/* a / b; y = em_recip(b), ok = em_recip_ok(b) */
+EM_FN double em_div(double a, double b, double y, int ok) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double aa = em_fabs(a);
+  if (ok && aa >= 0x1.0p-600 && aa <= 0x1.0p+600) {
+    double q = a * y;
+    double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+  }
+#else
+  (void)y;
+  (void)ok;
+#endif
+  return a / b;
+}
+
+EM_FN double em_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (x >= 0x1.0p-700 && x <= 0x1.0p+1000) {
+    double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, e, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  }
+#endif
+  return __builtin_sqrt(x);
+}
+
 #endif

@@ -12,6 +12,7 @@
 
 #include "eh_consts.h"
 #include "eh_host.h"
+#include "emath.h"
 
 static double sqr(double x) { return x * x; }
 
@@ -453,6 +454,10 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
   op->d[EH_D_RHO0] = rho0;
   op->d[EH_D_RHOA] = rho_actual;
   op->d[EH_D_RHOA_SGN] = rho_actual < 0 ? -1.0 : 1.0;
+  op->d[EH_D_RRHO0] = 1.0 / rho0;
+  op->d[EH_D_RRHOA] = 1.0 / rho_actual;
+  if (em_recip_ok(rho0) && em_recip_ok(rho_actual))
+    op->flags |= EH_F_RCP;
   op->d[EH_D_TANE1] = tan(e1);
   op->d[EH_D_TANE2] = tan(e2);
   op->d[EH_D_E1] = e1;

@@ -80,6 +80,91 @@ def test_reduction_exact_near_multiples_of_pi_over_2():
     assert np.max(ulp_err(c, ref_c)) < 1.0
 
 
+def wide_operands(n, seed):
+    """doubles of every exponent and both signs, the special values, the
+    edges of the unscaled ranges of em_div / em_sqrt and their neighbours,
+    and dividend/divisor pairs whose quotient lies next to a rounding
+    midpoint (a = b (q + ulp/2) rounded, the hardest cases for a quotient)"""
+    rng = np.random.default_rng(seed)
+    bits = rng.integers(0, 2**63, n, dtype=np.int64) | (rng.integers(0, 2, n, dtype=np.int64) << 63)
+    a = bits.view(np.float64).copy()
+    b = (rng.integers(0, 2**63, n, dtype=np.int64) | (rng.integers(0, 2, n, dtype=np.int64) << 63)).view(np.float64).copy()
+    # moderate magnitudes (where the unscaled sequences run) for half of them
+    m = rng.random(n) < 0.5
+    a[m] = rng.uniform(1, 2, m.sum()) * np.exp2(rng.integers(-650, 650, m.sum()))
+    b[m] = rng.uniform(1, 2, m.sum()) * np.exp2(rng.integers(-70, 70, m.sum()))
+    # quotients next to a midpoint
+    k = n // 8
+    q = rng.uniform(1, 2, k)
+    ulp = np.spacing(q)
+    bb = rng.uniform(0.5, 2, k)
+    a[:k] = bb * (q + ulp / 2)
+    b[:k] = bb
+    edges = []
+    for e in (-1074, -1022, -767, -700, -600, -64, 0, 64, 600, 1000, 1023):
+        v = np.float64(2.0) ** e if e > -1074 else np.float64(5e-324)
+        edges += [np.nextafter(v, 0), v, np.nextafter(v, np.inf)]
+    edges = np.array(edges + [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0])
+    ea, eb = np.meshgrid(edges, np.concatenate([edges, -edges]))
+    return np.concatenate([a, ea.ravel()]), np.concatenate([b, eb.ravel()])
+
+
+def _round_p(x, p):
+    """x (a Fraction) rounded to nearest-even in p-bit binary floating point"""
+    if x == 0:
+        return Fraction(0)
+    sgn = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    while Fraction(2) ** e > x:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= x:
+        e += 1
+    ulp = Fraction(2) ** (e - p + 1)
+    m = x / ulp
+    fl = m.numerator // m.denominator
+    rem = m - fl
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2):
+        fl += 1
+    return sgn * fl * ulp
+
+
+@pytest.mark.parametrize("p", [6, 8])
+def test_one_correction_quotient_exhaustive_low_precision(p):
+    """The quotient step em_div runs with a reciprocal rounded to nearest
+    (q = RN(a y), r = RN(a - b q), RN(q + r y) with y = RN(1/b)) is
+    correctly rounded for every pair of p-bit significands, dividends over
+    two binades: the same step in double precision is what the bend radii's
+    host-rounded reciprocals feed."""
+    sig = [Fraction(1) + Fraction(k, 2 ** (p - 1)) for k in range(2 ** (p - 1))]
+    for b in sig:
+        y = _round_p(1 / b, p)
+        for a0 in sig:
+            for a in (a0, 2 * a0):
+                q = _round_p(a * y, p)
+                r = _round_p(a - b * q, p)
+                assert _round_p(q + r * y, p) == _round_p(a / b, p), (p, a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fn,host", [("sqrt_unscaled", "sqrt"), ("div_shared_recip", "div"),
+                                     ("div_rounded_recip", "div")])
+def test_device_unscaled_sequences_match_ieee(tracker, fn, host):
+    """em_sqrt and em_div (emath.h) against the host's correctly rounded
+    operators on every kind of operand: bit-identical, NaN for NaN.  em_div
+    runs with the device's refined reciprocal (divisions by a particle's
+    1 + delta and by fresh denominators) and with 1/b rounded to nearest
+    (the bend radii, whose reciprocals the host rounds)"""
+    x, y = wide_operands(1 << 20, seed=23)
+    if fn == "sqrt_unscaled":
+        x = np.abs(x)
+        x[::7] *= -1
+    gpu = tracker.math_probe({"sqrt_unscaled": 7, "div_shared_recip": 8, "div_rounded_recip": 9}[fn], x, y)
+    cpu = O.math_eval(FN[host], x, y, mode="emath")
+    same = (gpu.view(np.int64) == cpu.view(np.int64)) | (np.isnan(gpu) & np.isnan(cpu))
+    assert same.all(), (fn, int((~same).sum()), x[~same][:4], y[~same][:4])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("fn", ["sin", "cos", "asin", "log", "tan", "sqrt", "div"])
 def test_device_math_bit_identical_to_cpu(tracker, fn):
