@@ -50,6 +50,9 @@
 #define EH_F_MATRIX (1 << 17)  /* RFCA matrix method */
 #define EH_F_HASLEN (1 << 18)  /* nonzero length (drift branches) */
 #define EH_F_RCP (1 << 19)     /* CSBEND: rho0 and rho_actual within em_recip_ok's range (emath.h) */
+#define EH_F_FAST (1 << 20)    /* CSBEND: radii, substep angles and kick lengths admit the branch-free common path */
+#define EH_F_ALONE (1 << 21)   /* single-element call: no limit_amplitudes cull after the element */
+#define EH_F_DFAST (1 << 22)   /* DRIF: 2^-300 <= |L| <= 2^300, the short form of the order-2 drift applies */
 
 /* loss stages recorded with the element index (loss_code = 4*op_index + stage) */
 #define EH_STAGE_ELEMENT 0  /* inside the element routine's own compaction */

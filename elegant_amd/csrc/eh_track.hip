@@ -39,6 +39,15 @@ struct particle {
   double x, xp, y, yp, s, dp;
 };
 
+/* true on every active lane: a scalar test, so the rarely taken side of a
+ * branch costs one uniform jump instead of a divergent if/else */
+#define ALL_LANES(c) (__builtin_amdgcn_ballot_w64(!(c)) == 0)
+
+DEV int imin3(int a, int b, int c) {
+  int m = a < b ? a : b;
+  return m < c ? m : c;
+}
+
 /* ---- frozen rows of lost particles ---- */
 
 /* limit_amplitudes loss (limit_amplitudes.c:232-268): extrapolate along the
@@ -67,6 +76,8 @@ DEV void aperture_loss(particle &p, double xmax, double ymax, int is_out, bool e
 /* the two checks that follow every element (do_tracking.c:2459-2477, 2628-2637).
  * Returns 0 if the particle survives, else stage + 1 with the row frozen. */
 DEV int post_element_checks(particle &p, const eh_op &op) {
+  if (op.flags & EH_F_ALONE)
+    return 0;
   if (op.flags & EH_F_AP) {
     double xmax = op.ap_xmax, ymax = op.ap_ymax;
     int is_out = 0;
@@ -100,8 +111,57 @@ DEV int post_element_checks(particle &p, const eh_op &op) {
 
 /* ---- drifts ---- */
 
-/* order-2 drift matrix: the sparse loop of matrix.c:389-429 with the drift
- * matrix reduces exactly to these sums (R/T zeros only add signed zeros) */
+/* The drift's matrix applied as the sparse order-2 loop of matrix.c:389-429
+ * does it: for each output i (5..0), sum = C[i], then for each input j
+ * (5..0) with coord[j] != 0 (NaN counts as nonzero), sum1 = R[i][j] plus
+ * T[i][j][k] coord[k] for k = j..0, and sum += sum1 coord[j].  For the drift
+ * (R = 1 + L at (0,1) and (2,3), T[4][1][1] = T[4][3][3] = L/2, C[4] = L)
+ * every zero entry still multiplies a coordinate, so an infinite or NaN
+ * coordinate turns others into NaN, and zero sums take the sign the loop
+ * gives them.  drift2_exact is that loop term for term; drift2 is the same
+ * result in a few operations for rows where the zero terms cannot matter. */
+DEV double drift_R(int i, int j, double L) {
+  return i == j ? 1.0 : (((i == 0 && j == 1) || (i == 2 && j == 3)) ? L : 0.0);
+}
+
+DEV void drift2_exact(particle &p, double L, double hL) {
+  const double v[6] = {p.x, p.xp, p.y, p.yp, p.s, p.dp};
+  double out[6];
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    double sum = i == 4 ? L : 0.0;
+#pragma unroll
+    for (int j = 5; j >= 0; j--) {
+      if (v[j] != 0 || v[j] != v[j]) {
+        double sum1 = drift_R(i, j, L);
+#pragma unroll
+        for (int k = j; k >= 0; k--)
+          sum1 += ((i == 4 && ((j == 1 && k == 1) || (j == 3 && k == 3))) ? hL : 0.0) * v[k];
+        sum += sum1 * v[j];
+      }
+    }
+    out[i] = sum;
+  }
+  p.x = out[0];
+  p.xp = out[1];
+  p.y = out[2];
+  p.yp = out[3];
+  p.s = out[4];
+  p.dp = out[5];
+}
+
+/* drift2 equals drift2_exact when every coordinate is finite, |x'| and |y'|
+ * are 0 or within [2^-300, 2^300] and the element's 2^-300 <= |L| <= 2^300
+ * (EH_F_DFAST): then each zero term is a +0 that leaves a sum unchanged
+ * (the sums start at +0 or L), no product of a nonzero slope rounds to zero,
+ * and a zero x', y' or delta comes out as +0 (the loop skips it and keeps
+ * sum = C[i] = +0). */
+DEV bool drift2_ok(const particle &p) {
+  const double fin = (__builtin_fabs(p.x) + __builtin_fabs(p.y)) + (__builtin_fabs(p.s) + __builtin_fabs(p.dp));
+  return fin <= 0x1.0p+1000 && __builtin_fabs(p.xp) + __builtin_fabs(p.yp) <= 0x1.0p+300 &&
+         imin3(__builtin_amdgcn_frexp_exp(p.xp), __builtin_amdgcn_frexp_exp(p.yp), 0) >= -299;
+}
+
 DEV void drift2(particle &p, double L, double hL) {
   double s = L;
   if (p.s != 0)
@@ -119,17 +179,21 @@ DEV void drift2(particle &p, double L, double hL) {
   p.x = x;
   p.y = y;
   p.s = s;
+  p.xp = p.xp + 0.0;
+  p.yp = p.yp + 0.0;
+  p.dp = p.dp + 0.0;
 }
 
-/* order-1 matrix branch (matrix.c:430-455) */
+/* order-1 matrix branch (matrix.c:430-455): sum = C[i] + R[i][j] ini[j] over
+ * j = 5..0 for every output, zeros included */
 DEV void drift1(particle &p, double L) {
-  /* sum = C[i] + sum_j R[i][j] ini[j], j = 5..0 */
-  double x = 0.0 + 0.0 * p.dp + 0.0 * p.s + 0.0 * p.yp + 0.0 * p.y + L * p.xp + 1.0 * p.x;
-  double y = 0.0 + 0.0 * p.dp + 0.0 * p.s + L * p.yp + 1.0 * p.y + 0.0 * p.xp + 0.0 * p.x;
-  double s = L + 0.0 * p.dp + 1.0 * p.s + 0.0 * p.yp + 0.0 * p.y + 0.0 * p.xp + 0.0 * p.x;
-  p.x = x;
-  p.y = y;
-  p.s = s;
+  const double x0 = p.x, xp0 = p.xp, y0 = p.y, yp0 = p.yp, s0 = p.s, dp0 = p.dp;
+  p.dp = 0.0 + 1.0 * dp0 + 0.0 * s0 + 0.0 * yp0 + 0.0 * y0 + 0.0 * xp0 + 0.0 * x0;
+  p.s = L + 0.0 * dp0 + 1.0 * s0 + 0.0 * yp0 + 0.0 * y0 + 0.0 * xp0 + 0.0 * x0;
+  p.yp = 0.0 + 0.0 * dp0 + 0.0 * s0 + 1.0 * yp0 + 0.0 * y0 + 0.0 * xp0 + 0.0 * x0;
+  p.y = 0.0 + 0.0 * dp0 + 0.0 * s0 + L * yp0 + 1.0 * y0 + 0.0 * xp0 + 0.0 * x0;
+  p.xp = 0.0 + 0.0 * dp0 + 0.0 * s0 + 0.0 * yp0 + 0.0 * y0 + 1.0 * xp0 + 0.0 * x0;
+  p.x = 0.0 + 0.0 * dp0 + 0.0 * s0 + 0.0 * yp0 + 0.0 * y0 + L * xp0 + 1.0 * x0;
 }
 
 DEV void exact_drift(particle &p, double L) {
@@ -155,6 +219,7 @@ DEV void rotate(particle &p, double c, double s) {
 }
 
 /* ---- multipoles ---- */
+
 
 DEV bool in_aperture(double x, double y, const eh_op &op) {
   if (!(op.flags & EH_F_INAP))
@@ -290,6 +355,107 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
   return 0;
 }
 
+/* multipole_body on its common path (EXPAND_HAMILTONIAN=0), with no
+ * per-lane branches: the unscaled square roots and quotients by positive
+ * divisors (emath.h em_sqrt_mid, em_div_pos), no loss.  Returns false for a
+ * lane where that may not hold; the caller then recomputes the lane from
+ * the row it received with the general code.  Checked per substep, into
+ * one lane mask:
+ *   - 1 + x'^2 + y'^2 <= 2^1000 for the drift's square root (a NaN fails);
+ *   - after the kicks, |qx| + |qy| <= 2^700 and qx, qy are 0 or >= 2^-700
+ *     in magnitude (binary exponent >= -699), so every dividend below is 0
+ *     or within [2^-900, 2^900];
+ *   - (1 + delta)^2 - qx^2 - qy^2 >= 2^-128 at both conversions to slopes:
+ *     no slope loss, the root lies in [2^-64, 2^64];
+ *   - 1 + delta in [2^-64, 2^64]; the aperture holds at every slice.
+ * dzLoss is not accumulated: the caller sets the full length's sum.
+ * Arithmetic is otherwise multipole_body's, operation for operation. */
+DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
+                             const double dp, const opd_recip &o, const eh_op &op) {
+  const int nsub = op.nsub;
+  const int flags = op.flags;
+  const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
+  const bool has_drift = op.d[EH_D_DRIFT] != 0;
+  const int nko = op.nkick_orders;
+  bool good = o.v >= 0x1.0p-64 && o.v <= 0x1.0p+64;
+  for (int ik = 0; ik < op.nslices; ik++) {
+    good = good && in_aperture(x + dxa, y + dya, op);
+    for (int step = 0; step < nsub; step++) {
+      if (has_drift) {
+        const double dsh = opc(&op.d[EH_D_DSH + step]);
+        x += xp * dsh;
+        y += yp * dsh;
+        const double t = 1 + sq(xp) + sq(yp);
+        good = good && t <= 0x1.0p+1000;
+        s += dsh * em_sqrt_mid(t);
+      }
+      if (step == nsub - 1)
+        break; /* kickFrac[last] == 0 */
+      if (nko > 0)
+        apply_kick(qx, qy, op.order0, op.skew0, opc(&op.d[EH_D_KICK + step]), x, y);
+      if (nko > 1)
+        apply_kick(qx, qy, op.order1, op.skew1, opc(&op.d[EH_D_KICK + 8 + step]), x, y);
+      if (nko > 2)
+        apply_kick(qx, qy, op.order2, op.skew2, opc(&op.d[EH_D_KICK + 16 + step]), x, y);
+      if (flags & EH_F_XKICK)
+        apply_kick(qx, qy, 0, 0, opc(&op.d[EH_D_XK + step]), x, y);
+      if (flags & EH_F_YKICK)
+        apply_kick(qx, qy, 0, 1, opc(&op.d[EH_D_YK + step]), x, y);
+      good = good && __builtin_fabs(qx) + __builtin_fabs(qy) <= 0x1.0p+700 &&
+             imin3(__builtin_amdgcn_frexp_exp(qx), __builtin_amdgcn_frexp_exp(qy), 0) >= -699;
+      double den = sq(1 + dp) - sq(qx) - sq(qy);
+      double dmin = den;
+      den = em_sqrt_mid(den);
+      double r = em_recip(den);
+      xp = em_div_pos(qx, den, r);
+      yp = em_div_pos(qy, den, r);
+      if (has_drift) {
+        /* rad_coef = 0, isr_coef < 0: the q/(1+dp)*(1+dp) round trip (multipole.c:1321-1351) */
+        qx = em_div_pos(qx, o.v, o.r);
+        qy = em_div_pos(qy, o.v, o.r);
+        qx *= (1 + dp);
+        qy *= (1 + dp);
+        den = sq(1 + dp) - sq(qx) - sq(qy);
+        dmin = __builtin_fmin(dmin, den);
+        den = em_sqrt_mid(den);
+        r = em_recip(den);
+        xp = em_div_pos(qx, den, r);
+        yp = em_div_pos(qy, den, r);
+      }
+      good = good && dmin >= 0x1.0p-128;
+    }
+  }
+  return good;
+}
+
+/* the integrator's entry (multipole.c:1176-1205): limits, slopes to
+ * momenta and back.  Returns 0, or 2 for a slope loss (dzLoss set). */
+DEV int multipole_entry(const particle &p, double &xp, double &yp, double &qx, double &qy, double &dzLoss,
+                        const bool expand, const opd_recip &o, const eh_op &op) {
+  const double dp = p.dp;
+  xp = p.xp;
+  yp = p.yp;
+  if (__builtin_isnan(p.x) || __builtin_isnan(xp) || __builtin_isnan(p.y) || __builtin_isnan(yp) ||
+      __builtin_fabs(p.x) > EH_COORD_LIMIT || __builtin_fabs(p.y) > EH_COORD_LIMIT ||
+      __builtin_fabs(xp) > EH_SLOPE_LIMIT || __builtin_fabs(yp) > EH_SLOPE_LIMIT) {
+    dzLoss = op.d[EH_D_TOTLEN];
+    return 2;
+  }
+  if (expand) {
+    qx = (1 + dp) * xp;
+    qy = (1 + dp) * yp;
+  } else {
+    double denom = em_sqrt(1 + sq(xp) + sq(yp));
+    qx = (1 + dp) * xp / denom;
+    qy = (1 + dp) * yp / denom;
+  }
+  if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o)) {
+    dzLoss = op.d[EH_D_TOTLEN];
+    return 2;
+  }
+  return 0;
+}
+
 /* multipole_tracking2 per particle: entry misalignment, the integrator of
  * integrate_kick_multipole_ordn (multipole.c:1122-1410), exit misalignment.
  * Returns 0 if alive, EH_STAGE_ELEMENT + 1 if lost (row frozen). */
@@ -304,31 +470,26 @@ DEV int track_multipole(particle &p, const eh_op &op) {
   /* the row as the integrator receives it stays in p until the end */
   const double dp = p.dp;
   const opd_recip o = make_opd(dp);
-  double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, qx = 0, qy = 0, dzLoss = 0;
-  int lost = 0; /* 1: aperture (x, y written), 2: slopes (nothing written), 3: final limits */
+  double x = p.x, xp = 0, y = p.y, yp = 0, s = 0, qx = 0, qy = 0, dzLoss = 0;
   const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
-
-  if (__builtin_isnan(x) || __builtin_isnan(xp) || __builtin_isnan(y) || __builtin_isnan(yp) ||
-      __builtin_fabs(x) > EH_COORD_LIMIT || __builtin_fabs(y) > EH_COORD_LIMIT ||
-      __builtin_fabs(xp) > EH_SLOPE_LIMIT || __builtin_fabs(yp) > EH_SLOPE_LIMIT) {
-    lost = 2;
-    dzLoss = op.d[EH_D_TOTLEN];
-  } else {
-    if (expand) {
-      qx = (1 + dp) * xp;
-      qy = (1 + dp) * yp;
-    } else {
-      double denom = em_sqrt(1 + sq(xp) + sq(yp));
-      qx = (1 + dp) * xp / denom;
-      qy = (1 + dp) * yp / denom;
-    }
-    if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o)) {
-      lost = 2;
-      dzLoss = op.d[EH_D_TOTLEN];
-    }
-  }
+  /* 1: aperture (x, y written), 2: slopes (nothing written), 3: final limits */
+  int lost = multipole_entry(p, xp, yp, qx, qy, dzLoss, expand, o, op);
   if (!lost) {
-    lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, o, op);
+    bool done = false;
+    if (!expand) {
+      done = multipole_body_fast(x, xp, y, yp, s, qx, qy, dp, o, op);
+      /* the checks after the integrator may still lose the particle: at the
+       * element's full length, the sum the general code would have made */
+      dzLoss = done ? op.d[EH_D_TOTLEN] : 0.0;
+      if (!ALL_LANES(done) && !done) {
+        x = p.x;
+        y = p.y;
+        s = 0;
+        multipole_entry(p, xp, yp, qx, qy, dzLoss, expand, o, op);
+      }
+    }
+    if (!ALL_LANES(done) && !done)
+      lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, o, op);
     if (!lost) {
       if (!in_aperture(x + dxa, y + dya, op))
         lost = 1;
@@ -428,6 +589,88 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
   return false;
 }
 
+
+/* csbend_body on its common path, with no per-lane branches: every substep
+ * takes the narrow bands of asin/sin/cos (z < 2^-10, emath.h *_small), the
+ * unscaled quotients by positive divisors (em_div_pos) and the unscaled
+ * square root, and no loss occurs.  Returns false for a lane where any of
+ * that may not hold; the caller then discards the lane's results and runs
+ * csbend_body from the saved state.  What makes every step valid, checked
+ * once at the end from running extrema (which add no branches):
+ *   - |X| + |QX| + |QY| <= 2^700 at every substep (a NaN or infinity in the
+ *     state fails it, and the state is the only source of one);
+ *   - X, QX, QY are 0 or >= 2^-700 in magnitude at every substep (binary
+ *     exponent >= -699); then every dividend below is 0 or within
+ *     [2^-900, 2^900]: sums of such terms with the radii or with 1 are 0 or
+ *     at least an ulp of 2^-764;
+ *   - opd^2 - QY^2 >= 2^-128: the square root is in range, f lies in
+ *     [2^-64, 2^64] and is positive, and the f <= 0 loss cannot occur;
+ *   - sin(phi)^2, (angle)^2 and phi^2 < 2^-10 (the narrow bands; also rules
+ *     out |sin(phi)| > 1 and cos(angle) = 0), |angle| >= 2^-900, which given
+ *     the rest means angle != 0 (em_sincos's zero case), so cos(angle) and
+ *     cos(phi) exceed 0.999 (this bound and the one above share a running
+ *     minimum, of (opd^2 - QY^2) 2^-772 and |angle|);
+ *   - dz_lost is not accumulated: it only matters for a lost particle, and
+ *     the fast path never loses one;
+ *   - 1 + delta in [2^-64, 2^64]; the radii, angles and kick lengths are
+ *     checked on the host (EH_F_FAST);
+ *   - the aperture holds at every slice.
+ * Arithmetic is otherwise csbend_body's, operation for operation. */
+DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const double DP, double &dist,
+                          const eh_op &op) {
+  const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA], rsgn = op.d[EH_D_RHOA_SGN];
+  const double opd = 1 + DP;
+  const int nsub = op.nsub;
+  bool good = opd >= 0x1.0p-64 && opd <= 0x1.0p+64;
+  const double r_opd = em_recip(opd);
+  double acc = 0, zmax = 0, lo = 1;
+  int emin = 0;
+  for (int i = 0; i < op.nslices; i++) {
+    good = good && in_aperture(X, Y, op);
+    for (int j = 0; j < nsub; j++) {
+      acc += (__builtin_fabs(X) + __builtin_fabs(QX)) + __builtin_fabs(QY);
+      emin = imin3(emin, __builtin_amdgcn_frexp_exp(X), imin3(emin, __builtin_amdgcn_frexp_exp(QX),
+                                                              __builtin_amdgcn_frexp_exp(QY)));
+      double f = sq(opd) - sq(QY);
+      const double f_in = f;
+      f = em_sqrt_mid(f);
+      const double r_f = em_recip(f);
+      double sin_phi = em_div_pos(QX, f, r_f);
+      const double zs = sin_phi * sin_phi;
+      double phi = em_asin_small(sin_phi, zs);
+      const double ang = opc(&op.d[EH_D_BDSHR + j]) + phi;
+      const double za = ang * ang;
+      lo = __builtin_fmin(lo, __builtin_fmin(f_in * 0x1.0p-772, __builtin_fabs(ang)));
+      double sine = em_sin_small(ang, za), cosi = em_cos_small(ang, za);
+      const double r_c = em_recip(cosi);
+      double tang = em_div_pos(sine, cosi, r_c);
+      const double zp = phi * phi;
+      zmax = __builtin_fmax(zmax, __builtin_fmax(zs, __builtin_fmax(za, zp)));
+      double cos_phi = em_cos_small(phi, zp);
+      QX = f * sine;
+      double factor = em_div_pos((rho0 + X) * cos_phi, f, r_f) *
+                      (tang - em_div_pos(sin_phi, cos_phi, em_recip(cos_phi)));
+      Y += QY * factor;
+      dist += factor * opd;
+      f = em_div_pos(cos_phi, cosi, r_c);
+      X = rho0 * (f - 1) + f * X;
+      if (j == nsub - 1)
+        break; /* kickFrac[last] == 0 */
+      double ds = opc(&op.d[EH_D_BDS + j]);
+      double g = 1 + em_div_pos(X, rho0, opc(&op.d[EH_D_RRHO0]));
+      QX += em_div_pos(-ds * g * 1.0, rhoa, opc(&op.d[EH_D_RRHOA]));
+      QY += ds * g * 0.0 * rsgn;
+      QX = em_div_pos(QX, opd, r_opd);
+      QY = em_div_pos(QY, opd, r_opd);
+      QX *= opd;
+      QY *= opd;
+    }
+  }
+  /* the last substep's X and the last kick's QX, QY need no check: no
+   * quotient takes them before the next element re-checks */
+  return good && acc <= 0x1.0p+700 && emin >= -699 && lo >= 0x1.0p-900 && zmax < EM_SMALL_Z;
+}
+
 DEV int track_csbend(particle &p, const eh_op &op) {
   const int flags = op.flags;
   const double ct = op.d[EH_D_CT], st = op.d[EH_D_ST];
@@ -461,8 +704,20 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   }
   /* integrate_csbend_ordn (csbend.c:1434-1693), Fx = 0, Fy = 1 */
   double dist = 0, dz_lost = 0;
-  bool lost;
-  lost = csbend_body(X, QX, Y, QY, DP, dist, dz_lost, op);
+  bool lost = false, done = false;
+  if (flags & EH_F_FAST) {
+    const double X0 = X, QX0 = QX, Y0 = Y, QY0 = QY;
+    done = csbend_body_fast(X, QX, Y, QY, DP, dist, op);
+    if (!ALL_LANES(done) && !done) {
+      X = X0;
+      QX = QX0;
+      Y = Y0;
+      QY = QY0;
+      dist = 0;
+    }
+  }
+  if (!ALL_LANES(done) && !done)
+    lost = csbend_body(X, QX, Y, QY, DP, dist, dz_lost, op);
   if (!lost) {
     dz_lost = op.d[EH_D_NS];
     if (!in_aperture(X, Y, op))
@@ -642,7 +897,10 @@ DEV int run_op(particle &p, const eh_op &op) {
   int lost = 0;
   switch (op.op) {
   case EH_OP_DRIFT2:
-    drift2(p, op.d[0], op.d[1]);
+    if ((op.flags & EH_F_DFAST) && ALL_LANES(drift2_ok(p)))
+      drift2(p, op.d[0], op.d[1]);
+    else
+      drift2_exact(p, op.d[0], op.d[1]);
     break;
   case EH_OP_DRIFT1:
     drift1(p, op.d[0]);
@@ -676,7 +934,7 @@ DEV int run_op(particle &p, const eh_op &op) {
  * dependency and scalar-load latency; the registers the compiler then has
  * to spill belong to rarely taken paths (measured: the storage ring runs
  * 2.5 % faster than at five waves). */
-__global__ void __launch_bounds__(256, 6) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
+__global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
                                                        int64_t pass0, int32_t npasses, int32_t op_first,
                                                        int32_t op_last) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

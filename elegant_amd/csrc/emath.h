@@ -130,13 +130,19 @@ EM_FN int em_reduce_pio2(double x, double *rh, double *rl) {
 #define EM_S1 (-0x1.5555555555555p-3)
 #define EM_SMALL_Z 0x1.0p-10
 
-/* (sin(x) - x - S1 x^3) / x^5 as a polynomial in z = x*x */
+/* (sin(x) - x - S1 x^3) / x^5 as a polynomial in z = x*x; the narrow band
+ * is its own function so that branch-free callers evaluate the same bits */
+EM_FN double em_sin_r_small(double z) {
+  const double s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13, s4 = 0x1.71de3a556c734p-19;
+  return __builtin_fma(z, __builtin_fma(z, em_k(s4), em_k(s3)), em_k(s2));
+}
+
 EM_FN double em_sin_r(double z) {
   const double s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13, s4 = 0x1.71de3a556c734p-19,
                s5 = -0x1.ae64567f544e4p-26, s6 = 0x1.6124613a86d09p-33, s7 = -0x1.ae7f3e733b81fp-41,
                s8 = 0x1.952c77030ad4ap-49;
   if (z < EM_SMALL_Z)
-    return __builtin_fma(z, __builtin_fma(z, em_k(s4), em_k(s3)), em_k(s2));
+    return em_sin_r_small(z);
   double r = __builtin_fma(z, em_k(s8), em_k(s7));
   r = __builtin_fma(z, r, em_k(s6));
   r = __builtin_fma(z, r, em_k(s5));
@@ -146,12 +152,17 @@ EM_FN double em_sin_r(double z) {
 }
 
 /* (cos(x) - 1 + z/2) / z^2 as a polynomial in z = x*x */
+EM_FN double em_cos_r_small(double z) {
+  const double c2 = 0x1.5555555555555p-5, c3 = -0x1.6c16c16c16c17p-10, c4 = 0x1.a01a01a01a01ap-16;
+  return __builtin_fma(z, __builtin_fma(z, em_k(c4), em_k(c3)), em_k(c2));
+}
+
 EM_FN double em_cos_r(double z) {
   const double c2 = 0x1.5555555555555p-5, c3 = -0x1.6c16c16c16c17p-10, c4 = 0x1.a01a01a01a01ap-16,
                c5 = -0x1.27e4fb7789f5cp-22, c6 = 0x1.1eed8eff8d898p-29, c7 = -0x1.93974a8c07c9dp-37,
                c8 = 0x1.ae7f3e733b81fp-45;
   if (z < EM_SMALL_Z)
-    return __builtin_fma(z, __builtin_fma(z, em_k(c4), em_k(c3)), em_k(c2));
+    return em_cos_r_small(z);
   double r = __builtin_fma(z, em_k(c8), em_k(c7));
   r = __builtin_fma(z, r, em_k(c6));
   r = __builtin_fma(z, r, em_k(c5));
@@ -186,6 +197,19 @@ EM_FN double em_sin_kernel0(double x) {
 EM_FN double em_cos_kernel0(double x) {
   double z = x * x;
   double r = z * z * em_cos_r(z);
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  return w + (((1.0 - w) - hz) + r);
+}
+
+/* the kernels on the narrow band, z = x*x < 2^-10: em_sin(x) for x != 0 and
+ * em_cos(x) there, term for term */
+EM_FN double em_sin_small(double x, double z) {
+  return x + (z * x) * __builtin_fma(z, em_sin_r_small(z), em_k(EM_S1));
+}
+
+EM_FN double em_cos_small(double x, double z) {
+  double r = z * z * em_cos_r_small(z);
   double hz = 0.5 * z;
   double w = 1.0 - hz;
   return w + (((1.0 - w) - hz) + r);
@@ -263,6 +287,17 @@ EM_FN double em_tan(double x) {
   return s / c;
 }
 
+/* the five-term band of em_asin_poly_small */
+EM_FN double em_asin_poly5(double z) {
+  const double a1 = 0x1.5555555555555p-3, a2 = 0x1.3333333333333p-4, a3 = 0x1.6db6db6db6db7p-5,
+               a4 = 0x1.f1c71c71c71c7p-6, a5 = 0x1.6e8ba2e8ba2e9p-6;
+  double p = __builtin_fma(z, em_k(a5), em_k(a4));
+  p = __builtin_fma(z, p, em_k(a3));
+  p = __builtin_fma(z, p, em_k(a2));
+  p = __builtin_fma(z, p, em_k(a1));
+  return z * p;
+}
+
 /* (asin(x) - x)/x as a polynomial in z = x*x: five terms for z < 2^-10
  * (next term below 2^-65 relative), nine for z <= 1/64 */
 EM_FN double em_asin_poly_small(double z) {
@@ -270,20 +305,21 @@ EM_FN double em_asin_poly_small(double z) {
                a4 = 0x1.f1c71c71c71c7p-6, a5 = 0x1.6e8ba2e8ba2e9p-6, a6 = 0x1.1c4ec4ec4ec4fp-6,
                a7 = 0x1.c99999999999ap-7, a8 = 0x1.7a87878787878p-7, a9 = 0x1.3fde50d79435ep-7;
   double p;
-  if (z < EM_SMALL_Z) {
-    p = __builtin_fma(z, em_k(a5), em_k(a4));
-  } else {
-    p = __builtin_fma(z, em_k(a9), em_k(a8));
-    p = __builtin_fma(z, p, em_k(a7));
-    p = __builtin_fma(z, p, em_k(a6));
-    p = __builtin_fma(z, p, em_k(a5));
-    p = __builtin_fma(z, p, em_k(a4));
-  }
+  if (z < EM_SMALL_Z)
+    return em_asin_poly5(z);
+  p = __builtin_fma(z, em_k(a9), em_k(a8));
+  p = __builtin_fma(z, p, em_k(a7));
+  p = __builtin_fma(z, p, em_k(a6));
+  p = __builtin_fma(z, p, em_k(a5));
+  p = __builtin_fma(z, p, em_k(a4));
   p = __builtin_fma(z, p, em_k(a3));
   p = __builtin_fma(z, p, em_k(a2));
   p = __builtin_fma(z, p, em_k(a1));
   return z * p;
 }
+
+/* em_asin(x) for z = x*x < 2^-10 */
+EM_FN double em_asin_small(double x, double z) { return __builtin_fma(x, em_asin_poly5(z), x); }
 
 /* same series, long form for z <= 1/4 (25 terms) */
 EM_FN double em_asin_poly_large(double z) {
@@ -385,7 +421,11 @@ EM_FN double em_recip(double b) {
   double e = __builtin_fma(-b, y, 1.0);
   y = __builtin_fma(y, e, y);
   e = __builtin_fma(-b, y, 1.0);
-  return __builtin_fma(y, e, y);
+  y = __builtin_fma(y, e, y);
+  /* computed once here: without the barrier the compiler may sink the steps
+   * into the branch of each quotient and repeat them */
+  asm volatile("" : "+v"(y));
+  return y;
 #else
   return 1.0 / b;
 #endif
@@ -407,19 +447,42 @@ EM_FN double em_div(double a, double b, double y, int ok) {
   return a / b;
 }
 
+/* The same sequences for callers that have established the ranges
+ * themselves: em_div_pos for 2^-64 <= b <= 2^64 (positive) and a = 0 or
+ * 2^-900 <= |a| <= 2^900, written with the negated residual so that a zero
+ * dividend keeps its sign (the residual is exact, so the quotient is the
+ * same as em_div's otherwise); em_sqrt_mid for 2^-700 <= x <= 2^1000. */
+EM_FN double em_div_pos(double a, double b, double y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double q = a * y;
+  double r = __builtin_fma(b, q, -a);
+  return __builtin_fma(-r, y, q);
+#else
+  (void)y;
+  return a / b;
+#endif
+}
+
+EM_FN double em_sqrt_mid(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = r * 0.5;
+  double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  double d = __builtin_fma(-g, g, x);
+  h = __builtin_fma(h, e, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+#else
+  return __builtin_sqrt(x);
+#endif
+}
+
 EM_FN double em_sqrt(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (x >= 0x1.0p-700 && x <= 0x1.0p+1000) {
-    double r = __builtin_amdgcn_rsq(x);
-    double g = x * r, h = r * 0.5;
-    double e = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, e, g);
-    double d = __builtin_fma(-g, g, x);
-    h = __builtin_fma(h, e, h);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
-  }
+  if (x >= 0x1.0p-700 && x <= 0x1.0p+1000)
+    return em_sqrt_mid(x);
 #endif
   return __builtin_sqrt(x);
 }

@@ -486,6 +486,19 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
       op->d[EH_D_BKF + j] = kf[j];
     }
   }
+  /* the branch-free common path of the kernel needs positive radii within
+   * the shared-reciprocal range, substep angles that are 0 or moderate and
+   * below 2^-5, and kick lengths that are 0 or moderate */
+  if (rho0 > 0 && rho_actual > 0 && em_recip_ok(rho0) && em_recip_ok(rho_actual)) {
+    int fast = 1;
+    for (int j = 0; j < nsub; j++) {
+      double a = fabs(op->d[EH_D_BDSHR + j]), k = fabs(op->d[EH_D_BDS + j]);
+      if (!(a == 0 || (a >= 0x1.0p-700 && a < 0x1.0p-5)) || !(k == 0 || em_recip_ok(k)))
+        fast = 0;
+    }
+    if (fast)
+      op->flags |= EH_F_FAST;
+  }
   set_in_aperture(op, -tilt, maxamp);
   return EH_OK;
 }
@@ -591,6 +604,8 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
       op.d[0] = e->p.drift.length;
       op.d[1] = e->p.drift.length / 2;
       op.flags |= EH_F_EXTRAP;
+      if (fabs(op.d[0]) >= 0x1.0p-300 && fabs(op.d[0]) <= 0x1.0p+300)
+        op.flags |= EH_F_DFAST;
       break;
     }
     case EH_T_EDRIFT:

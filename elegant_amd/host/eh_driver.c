@@ -230,9 +230,11 @@ static int alloc_particles(eh_ctx *c, int64_t n) {
   return EH_OK;
 }
 
-/* AoS rows -> SoA, with do_tracking's initial NaN/Inf cull (do_tracking.c:468-473):
- * such rows are recorded as lost on pass 0 before the first element. */
-int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) {
+/* AoS rows -> SoA, with do_tracking's initial NaN/Inf cull (do_tracking.c:468-473)
+ * when cull is set: such rows are recorded as lost on pass 0 before the first
+ * element.  The single-element entry points upload without it, as the
+ * element routines leave such rows to their caller. */
+static int upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cull) {
   int rc;
   double *soa;
   if (!c || (!coord && n > 0) || n < 0 || nprops < 7)
@@ -254,7 +256,7 @@ int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) {
       c->ids[i] = r[6];
       c->h_pass[i] = -1;
       c->h_code[i] = 0;
-      if (isinf(r[0]) || isinf(r[2]) || isnan(r[0]) || isnan(r[2])) {
+      if (cull && (isinf(r[0]) || isinf(r[2]) || isnan(r[0]) || isnan(r[2]))) {
         /* limit_amplitudes(DBL_MAX) at z = 0: s = 0, delta = Po(1+delta) */
         soa[4 * n + i] = 0.0;
         soa[5 * n + i] = c->Po * (1 + r[5]);
@@ -274,6 +276,8 @@ int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) {
     return devfail(c, rc);
   return EH_OK;
 }
+
+int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) { return upload_rows(c, coord, n, nprops, 1); }
 
 /* ----------------------------------------------------- reference order */
 
@@ -661,20 +665,21 @@ static int64_t run_single(eh_ctx *c, double **rows, int64_t n, const eh_element 
   rc = eh_compile(bl, nb, Po, c->prog, c->msg, sizeof(c->msg));
   if (rc)
     return rc;
-  /* keep only the element itself, without the checks do_tracking adds after it */
+  /* keep only the element itself, without the checks do_tracking adds after
+   * it (the MAXAMP aperture and the NaN/infinity cull of limit_amplitudes):
+   * the element routines leave such rows to their caller */
   c->prog->ops[0] = c->prog->ops[c->prog->nops - 1];
   c->prog->nops = 1;
   c->prog->ops[0].flags &= ~EH_F_AP;
-  c->prog->ops[0].flags |= 0;
+  c->prog->ops[0].flags |= EH_F_ALONE;
   c->prog->ops[0].z_start = z_start;
   c->prog->ops[0].z_end = z_start + eh_element_length(e);
-  c->prog->ops[0].op = c->prog->ops[0].op;
   c->Po = Po;
   c->loaded = 1;
   rc = ehd_put_ops(c->dev, c->prog->ops, 1);
   if (rc)
     return devfail(c, rc);
-  rc = eh_upload(c, rows, n, 8);
+  rc = upload_rows(c, rows, n, 8, 0);
   if (rc)
     return rc;
   rc = ehd_track(c->dev, pass, 1, 0, 1);
@@ -843,7 +848,8 @@ int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax,
     rc = ehd_put_ops(c->dev, c->prog->ops, 1);
     if (rc)
       return devfail(c, rc);
-    rc = eh_upload(c, coord, np, 8);
+    /* NaN and infinite rows are limit_amplitudes' own to remove, at z */
+    rc = upload_rows(c, coord, np, 8, 0);
     if (rc)
       return rc;
     rc = ehd_track(c->dev, 0, 1, 0, 1);

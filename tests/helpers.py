@@ -28,6 +28,17 @@ def lossy_ring(n, seed, lattice_seed=1):
     return bl, Po, rows
 
 
+def assert_rows_bitwise(a, b, what=""):
+    """Bit-identical rows including the sign of zero (any NaN equals any NaN)."""
+    assert a.shape == b.shape
+    same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        bad = np.argwhere(~same)
+        i, j = bad[0]
+        raise AssertionError("%s: %d values differ; first row %d col %d: %r vs %r (ids %r %r)" % (
+            what, len(bad), i, j, a[i, j], b[i, j], a[i, 6], b[i, 6]))
+
+
 def assert_rows_identical(a, b, what=""):
     """Bit-identical rows (signed zeros compare equal, NaN equals NaN)."""
     assert a.shape == b.shape

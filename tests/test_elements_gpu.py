@@ -12,7 +12,7 @@ from elegant_amd import _abi as A
 from elegant_amd import elements as E
 from elegant_amd import lattices as Lt
 from oracle import oracle_ctypes as O
-from tests.helpers import assert_rows_identical
+from tests.helpers import assert_rows_bitwise, assert_rows_identical
 
 pytestmark = pytest.mark.gpu
 
@@ -57,6 +57,33 @@ def test_multipole_tracking2(tracker, make):
     assert_rows_identical(g[:, :7], c[:, :7], "multipole")
 
 
+@pytest.mark.parametrize("make,aperture", [
+    (lambda: E.KQUAD(L=0.4, K1=1.8, N_KICKS=8, DX=2e-4, DY=-1e-4, TILT=3e-3), True),
+    (lambda: E.KQUAD(L=0.4, K1=-1.8, N_KICKS=8), False),
+    (lambda: E.KSEXT(L=0.2, K2=300.0, N_KICKS=4, DX=1e-4), False),
+    (lambda: E.KOCT(L=0.0, K3=4e4), False),
+    (lambda: E.KQUAD(L=0.4, K1=1.8, N_KICKS=8, EXPAND_HAMILTONIAN=1), False),
+])
+def test_multipole_fast_path_edges(tracker, make, aperture):
+    """multipole_tracking2 with rows of every kind in the same waves as
+    ordinary ones (the bend's edge set: zeros of both signs, tiny, huge,
+    NaN and infinite values, slopes that end the particle): the branch-free
+    common path and the general code together match the checker bit for
+    bit, signs of zeros included; a thin element and an expanded
+    Hamiltonian take the general code"""
+    e = make()
+    ap = A.EH_MAXAMP(0.05, 0.04, 0, 2, 2, 0) if aperture else None
+    r = csbend_edge_rows(1000, 21)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.multipole_tracking2(g, e, PO, 12.5, ap)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_multipole_tracking2(ptr, c.shape[0], C.byref(e), PO, 12.5,
+                                                C.byref(ap) if ap is not None else None)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "multipole edges")
+
+
 def test_track_through_csbend(tracker):
     b = E.CSBEND(L=1.0, ANGLE=0.08, N_KICKS=16, E1=0.02, E2=0.04, DX=1e-4, TILT=1e-3).p.csbend
     ap = A.EH_MAXAMP(0.005, 0.004, 0, 2, 2, 0)
@@ -68,6 +95,58 @@ def test_track_through_csbend(tracker):
     nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0, C.byref(ap))
     assert ng == nc and 0 < nc < r.shape[0]
     assert_rows_identical(g[:, :7], c[:, :7], "csbend")
+
+
+def assert_same_survivors(r, g, ng, c, nc):
+    """the same particles survive on both sides (names the rows that differ)"""
+    kept_g = set(g[:ng, 6].astype(int).tolist())
+    kept_c = set(c[:nc, 6].astype(int).tolist())
+    diff = sorted(kept_g ^ kept_c)
+    assert not diff, "survivors differ (ids, initial rows, kept on the GPU): %r" % [
+        (i, r[i - 1, :6].tolist(), i in kept_g) for i in diff[:6]]
+    assert ng == nc and 0 < nc < r.shape[0]
+
+
+def csbend_edge_rows(n, seed):
+    """an ordinary beam with, every 7th row, a case the bend's branch-free
+    path must either reproduce bit for bit or hand to the general code:
+    exact and negative zeros, tiny and subnormal values, slopes outside the
+    narrow polynomial bands, momenta that end the particle inside the bend,
+    1 + delta at or near 0, huge offsets, NaN and infinities"""
+    r = beam(n, seed, halo=0.0)
+    nan, inf = float("nan"), float("inf")
+    cases = [
+        (0, 0, 0, 0, 0), (-0.0, -0.0, -0.0, -0.0, -0.0), (0, 1e-4, 0, 0, 0), (1e-3, 0, 0, -0.0, 0),
+        (1e-300, 1e-310, 0, -1e-300, 0), (5e-324, -5e-324, 1e-320, 0, 0), (0, 0.2, 0, 0, 0),
+        (0, -0.9, 0, 0, 0), (0, 0, 0, 2.0, 0), (0, 1.5, 0, 0, 0), (1e-3, 1e-4, 0, 0, -1.0),
+        (1e-3, 1e-4, 0, 0, -0.9999999), (1e200, 0, 0, 0, 0), (1e300, 0, 0, 0, 0), (-12.7, 0, 0, 0, 0),
+        (nan, 0, 0, 0, 0), (0, inf, 0, 0, 0), (0, 0, -inf, 0, 0), (1e-3, 0, 0, 0, nan), (0, 0, 0, 0, 1e6),
+        (0.03, 0.0312, 0, 0, 0), (0, 0.03125, 0, 0.0, 0), (2e-3, -2e-3, 1e-3, 1e-3, 0.05),
+    ]
+    for k, (x, xp, y, yp, dp) in enumerate(cases):
+        i = 7 * k + 3
+        r[i, [0, 1, 2, 3, 5]] = (x, xp, y, yp, dp)
+    return r
+
+
+@pytest.mark.parametrize("angle,aperture", [(0.08, True), (0.08, False), (-0.08, False), (0.3, False)])
+def test_csbend_fast_path_edges(tracker, angle, aperture):
+    """CSBEND rows of every kind in the same waves as ordinary ones: the
+    branch-free common path and the general code together must match the
+    checker bit for bit, signs of zeros included (a negative bend and a
+    strong one exercise the general code for whole elements and for most
+    substeps)"""
+    b = E.CSBEND(L=1.0, ANGLE=angle, N_KICKS=16, E1=0.02, E2=0.04, DX=1e-4, TILT=1e-3).p.csbend
+    ap = A.EH_MAXAMP(0.05, 0.04, 0, 2, 2, 0) if aperture else None
+    r = csbend_edge_rows(1000, 12)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.track_through_csbend(g, b, PO, 3.0, ap)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0,
+                                                 C.byref(ap) if ap is not None else None)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "csbend edges angle %g" % angle)
 
 
 def test_simple_rf_cavity_two_calls(tracker):
@@ -107,6 +186,23 @@ def test_drift_and_limit_amplitudes(tracker):
     nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1)
     assert ng == nc and 0 < nc < r.shape[0]
     assert_rows_identical(g[:, :7], c[:, :7], "limit_amplitudes")
+
+
+def test_drift_and_limit_amplitudes_edges(tracker):
+    """the element entry points leave NaN and infinite rows to limit_amplitudes,
+    which removes them at its own z (bit for bit, signs of zeros included)"""
+    d = E.DRIF(L=1.3).p.drift
+    r = csbend_edge_rows(1000, 8)
+    g = r.copy()
+    c = r.copy()
+    assert tracker.track_particles_drift(g, d) == r.shape[0]
+    ptr, keep = rows_ptr(c)
+    O.lib("glibc").or_track_particles_drift(ptr, c.shape[0], 1.3, 2)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "drift edges")
+    ng = tracker.limit_amplitudes(g, 0.004, 0.002, 9.0, PO, 1)
+    nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "limit_amplitudes edges")
 
 
 def test_wake_and_zlongit_entry_points(tracker):
