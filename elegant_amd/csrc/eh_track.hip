@@ -930,10 +930,10 @@ DEV int run_op(particle &p, const eh_op &op) {
   return post_element_checks(p, op);
 }
 
-/* Six waves per SIMD (at most 80 VGPRs): the extra wave hides more of the
- * dependency and scalar-load latency; the registers the compiler then has
- * to spill belong to rarely taken paths (measured: the storage ring runs
- * 2.5 % faster than at five waves). */
+/* Four waves per SIMD (up to 128 VGPRs): the branch-free bend body needs
+ * about 111 registers in this kernel; squeezing it into five or six waves
+ * spills to scratch or parks state in LDS, and measured no faster on the
+ * storage ring (29.0-29.3 M against 29.4 M particle-turns/s). */
 __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
                                                        int64_t pass0, int32_t npasses, int32_t op_first,
                                                        int32_t op_last) {
