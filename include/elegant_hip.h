@@ -226,7 +226,11 @@ int64_t eh_do_tracking(eh_ctx *ctx, double **coord, int64_t nOriginal, int64_t *
 /* ---- element-by-element entry points (host-row API of the reference) ----
  * Each uploads the rows, runs one element on the GPU and writes the rows
  * back with the reference's compaction.  The aperture is the MAXAMP in force
- * (NULL for none).  They return nLeft >= 0 or a negative error code. */
+ * (NULL for none).  They return nLeft >= 0 or a negative error code.  Like
+ * the routines they replace, they leave NaN and infinite coordinates to the
+ * caller: do_tracking's culls (before the first element and
+ * limit_amplitudes(DBL_MAX) after each one) belong to eh_do_tracking and
+ * eh_track; eh_limit_amplitudes removes such rows at its own z. */
 /* multipole_tracking2 (multipole.c:693-1117) for KQUAD/KSEXT/KOCT */
 int64_t eh_multipole_tracking2(eh_ctx *ctx, double **particle, int64_t n_part, const eh_element *elem,
                                double Po, double z_start, const EH_MAXAMP *maxamp);
