@@ -449,9 +449,10 @@ EM_FN double em_div(double a, double b, double y, int ok) {
 
 /* The same sequences for callers that have established the ranges
  * themselves: em_div_pos for 2^-64 <= b <= 2^64 (positive) and a = 0 or
- * 2^-900 <= |a| <= 2^900, written with the negated residual so that a zero
- * dividend keeps its sign (the residual is exact, so the quotient is the
- * same as em_div's otherwise); em_sqrt_mid for 2^-700 <= x <= 2^1000. */
+ * 2^-950 <= |a| <= 2^900 (then a*y is normal and the residual exact),
+ * written with the negated residual so that a zero dividend keeps its sign
+ * (RN(b q - a) = -RN(a - b q), so the quotient is em_div's otherwise);
+ * em_sqrt_mid for 2^-700 <= x <= 2^1000. */
 EM_FN double em_div_pos(double a, double b, double y) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double q = a * y;
