@@ -918,9 +918,12 @@ DEV int run_op(particle &p, const eh_op &op) {
     lost = track_rfca(p, op);
     break;
   case EH_OP_RFCA_DRIFT:
-    p.x += p.xp * op.d[EH_D_RFLEN_TOTAL];
-    p.y += p.yp * op.d[EH_D_RFLEN_TOTAL];
-    p.s += op.d[EH_D_RFLEN_TOTAL] * em_sqrt(1 + sq(p.xp) + sq(p.yp));
+    /* VOLT=0: a drift when the cavity has length, else nothing (simple_rfca.c:389-402) */
+    if (op.d[EH_D_RFLEN_TOTAL] != 0) {
+      p.x += p.xp * op.d[EH_D_RFLEN_TOTAL];
+      p.y += p.yp * op.d[EH_D_RFLEN_TOTAL];
+      p.s += op.d[EH_D_RFLEN_TOTAL] * em_sqrt(1 + sq(p.xp) + sq(p.yp));
+    }
     break;
   default:
     break;

@@ -779,14 +779,17 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
       return rc;
     if (c->prog->ops[0].op == EH_OP_RFCA)
       eh_rfca_set_phase(&c->prog->ops[0], rfca, *phase_fiducial);
+    /* the cavity's own removeInvalidParticles is the only cull, as in the
+     * routine it replaces */
     c->prog->ops[0].flags &= ~EH_F_AP;
+    c->prog->ops[0].flags |= EH_F_ALONE;
     c->prog->ops[0].z_end = zEnd;
     c->Po = *P_central;
     c->loaded = 1;
     rc = ehd_put_ops(c->dev, c->prog->ops, 1);
     if (rc)
       return devfail(c, rc);
-    rc = eh_upload(c, part, np, 8);
+    rc = upload_rows(c, part, np, 8, 0);
     if (rc)
       return rc;
     rc = ehd_track(c->dev, 0, 1, 0, 1);

@@ -173,6 +173,28 @@ def test_simple_rf_cavity_two_calls(tracker):
     assert_rows_identical(g2[:, :7], c2[:, :7], "rfca pass 1")
 
 
+@pytest.mark.parametrize("volt", [5e6, 0.0])
+def test_simple_rf_cavity_edges(tracker, volt):
+    """the cavity with rows of every kind: its own removeInvalidParticles
+    (any NaN coordinate, delta <= -1) is the only cull; infinite offsets pass
+    through as the reference lets them (bit for bit, signs of zeros
+    included).  T_REFERENCE fixes the phase without averaging the rows."""
+    rf = E.RFCA(L=0.0, VOLT=volt, PHASE=160.0, FREQ=352e6, T_REFERENCE=2e-8).p.rfca
+    r = csbend_edge_rows(1000, 31)
+    g = r.copy()
+    c = r.copy()
+    ng, ph = tracker.simple_rf_cavity(g, rf, PO, 7.0)
+    o = O.Oracle("emath")
+    ptr, keep = rows_ptr(c)
+    pc = C.c_double(PO)
+    nc = o.L.or_simple_rf_cavity(o.ctx, 0, ptr, c.shape[0], C.byref(rf), C.byref(pc), 7.0)
+    if volt == 0:
+        assert ng == nc == r.shape[0]
+    else:
+        assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "rfca edges volt %g" % volt)
+
+
 def test_drift_and_limit_amplitudes(tracker):
     d = E.DRIF(L=1.3).p.drift
     r = beam(4097, 4)
