@@ -1616,6 +1616,12 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
   for (long ip = 0; ip < nOriginal; ip++)
     coord[ip][7] = -1;
   c->effort = 0;
+  /* check_nan: limit_amplitudes(DBL_MAX) once before the pass loop, at z = 0,
+   * losses recorded on pass 0 (do_tracking.c:398, 468-473) */
+  nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nToTrack, 0.0, *P_central, 0);
+  for (long ip = nLeft; ip < nToTrack; ip++)
+    coord[ip][7] = 0;
+  nToTrack = nLeft;
   for (i_pass = passOffset; i_pass < n_passes + passOffset; i_pass++) {
     double z = 0, last_z = 0;
     if (!nToTrack)

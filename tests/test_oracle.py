@@ -373,3 +373,19 @@ def test_drift_aperture_loss_point_is_extrapolated():
     assert a[5] == PO * (1 + 2e-3)
     b = lost[3]
     assert abs(b[4] - 0.5) < 1e-12 and abs(b[2] - 0.01) < 1e-15
+
+
+def test_initial_nan_cull_precedes_the_first_element():
+    """do_tracking culls NaN/infinite x or y once before the pass loop, at
+    z = 0, recording pass 0 (do_tracking.c:398, 468-473): such rows are
+    lost before, and swapped to the tail ahead of, the losses of the first
+    element's aperture; delta becomes Po(1 + delta), s = 0"""
+    els = [E.MAXAMP(X_MAX=0.02, Y_MAX=0.01), E.DRIF(L=1.0)]
+    r, n, _ = track(els, rows_of([[0.03, 0, 0, 0, 0, 0], [float("nan"), 0, 0, 0, 0.7, 1e-3],
+                                  [0, 0, float("-inf"), 0, 0, 0], [0.001, 0, 0, 0, 0, 0]]))
+    assert n == 1 and r[0, 6] == 4
+    # swap-to-tail: the first cull loses id 2 (to the last row), then id 3;
+    # the aperture after MAXAMP then loses id 1
+    assert [int(x) for x in r[1:, 6]] == [1, 3, 2]
+    b = r[3]
+    assert b[7] == 0 and b[4] == 0.0 and b[5] == PO * (1 + 1e-3)

@@ -53,3 +53,19 @@ def test_ring_vs_glibc_checker_one_turn(tracker):
     for col, scale in [(0, 1e-3), (1, 1e-4), (2, 1e-4), (3, 1e-5), (4, 1.0), (5, 1e-3)]:
         err = np.abs(gi[:, col] - ci[:, col]) / np.maximum(np.abs(ci[:, col]), scale)
         assert err.max() <= 5e-10, (col, err.max())
+
+
+def test_ring_edge_rows_bitwise(tracker):
+    """do_tracking over the storage ring with rows of every kind mixed into
+    an ordinary beam (zeros of both signs, subnormals, large slopes, 1 + delta
+    at 0, huge offsets, NaN and infinities): the initial and per-element
+    culls, the branch-free bodies and the general code together reproduce
+    the checker bit for bit, signs of zeros included"""
+    from tests.helpers import assert_rows_bitwise
+    from tests.test_elements_gpu import csbend_edge_rows
+    bl, Po, _ = Lt.storage_ring_workload(n=10, seed=9)
+    rows = csbend_edge_rows(700, 41)
+    rows[:, :4] *= 0.5
+    g, ng, eg, c, nc, ec = run_both(tracker, bl, Po, rows, 2, "emath")
+    assert ng == nc and eg == ec and 0 < nc < rows.shape[0]
+    assert_rows_bitwise(g, c, "ring edges")
