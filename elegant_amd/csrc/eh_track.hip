@@ -618,7 +618,7 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
  * Arithmetic is otherwise csbend_body's, operation for operation. */
 DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const double DP, double &dist,
                           const eh_op &op) {
-  const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA], rsgn = op.d[EH_D_RHOA_SGN];
+  const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA];
   const double opd = 1 + DP;
   const int nsub = op.nsub;
   bool good = opd >= 0x1.0p-64 && opd <= 0x1.0p+64;
@@ -658,8 +658,11 @@ DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const do
         break; /* kickFrac[last] == 0 */
       double ds = opc(&op.d[EH_D_BDS + j]);
       double g = 1 + em_div_pos(X, rho0, opc(&op.d[EH_D_RRHO0]));
-      QX += em_div_pos(-ds * g * 1.0, rhoa, opc(&op.d[EH_D_RRHOA]));
-      QY += ds * g * 0.0 * rsgn;
+      /* (-ds) g 1.0 = -(ds g) and rsgn = +1 here (rho_actual > 0): the same
+       * bits as the general code's products, one product fewer */
+      const double dsg = ds * g;
+      QX += em_div_pos(-dsg, rhoa, opc(&op.d[EH_D_RRHOA]));
+      QY += dsg * 0.0;
       QX = em_div_pos(QX, opd, r_opd);
       QY = em_div_pos(QY, opd, r_opd);
       QX *= opd;
