@@ -370,12 +370,13 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
  *   - 1 + delta in [2^-64, 2^64]; the aperture holds at every slice.
  * dzLoss is not accumulated: the caller sets the full length's sum.
  * Arithmetic is otherwise multipole_body's, operation for operation. */
+template <int ORD>
 DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
                              const double dp, const opd_recip &o, const eh_op &op) {
   const int nsub = op.nsub;
   const int flags = op.flags;
   const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
-  const bool has_drift = op.d[EH_D_DRIFT] != 0;
+  const bool has_drift = ORD > 0 || op.d[EH_D_DRIFT] != 0;
   const int nko = op.nkick_orders;
   bool good = o.v >= 0x1.0p-64 && o.v <= 0x1.0p+64;
   for (int ik = 0; ik < op.nslices; ik++) {
@@ -391,16 +392,20 @@ DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, doubl
       }
       if (step == nsub - 1)
         break; /* kickFrac[last] == 0 */
-      if (nko > 0)
-        apply_kick(qx, qy, op.order0, op.skew0, opc(&op.d[EH_D_KICK + step]), x, y);
-      if (nko > 1)
-        apply_kick(qx, qy, op.order1, op.skew1, opc(&op.d[EH_D_KICK + 8 + step]), x, y);
-      if (nko > 2)
-        apply_kick(qx, qy, op.order2, op.skew2, opc(&op.d[EH_D_KICK + 16 + step]), x, y);
-      if (flags & EH_F_XKICK)
-        apply_kick(qx, qy, 0, 0, opc(&op.d[EH_D_XK + step]), x, y);
-      if (flags & EH_F_YKICK)
-        apply_kick(qx, qy, 0, 1, opc(&op.d[EH_D_YK + step]), x, y);
+      if (ORD > 0) {
+        apply_kick(qx, qy, ORD, 0, opc(&op.d[EH_D_KICK + step]), x, y);
+      } else {
+        if (nko > 0)
+          apply_kick(qx, qy, op.order0, op.skew0, opc(&op.d[EH_D_KICK + step]), x, y);
+        if (nko > 1)
+          apply_kick(qx, qy, op.order1, op.skew1, opc(&op.d[EH_D_KICK + 8 + step]), x, y);
+        if (nko > 2)
+          apply_kick(qx, qy, op.order2, op.skew2, opc(&op.d[EH_D_KICK + 16 + step]), x, y);
+        if (flags & EH_F_XKICK)
+          apply_kick(qx, qy, 0, 0, opc(&op.d[EH_D_XK + step]), x, y);
+        if (flags & EH_F_YKICK)
+          apply_kick(qx, qy, 0, 1, opc(&op.d[EH_D_YK + step]), x, y);
+      }
       good = good && __builtin_fabs(qx) + __builtin_fabs(qy) <= 0x1.0p+700 &&
              imin3(__builtin_amdgcn_frexp_exp(qx), __builtin_amdgcn_frexp_exp(qy), 0) >= -699;
       double den = sq(1 + dp) - sq(qx) - sq(qy);
@@ -477,7 +482,20 @@ DEV int track_multipole(particle &p, const eh_op &op) {
   if (!lost) {
     bool done = false;
     if (!expand) {
-      done = multipole_body_fast(x, xp, y, yp, s, qx, qy, dp, o, op);
+      /* the common thick element with one normal multipole and no steering
+       * gets a body specialised to its order: the kick is straight-line code */
+      const int ord = (op.nkick_orders == 1 && !op.skew0 && !(flags & (EH_F_XKICK | EH_F_YKICK)) &&
+                       op.d[EH_D_DRIFT] != 0)
+                          ? op.order0
+                          : 0;
+      if (ord == 1)
+        done = multipole_body_fast<1>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      else if (ord == 2)
+        done = multipole_body_fast<2>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      else if (ord == 3)
+        done = multipole_body_fast<3>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      else
+        done = multipole_body_fast<0>(x, xp, y, yp, s, qx, qy, dp, o, op);
       /* the checks after the integrator may still lose the particle: at the
        * element's full length, the sum the general code would have made */
       dzLoss = done ? op.d[EH_D_TOTLEN] : 0.0;
