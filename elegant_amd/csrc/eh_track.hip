@@ -634,17 +634,19 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
  *     checked on the host (EH_F_FAST);
  *   - the aperture holds at every slice.
  * Arithmetic is otherwise csbend_body's, operation for operation. */
+template <int NSUB>
 DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const double DP, double &dist,
                           const eh_op &op) {
   const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA];
   const double opd = 1 + DP;
-  const int nsub = op.nsub;
+  const int nsub = NSUB ? NSUB : op.nsub;
   bool good = opd >= 0x1.0p-64 && opd <= 0x1.0p+64;
   const double r_opd = em_recip(opd);
   double acc = 0, zmax = 0, lo = 1;
   int emin = 0;
   for (int i = 0; i < op.nslices; i++) {
     good = good && in_aperture(X, Y, op);
+#pragma unroll
     for (int j = 0; j < nsub; j++) {
       acc += (__builtin_fabs(X) + __builtin_fabs(QX)) + __builtin_fabs(QY);
       emin = imin3(emin, __builtin_amdgcn_frexp_exp(X), imin3(emin, __builtin_amdgcn_frexp_exp(QX),
@@ -728,7 +730,11 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   bool lost = false, done = false;
   if (flags & EH_F_FAST) {
     const double X0 = X, QX0 = QX, Y0 = Y, QY0 = QY;
-    done = csbend_body_fast(X, QX, Y, QY, DP, dist, op);
+    /* the 4th-order integrator's four substeps of a slice unrolled (NSUB =
+     * 4): the substep coefficients load once per slice and the scheduler
+     * overlaps neighbouring substeps */
+    done = op.nsub == 4 ? csbend_body_fast<4>(X, QX, Y, QY, DP, dist, op)
+                        : csbend_body_fast<0>(X, QX, Y, QY, DP, dist, op);
     if (!ALL_LANES(done) && !done) {
       X = X0;
       QX = QX0;
