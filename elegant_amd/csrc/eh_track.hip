@@ -370,10 +370,10 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
  *   - 1 + delta in [2^-64, 2^64]; the aperture holds at every slice.
  * dzLoss is not accumulated: the caller sets the full length's sum.
  * Arithmetic is otherwise multipole_body's, operation for operation. */
-template <int ORD>
+template <int ORD, int NSUB>
 DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
                              const double dp, const opd_recip &o, const eh_op &op) {
-  const int nsub = op.nsub;
+  const int nsub = NSUB ? NSUB : op.nsub;
   const int flags = op.flags;
   const double dxa = op.d[EH_D_DX], dya = op.d[EH_D_DY];
   const bool has_drift = ORD > 0 || op.d[EH_D_DRIFT] != 0;
@@ -381,6 +381,7 @@ DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, doubl
   bool good = o.v >= 0x1.0p-64 && o.v <= 0x1.0p+64;
   for (int ik = 0; ik < op.nslices; ik++) {
     good = good && in_aperture(x + dxa, y + dya, op);
+#pragma unroll
     for (int step = 0; step < nsub; step++) {
       if (has_drift) {
         const double dsh = opc(&op.d[EH_D_DSH + step]);
@@ -488,14 +489,15 @@ DEV int track_multipole(particle &p, const eh_op &op) {
                        op.d[EH_D_DRIFT] != 0)
                           ? op.order0
                           : 0;
-      if (ord == 1)
-        done = multipole_body_fast<1>(x, xp, y, yp, s, qx, qy, dp, o, op);
-      else if (ord == 2)
-        done = multipole_body_fast<2>(x, xp, y, yp, s, qx, qy, dp, o, op);
-      else if (ord == 3)
-        done = multipole_body_fast<3>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      const bool four = op.nsub == 4;
+      if (ord == 1 && four)
+        done = multipole_body_fast<1, 4>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      else if (ord == 2 && four)
+        done = multipole_body_fast<2, 4>(x, xp, y, yp, s, qx, qy, dp, o, op);
+      else if (ord == 3 && four)
+        done = multipole_body_fast<3, 4>(x, xp, y, yp, s, qx, qy, dp, o, op);
       else
-        done = multipole_body_fast<0>(x, xp, y, yp, s, qx, qy, dp, o, op);
+        done = multipole_body_fast<0, 0>(x, xp, y, yp, s, qx, qy, dp, o, op);
       /* the checks after the integrator may still lose the particle: at the
        * element's full length, the sum the general code would have made */
       dzLoss = done ? op.d[EH_D_TOTLEN] : 0.0;
