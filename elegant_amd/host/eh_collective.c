@@ -25,6 +25,7 @@ int32_t *eh_ctx_zl_nbins(eh_ctx *c);
 double *eh_ctx_zl_binsize(eh_ctx *c);
 const char *ehd_message(const ehd_dev *d);
 int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops);
+int eh_upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cull);
 int eh_download(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int64_t *nLeft);
 
 static double sqr(double x) { return x * x; }
@@ -234,7 +235,8 @@ int eh_collective_probe(eh_ctx *c, double **part, int64_t np, const eh_element *
   if (rc)
     return rc;
   memset(eh_ctx_coll_loaded(c), 0, sizeof(int) * EH_MAX_COLL);
-  rc = eh_upload(c, part, np, 8);
+  /* the element routines leave NaN rows to their caller: no initial cull */
+  rc = eh_upload_rows(c, part, np, 8, 0);
   if (rc)
     return rc;
   rc = collective_run(c, 0, e, &prog->ops[0], 0, i_pass, macro, EHD_COLL_NOCHECKS, hist, volt, nb, tmin);

@@ -234,7 +234,7 @@ static int alloc_particles(eh_ctx *c, int64_t n) {
  * when cull is set: such rows are recorded as lost on pass 0 before the first
  * element.  The single-element entry points upload without it, as the
  * element routines leave such rows to their caller. */
-static int upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cull) {
+int eh_upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cull) {
   int rc;
   double *soa;
   if (!c || (!coord && n > 0) || n < 0 || nprops < 7)
@@ -277,7 +277,7 @@ static int upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int
   return EH_OK;
 }
 
-int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) { return upload_rows(c, coord, n, nprops, 1); }
+int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops) { return eh_upload_rows(c, coord, n, nprops, 1); }
 
 /* ----------------------------------------------------- reference order */
 
@@ -679,7 +679,7 @@ static int64_t run_single(eh_ctx *c, double **rows, int64_t n, const eh_element 
   rc = ehd_put_ops(c->dev, c->prog->ops, 1);
   if (rc)
     return devfail(c, rc);
-  rc = upload_rows(c, rows, n, 8, 0);
+  rc = eh_upload_rows(c, rows, n, 8, 0);
   if (rc)
     return rc;
   rc = ehd_track(c->dev, pass, 1, 0, 1);
@@ -789,7 +789,7 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     rc = ehd_put_ops(c->dev, c->prog->ops, 1);
     if (rc)
       return devfail(c, rc);
-    rc = upload_rows(c, part, np, 8, 0);
+    rc = eh_upload_rows(c, part, np, 8, 0);
     if (rc)
       return rc;
     rc = ehd_track(c->dev, 0, 1, 0, 1);
@@ -852,7 +852,7 @@ int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax,
     if (rc)
       return devfail(c, rc);
     /* NaN and infinite rows are limit_amplitudes' own to remove, at z */
-    rc = upload_rows(c, coord, np, 8, 0);
+    rc = eh_upload_rows(c, coord, np, 8, 0);
     if (rc)
       return rc;
     rc = ehd_track(c->dev, 0, 1, 0, 1);
