@@ -586,21 +586,18 @@ static int cfail(ehd_dev *d, hipError_t e, const char *what) {
       return cfail(d, e_, #call);                                                                                      \
   } while (0)
 
+/* teardown: errors are ignored, there is nothing left to undo */
 void ehd_free_collective(ehd_dev *d) {
   for (int k = 0; k < EHD_MAX_COLL; k++) {
     ehd_coll_slot *sl = &d->slot[k];
-    if (sl->W)
-      hipFree(sl->W);
-    if (sl->Z)
-      hipFree(sl->Z);
-    if (sl->tc)
-      hipFree(sl->tc);
-    if (sl->ts)
-      hipFree(sl->ts);
+    void *bufs[4] = {sl->W, sl->Z, sl->tc, sl->ts};
+    for (int b = 0; b < 4; b++)
+      if (bufs[b])
+        (void)hipFree(bufs[b]);
     memset(sl, 0, sizeof(*sl));
   }
   if (d->cwork)
-    hipFree(d->cwork);
+    (void)hipFree(d->cwork);
   d->cwork = NULL;
   d->cwork_bytes = 0;
 }

@@ -1,6 +1,7 @@
 /*
  * eh_device.h - internal C ABI between the C host (elegant_amd/host) and the
- * HIP launch layer (elegant_amd/csrc/*.hip).  Not part of the public API.
+ * HIP launch layer (the .hip sources in elegant_amd/csrc).  Not part of the
+ * public API.
  */
 #ifndef ELEGANT_AMD_EH_DEVICE_H
 #define ELEGANT_AMD_EH_DEVICE_H
@@ -81,7 +82,7 @@ int ehd_put_wake(ehd_dev *d, int32_t slot, const double *W, int64_t npoints);
 int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc, const double *ts, int32_t nb);
 /* one WAKE or ZLONGIT step on all alive particles.  Returns EH_ERR_LONGBEAM if the
  * bunch is longer than allowed.  hist/volt (optional, may be NULL) receive the
- * histogram and the scaled voltage; *nb_out/*tmin_out describe them. */
+ * histogram and the scaled voltage; *nb_out and *tmin_out describe them. */
 int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist, double *volt,
                         int64_t hist_cap, int64_t *nb_out, double *tmin_out);
 int ehd_collective_timing(ehd_dev *d, int64_t *launches, double *ms);

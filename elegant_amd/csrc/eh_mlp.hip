@@ -225,6 +225,8 @@ static int ensure_batch(ehd_mlp *m, int32_t B) {
   return EH_OK;
 }
 
+void ehd_mlp_destroy(ehd_mlp *m);
+
 int ehd_mlp_create(ehd_dev *d, int32_t nin, int32_t hidden, const float *init, ehd_mlp **out) {
   ehd_mlp *m = (ehd_mlp *)calloc(1, sizeof(ehd_mlp));
   if (!m)
@@ -242,42 +244,45 @@ int ehd_mlp_create(ehd_dev *d, int32_t nin, int32_t hidden, const float *init, e
     off += m->out_dim[l];
   }
   m->nparams = off;
-  hipSetDevice(d->device);
-  if (hipMalloc((void **)&m->params, sizeof(float) * off) != hipSuccess ||
+  if (hipSetDevice(d->device) != hipSuccess || hipMalloc((void **)&m->params, sizeof(float) * off) != hipSuccess ||
       hipMalloc((void **)&m->grads, sizeof(float) * off) != hipSuccess ||
       hipMalloc((void **)&m->adam_m, sizeof(float) * off) != hipSuccess ||
       hipMalloc((void **)&m->adam_v, sizeof(float) * off) != hipSuccess) {
-    free(m);
+    ehd_mlp_destroy(m);
     return EH_ERR_NOMEM;
   }
-  hipMemcpy(m->params, init, sizeof(float) * off, hipMemcpyHostToDevice);
-  hipMemset(m->adam_m, 0, sizeof(float) * off);
-  hipMemset(m->adam_v, 0, sizeof(float) * off);
+  if (hipMemcpy(m->params, init, sizeof(float) * off, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->adam_m, 0, sizeof(float) * off) != hipSuccess ||
+      hipMemset(m->adam_v, 0, sizeof(float) * off) != hipSuccess) {
+    ehd_mlp_destroy(m);
+    return EH_ERR_HIP;
+  }
   *out = m;
   return EH_OK;
 }
 
 int64_t ehd_mlp_nparams(const ehd_mlp *m) { return m->nparams; }
 
+/* teardown: errors are ignored, there is nothing left to undo */
 void ehd_mlp_destroy(ehd_mlp *m) {
   if (!m)
     return;
-  hipSetDevice(m->d->device);
-  hipStreamSynchronize(m->d->stream);
-  hipFree(m->params);
-  hipFree(m->grads);
-  hipFree(m->adam_m);
-  hipFree(m->adam_v);
+  (void)hipSetDevice(m->d->device);
+  (void)hipStreamSynchronize(m->d->stream);
+  void *bufs[4] = {m->params, m->grads, m->adam_m, m->adam_v};
+  for (int k = 0; k < 4; k++)
+    if (bufs[k])
+      (void)hipFree(bufs[k]);
   for (int l = 0; l <= MLP_LAYERS; l++) {
     if (m->act[l])
-      hipFree(m->act[l]);
+      (void)hipFree(m->act[l]);
     if (m->dact[l])
-      hipFree(m->dact[l]);
+      (void)hipFree(m->dact[l]);
   }
   if (m->labels)
-    hipFree(m->labels);
+    (void)hipFree(m->labels);
   if (m->loss_part)
-    hipFree(m->loss_part);
+    (void)hipFree(m->loss_part);
   free(m);
 }
 

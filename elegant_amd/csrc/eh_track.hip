@@ -1154,43 +1154,24 @@ int ehd_create(int device, ehd_dev **out, char *msg, int msglen) {
 
 void ehd_free_collective(ehd_dev *d);
 
+/* teardown: errors are ignored, there is nothing left to undo */
 void ehd_destroy(ehd_dev *d) {
   if (!d)
     return;
-  hipSetDevice(d->device);
-  hipStreamSynchronize(d->stream);
-  for (int k = 0; k < 6; k++)
-    if (d->coord[k])
-      hipFree(d->coord[k]);
-  if (d->loss_pass)
-    hipFree(d->loss_pass);
-  if (d->loss_code)
-    hipFree(d->loss_code);
-  if (d->ops)
-    hipFree(d->ops);
-  if (d->counter)
-    hipFree(d->counter);
-  if (d->vdesc)
-    hipFree(d->vdesc);
-  if (d->chunks)
-    hipFree(d->chunks);
-  if (d->perm)
-    hipFree(d->perm);
-  if (d->queue)
-    hipFree(d->queue);
-  for (int k = 0; k < 6; k++)
-    if (d->snap[k])
-      hipFree(d->snap[k]);
-  if (d->snap_lp)
-    hipFree(d->snap_lp);
-  if (d->snap_lc)
-    hipFree(d->snap_lc);
+  (void)hipSetDevice(d->device);
+  (void)hipStreamSynchronize(d->stream);
+  void *bufs[] = {d->coord[0], d->coord[1], d->coord[2], d->coord[3], d->coord[4], d->coord[5],
+                  d->loss_pass, d->loss_code, d->ops, d->counter, d->vdesc, d->chunks, d->perm, d->queue,
+                  d->snap[0], d->snap[1], d->snap[2], d->snap[3], d->snap[4], d->snap[5], d->snap_lp, d->snap_lc};
+  for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
+    if (bufs[k])
+      (void)hipFree(bufs[k]);
   ehd_free_collective(d);
-  hipEventDestroy(d->ev0);
-  hipEventDestroy(d->ev1);
-  hipEventDestroy(d->cev0);
-  hipEventDestroy(d->cev1);
-  hipStreamDestroy(d->stream);
+  (void)hipEventDestroy(d->ev0);
+  (void)hipEventDestroy(d->ev1);
+  (void)hipEventDestroy(d->cev0);
+  (void)hipEventDestroy(d->cev1);
+  (void)hipStreamDestroy(d->stream);
   free(d);
 }
 
