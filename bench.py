@@ -155,7 +155,7 @@ def learned_da(args):
     rep = learned.study(tr, range(half), range(half, 2 * half), passes=args.steps,
                         log=lambda m: print(m, file=sys.stderr, flush=True))
     tr.close()
-    best = max(("learned", "learned_ladder"), key=lambda k: rep[k]["maps_per_hour"])
+    best = max(("learned", "learned_ladder", "learned_sweep"), key=lambda k: rep[k]["maps_per_hour"])
     return {"metric": "DA maps/hour learned-proposal vs plain", "value": rep[best]["maps_per_hour"],
             "unit": "maps/hour", "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -137,7 +137,7 @@ def propose(mlp, vs, rays, deltas, confidence=False):
 
 
 def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=1, max_rounds=64, ladder=(),
-                 ladder_rays=None):
+                 ladder_rays=None, sweep_from=None):
     """Find, on every ray of every (variant, offset), a grid index lo that
     survives with lo + 1 lost (or lo = nx - 1 surviving, or lo = -1 with the
     origin lost), assuming survival is monotone along the ray.  Each round is
@@ -154,8 +154,12 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
     bracket a wrong guess at once, trading particle-turns for rounds (each
     round costs the latency of `passes` passes however few particles it
     holds); `ladder_rays` (per variant, offsets x rays) limits it to the
-    rays whose guess is uncertain.  Returns (limits per variant, effort, rounds, wall seconds,
-    fraction of rays settled by the first round)."""
+    rays whose guess is uncertain.  From round `sweep_from` on, every
+    untested index of an unsettled ray is tracked at once, which settles it in
+    that round: the search then costs at most sweep_from + 1 rounds, each the
+    latency of `passes` passes, and the sweep reports the first loss inside
+    the bracket as the plain grid does.  Returns (limits per variant, effort,
+    rounds, wall seconds, fraction of rays settled by the first round)."""
     nd, nl = len(deltas), rays.lines
     nx = np.concatenate([np.full(nd * nl, v.nx) for v in vs])
     R = nx.size
@@ -177,7 +181,11 @@ def index_search(tracker, vs, passes, rays, deltas, guess=None, tune=None, kary=
         active = hi - lo > 1
         if not active.any():
             break
-        if rounds == 0 and g is not None:
+        if sweep_from is not None and rounds >= sweep_from:
+            gap = hi - lo - 1                      # untested indices lo+1 .. hi-1
+            j = np.arange(1, int(gap[active].max()) + 1)[None, :]
+            T = np.where(active[:, None] & (j <= gap[:, None]), lo[:, None] + j, -1)
+        elif rounds == 0 and g is not None:
             offs = (0, 1) + tuple(ladder)
             T = np.stack([np.where(active & (g + o >= 0) & (g + o < nx) & (lm | (k < 2)), g + o, -1)
                           for k, o in enumerate(offs)], axis=1)
@@ -278,6 +286,10 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
     lim_f, eff_f, rounds_f, wall_f, first_f = index_search(tracker, ev, passes, rays, deltas, guess, tune,
                                                            kary=16, ladder=ladder, ladder_rays=unsure)
     say("learned search, ladder %s: %d particle-turns, %d rounds, %.1f s" % (ladder, eff_f, rounds_f, wall_f))
+    lim_s, eff_s, rounds_s, wall_s, first_s = index_search(tracker, ev, passes, rays, deltas, guess, tune,
+                                                           kary=16, ladder=ladder, ladder_rays=unsure, sweep_from=1)
+    say("learned search, ladder %s, then a sweep of the brackets: %d particle-turns, %d rounds, %.1f s" % (
+        ladder, eff_s, rounds_s, wall_s))
     lim_b, eff_b, rounds_b, wall_b, _ = index_search(tracker, ev, passes, rays, deltas, None, tune)
     say("bisection search: %d particle-turns, %d rounds, %.1f s" % (eff_b, rounds_b, wall_b))
     mlp.close()
@@ -319,5 +331,9 @@ def study(tracker, train_seeds, eval_seeds, passes=1024, epochs=6, batch=4096, l
                                maps_per_hour=maps / (wall_f + t_prop) * 3600,
                                maps_per_hour_incl_training=maps / (wall_f + t_prop + gtrain["wall"] + t_train) * 3600,
                                settled_first_round=first_f, **compare(lim_f)),
+        "learned_sweep": dict(ladder=list(ladder), sweep_from=1, effort=eff_s, rounds=rounds_s,
+                              wall_s=wall_s + t_prop, maps_per_hour=maps / (wall_s + t_prop) * 3600,
+                              maps_per_hour_incl_training=maps / (wall_s + t_prop + gtrain["wall"] + t_train) * 3600,
+                              settled_first_round=first_s, **compare(lim_s)),
     }
     return rep

@@ -26,6 +26,9 @@ def test_small_study(tracker):
     assert l["effort"] < 0.5 * g["effort_sequential"]
     assert l["settled_first_round"] > 0.3
     assert rep["training"]["loss_last"] < rep["training"]["loss_first"]
+    # the sweep of the brackets settles every ray by the second round
+    sw = rep["learned_sweep"]
+    assert sw["rounds"] <= 2 and sw["rays_identical"] > 0.85
 
 
 def test_reported_boundaries_are_tracked_pairs(tracker):

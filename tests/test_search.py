@@ -71,6 +71,13 @@ def test_index_search_finds_the_grid_boundary(monkeypatch, guess_kind):
         lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g, kary=kary, ladder=ladder)
         for a, b in zip(lim, ref):
             assert np.array_equal(a, b), (kary, ladder)
+    # a sweep of the brackets from the second round settles every ray in it
+    for sweep in (0, 1, 2):
+        lim, effort, rounds, _, first = learned.index_search(None, vs, 10, rays, deltas, g, kary=16,
+                                                             ladder=(-4, 4), sweep_from=sweep)
+        for a, b in zip(lim, ref):
+            assert np.array_equal(a, b), sweep
+        assert rounds <= sweep + 1
     if guess_kind == "exact":
         assert rounds == 1 and first == 1.0
     if guess_kind == "none":
@@ -88,6 +95,28 @@ def test_a_good_guess_costs_fewer_points_than_bisection(monkeypatch):
     _, e_guess, _, _, _ = learned.index_search(None, vs, 10, rays, deltas, near, kary=4)
     _, e_bis, _, _, _ = learned.index_search(None, vs, 10, rays, deltas, None)
     assert e_guess < 0.6 * e_bis
+
+
+def test_sweep_reports_the_first_loss_like_the_grid(monkeypatch):
+    """on a ray with a hole (a lost point below survivors) the sweep of a
+    bracket finds the first loss inside it, as the plain grid's outward scan
+    does"""
+    rays = da.Rays(lines=3, xmax=0.01, ymax=0.01)
+    deltas = (0.0,)
+    v = FakeVariant(20, np.array([0.8]))
+    frac = np.arange(20) / 19
+    hole = (frac[5] * rays.xmax * rays.dxf[1], frac[5] * rays.ymax * rays.dyf[1])
+    install(monkeypatch, rays, deltas, holes=[hole])
+    # on the middle ray the guess (12) and guess + 1 survive, the ladder's
+    # -8 point (index 4) survives and its +4 point (16) is lost; round two
+    # sweeps (13, 16), and the bracket below the guess is never reopened
+    lim, _, rounds, _, _ = learned.index_search(None, [v], 10, rays, deltas, [np.full((1, 3), 12)], kary=16,
+                                                ladder=(-8, 4), sweep_from=1)
+    assert rounds <= 2
+    assert lim[0][0].tolist() == [15, 15, 15]
+    # with no guess the sweep from the first round is the plain grid itself
+    lim, _, rounds, _, _ = learned.index_search(None, [v], 10, rays, deltas, None, sweep_from=0)
+    assert rounds == 1 and lim[0][0].tolist() == [15, 4, 15]
 
 
 def test_grid_limits_and_sequential_cost():
