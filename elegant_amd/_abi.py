@@ -34,6 +34,7 @@ TYPE_NAMES = {
     EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE",
 }
 
+EH_OPEN_PLUS_X, EH_OPEN_PLUS_Y, EH_OPEN_MINUS_X, EH_OPEN_MINUS_Y = 1, 2, 3, 4
 EH_FID_TMEAN = 0
 EH_FID_LIGHT = 1
 

@@ -25,6 +25,7 @@
 #include <string.h>
 
 #include "../../include/elegant_hip.h"
+#include "eh_aperture.h"
 #include "eh_consts.h"
 #include "eh_device.h"
 #include "eh_fixedsum.h"
@@ -489,7 +490,7 @@ __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, dou
                                            int32_t *lp, int32_t *lc, int64_t n, double Po, const coll_plan *plan,
                                            double dt, const double *__restrict__ volt, int32_t kind,
                                            int32_t interpolate, double z_end, double ap_x, double ap_y,
-                                           int32_t ap_on, int64_t pass, int32_t code_base) {
+                                           int32_t ap_on, int32_t ap_code, int64_t pass, int32_t code_base) {
   int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   const int32_t nb = plan->nb;
   const double tmin = plan->tmin;
@@ -545,6 +546,20 @@ __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, dou
   if (ap_on < 0)
     return;
   double X = x[i], Y = y[i];
+  if (ap_on && ap_code) {
+    /* a shaped MAXAMP (ellipse or open side); no drift extrapolation here */
+    double S1 = S, D1 = D;
+    if (ap_maxamp_shaped(X, XP, Y, YP, S1, D1, ap_x, ap_y, ap_code, false, z_end, Po)) {
+      x[i] = X;
+      y[i] = Y;
+      s[i] = S1;
+      dp[i] = D1;
+      lp[i] = (int32_t)pass;
+      lc[i] = code_base + 1;
+      return;
+    }
+    ap_on = 0;
+  }
   for (int stage = 1; stage <= 2; stage++) {
     double xmax = stage == 1 ? ap_x : 1.7976931348623157e308;
     double ymax = stage == 1 ? ap_y : 1.7976931348623157e308;
@@ -727,7 +742,7 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
     hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[0], d->coord[1], d->coord[2],
                        d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po, plan, c->dt, volt,
                        kind, c->interpolate, c->z_end, c->ap_xmax, c->ap_ymax,
-                       (c->flags & EHD_COLL_NOCHECKS) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0), c->pass,
+                       (c->flags & EHD_COLL_NOCHECKS) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0), c->ap_code, c->pass,
                        4 * c->op_index);
   CHK(hipGetLastError());
   CHK(hipEventRecord(d->cev1, d->stream));

@@ -74,7 +74,7 @@ typedef struct {
   int32_t nb;           /* ZLONGIT n_bins */
   double wake_span;     /* WAKE: t[last] - t[0] (long-beam check) */
   int32_t allow_long_beam;
-  int32_t pad;
+  int32_t ap_code;      /* EH_AP_* shape of the MAXAMP check, 0 = rectangle */
 } ehd_collective;
 
 /* WAKE Green function samples; ZLONGIT impedance Z[nb] and twiddles */

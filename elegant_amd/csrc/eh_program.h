@@ -54,6 +54,18 @@
 #define EH_F_ALONE (1 << 21)   /* single-element call: no limit_amplitudes cull after the element */
 #define EH_F_DFAST (1 << 22)   /* DRIF: 2^-300 <= |L| <= 2^300, the short form of the order-2 drift applies */
 
+/* Aperture shapes other than the closed rectangle, packed in one int32:
+ * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the
+ * x exponent, bits 16-23 the y exponent.  The checks after an element
+ * follow limit_amplitudes / elimit_amplitudes (limit_amplitudes.c:193-272,
+ * 472-598), the ones inside an integrator checkMultAperture
+ * (multipole.c:1786-1833); 0 means the closed rectangle of the fast path. */
+#define EH_AP_ELL 1
+#define EH_AP_CODE(ell, open, xe, ye) ((ell) | ((open) << 1) | ((xe) << 8) | ((ye) << 16))
+#define EH_AP_OPEN(code) (((code) >> 1) & 7)
+#define EH_AP_XE(code) (((code) >> 8) & 0xff)
+#define EH_AP_YE(code) (((code) >> 16) & 0xff)
+
 /* loss stages recorded with the element index (loss_code = 4*op_index + stage) */
 #define EH_STAGE_ELEMENT 0  /* inside the element routine's own compaction */
 #define EH_STAGE_MAXAMP 1   /* limit_amplitudes with the MAXAMP aperture */
@@ -63,7 +75,7 @@ typedef struct {
   int32_t op, elem, flags, nslices;
   int32_t nsub, nkick_orders, order0, order1;
   int32_t order2, skew0, skew1, skew2;
-  int32_t nkicks_rf, pad0, pad1, pad2;
+  int32_t nkicks_rf, ap_code, in_code, pad2; /* ap_code, in_code: EH_AP_* shape of the two checks, 0 = rectangle */
   /* common */
   double z_start, z_end, Po;
   double ap_xmax, ap_ymax;      /* post-element rectangle (0 = open in that plane) */

@@ -20,6 +20,7 @@
 #include "../../include/elegant_hip.h"
 #include "eh_consts.h"
 #include "eh_device.h"
+#include "eh_aperture.h"
 #include "eh_program.h"
 #include "emath.h"
 #include "ehd_internal.h"
@@ -75,10 +76,15 @@ DEV void aperture_loss(particle &p, double xmax, double ymax, int is_out, bool e
 
 /* the two checks that follow every element (do_tracking.c:2459-2477, 2628-2637).
  * Returns 0 if the particle survives, else stage + 1 with the row frozen. */
+template <bool SH>
 DEV int post_element_checks(particle &p, const eh_op &op) {
   if (op.flags & EH_F_ALONE)
     return 0;
-  if (op.flags & EH_F_AP) {
+  if (SH && (op.flags & EH_F_AP) && op.ap_code) {
+    if (ap_maxamp_shaped(p.x, p.xp, p.y, p.yp, p.s, p.dp, op.ap_xmax, op.ap_ymax, op.ap_code,
+                         (op.flags & EH_F_EXTRAP) != 0, op.z_end, op.Po))
+      return EH_STAGE_MAXAMP + 1;
+  } else if (op.flags & EH_F_AP) {
     double xmax = op.ap_xmax, ymax = op.ap_ymax;
     int is_out = 0;
     if (xmax > 0 && __builtin_fabs(p.x) > xmax)
@@ -221,7 +227,9 @@ DEV void rotate(particle &p, double c, double s) {
 /* ---- multipoles ---- */
 
 
-DEV bool in_aperture(double x, double y, const eh_op &op) {
+/* checkMultAperture (multipole.c:1786-1833) for the closed rectangle: the
+ * only shape the branch-free bodies see */
+DEV bool in_rect(double x, double y, const eh_op &op) {
   if (!(op.flags & EH_F_INAP))
     return true;
   if (op.flags & EH_F_INTILT) {
@@ -232,6 +240,24 @@ DEV bool in_aperture(double x, double y, const eh_op &op) {
   if ((op.in_xmax > 0 && __builtin_fabs(x) > op.in_xmax) || (op.in_ymax > 0 && __builtin_fabs(y) > op.in_ymax))
     return false;
   return true;
+}
+
+/* checkMultAperture for any MAXAMP shape (in_code != 0: ellipse or open
+ * side).  SH: the kernel instance built for programs with such shapes; the
+ * other instance keeps the closed rectangle only, so the code of the common
+ * case is the same as without them. */
+template <bool SH>
+DEV bool in_aperture(double x, double y, const eh_op &op) {
+  if (!SH || !op.in_code)
+    return in_rect(x, y, op);
+  if (!(op.flags & EH_F_INAP))
+    return true;
+  if (op.flags & EH_F_INTILT) {
+    double x0 = x, y0 = y;
+    x = x0 * op.in_cs0 + y0 * op.in_cs1;
+    y = -x0 * op.in_cs1 + y0 * op.in_cs0;
+  }
+  return ap_inside_shaped(x, y, op.in_xmax, op.in_ymax, op.in_code);
 }
 
 /* 1 + delta and its reciprocal for the IEEE quotients of emath.h, fixed for
@@ -306,6 +332,7 @@ DEV void apply_kick(double &qx, double &qy, int order, int skew, double KnL, dou
  * loads of the element record).
  * Returns 0, or the loss kind: 1 aperture (x, y hold the loss point),
  * 2 slopes (the row is left as received). */
+template <bool SH>
 DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, double &qx, double &qy,
                        double &dzLoss, const double dp, const bool expand, const opd_recip &o, const eh_op &op) {
   const int nsub = op.nsub;
@@ -314,7 +341,7 @@ DEV int multipole_body(double &x, double &xp, double &y, double &yp, double &s, 
   const bool has_drift = op.d[EH_D_DRIFT] != 0;
   const int nko = op.nkick_orders;
   for (int ik = 0; ik < op.nslices; ik++) {
-    if (!in_aperture(x + dxa, y + dya, op))
+    if (!in_aperture<SH>(x + dxa, y + dya, op))
       return 1;
     for (int step = 0; step < nsub; step++) {
       if (has_drift) {
@@ -380,7 +407,7 @@ DEV bool multipole_body_fast(double &x, double &xp, double &y, double &yp, doubl
   const int nko = op.nkick_orders;
   bool good = o.v >= 0x1.0p-64 && o.v <= 0x1.0p+64;
   for (int ik = 0; ik < op.nslices; ik++) {
-    good = good && in_aperture(x + dxa, y + dya, op);
+    good = good && in_rect(x + dxa, y + dya, op);
 #pragma unroll
     for (int step = 0; step < nsub; step++) {
       if (has_drift) {
@@ -465,6 +492,7 @@ DEV int multipole_entry(const particle &p, double &xp, double &yp, double &qx, d
 /* multipole_tracking2 per particle: entry misalignment, the integrator of
  * integrate_kick_multipole_ordn (multipole.c:1122-1410), exit misalignment.
  * Returns 0 if alive, EH_STAGE_ELEMENT + 1 if lost (row frozen). */
+template <bool SH>
 DEV int track_multipole(particle &p, const eh_op &op) {
   const int flags = op.flags;
   const bool expand = (flags & EH_F_EXPAND) != 0;
@@ -482,7 +510,7 @@ DEV int track_multipole(particle &p, const eh_op &op) {
   int lost = multipole_entry(p, xp, yp, qx, qy, dzLoss, expand, o, op);
   if (!lost) {
     bool done = false;
-    if (!expand) {
+    if (!expand && !(SH && op.in_code)) {
       /* the common thick element with one normal multipole and no steering
        * gets a body specialised to its order: the kick is straight-line code */
       const int ord = (op.nkick_orders == 1 && !op.skew0 && !(flags & (EH_F_XKICK | EH_F_YKICK)) &&
@@ -509,9 +537,9 @@ DEV int track_multipole(particle &p, const eh_op &op) {
       }
     }
     if (!ALL_LANES(done) && !done)
-      lost = multipole_body(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, o, op);
+      lost = multipole_body<SH>(x, xp, y, yp, s, qx, qy, dzLoss, dp, expand, o, op);
     if (!lost) {
-      if (!in_aperture(x + dxa, y + dya, op))
+      if (!in_aperture<SH>(x + dxa, y + dya, op))
         lost = 1;
       else if (!momenta_to_slopes(xp, yp, qx, qy, dp, expand, o))
         lost = 2;
@@ -553,6 +581,7 @@ DEV int track_multipole(particle &p, const eh_op &op) {
  * slice; per-substep coefficients are scalar loads of the element record.  Returns true if the particle is lost; X, QX, Y, QY, dist
  * and dz_lost then hold their values at the loss, as the reference leaves
  * them. */
+template <bool SH>
 DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double DP, double &dist, double &dz_lost,
                      const eh_op &op) {
   const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA], rsgn = op.d[EH_D_RHOA_SGN];
@@ -564,7 +593,7 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
   const int ok_opd = em_recip_ok(opd);
   const int ok_rho = (op.flags & EH_F_RCP) != 0;
   for (int i = 0; i < op.nslices; i++) {
-    if (!in_aperture(X, Y, op))
+    if (!in_aperture<SH>(X, Y, op))
       return true;
     for (int j = 0; j < nsub; j++) {
       double f = sq(opd) - sq(QY);
@@ -647,7 +676,7 @@ DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const do
   double acc = 0, zmax = 0, lo = 1;
   int emin = 0;
   for (int i = 0; i < op.nslices; i++) {
-    good = good && in_aperture(X, Y, op);
+    good = good && in_rect(X, Y, op);
 #pragma unroll
     for (int j = 0; j < nsub; j++) {
       acc += (__builtin_fabs(X) + __builtin_fabs(QX)) + __builtin_fabs(QY);
@@ -696,6 +725,7 @@ DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const do
   return good && acc <= 0x1.0p+700 && emin >= -699 && lo >= 0x1.0p-900 && zmax < EM_SMALL_Z;
 }
 
+template <bool SH>
 DEV int track_csbend(particle &p, const eh_op &op) {
   const int flags = op.flags;
   const double ct = op.d[EH_D_CT], st = op.d[EH_D_ST];
@@ -730,7 +760,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
   /* integrate_csbend_ordn (csbend.c:1434-1693), Fx = 0, Fy = 1 */
   double dist = 0, dz_lost = 0;
   bool lost = false, done = false;
-  if (flags & EH_F_FAST) {
+  if ((flags & EH_F_FAST) && !(SH && op.in_code)) {
     const double X0 = X, QX0 = QX, Y0 = Y, QY0 = QY;
     /* the 4th-order integrator's four substeps of a slice unrolled (NSUB =
      * 4): the substep coefficients load once per slice and the scheduler
@@ -746,10 +776,10 @@ DEV int track_csbend(particle &p, const eh_op &op) {
     }
   }
   if (!ALL_LANES(done) && !done)
-    lost = csbend_body(X, QX, Y, QY, DP, dist, dz_lost, op);
+    lost = csbend_body<SH>(X, QX, Y, QY, DP, dist, dz_lost, op);
   if (!lost) {
     dz_lost = op.d[EH_D_NS];
-    if (!in_aperture(X, Y, op))
+    if (!in_aperture<SH>(X, Y, op))
       lost = true;
     else
       S += dist;
@@ -922,6 +952,7 @@ DEV int track_rfca(particle &p, const eh_op &op) {
   return 0;
 }
 
+template <bool SH>
 DEV int run_op(particle &p, const eh_op &op) {
   int lost = 0;
   switch (op.op) {
@@ -938,10 +969,10 @@ DEV int run_op(particle &p, const eh_op &op) {
     exact_drift(p, op.d[0]);
     break;
   case EH_OP_MULT:
-    lost = track_multipole(p, op);
+    lost = track_multipole<SH>(p, op);
     break;
   case EH_OP_CSBEND:
-    lost = track_csbend(p, op);
+    lost = track_csbend<SH>(p, op);
     break;
   case EH_OP_RFCA:
     lost = track_rfca(p, op);
@@ -959,13 +990,14 @@ DEV int run_op(particle &p, const eh_op &op) {
   }
   if (lost)
     return lost;
-  return post_element_checks(p, op);
+  return post_element_checks<SH>(p, op);
 }
 
 /* Four waves per SIMD (up to 128 VGPRs): the branch-free bend body needs
  * about 111 registers in this kernel; squeezing it into five or six waves
  * spills to scratch or parks state in LDS, and measured no faster on the
  * storage ring (29.0-29.3 M against 29.4 M particle-turns/s). */
+template <bool SH>
 __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
                                                        int64_t pass0, int32_t npasses, int32_t op_first,
                                                        int32_t op_last) {
@@ -987,7 +1019,7 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
     const int32_t ob = (ip == 0) ? op_first : 0;
     const int32_t oe = (ip == npasses - 1) ? op_last : nops;
     for (int32_t io = ob; io < oe; io++) {
-      lost = run_op(p, ops[io]);
+      lost = run_op<SH>(p, ops[io]);
       if (lost) {
         lost_op = io;
         lost_pass = pass0 + ip;
@@ -1012,7 +1044,7 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
  * particles of the chunk through the variant's program.  The chunk queue is
  * ordered by variant so that workgroups resident at the same time share a few
  * programs in L2. */
-template <int PPL>
+template <int PPL, bool SH>
 __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op *__restrict__ ops,
                                                        const eh_vdesc *__restrict__ vars,
                                                        const eh_chunk *__restrict__ chunks, int32_t nchunks,
@@ -1063,7 +1095,7 @@ __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op 
         for (int k = 0; k < PPL; k++) {
           if (!live[k])
             continue;
-          int l = run_op(p[k], op);
+          int l = run_op<SH>(p[k], op);
           if (l) {
             live[k] = false;
             lost[k] = l;
@@ -1251,6 +1283,12 @@ int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops) {
     HIPCHK(d, hipMemcpyAsync(d->ops, ops, sizeof(eh_op) * nops, hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
   d->nops = nops;
+  /* programs with an elliptical or open-sided MAXAMP run on the kernel
+   * instance that evaluates those shapes */
+  d->shaped = 0;
+  for (int32_t k = 0; k < nops; k++)
+    if (ops[k].ap_code || ops[k].in_code)
+      d->shaped = 1;
   return EH_OK;
 }
 
@@ -1260,6 +1298,8 @@ int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op) {
   HIPCHK(d, hipSetDevice(d->device));
   HIPCHK(d, hipMemcpyAsync(d->ops + index, op, sizeof(eh_op), hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
+  if (op->ap_code || op->in_code)
+    d->shaped = 1;
   return EH_OK;
 }
 
@@ -1287,8 +1327,12 @@ int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int3
   const int block = d->block;
   const int64_t grid = (d->n + block - 1) / block;
   HIPCHK(d, hipEventRecord(d->ev0, d->stream));
-  hipLaunchKernelGGL(eh_track_kernel, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops, pass0,
-                     npasses, op_first, op_last);
+  if (d->shaped)
+    hipLaunchKernelGGL(eh_track_kernel<true>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops,
+                       pass0, npasses, op_first, op_last);
+  else
+    hipLaunchKernelGGL(eh_track_kernel<false>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops,
+                       pass0, npasses, op_first, op_last);
   HIPCHK(d, hipGetLastError());
   HIPCHK(d, hipEventRecord(d->ev1, d->stream));
   HIPCHK(d, hipEventSynchronize(d->ev1));
@@ -1401,12 +1445,19 @@ int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_c
 static int batch_launch(ehd_dev *d, eh_state st, int64_t pass0, int32_t npasses, int32_t ppl, int32_t block,
                         int grid) {
   hipLaunchKernelGGL(eh_queue_reset, dim3(1), dim3(1), 0, d->stream, d->queue);
-  if (ppl == 2)
-    hipLaunchKernelGGL(eh_batch_kernel<2>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
-                       d->nchunks, d->use_perm ? d->perm : (const int32_t *)NULL, pass0, npasses, d->queue);
+  const int32_t *perm = d->use_perm ? d->perm : (const int32_t *)NULL;
+  if (ppl == 2 && d->shaped)
+    hipLaunchKernelGGL((eh_batch_kernel<2, true>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
+                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+  else if (ppl == 2)
+    hipLaunchKernelGGL((eh_batch_kernel<2, false>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
+                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+  else if (d->shaped)
+    hipLaunchKernelGGL((eh_batch_kernel<1, true>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
+                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
   else
-    hipLaunchKernelGGL(eh_batch_kernel<1>, dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc, d->chunks,
-                       d->nchunks, d->use_perm ? d->perm : (const int32_t *)NULL, pass0, npasses, d->queue);
+    hipLaunchKernelGGL((eh_batch_kernel<1, false>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
+                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
   return EH_OK;
 }
 
@@ -1431,10 +1482,14 @@ int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_p
   st.loss_code = d->loss_code;
   st.n = d->n;
   int per_cu = 0;
-  if (ppl == 2)
-    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<2>, block, 0));
+  if (ppl == 2 && d->shaped)
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<2, true>, block, 0));
+  else if (ppl == 2)
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<2, false>, block, 0));
+  else if (d->shaped)
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<1, true>, block, 0));
   else
-    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<1>, block, 0));
+    HIPCHK(d, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eh_batch_kernel<1, false>, block, 0));
   if (per_cu < 1)
     per_cu = 1;
   int grid = d->num_cu * per_cu;
@@ -1540,6 +1595,9 @@ __global__ void eh_math_probe_kernel(int32_t fn, const double *in, const double 
     break;
   case 9: /* the same quotient from the rounded reciprocal (bend radii) */
     r = em_div(v, in2[i], 1.0 / in2[i], em_recip_ok(in2[i]));
+    break;
+  case 10: /* the aperture's integer power, exponent in in2 (1..EH_MAX_AP_EXPONENT) */
+    r = ap_ipow(v, (int)in2[i]);
     break;
   default:
     r = v / in2[i];

@@ -43,6 +43,7 @@ struct ehd_dev {
   int32_t *perm;     /* compacted particle indices the chunks refer to (NULL: identity) */
   int64_t perm_cap;
   int use_perm;
+  int shaped; /* the loaded program has an elliptical or open-sided MAXAMP */
   double *snap[6];
   int32_t *snap_lp, *snap_lc;
   int64_t snap_cap;

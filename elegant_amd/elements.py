@@ -59,7 +59,7 @@ _NAMES = {
                      "SMOOTHING": "smoothing", "ALLOW_LONG_BEAM": "allowLongBeam",
                      "BUNCHED_BEAM_MODE": "bunchedBeamMode"},
     A.EH_T_MAXAMP: {"X_MAX": "x_max", "Y_MAX": "y_max", "ELLIPTICAL": "elliptical", "EXPONENT": "exponent",
-                    "YEXPONENT": "yExponent"},
+                    "YEXPONENT": "yExponent", "OPEN_SIDE": "openSide"},
     A.EH_T_CHARGE: {"TOTAL": "charge", "PER_PARTICLE": "chargePerParticle"},
     A.EH_T_MARK: {},
 }
@@ -77,6 +77,22 @@ _DEFAULTS = {
     A.EH_T_ZLONGIT: {"n_bins": 128, "factor": 1.0, "bunchedBeamMode": 1},
     A.EH_T_MAXAMP: {"exponent": 2},
 }
+
+
+# determineOpenSideCode (limit_amplitudes.c:1130-1162; codes track.h:1515-1518)
+OPEN_SIDE_CODES = {"+x": A.EH_OPEN_PLUS_X, "+y": A.EH_OPEN_PLUS_Y, "-x": A.EH_OPEN_MINUS_X, "-y": A.EH_OPEN_MINUS_Y}
+
+
+def open_side_code(text):
+    """MAXAMP OPEN_SIDE string -> code: "+x"/"+h", "+y"/"+v", "-x"/"-h",
+    "-y"/"-v" in either case; "" -> 0 (no open side); anything else is an
+    error, as in the reference"""
+    if text == "":
+        return 0
+    t = text.lower().replace("h", "x").replace("v", "y")
+    if len(text) != 2 or t not in OPEN_SIDE_CODES:
+        raise ValueError("open_side=%s is not valid" % text)
+    return OPEN_SIDE_CODES[t]
 
 
 def make(type_code, name="", **kw):
@@ -97,6 +113,8 @@ def make(type_code, name="", **kw):
                 raise KeyError("%s has no parameter %s" % (A.TYPE_NAMES[type_code], k))
             if key == "FIDUCIAL" and isinstance(v, str):
                 v = {"tmean": A.EH_FID_TMEAN, "light": A.EH_FID_LIGHT}[v.lower()]
+            if key == "OPEN_SIDE" and isinstance(v, str):
+                v = open_side_code(v)
             setattr(sub, names[key], v)
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])

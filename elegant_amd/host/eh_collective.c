@@ -175,6 +175,7 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
   cc.ap_xmax = op->ap_xmax;
   cc.ap_ymax = op->ap_ymax;
   cc.flags = (op->flags & EH_F_AP) | flags;
+  cc.ap_code = op->ap_code;
   if (e->type == EH_T_WAKE) {
     const EH_WAKE *w = &e->p.wake;
     double dt, span;

@@ -184,17 +184,77 @@ static void rotation_cs(double angle, double *c, double *s) {
 
 static int rotation_active(double angle) { return angle && !(fabs(fabs(angle) - EH_PIx2) < 1e-12); }
 
+/* x and y exponents of an elliptical MAXAMP (YEXPONENT 0 = EXPONENT) */
+static int32_t maxamp_xe(const EH_MAXAMP *m) { return m->exponent; }
+static int32_t maxamp_ye(const EH_MAXAMP *m) { return m->yExponent ? m->yExponent : m->exponent; }
+
+/* setupMultApertureData from a MAXAMP (multipole.c:1725-1763): the
+ * integrators' checkMultAperture takes the elliptical branch only with both
+ * half-axes positive (multipole.c:1807) */
 static void set_in_aperture(eh_op *op, double reverseTilt, const EH_MAXAMP *maxamp) {
   op->in_cs0 = cos(reverseTilt);
   op->in_cs1 = sin(reverseTilt);
   if (reverseTilt)
     op->flags |= EH_F_INTILT;
   if (maxamp) {
+    int ell = maxamp->elliptical && maxamp->x_max > 0 && maxamp->y_max > 0;
     op->in_xmax = maxamp->x_max;
     op->in_ymax = maxamp->y_max;
     if (op->in_xmax > 0 || op->in_ymax > 0)
       op->flags |= EH_F_INAP;
+    op->in_code = ell ? EH_AP_CODE(1, maxamp->openSide, maxamp_xe(maxamp), maxamp_ye(maxamp))
+                      : EH_AP_CODE(0, maxamp->openSide, 0, 0);
   }
+}
+
+/* MAXAMP parameters the checks can evaluate: an open-side code of
+ * determineOpenSideCode (limit_amplitudes.c:1130-1162) and, for an ellipse,
+ * even exponents >= 2 (elimit_amplitudes rejects others,
+ * limit_amplitudes.c:507-521) */
+static int check_maxamp(const eh_element *e, char *msg, int msglen) {
+  const EH_MAXAMP *m = &e->p.maxamp;
+  if (m->openSide < 0 || m->openSide > 4) {
+    snprintf(msg, msglen, "%s: OPEN_SIDE code %d is not one of 0 (none), +x, +y, -x, -y", e->name, m->openSide);
+    return EH_ERR_INVALID;
+  }
+  if (m->elliptical && (m->x_max || m->y_max)) {
+    int32_t xe = maxamp_xe(m), ye = maxamp_ye(m);
+    if (xe < 2 || xe % 2 || ye < 2 || ye % 2) {
+      snprintf(msg, msglen, "%s: exponent=%d is not valid.  Give even integer >=2", e->name, xe < 2 || xe % 2 ? xe : ye);
+      return EH_ERR_INVALID;
+    }
+    if (xe > EH_MAX_AP_EXPONENT || ye > EH_MAX_AP_EXPONENT) {
+      snprintf(msg, msglen, "%s: exponents above %d are not supported", e->name, EH_MAX_AP_EXPONENT);
+      return EH_ERR_UNSUPPORTED;
+    }
+  }
+  return EH_OK;
+}
+
+/* The MAXAMP check do_tracking makes after an element (do_tracking.c:2461-2470),
+ * with the early returns of limit_amplitudes (xmax < 0 and ymax < 0: no
+ * check at all, limit_amplitudes.c:215-217) and elimit_amplitudes (a
+ * non-positive half-axis: the rectangle in the other plane, or nothing,
+ * limit_amplitudes.c:523-528) resolved here.  Returns 0 when no check is
+ * made, else 1 with *code the EH_AP_* shape (0 = closed rectangle). */
+static int post_aperture(const EH_MAXAMP *m, double x_max, double y_max, int32_t *code) {
+  *code = 0;
+  if (!(x_max || y_max))
+    return 0;
+  if (m && m->elliptical) {
+    if (x_max <= 0 || y_max <= 0) {
+      if (!(x_max > 0 || y_max > 0))
+        return 0;
+      *code = EH_AP_CODE(0, m->openSide, 0, 0);
+      return 1;
+    }
+    *code = EH_AP_CODE(1, m->openSide, maxamp_xe(m), maxamp_ye(m));
+    return 1;
+  }
+  if (x_max < 0 && y_max < 0)
+    return 0;
+  *code = m ? EH_AP_CODE(0, m->openSide, 0, 0) : 0;
+  return 1;
 }
 
 /* multipole_tracking2 set-up (multipole.c:765-1017) */
@@ -589,10 +649,9 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
       keep = 0; /* NO_APERTURE; the NaN cull after them cannot fire (every element ends with it) */
       break;
     case EH_T_MAXAMP:
-      if (e->p.maxamp.elliptical || e->p.maxamp.openSide) {
-        snprintf(msg, msglen, "%s: only rectangular MAXAMP without OPEN_SIDE is supported", e->name);
-        return EH_ERR_UNSUPPORTED;
-      }
+      rc = check_maxamp(e, msg, msglen);
+      if (rc)
+        return rc;
       maxamp = &e->p.maxamp;
       x_max = maxamp->x_max;
       y_max = maxamp->y_max;
@@ -652,7 +711,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
       return rc;
     if (!keep)
       continue;
-    if (x_max || y_max) {
+    if (post_aperture(maxamp, x_max, y_max, &op.ap_code)) {
       op.flags |= EH_F_AP;
       op.ap_xmax = x_max;
       op.ap_ymax = y_max;

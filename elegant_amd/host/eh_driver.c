@@ -819,66 +819,102 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
   return r;
 }
 
-int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
+/* one MAXAMP check on its own, at z: a zero-length step whose only work is
+ * the check do_tracking makes after an element (do_tracking.c:2461-2470)
+ * with the given aperture */
+static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, const EH_MAXAMP *m, double z, double Po,
                             int32_t extrapolate_z) {
-  /* the aperture check of do_tracking.c:2463-2467 after a zero-length MARK-like step */
-  eh_element e;
-  if (!c)
-    return EH_ERR_INVALID;
-  if (np <= 0 || (xmax < 0 && ymax < 0))
-    return np;
-  eh_element_defaults(&e, EH_T_DRIF);
-  e.p.drift.length = 0;
+  eh_element bl[2];
+  int rc;
+  int64_t nLeft = 0;
+  eh_element_defaults(&bl[0], EH_T_MAXAMP);
+  bl[0].p.maxamp = *m;
+  eh_element_defaults(&bl[1], EH_T_DRIF);
+  bl[1].p.drift.length = 0;
+  rc = eh_compile(bl, 2, Po, c->prog, c->msg, sizeof(c->msg));
+  if (rc)
+    return rc;
+  c->prog->ops[0] = c->prog->ops[1];
+  c->prog->ops[0].op = EH_OP_NOP;
+  c->prog->ops[0].z_end = z;
+  if (!extrapolate_z)
+    c->prog->ops[0].flags &= ~EH_F_EXTRAP;
+  c->prog->nops = 1;
+  c->Po = Po;
+  c->loaded = 1;
+  rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+  if (rc)
+    return devfail(c, rc);
+  /* NaN and infinite rows are the check's own to remove, at z */
+  rc = eh_upload_rows(c, coord, np, 8, 0);
+  if (rc)
+    return rc;
+  rc = ehd_track(c->dev, 0, 1, 0, 1);
+  if (rc)
+    return devfail(c, rc);
   {
-    eh_element bl[2];
-    int rc;
-    int64_t nLeft = 0;
-    eh_element_defaults(&bl[0], EH_T_MAXAMP);
-    bl[0].p.maxamp.x_max = xmax;
-    bl[0].p.maxamp.y_max = ymax;
-    bl[1] = e;
-    rc = eh_compile(bl, 2, Po, c->prog, c->msg, sizeof(c->msg));
-    if (rc)
-      return rc;
-    c->prog->ops[0] = c->prog->ops[1];
-    c->prog->ops[0].op = EH_OP_NOP;
-    c->prog->ops[0].z_end = z;
-    if (!extrapolate_z)
-      c->prog->ops[0].flags &= ~EH_F_EXTRAP;
-    c->prog->nops = 1;
-    c->Po = Po;
-    c->loaded = 1;
-    rc = ehd_put_ops(c->dev, c->prog->ops, 1);
-    if (rc)
-      return devfail(c, rc);
-    /* NaN and infinite rows are limit_amplitudes' own to remove, at z */
-    rc = eh_upload_rows(c, coord, np, 8, 0);
-    if (rc)
-      return rc;
-    rc = ehd_track(c->dev, 0, 1, 0, 1);
-    if (rc)
-      return devfail(c, rc);
-    {
-      double **tmp = (double **)malloc(sizeof(double *) * np);
-      double *buf = (double *)malloc(sizeof(double) * 8 * np);
-      if (!tmp || !buf) {
-        free(tmp);
-        free(buf);
-        return EH_ERR_NOMEM;
-      }
-      for (int64_t i = 0; i < np; i++)
-        tmp[i] = buf + 8 * i;
-      rc = eh_download(c, tmp, np, 8, &nLeft);
-      if (!rc)
-        for (int64_t i = 0; i < np; i++)
-          memcpy(coord[i], tmp[i], sizeof(double) * 7);
+    double **tmp = (double **)malloc(sizeof(double *) * np);
+    double *buf = (double *)malloc(sizeof(double) * 8 * np);
+    if (!tmp || !buf) {
       free(tmp);
       free(buf);
-      if (rc)
-        return rc;
+      return EH_ERR_NOMEM;
     }
-    return nLeft;
+    for (int64_t i = 0; i < np; i++)
+      tmp[i] = buf + 8 * i;
+    rc = eh_download(c, tmp, np, 8, &nLeft);
+    if (!rc)
+      for (int64_t i = 0; i < np; i++)
+        memcpy(coord[i], tmp[i], sizeof(double) * 7);
+    free(tmp);
+    free(buf);
+    if (rc)
+      return rc;
   }
+  return nLeft;
+}
+
+int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
+                            int32_t extrapolate_z, int32_t openCode) {
+  EH_MAXAMP m;
+  if (!c || openCode < 0 || openCode > 4)
+    return EH_ERR_INVALID;
+  /* limit_amplitudes.c:215-217 */
+  if (np <= 0 || (xmax < 0 && ymax < 0))
+    return np;
+  memset(&m, 0, sizeof(m));
+  m.x_max = xmax;
+  m.y_max = ymax;
+  m.exponent = 2;
+  m.openSide = openCode;
+  return run_aperture(c, coord, np, &m, z, Po, extrapolate_z);
+}
+
+int64_t eh_elimit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
+                             int32_t extrapolate_z, int32_t openCode, int32_t exponent, int32_t yexponent) {
+  EH_MAXAMP m;
+  if (!c || openCode < 0 || openCode > 4)
+    return EH_ERR_INVALID;
+  /* limit_amplitudes.c:507-521 */
+  if (exponent < 2 || exponent % 2 || (yexponent && (yexponent < 2 || yexponent % 2)))
+    return EH_ERR_INVALID;
+  /* a non-positive half-axis: the rectangle in the other plane, or no
+   * check at all (limit_amplitudes.c:523-528) */
+  if (xmax <= 0 || ymax <= 0) {
+    if (xmax > 0 || ymax > 0)
+      return eh_limit_amplitudes(c, coord, xmax, ymax, np, z, Po, extrapolate_z, openCode);
+    return np;
+  }
+  if (np <= 0)
+    return np;
+  memset(&m, 0, sizeof(m));
+  m.x_max = xmax;
+  m.y_max = ymax;
+  m.elliptical = 1;
+  m.exponent = exponent;
+  m.yExponent = yexponent;
+  m.openSide = openCode;
+  return run_aperture(c, coord, np, &m, z, Po, extrapolate_z);
 }
 
 /* debugging/measurement helpers (not in the public header) */

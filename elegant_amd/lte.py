@@ -274,7 +274,7 @@ class Lattice:
             return E.WAKE(table, label, **kw)
         try:
             return E.make(tcode, label, **kw)
-        except KeyError as err:
+        except (KeyError, ValueError) as err:
             raise LatticeError("%s: %s" % (label, err))
 
 
@@ -318,6 +318,10 @@ def dumps(beamline, line_name="RING"):
                 if key in ("N_KICKS",) and e.type == A.EH_T_CSBEND:
                     continue   # alias of N_SLICES for CSBEND
                 v = getattr(sub, attr)
+                if key == "OPEN_SIDE" and e.type == A.EH_T_MAXAMP:
+                    if v:
+                        parts.append('OPEN_SIDE="%s"' % {c: t for t, c in E.OPEN_SIDE_CODES.items()}[v])
+                    continue
                 if isinstance(v, (int, float)) and v != getattr(ref, attr):
                     parts.append("%s=%r" % (key, v))
         if e.type == A.EH_T_WAKE:

@@ -51,7 +51,9 @@ SIGNATURES = [
     ("eh_simple_rf_cavity", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_RFCA), PD, C.c_double, PD]),
     ("eh_track_particles_drift", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_DRIFT)]),
     ("eh_limit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
-                                        C.c_int32]),
+                                        C.c_int32, C.c_int32]),
+    ("eh_elimit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("eh_track_through_wake", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_WAKE), C.c_double, C.c_int64,
                                           C.c_double]),
     ("eh_track_through_zlongit", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_ZLONGIT), C.c_double,
@@ -333,10 +335,15 @@ class Tracker:
         ptr, keep = A.rows_pointer(rows)
         return check(self._ctx, lib().eh_track_particles_drift(self._ctx, ptr, rows.shape[0], C.byref(drift)))
 
-    def limit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z):
+    def limit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0):
         ptr, keep = A.rows_pointer(rows)
         return check(self._ctx, lib().eh_limit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], z, Po,
-                                                          extrapolate_z))
+                                                          extrapolate_z, open_code))
+
+    def elimit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0, exponent=2, yexponent=0):
+        ptr, keep = A.rows_pointer(rows)
+        return check(self._ctx, lib().eh_elimit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], z, Po,
+                                                           extrapolate_z, open_code, exponent, yexponent))
 
     def track_through_wake(self, rows, wake, Po, i_pass, macro_charge):
         ptr, keep = A.rows_pointer(rows)

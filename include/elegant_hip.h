@@ -142,10 +142,19 @@ typedef struct {
   int32_t smoothing, allowLongBeam, bunchedBeamMode, pad;
 } EH_ZLONGIT;
 
-/* MAXAMP (track.h:1955-1961; defaults track_data.c:990-997) */
+/* MAXAMP (track.h:1955-1961; defaults track_data.c:990-997).  ELLIPTICAL
+ * selects (x/x_max)^exponent + (y/y_max)^yExponent (yExponent 0: exponent;
+ * even, >= 2, at most EH_MAX_AP_EXPONENT here).  OPEN_SIDE is held as the
+ * code determineOpenSideCode returns (limit_amplitudes.c:1130-1162) for
+ * "+x"/"+h", "+y"/"+v", "-x"/"-h", "-y"/"-v" (track.h:1515-1518); 0 = none. */
+#define EH_OPEN_PLUS_X 1
+#define EH_OPEN_PLUS_Y 2
+#define EH_OPEN_MINUS_X 3
+#define EH_OPEN_MINUS_Y 4
+#define EH_MAX_AP_EXPONENT 254
 typedef struct {
   double x_max, y_max;
-  int32_t elliptical, exponent, yExponent, openSide; /* openSide: 0 = none */
+  int32_t elliptical, exponent, yExponent, openSide;
 } EH_MAXAMP;
 
 /* CHARGE (track.h:3220-3226) */
@@ -244,9 +253,15 @@ int64_t eh_simple_rf_cavity(eh_ctx *ctx, double **part, int64_t np, const EH_RFC
                             double zEnd, double *phase_fiducial);
 /* track_particles with the order-2 drift matrix (matrix.c:294-468) */
 int64_t eh_track_particles_drift(eh_ctx *ctx, double **part, int64_t n_part, const EH_DRIFT *drift);
-/* limit_amplitudes (limit_amplitudes.c:193-272) */
+/* limit_amplitudes (limit_amplitudes.c:193-272); openCode as EH_MAXAMP.openSide */
 int64_t eh_limit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np, double z,
-                            double Po, int32_t extrapolate_z);
+                            double Po, int32_t extrapolate_z, int32_t openCode);
+/* elimit_amplitudes (limit_amplitudes.c:472-598): the (super-)elliptical
+ * aperture; lost rows get delta = sqrt((Po(1+delta))^2 + 1) as there.
+ * Odd or < 2 exponents give EH_ERR_INVALID (the reference exits). */
+int64_t eh_elimit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np, double z,
+                             double Po, int32_t extrapolate_z, int32_t openCode, int32_t exponent,
+                             int32_t yexponent);
 /* track_through_wake (wake.c:23-303); macroParticleCharge as set by CHARGE */
 int64_t eh_track_through_wake(eh_ctx *ctx, double **part, int64_t np, const EH_WAKE *wake, double Po,
                               int64_t i_pass, double macroParticleCharge);

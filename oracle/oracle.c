@@ -27,7 +27,10 @@
  * checked against mathematics, not against the toolkit's bits; the physical
  * constants (CODATA 2018, eh_consts.h) are this build's choice; and the
  * bit-level operation order inside each restated routine rests on review of
- * the cited lines, since the reference cannot be compiled here.
+ * the cited lines, since the reference cannot be compiled here.  The
+ * toolkit's ipow (aperture powers) is the exception among the toolkit
+ * routines: the reference ships a device copy of it whose multiplication
+ * order is followed here.
  */
 #include "oracle.h"
 
@@ -62,6 +65,8 @@ int64_t or_ctx_effort(const or_ctx *c) { return c->effort; }
 
 /* evaluate the per-particle math of this build on an array (fn: 0 sin, 1 cos,
  * 2 asin, 3 log, 4 tan, 5 sqrt, 6 in/in2) */
+static double or_ipow(double x, long p);
+
 void or_math_eval(int fn, const double *in, const double *in2, double *out, long n) {
   for (long i = 0; i < n; i++) {
     double v = in[i];
@@ -83,6 +88,9 @@ void or_math_eval(int fn, const double *in, const double *in2, double *out, long
       break;
     case 5:
       out[i] = sqrt(v);
+      break;
+    case 7:
+      out[i] = or_ipow(v, (long)in2[i]);
       break;
     default:
       out[i] = v / in2[i];
@@ -228,27 +236,105 @@ static void offset_for_misalignment(double **coord, long np, double dx, double d
 
 /* ------------------------------------------------------------ apertures */
 
+/* ipow: x^p by the fixed multiplication order of the SDDS toolkit's ipow,
+ * as the reference's own device copy spells it out
+ * (gpuElegant/gpu_bin_transverse_distribution.cu:8-56): explicit products
+ * up to p = 8, then the square of ipow(x, p/2), times x when p is odd.
+ * x = 0 gives 0 (p > 0).  Only p >= 1 occurs here. */
+static double or_ipow(double x, long p) {
+  double h;
+  if (!x)
+    return p == 0 ? 1.0 : 0.0;
+  switch (p) {
+  case 0:
+    return 1.0;
+  case 1:
+    return x;
+  case 2:
+    return x * x;
+  case 3:
+    h = x * x;
+    return h * x;
+  case 4:
+    h = x * x;
+    return h * h;
+  case 5:
+    h = x * x;
+    return h * h * x;
+  case 6:
+    h = x * x;
+    return h * h * h;
+  case 7:
+    h = x * x * x;
+    return h * h * x;
+  case 8:
+    h = x * x;
+    h = h * h;
+    return h * h;
+  default:
+    h = or_ipow(x, p / 2);
+    return (p & 1) ? h * h * x : h * h;
+  }
+}
+
+/* evaluateLostWithOpenSides (limit_amplitudes.c:1105-1128): a particle
+ * outside the aperture on its open side, within the other plane's
+ * half-width (or with that plane open), is not lost.  Codes as track.h:1515-1518. */
+static long or_open_side_lost(long code, double dx, double dy, double xsize, double ysize) {
+  long lost = 1;
+  switch (code) {
+  case EH_OPEN_PLUS_X:
+    if (dx > 0 && (fabs(dy) < ysize || ysize <= 0))
+      lost = 0;
+    break;
+  case EH_OPEN_MINUS_X:
+    if (dx < 0 && (fabs(dy) < ysize || ysize <= 0))
+      lost = 0;
+    break;
+  case EH_OPEN_PLUS_Y:
+    if (dy > 0 && (fabs(dx) < xsize || xsize <= 0))
+      lost = 0;
+    break;
+  case EH_OPEN_MINUS_Y:
+    if (dy < 0 && (fabs(dx) < xsize || xsize <= 0))
+      lost = 0;
+    break;
+  default:
+    break;
+  }
+  return lost;
+}
+
 typedef struct {
   double xMax, yMax, xCen, yCen, reverseTilt, cs0, cs1;
-  int present;
+  int present, elliptical, xExponent, yExponent, openSide;
 } mult_ap;
 
-/* setupMultApertureData for a rectangular MAXAMP (multipole.c:1725-1777) */
+/* setupMultApertureData from a MAXAMP (multipole.c:1725-1763; no
+ * aperture file, no APCONTOUR) */
 static void setup_mult_ap(mult_ap *ap, double reverseTilt, const EH_MAXAMP *maxamp) {
   ap->reverseTilt = reverseTilt;
   ap->cs0 = cos(reverseTilt);
   ap->cs1 = sin(reverseTilt);
   ap->xCen = ap->yCen = 0;
   ap->xMax = ap->yMax = 0;
-  ap->present = 0;
+  ap->present = ap->elliptical = ap->openSide = 0;
+  ap->xExponent = ap->yExponent = 0;
   if (maxamp) {
     ap->xMax = maxamp->x_max;
     ap->yMax = maxamp->y_max;
+    ap->elliptical = maxamp->elliptical;
     ap->present = ap->xMax > 0 || ap->yMax > 0;
+    ap->xExponent = ap->yExponent = maxamp->exponent;
+    if (maxamp->yExponent)
+      ap->yExponent = maxamp->yExponent;
+    ap->openSide = maxamp->openSide;
   }
 }
 
-/* checkMultAperture, rectangular branch (multipole.c:1779-1833) */
+/* checkMultAperture (multipole.c:1786-1833): rectangle (strict >), or
+ * (super-)ellipse (x/a)^p + (y/b)^q >= 1 when elliptical with both
+ * half-axes positive; an open side can spare the particle either way */
 static int check_mult_ap(double xInput, double yInput, const mult_ap *ap) {
   double x = xInput - ap->xCen;
   double y = yInput - ap->yCen;
@@ -257,13 +343,25 @@ static int check_mult_ap(double xInput, double yInput, const mult_ap *ap) {
     x = x0 * ap->cs0 + y0 * ap->cs1;
     y = -x0 * ap->cs1 + y0 * ap->cs0;
   }
-  if ((ap->xMax > 0 && fabs(x) > ap->xMax) || (ap->yMax > 0 && fabs(y) > ap->yMax))
-    return 0;
+  if (ap->elliptical == 0 || ap->xMax <= 0 || ap->yMax <= 0) {
+    if ((ap->xMax > 0 && fabs(x) > ap->xMax) || (ap->yMax > 0 && fabs(y) > ap->yMax)) {
+      if (ap->openSide == 0 || or_open_side_lost(ap->openSide, x, y, ap->xMax, ap->yMax))
+        return 0;
+    }
+  } else {
+    double xa = x / ap->xMax;
+    double yb = y / ap->yMax;
+    if ((or_ipow(xa, ap->xExponent) + or_ipow(yb, ap->yExponent)) >= 1) {
+      if (ap->openSide == 0 || or_open_side_lost(ap->openSide, x, y, ap->xMax, ap->yMax))
+        return 0;
+    }
+  }
   return 1;
 }
 
-/* limit_amplitudes (limit_amplitudes.c:193-272) for a closed rectangle */
-long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z) {
+/* limit_amplitudes (limit_amplitudes.c:193-272) */
+long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                         long openCode) {
   long ip, itop, is_out;
   double *part, dz, dzx, dzy;
   if (xmax < 0 && ymax < 0)
@@ -276,10 +374,12 @@ long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, doub
       is_out += 1;
     if (ymax > 0 && fabs(part[2]) > ymax)
       is_out += 2;
+    if (openCode)
+      is_out *= or_open_side_lost(openCode, part[0], part[2], xmax, ymax);
     if (isinf(part[0]) || isinf(part[2]) || isnan(part[0]) || isnan(part[2]))
       is_out += 4;
     dz = 0;
-    if (is_out && !(is_out & 4) && extrapolate_z) {
+    if (is_out && !(is_out & 4) && !openCode && extrapolate_z) {
       dzx = dzy = -DBL_MAX;
       if (is_out & 1 && part[1] != 0)
         dzx = (part[0] - (part[1] > 0 ? 1 : -1) * xmax) / part[1];
@@ -305,6 +405,85 @@ long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, doub
     }
   }
   return np;
+}
+
+/* elimit_amplitudes (limit_amplitudes.c:472-598): the (super-)elliptical
+ * MAXAMP after an element.  Its loss rows differ from limit_amplitudes':
+ * delta becomes sqrt((Po(1+delta))^2 + 1), and the loss point is
+ * extrapolated back along a drift only for a closed true ellipse.  Invalid
+ * exponents are an error (the reference exits). */
+long or_elimit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                          long openCode, long exponent, long yexponent) {
+  long ip, itop, lost, xe, ye;
+  double *part, a2, b2, c0, c1, c2, dz, det;
+  if ((xe = exponent) < 2 || xe % 2)
+    return EH_ERR_INVALID;
+  ye = xe;
+  if (yexponent) {
+    if ((ye = yexponent) < 2 || ye % 2)
+      return EH_ERR_INVALID;
+  }
+  if (xmax <= 0 || ymax <= 0) {
+    if (xmax > 0 || ymax > 0)
+      return or_limit_amplitudes(coord, xmax, ymax, np, z, Po, extrapolate_z, openCode);
+    return np;
+  }
+  a2 = or_ipow(xmax, xe);
+  b2 = or_ipow(ymax, ye);
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    part = coord[ip];
+    if (isinf(part[0]) || isinf(part[2]) || isnan(part[0]) || isnan(part[2])) {
+      swap_rows(coord[ip], coord[itop], OR_NPROPS);
+      coord[itop][4] = z;
+      coord[itop][5] = sqrt((Po * (1 + coord[itop][5])) * (Po * (1 + coord[itop][5])) + 1);
+      --itop;
+      --ip;
+      np--;
+      continue;
+    }
+    lost = 0;
+    if ((or_ipow(part[0], xe) / a2 + or_ipow(part[2], ye) / b2) > 1)
+      lost = 1;
+    else
+      continue;
+    if (openCode)
+      lost *= or_open_side_lost(openCode, part[0], part[2], xmax, ymax);
+    if (lost) {
+      dz = 0;
+      if (extrapolate_z && !openCode && xe == 2 && ye == 2) {
+        c0 = part[0] * part[0] / a2 + part[2] * part[2] / b2 - 1;
+        c1 = 2 * (part[0] * part[1] / a2 + part[2] * part[3] / b2);
+        c2 = part[1] * part[1] / a2 + part[3] * part[3] / b2;
+        det = c1 * c1 - 4 * c0 * c2;
+        if (z > 0 && c2 && det >= 0) {
+          if ((dz = (-c1 + sqrt(det)) / (2 * c2)) > 0)
+            dz = (-c1 - sqrt(det)) / (2 * c2);
+          if ((z + dz) < 0)
+            dz = -z;
+          part[0] += dz * part[1];
+          part[2] += dz * part[3];
+        }
+      }
+      swap_rows(coord[ip], coord[itop], OR_NPROPS);
+      coord[itop][4] = z + dz;
+      coord[itop][5] = sqrt((Po * (1 + coord[itop][5])) * (Po * (1 + coord[itop][5])) + 1);
+      --itop;
+      --ip;
+      np--;
+    }
+  }
+  return np;
+}
+
+/* the MAXAMP check do_tracking makes after an element (do_tracking.c:2461-2470) */
+static long or_maxamp_check(double **coord, const EH_MAXAMP *maxamp, double x_max, double y_max, long np, double z,
+                            double Po, long extrapolate_z) {
+  long openCode = maxamp ? maxamp->openSide : 0;
+  if (!maxamp || !maxamp->elliptical)
+    return or_limit_amplitudes(coord, x_max, y_max, np, z, Po, extrapolate_z, openCode);
+  return or_elimit_amplitudes(coord, x_max, y_max, np, z, Po, extrapolate_z, openCode, maxamp->exponent,
+                              maxamp->yExponent);
 }
 
 /* removeInvalidParticles (limit_amplitudes.c:274-320) */
@@ -1618,7 +1797,7 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
   c->effort = 0;
   /* check_nan: limit_amplitudes(DBL_MAX) once before the pass loop, at z = 0,
    * losses recorded on pass 0 (do_tracking.c:398, 468-473) */
-  nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nToTrack, 0.0, *P_central, 0);
+  nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nToTrack, 0.0, *P_central, 0, 0);
   for (long ip = nLeft; ip < nToTrack; ip++)
     coord[ip][7] = 0;
   nToTrack = nLeft;
@@ -1648,8 +1827,8 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
           c->macroParticleCharge = e->p.charge.chargePerParticle;
         break;
       case EH_T_MAXAMP:
-        if (e->p.maxamp.elliptical || e->p.maxamp.openSide)
-          return EH_ERR_UNSUPPORTED;
+        if (e->p.maxamp.openSide < 0 || e->p.maxamp.openSide > 4)
+          return EH_ERR_INVALID;
         maxamp = &e->p.maxamp;
         x_max = maxamp->x_max;
         y_max = maxamp->y_max;
@@ -1689,9 +1868,12 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
       if (e->type == EH_T_KQUAD || e->type == EH_T_KSEXT || e->type == EH_T_KOCT || e->type == EH_T_CSBEND ||
           e->type == EH_T_RFCA)
         nLeft = rc;
-      if (!no_aperture && (x_max || y_max))
-        nLeft = or_limit_amplitudes(coord, x_max, y_max, nLeft, z, *P_central, e->type == EH_T_DRIF);
-      nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nLeft, z, *P_central, 0);
+      if (!no_aperture && (x_max || y_max)) {
+        nLeft = or_maxamp_check(coord, maxamp, x_max, y_max, nLeft, z, *P_central, e->type == EH_T_DRIF);
+        if (nLeft < 0)
+          return nLeft;
+      }
+      nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nLeft, z, *P_central, 0, 0);
       for (long ip = nLeft; ip < nToTrack; ip++)
         coord[ip][7] = i_pass;
       nToTrack = nLeft;

@@ -63,7 +63,10 @@ long or_simple_rf_cavity(or_ctx *c, long elem_index, double **part, long np, con
 long or_track_through_wake(or_ctx *c, double **part, long np, const EH_WAKE *wake, double Po, long i_pass);
 long or_track_through_zlongit(or_ctx *c, long elem_index, double **part, long np, const EH_ZLONGIT *zl, double Po,
                               long i_pass);
-long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z);
+long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                         long openCode);
+long or_elimit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                          long openCode, long exponent, long yexponent);
 long or_remove_invalid_particles(double **coord, long np, double z, double Po);
 void or_add_to_particle_energy(double *coord, double timeOfFlight, double Po, double dgamma);
 double or_kahan_plus(double oldSum, double b, double *error);

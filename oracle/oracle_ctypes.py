@@ -45,7 +45,11 @@ def lib(mode="glibc"):
         L.or_track_particles_drift.argtypes = [PP, C.c_long, C.c_double, C.c_int]
         L.or_exact_drift.argtypes = [PP, C.c_long, C.c_double]
         L.or_limit_amplitudes.restype = C.c_long
-        L.or_limit_amplitudes.argtypes = [PP, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long]
+        L.or_limit_amplitudes.argtypes = [PP, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long,
+                                          C.c_long]
+        L.or_elimit_amplitudes.restype = C.c_long
+        L.or_elimit_amplitudes.argtypes = [PP, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long,
+                                           C.c_long, C.c_long, C.c_long]
         L.or_add_to_particle_energy.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.or_kahan_plus.restype = C.c_double
         L.or_kahan_plus.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]

@@ -65,3 +65,34 @@ def test_autotune_keeps_results(tracker):
     assert ng == nc
     assert_rows_identical(g, c, "after autotune")
     b.close()
+
+
+@pytest.mark.parametrize("ppl", [1, 2])
+def test_batch_with_shaped_apertures(tracker, ppl):
+    """variants whose MAXAMP is elliptical (one with an open side) next to
+    a rectangular one: the batch runs on the kernel instance that evaluates
+    the shapes, every variant bit-identical to the checker"""
+    from elegant_amd import _abi as A
+    from elegant_amd import elements as E
+    vs = []
+    for k, (bl, Po, rows) in enumerate(variants()):
+        if k < 2:
+            els = list(bl.elements)
+            assert els[0].type == A.EH_T_MAXAMP
+            els[0] = E.MAXAMP("AP", X_MAX=0.015, Y_MAX=0.008, ELLIPTICAL=1, EXPONENT=2 + 2 * k,
+                              OPEN_SIDE="-x" if k else "")
+            bl = E.Beamline(els, keep=bl.keep)
+        vs.append((bl, Po, rows))
+    b = native.Batch(tracker)
+    for bl, Po, rows in vs:
+        b.add_variant(bl, Po, rows)
+    b.set_tuning(ppl, 128, 2, 1, 0)
+    b.track(0, 3)
+    for k, (bl, Po, rows) in enumerate(vs):
+        g, ng = b.download(k)
+        c = rows.copy()
+        o = O.Oracle("emath")
+        nc = o.do_tracking(c, bl, Po, 3)
+        assert ng == nc and ng < rows.shape[0]
+        assert_rows_identical(g, c, "shaped variant %d" % k)
+    b.close()

@@ -4,10 +4,11 @@ collective elements over several passes."""
 import numpy as np
 import pytest
 
+from elegant_amd import _abi as A
 from elegant_amd import elements as E
 from elegant_amd import lattices as Lt
 from oracle import oracle_ctypes as O
-from tests.helpers import assert_rows_identical
+from tests.helpers import assert_rows_bitwise, assert_rows_identical
 from tests.test_oracle import PO, wake_bunch
 
 pytestmark = pytest.mark.gpu
@@ -63,3 +64,24 @@ def test_ring_with_wakes_bit_exact_vs_emath_checker(tracker):
     assert ng == nc and eg == o.effort()
     assert rows.shape[0] - ng > 10
     assert_rows_identical(g, c, "ring with wakes")
+
+
+def test_ring_with_wakes_and_an_elliptical_aperture(tracker):
+    """the check after WAKE and ZLONGIT (inside the kick kernel) with an
+    elliptical MAXAMP open on the +x side"""
+    bl, Po = Lt.storage_ring_wake()
+    assert bl.elements[1].type == A.EH_T_MAXAMP
+    els = list(bl.elements)
+    els[1] = E.MAXAMP("AP", X_MAX=0.015, Y_MAX=0.008, ELLIPTICAL=1, EXPONENT=2, OPEN_SIDE="+x")
+    ring = E.Beamline(els, keep=bl.keep)
+    rows = Lt.matched_beam(ring, 3000, 1e-9, 1e-11, 1e-3, 20e-12, 31, Po)
+    rows[:300, 0] += 0.0149
+    rows[300:400, 2] += 0.0079
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, ring, Po, 3)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, ring, Po, 3)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 10
+    assert_rows_bitwise(g, c, "ring with wakes, elliptical aperture")

@@ -205,7 +205,7 @@ def test_drift_and_limit_amplitudes(tracker):
     O.lib("glibc").or_track_particles_drift(ptr, c.shape[0], 1.3, 2)
     assert_rows_identical(g[:, :7], c[:, :7], "drift")
     ng = tracker.limit_amplitudes(g, 0.004, 0.002, 9.0, PO, 1)
-    nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1)
+    nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1, 0)
     assert ng == nc and 0 < nc < r.shape[0]
     assert_rows_identical(g[:, :7], c[:, :7], "limit_amplitudes")
 
@@ -222,9 +222,118 @@ def test_drift_and_limit_amplitudes_edges(tracker):
     O.lib("glibc").or_track_particles_drift(ptr, c.shape[0], 1.3, 2)
     assert_rows_bitwise(g[:, :7], c[:, :7], "drift edges")
     ng = tracker.limit_amplitudes(g, 0.004, 0.002, 9.0, PO, 1)
-    nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1)
+    nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, 1, 0)
     assert_same_survivors(r, g, ng, c, nc)
     assert_rows_bitwise(g[:, :7], c[:, :7], "limit_amplitudes edges")
+
+
+# (elliptical, exponent, yExponent, open side, extrapolate): a true ellipse
+# with the drift loss point, super-ellipses, each open side, a rectangle
+# with an open side, one half-axis non-positive
+SHAPES = [(1, 2, 0, 0, 1), (1, 2, 0, 0, 0), (1, 4, 0, 0, 1), (1, 6, 2, 0, 0), (1, 10, 40, 0, 0),
+          (1, 2, 0, A.EH_OPEN_PLUS_X, 1), (1, 4, 0, A.EH_OPEN_MINUS_Y, 0), (0, 2, 0, A.EH_OPEN_MINUS_X, 1),
+          (0, 2, 0, A.EH_OPEN_PLUS_Y, 0)]
+
+
+def shaped_rows(n, seed):
+    """edge rows plus a halo spread over the aperture's boundary in both
+    planes, with slopes of both signs (the drift loss point)"""
+    r = csbend_edge_rows(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    k = n // 3
+    r[:k, 0] = rng.uniform(-0.006, 0.006, k)
+    r[:k, 2] = rng.uniform(-0.003, 0.003, k)
+    r[:k, 1] = rng.normal(0, 2e-3, k)
+    r[:k, 3] = rng.normal(0, 1e-3, k)
+    return r
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_shaped_limit_amplitudes(tracker, shape):
+    """limit_amplitudes with an open side and elimit_amplitudes for the
+    ellipse and super-ellipses (limit_amplitudes.c:193-272, 472-598)
+    against the checker: survivors, order, loss point and the ellipse's
+    sqrt((Po(1+delta))^2 + 1) in the lost rows, bit for bit"""
+    ell, xe, ye, op, ex = shape
+    r = shaped_rows(1200, 31)
+    g = r.copy()
+    c = r.copy()
+    ptr, keep = rows_ptr(c)
+    if ell:
+        ng = tracker.elimit_amplitudes(g, 0.004, 0.002, 9.0, PO, ex, op, xe, ye)
+        nc = O.lib("glibc").or_elimit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, ex, op, xe, ye)
+    else:
+        ng = tracker.limit_amplitudes(g, 0.004, 0.002, 9.0, PO, ex, op)
+        nc = O.lib("glibc").or_limit_amplitudes(ptr, 0.004, 0.002, c.shape[0], 9.0, PO, ex, op)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "shape %r" % (shape,))
+
+
+def test_shaped_limit_amplitudes_early_returns(tracker):
+    """a non-positive half-axis: the ellipse falls back to the rectangle in
+    the other plane, none positive checks nothing (NaN rows stay); odd
+    exponents are refused"""
+    r = shaped_rows(600, 5)
+    for xmax, ymax in ((0.004, 0.0), (0.0, -1.0), (-1.0, -1.0)):
+        g = r.copy()
+        c = r.copy()
+        ptr, keep = rows_ptr(c)
+        ng = tracker.elimit_amplitudes(g, xmax, ymax, 2.0, PO, 1, 0, 4, 0)
+        nc = O.lib("glibc").or_elimit_amplitudes(ptr, xmax, ymax, c.shape[0], 2.0, PO, 1, 0, 4, 0)
+        assert ng == nc
+        assert_rows_bitwise(g[:, :7], c[:, :7], "early return %g %g" % (xmax, ymax))
+    with pytest.raises(Exception):
+        tracker.elimit_amplitudes(r.copy(), 0.004, 0.002, 2.0, PO, 1, 0, 3, 0)
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 0, 0), (1, 4, 2, 0), (1, 2, 0, A.EH_OPEN_PLUS_X),
+                                   (0, 2, 0, A.EH_OPEN_MINUS_Y)])
+def test_shaped_apertures_inside_elements(tracker, shape):
+    """checkMultAperture's ellipse (>= 1) and open sides inside the
+    multipole and bend integrators (multipole.c:1786-1833), both elements
+    misaligned and tilted so the aperture frame is rotated"""
+    ell, xe, ye, op = shape
+    ap = A.EH_MAXAMP(0.005, 0.0025, ell, xe, ye, op)
+    q = E.KQUAD(L=0.4, K1=1.8, N_KICKS=8, DX=2e-4, DY=-1e-4, TILT=3e-3)
+    b = E.CSBEND(L=1.0, ANGLE=0.08, N_KICKS=16, E1=0.02, E2=0.04, DX=1e-4, TILT=1e-3).p.csbend
+    r = shaped_rows(1500, 17)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.multipole_tracking2(g, q, PO, 12.5, ap)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_multipole_tracking2(ptr, c.shape[0], C.byref(q), PO, 12.5, C.byref(ap))
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "multipole shape %r" % (shape,))
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.track_through_csbend(g, b, PO, 3.0, ap)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0, C.byref(ap))
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "csbend shape %r" % (shape,))
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 0, 0), (1, 6, 4, A.EH_OPEN_PLUS_X), (0, 2, 0, A.EH_OPEN_MINUS_Y)])
+def test_ring_with_shaped_aperture(tracker, shape):
+    """do_tracking over the storage ring with its MAXAMP made elliptical or
+    given an open side: every element's in-integrator check and the check
+    after each element (drifts with the loss point), three passes"""
+    ell, xe, ye, op = shape
+    bl, Po = Lt.storage_ring(seed=4)
+    els = [E.MAXAMP("AP", X_MAX=0.012, Y_MAX=0.006, ELLIPTICAL=ell, EXPONENT=xe, YEXPONENT=ye,
+                    OPEN_SIDE=op)] + bl.elements[1:]
+    assert bl.elements[0].type == A.EH_T_MAXAMP
+    bl = E.Beamline(els, keep=bl.keep)
+    rows = Lt.matched_beam(bl, 1000, 1e-9, 1e-11, 1e-3, 10e-12, 9, Po)
+    rows[:300, 0] += np.sign(rows[:300, 0]) * 0.0105
+    rows[300:450, 2] += np.sign(rows[300:450, 2]) * 0.0052
+    g = rows.copy()
+    c = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 3)
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 3)
+    assert ng == nc and eg == o.effort() and 0 < nc < 1000
+    assert_rows_bitwise(g, c, "ring shape %r" % (shape,))
 
 
 def test_wake_and_zlongit_entry_points(tracker):

@@ -89,3 +89,22 @@ def test_storage_ring_round_trip_tracks_identically():
     n2 = O.Oracle("emath").do_tracking(r2, bl2, Po, 2)
     assert n1 == n2
     assert np.array_equal(r1, r2)
+
+
+def test_maxamp_shapes_parse_and_round_trip():
+    """MAXAMP's ELLIPTICAL, EXPONENT, YEXPONENT and OPEN_SIDE (track_data.c:990-997);
+    OPEN_SIDE strings map as determineOpenSideCode does (limit_amplitudes.c:1130-1162)"""
+    text = """AP1: MAXAMP, X_MAX=0.02, Y_MAX=0.01, ELLIPTICAL=1, EXPONENT=4, YEXPONENT=2, OPEN_SIDE="-h"
+AP2: MAXAMP, X_MAX=0.02, Y_MAX=0.01, OPEN_SIDE="+Y"
+D: DRIF, L=1
+RING: LINE=(AP1, D, AP2, D)
+"""
+    bl = lte.Lattice().parse(text).beamline()
+    m1, m2 = bl.elements[0].p.maxamp, bl.elements[2].p.maxamp
+    assert (m1.elliptical, m1.exponent, m1.yExponent, m1.openSide) == (1, 4, 2, A.EH_OPEN_MINUS_X)
+    assert (m2.elliptical, m2.exponent, m2.openSide) == (0, 2, A.EH_OPEN_PLUS_Y)
+    bl2 = lte.Lattice().parse(lte.dumps(bl)).beamline()
+    for a, b in zip(bl.elements, bl2.elements):
+        assert a.type == b.type and bytes(a.p) == bytes(b.p)
+    with pytest.raises(lte.LatticeError):
+        lte.Lattice().parse('AP: MAXAMP, X_MAX=0.1, OPEN_SIDE="x"\nR: LINE=(AP)\n').beamline()

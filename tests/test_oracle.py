@@ -389,3 +389,162 @@ def test_initial_nan_cull_precedes_the_first_element():
     assert [int(x) for x in r[1:, 6]] == [1, 3, 2]
     b = r[3]
     assert b[7] == 0 and b[4] == 0.0 and b[5] == PO * (1 + 1e-3)
+
+
+# ---------------------------------------------------------------- MAXAMP shapes
+
+def ipow_listed(x, p):
+    """x^p with the multiplications the reference's device copy of the
+    toolkit's ipow lists (gpuElegant/gpu_bin_transverse_distribution.cu:8-56)"""
+    x = np.float64(x)
+    if x == 0:
+        return np.float64(0.0)
+    if p <= 8:
+        h = x * x
+        return {1: x, 2: h, 3: h * x, 4: h * h, 5: h * h * x, 6: h * h * h, 7: (h * x) * (h * x) * x,
+                8: (h * h) * (h * h)}[p]
+    hp = ipow_listed(x, p // 2)
+    return hp * hp * x if p % 2 else hp * hp
+
+
+def test_ipow_follows_the_listed_multiplication_order():
+    """The aperture power keeps the toolkit's multiplication order, which
+    differs in the last bits from plain repeated multiplication and from
+    numpy's power, and stays within (p - 1) half-ulps of the exact power."""
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-3, 3, 4000)
+    exps = rng.integers(1, 80, x.size)
+    got = O.math_eval(7, x, exps.astype(np.float64), mode="glibc")
+    want = np.array([ipow_listed(a, int(p)) for a, p in zip(x, exps)])
+    assert (got.view(np.int64) == want.view(np.int64)).all()
+    naive = np.array([np.prod(np.full(int(p), a)) for a, p in zip(x, exps)])
+    assert (got != naive).any()
+    for a, p, g in list(zip(x, exps, got))[:300]:
+        exact = Fraction(float(a)) ** int(p)
+        assert abs(Fraction(float(g)) - exact) <= abs(exact) * Fraction(int(p), 2 ** 53)
+    # integers whose powers are exact in double precision come out exact
+    assert O.math_eval(7, np.array([3.0, -2.0, 0.0, -0.0]), np.array([20.0, 31.0, 4.0, 2.0]), mode="glibc").tolist() \
+        == [3.0 ** 20, -(2.0 ** 31), 0.0, 0.0]
+
+
+def elimit(rows, xmax, ymax, z, extrap, open_code=0, xe=2, ye=0):
+    r = rows.copy()
+    ptr, keep = A.rows_pointer(r)
+    n = O.lib("glibc").or_elimit_amplitudes(ptr, xmax, ymax, r.shape[0], z, PO, extrap, open_code, xe, ye)
+    return r, n
+
+
+def test_elliptical_aperture_against_exact_arithmetic():
+    """elimit_amplitudes (limit_amplitudes.c:472-598) loses exactly the
+    particles with (x/a)^p + (y/b)^q > 1 in exact rational arithmetic (points
+    within 1e-9 of the boundary left out), for true and super-ellipses; lost
+    rows are swapped to the tail with s = z and delta replaced by
+    sqrt((Po(1 + delta))^2 + 1)"""
+    rng = np.random.default_rng(12)
+    a, b = 0.013, 0.007
+    for xe, ye in ((2, 0), (4, 0), (6, 2), (10, 40), (40, 40)):
+        q = ye or xe
+        pts = np.zeros((3000, 6))
+        pts[:, 0] = rng.uniform(-1.6 * a, 1.6 * a, 3000)
+        pts[:, 2] = rng.uniform(-1.6 * b, 1.6 * b, 3000)
+        pts[:, 5] = rng.normal(0, 1e-3, 3000)
+        lhs = [(Fraction(float(x)) / Fraction(a)) ** xe + (Fraction(float(y)) / Fraction(b)) ** q
+               for x, y in zip(pts[:, 0], pts[:, 2])]
+        keep = np.array([abs(v - 1) > Fraction(1, 10 ** 9) for v in lhs])
+        pts, lhs = pts[keep], [v for v, k in zip(lhs, keep) if k]
+        rows = rows_of(pts)
+        r, n = elimit(rows, a, b, 4.5, 0, 0, xe, ye)
+        lost_ids = {i + 1 for i, v in enumerate(lhs) if v > 1}
+        assert set(r[n:, 6].astype(int)) == lost_ids and n == len(lhs) - len(lost_ids)
+        for row in r[n:]:
+            P = float(PO) * (1 + float(rows[int(row[6]) - 1, 5]))
+            assert row[4] == 4.5 and row[5] == math.sqrt(P * P + 1)
+    # a point exactly on the ellipse survives the check after an element (> 1)
+    r, n = elimit(rows_of([[0.013, 0, 0, 0, 0, 0], [0, 0, -0.007, 0, 0, 0]]), a, b, 1.0, 0)
+    assert n == 2
+    # the powers come before the quotients: with b^254 below the smallest
+    # double, y^254 / b^254 is 0/0 = NaN and the comparison keeps the particle
+    r, n = elimit(rows_of([[0.02, 0, 0, 0, 0, 0]]), a, b, 1.0, 0, 0, 10, 254)
+    assert n == 1
+
+
+def test_elliptical_loss_point_on_a_circle():
+    """With a true ellipse after a drift the loss point is moved back along
+    the straight line to the last crossing of the boundary
+    (limit_amplitudes.c:564-575): on a circle of radius R, x0 = 1.5 R moving
+    outward with slope 1 was last at the boundary 0.5 R earlier"""
+    R = 0.01
+    pts = [[1.5 * R, 1.0, 0, 0, 0, 0], [0, 0, -1.2 * R, -0.5, 0, 0], [0.6 * R, 0.0, 0.9 * R, 0.0, 0, 0]]
+    r, n = elimit(rows_of(pts), R, R, 2.0, 1)
+    assert n == 0
+    lost = {int(row[6]): row for row in r}
+    assert abs(lost[1][4] - (2.0 - 0.5 * R)) < 1e-15 and abs(lost[1][0] - R) < 1e-17
+    assert abs(lost[2][4] - (2.0 - 0.4 * R)) < 1e-15 and abs(lost[2][2] + R) < 1e-17
+    # no slope: no crossing behind it, the particle is lost where it stands
+    assert lost[3][4] == 2.0 and lost[3][0] == 0.6 * R
+    # no extrapolation for super-ellipses or without the drift flag
+    r, n = elimit(rows_of(pts), R, R, 2.0, 1, 0, 4)
+    assert n == 1 and (r[1:, 4] == 2.0).all()
+    r, n = elimit(rows_of(pts), R, R, 2.0, 0)
+    assert (r[:, 4] == 2.0).all()
+
+
+def test_open_side_spares_particles_leaving_through_it():
+    """evaluateLostWithOpenSides (limit_amplitudes.c:1105-1128): with +x open
+    a particle beyond X_MAX on the + side within Y_MAX is kept, one on the
+    - side or beyond Y_MAX is lost; the same for the ellipse"""
+    lib = O.lib("glibc")
+    pts = [[0.03, 0, 0.001, 0, 0, 0], [-0.03, 0, 0.001, 0, 0, 0], [0.03, 0, 0.02, 0, 0, 0],
+           [0.001, 0, 0.02, 0, 0, 0], [0.001, 0, -0.002, 0, 0, 0]]
+    for code, kept in ((A.EH_OPEN_PLUS_X, {1, 5}), (A.EH_OPEN_MINUS_X, {2, 5}), (A.EH_OPEN_PLUS_Y, {4, 5}),
+                       (A.EH_OPEN_MINUS_Y, {5}), (0, {5})):
+        r = rows_of(pts)
+        ptr, keep = A.rows_pointer(r)
+        n = lib.or_limit_amplitudes(ptr, 0.02, 0.01, r.shape[0], 1.0, PO, 1, code)
+        assert set(r[:n, 6].astype(int)) == kept, code
+        if code:
+            # no loss-point extrapolation with an open side
+            assert (r[n:, 4] == 1.0).all()
+        else:
+            # closed: with zero slopes both candidate distances stay -DBL_MAX,
+            # dz = +DBL_MAX, and the sentinel test (dz == -DBL_MAX,
+            # limit_amplitudes.c:246-247) lets it through: s = z + DBL_MAX
+            assert (r[n:, 4] == 1.0 + np.finfo(np.float64).max).all()
+        r2, n2 = elimit(rows_of(pts), 0.02, 0.01, 1.0, 1, code)
+        assert set(r2[:n2, 6].astype(int)) == kept, code
+
+
+def test_aperture_early_returns():
+    """limit_amplitudes with both half-widths negative checks nothing, not
+    even NaN (limit_amplitudes.c:215-217); elimit_amplitudes with one
+    non-positive half-axis falls back to the rectangle in the other plane,
+    with none positive checks nothing (:523-528); odd or small exponents
+    are an error (:507-521)"""
+    lib = O.lib("glibc")
+    pts = [[float("nan"), 0, 0, 0, 0, 0], [0.05, 0, 0.05, 0, 0, 0]]
+    r = rows_of(pts)
+    ptr, keep = A.rows_pointer(r)
+    assert lib.or_limit_amplitudes(ptr, -1.0, -1.0, 2, 1.0, PO, 0, 0) == 2
+    r, n = elimit(rows_of(pts), 0.0, 0.0, 1.0, 0)
+    assert n == 2
+    r, n = elimit(rows_of(pts), 0.02, 0.0, 1.0, 0)
+    assert n == 0 and r[0, 5] == PO * (1 + 0.0)  # rectangle's loss rows: Po(1 + delta)
+    for xe, ye in ((3, 0), (0, 0), (2, 5), (2, 1)):
+        r, n = elimit(rows_of(pts), 0.02, 0.02, 1.0, 0, 0, xe, ye)
+        assert n == -1  # EH_ERR_INVALID
+
+
+def test_elliptical_aperture_inside_an_element_is_not_strict():
+    """Inside an integrator the ellipse test is >= 1 (multipole.c:1817-1822)
+    while the check after the element is > 1: a particle travelling exactly
+    on the boundary through a field-free KQUAD is lost at the first slice's
+    check (multipole.c:1240-1248), at the element's entrance"""
+    a, b = 0.01, 0.005
+    els = [E.MAXAMP(X_MAX=a, Y_MAX=b, ELLIPTICAL=1), E.KQUAD(L=0.5, K1=0.0, N_KICKS=4)]
+    r, n, _ = track(els, rows_of([[a, 0, 0, 0, 0, 0], [0.99 * a, 0, 0, 0, 0, 0]]))
+    assert n == 1 and r[0, 6] == 2
+    assert r[1, 6] == 1 and r[1, 4] == 0.0 and r[1, 0] == a
+    # the same particle survives a drift (only the strict check after it)
+    r, n, _ = track([E.MAXAMP(X_MAX=a, Y_MAX=b, ELLIPTICAL=1), E.DRIF(L=0.5)],
+                    rows_of([[a, 0, 0, 0, 0, 0]]))
+    assert n == 1
