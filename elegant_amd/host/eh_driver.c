@@ -443,6 +443,19 @@ static int rfca_fiducial(eh_ctx *c, int k, int64_t pass) {
 
 int eh_collective(eh_ctx *c, int slot, int64_t pass, int64_t n_alive);
 
+int eh_rfca_fiducial(eh_ctx *c, int64_t elem_index, double *phase_fiducial) {
+  if (!c || !phase_fiducial)
+    return EH_ERR_INVALID;
+  if (!c->loaded)
+    return EH_ERR_STATE;
+  for (int k = 0; k < c->prog->n_rfca; k++)
+    if (c->prog->rfca_elem[k] == elem_index) {
+      *phase_fiducial = c->fid_seen[k] ? c->phase_fid[k] : NAN;
+      return EH_OK;
+    }
+  return EH_ERR_INVALID;
+}
+
 /* index of the first op >= from that needs the host on this pass, or nops */
 static int32_t next_barrier(const eh_ctx *c, int32_t from) {
   const eh_program *p = c->prog;

@@ -48,6 +48,7 @@ SIGNATURES = [
                                            C.POINTER(A.EH_MAXAMP)]),
     ("eh_track_through_csbend", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_CSBEND), C.c_double, C.c_double,
                                             C.POINTER(A.EH_MAXAMP)]),
+    ("eh_rfca_fiducial", C.c_int, [P, C.c_int64, PD]),
     ("eh_simple_rf_cavity", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_RFCA), PD, C.c_double, PD]),
     ("eh_track_particles_drift", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_DRIFT)]),
     ("eh_limit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
@@ -322,6 +323,12 @@ class Tracker:
         mp = C.byref(maxamp) if maxamp is not None else None
         return check(self._ctx, lib().eh_track_through_csbend(self._ctx, ptr, rows.shape[0], C.byref(csbend), Po,
                                                               z_start, mp))
+
+    def rfca_fiducial(self, elem_index):
+        """the fiducial phase the last fused run fixed for the RFCA at elem_index (NaN if unset)"""
+        ph = C.c_double()
+        check(self._ctx, lib().eh_rfca_fiducial(self._ctx, elem_index, C.byref(ph)))
+        return ph.value
 
     def simple_rf_cavity(self, rows, rfca, Po, zEnd, phase_fiducial=float("nan")):
         ptr, keep = A.rows_pointer(rows)

@@ -246,6 +246,13 @@ int64_t eh_multipole_tracking2(eh_ctx *ctx, double **particle, int64_t n_part, c
 /* track_through_csbend (csbend.c:665-1409) */
 int64_t eh_track_through_csbend(eh_ctx *ctx, double **part, int64_t n_part, const EH_CSBEND *csbend,
                                 double Po, double z_start, const EH_MAXAMP *maxamp);
+/* The fiducial phase that eh_do_tracking / eh_track fixed for the RFCA at
+ * beamline index elem_index on its first pass (rfca->phase_fiducial once
+ * fiducial_seen is set, simple_rfca.c:466-471); NaN while unset.  An
+ * element-by-element caller continuing a fused run passes it on to
+ * eh_simple_rf_cavity.  EH_ERR_INVALID if that element is not an RFCA of
+ * the loaded beamline. */
+int eh_rfca_fiducial(eh_ctx *ctx, int64_t elem_index, double *phase_fiducial);
 /* simple_rf_cavity (simple_rfca.c:264-293); *phase_fiducial carries the
  * element's fiducial phase between calls (NaN = not yet set, as
  * rfca->fiducial_seen == 0) */

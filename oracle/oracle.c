@@ -1900,6 +1900,15 @@ int64_t or_last_hist(const or_ctx *c, double *hist, double *volt, int64_t cap, d
 /* compute_average as restated (exact fixed-point sum / n) */
 double or_compute_average(const double *v, long n) { return compute_average(v, n); }
 
+/* the fiducial phase of the RFCA at beamline index elem_index, as set by an
+ * earlier first pass (rfca->phase_fiducial, fiducial_seen; simple_rfca.c:466-471) */
+void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial) {
+  if (elem_index < 0 || elem_index >= OR_MAX_ELEMS)
+    return;
+  c->phase_fiducial[elem_index] = phase_fiducial;
+  c->fiducial_seen[elem_index] = 1;
+}
+
 void or_set_macro_charge(or_ctx *c, double q) {
   c->macroParticleCharge = q;
   c->charge_seen = 1;

@@ -79,6 +79,7 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
 int64_t or_last_hist(const or_ctx *c, double *hist, double *volt, int64_t cap, double *tmin, double *dt);
 double or_compute_average(const double *v, long n);
 void or_set_macro_charge(or_ctx *c, double q);
+void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial);
 
 /* FFT used by ZLONGIT (packed real layout, see oracle.c) */
 void or_real_fft(double *data, long n, int inverse);

@@ -68,6 +68,7 @@ def lib(mode="glibc"):
         L.or_compute_average.restype = C.c_double
         L.or_compute_average.argtypes = [PD, C.c_long]
         L.or_set_macro_charge.argtypes = [C.c_void_p, C.c_double]
+        L.or_set_fiducial.argtypes = [C.c_void_p, C.c_long, C.c_double]
         _libs[mode] = L
     return _libs[mode]
 
@@ -107,6 +108,10 @@ class Oracle:
 
     def set_macro_charge(self, q):
         self.L.or_set_macro_charge(self.ctx, q)
+
+    def set_fiducial(self, elem_index, phase_fiducial):
+        """the RFCA at elem_index keeps this fiducial phase (as after its first pass)"""
+        self.L.or_set_fiducial(self.ctx, elem_index, phase_fiducial)
 
     def wake(self, rows, wake, Po, i_pass):
         ptr, keep = A.rows_pointer(rows)
