@@ -75,7 +75,10 @@ DEV void aperture_loss(particle &p, double xmax, double ymax, int is_out, bool e
 }
 
 /* the two checks that follow every element (do_tracking.c:2459-2477, 2628-2637).
- * Returns 0 if the particle survives, else stage + 1 with the row frozen. */
+ * Returns 0 if the particle survives, else stage + 1 with the row frozen.
+ * In the SH instance an element with a shaped MAXAMP (ap_code != 0) takes
+ * elimit_amplitudes or the open-sided limit_amplitudes instead of the
+ * rectangle. */
 template <bool SH>
 DEV int post_element_checks(particle &p, const eh_op &op) {
   if (op.flags & EH_F_ALONE)
@@ -996,7 +999,10 @@ DEV int run_op(particle &p, const eh_op &op) {
 /* Four waves per SIMD (up to 128 VGPRs): the branch-free bend body needs
  * about 111 registers in this kernel; squeezing it into five or six waves
  * spills to scratch or parks state in LDS, and measured no faster on the
- * storage ring (29.0-29.3 M against 29.4 M particle-turns/s). */
+ * storage ring (29.0-29.3 M against 29.4 M particle-turns/s).
+ * SH: built with the elliptical / open-sided MAXAMP checks; launched only for
+ * programs that have such an aperture, so the rectangle-only instance keeps
+ * the registers and schedule of the common case. */
 template <bool SH>
 __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
                                                        int64_t pass0, int32_t npasses, int32_t op_first,
@@ -1043,7 +1049,7 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
  * of its particles) from a queue until it is empty; every lane tracks PPL
  * particles of the chunk through the variant's program.  The chunk queue is
  * ordered by variant so that workgroups resident at the same time share a few
- * programs in L2. */
+ * programs in L2.  SH as for eh_track_kernel. */
 template <int PPL, bool SH>
 __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op *__restrict__ ops,
                                                        const eh_vdesc *__restrict__ vars,
