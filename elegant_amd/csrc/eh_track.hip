@@ -1168,7 +1168,7 @@ int ehd_create(int device, ehd_dev **out, char *msg, int msglen) {
   if (!d)
     return EH_ERR_NOMEM;
   d->device = device;
-  d->block = 256;
+  d->block = 0; /* chosen per launch from the particle count */
   e = hipSetDevice(device);
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
@@ -1309,8 +1309,9 @@ int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op) {
   return EH_OK;
 }
 
+/* 64, 128, 192 or 256 lanes per workgroup; 0 = chosen per launch */
 void ehd_set_block(ehd_dev *d, int32_t block) {
-  if (block >= 64 && block <= 256 && block % 64 == 0)
+  if (block == 0 || (block >= 64 && block <= 256 && block % 64 == 0))
     d->block = block;
 }
 
@@ -1330,7 +1331,19 @@ int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int3
   st.loss_pass = d->loss_pass;
   st.loss_code = d->loss_code;
   st.n = d->n;
-  const int block = d->block;
+  int block = d->block;
+  if (block == 0) {
+    /* 256 lanes per workgroup, unless the particles would fill at most one
+     * such workgroup per CU: then 64, so the waves spread over all CUs
+     * instead of sharing a few (10^4 particles: 157 CUs busy instead of 40,
+     * 6 % faster on the FODO ring) */
+    if (!d->num_cu) {
+      int cu = 0;
+      HIPCHK(d, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
+      d->num_cu = cu > 0 ? cu : 256;
+    }
+    block = d->n <= (int64_t)d->num_cu * 256 ? 64 : 256;
+  }
   const int64_t grid = (d->n + block - 1) / block;
   HIPCHK(d, hipEventRecord(d->ev0, d->stream));
   if (d->shaped)
