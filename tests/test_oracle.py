@@ -548,3 +548,17 @@ def test_elliptical_aperture_inside_an_element_is_not_strict():
     r, n, _ = track([E.MAXAMP(X_MAX=a, Y_MAX=b, ELLIPTICAL=1), E.DRIF(L=0.5)],
                     rows_of([[a, 0, 0, 0, 0, 0]]))
     assert n == 1
+
+
+def test_open_side_inside_an_element():
+    """checkMultAperture spares a particle beyond X_MAX on the open side
+    within Y_MAX (multipole.c:1811-1815); on the closed side it is lost at
+    the element's entrance check; the same holds for the ellipse"""
+    for ell in (0, 1):
+        els = [E.MAXAMP(X_MAX=0.01, Y_MAX=0.005, ELLIPTICAL=ell, OPEN_SIDE="+x"),
+               E.KQUAD(L=0.5, K1=0.0, N_KICKS=4)]
+        r, n, _ = track(els, rows_of([[0.012, 0, 0.001, 0, 0, 0], [-0.012, 0, 0.001, 0, 0, 0],
+                                      [0.012, 0, 0.006, 0, 0, 0]]))
+        assert n == 1 and r[0, 6] == 1 and r[0, 0] == 0.012, ell
+        lost = {int(row[6]): row for row in r[1:]}
+        assert lost[2][4] == 0.0 and lost[3][4] == 0.0, ell
