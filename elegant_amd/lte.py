@@ -208,6 +208,8 @@ class Lattice:
                 sub = self._line_items(tok[1:-1])
                 items.append(("group", sub, rev, mult))
             else:
+                if len(tok) >= 2 and tok[0] == '"' and tok[-1] == '"':
+                    tok = tok[1:-1]   # quoted names, as in their definitions
                 items.append(("name", tok, rev, mult))
         return items
 

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from elegant_amd import _abi as A
+from elegant_amd import elements as E
 from elegant_amd import lattices as Lt
 from elegant_amd import lte
 from oracle import oracle_ctypes as O
@@ -108,3 +109,25 @@ RING: LINE=(AP1, D, AP2, D)
         assert a.type == b.type and bytes(a.p) == bytes(b.p)
     with pytest.raises(lte.LatticeError):
         lte.Lattice().parse('AP: MAXAMP, X_MAX=0.1, OPEN_SIDE="x"\nR: LINE=(AP)\n').beamline()
+
+
+def test_reference_deck_elliptical_open_apertures():
+    """MAXAMP definitions as the reference's APS deck writes them, quoted
+    names included (doc/examples/matching/IDCompensation/apsKick.lte:502,
+    2365): an insertion-device extrusion open on the +x side"""
+    text = """"ID3Extru": MAXAMP,X_MAX=0.015,Y_MAX=0.0024,ELLIPTICAL=1,OPEN_SIDE="+X"
+"extru": MAXAMP,X_MAX=0.042,Y_MAX=0.021,ELLIPTICAL=1
+D: DRIF, L=1
+R: LINE=("ID3Extru", D, "extru", D)
+"""
+    bl = lte.Lattice().parse(text).beamline()
+    m1, m2 = bl.elements[0].p.maxamp, bl.elements[2].p.maxamp
+    assert (m1.x_max, m1.y_max, m1.elliptical, m1.exponent, m1.openSide) == (0.015, 0.0024, 1, 2, A.EH_OPEN_PLUS_X)
+    assert (m2.x_max, m2.elliptical, m2.openSide) == (0.042, 1, 0)
+    # beyond the ellipse on the open side within Y_MAX: kept; on the closed side: lost
+    rows = np.zeros((2, 8))
+    rows[:, 0] = [0.02, -0.02]
+    rows[:, 6] = [1, 2]
+    rows[:, 7] = -1
+    n = O.Oracle("glibc").do_tracking(rows, E.Beamline(bl.elements[:2]), 1e4, 1)
+    assert n == 1 and rows[0, 6] == 1
