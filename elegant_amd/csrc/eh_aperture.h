@@ -16,14 +16,16 @@
 
 #define EHA_DEV static __device__ __forceinline__
 
-/* x^p (1 <= p <= EH_MAX_AP_EXPONENT) with the multiplications of the SDDS
+/* x^p (0 <= p <= EH_MAX_AP_EXPONENT) with the multiplications of the SDDS
  * toolkit's ipow, as the reference's device copy orders them
  * (gpuElegant/gpu_bin_transverse_distribution.cu:8-56): explicit products
  * for p <= 8; above, ipow(x, p/2) squared, times x for odd p.  The
  * recursion is unrolled: p >> L is the first exponent <= 8 of the chain
  * p, p/2, p/4, ..., and each level back up squares and, when that level's
- * exponent is odd, multiplies by x.  x = 0 gives +0. */
+ * exponent is odd, multiplies by x.  p = 0 gives 1; otherwise x = 0 gives +0. */
 EHA_DEV double ap_ipow(double x, int p) {
+  if (p <= 0)
+    return 1.0;
   if (x == 0)
     return 0.0;
   int L = 0;

@@ -1615,7 +1615,7 @@ __global__ void eh_math_probe_kernel(int32_t fn, const double *in, const double 
   case 9: /* the same quotient from the rounded reciprocal (bend radii) */
     r = em_div(v, in2[i], 1.0 / in2[i], em_recip_ok(in2[i]));
     break;
-  case 10: /* the aperture's integer power, exponent in in2 (1..EH_MAX_AP_EXPONENT) */
+  case 10: /* the aperture's integer power, exponent in in2 (0..EH_MAX_AP_EXPONENT) */
     r = ap_ipow(v, (int)in2[i]);
     break;
   default:
