@@ -39,7 +39,10 @@ def parse():
                     choices=["storage_ring", "fodo_ring", "storage_ring_wake", "error_seeds", "learned_da"])
     ap.add_argument("--variants", type=int, default=256, help="error_seeds / learned_da: lattice variants")
     ap.add_argument("--particles", type=int, default=0, help="override the workload's particle count")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0 = all, max 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads for the CPU baseline (0 = the cores allotted to this job, at most 16)")
+    ap.add_argument("--survivors", action="store_true",
+                    help="report the survivors after every timed pass (from the passes of loss)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="passes per kernel launch (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
@@ -61,12 +64,32 @@ def workload(name, n):
     return bl, Po, rows, desc
 
 
-def cpu_baseline(name, threads):
-    """CPU checker on a bounded sample: each thread tracks its own shard
-    (Pelegant-style static split) of the same workload for a few passes."""
+def host_cores():
+    """Cores this job may use: the CPU set the process is allowed to run on,
+    at most 16 (the host share that comes with one GPU on the measurement
+    box, which may show many more CPUs than are allotted to the job)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "?")
+    except OSError:
+        return "?"
+
+
+def _cpu_run(name, threads, per_thread, passes):
+    """The CPU checker (glibc build) on `threads` shards of `per_thread`
+    particles of the workload for `passes` passes, one thread per shard (the
+    static particle split of Pelegant, do_tracking.c:4736-4800; the shards of
+    these workloads never interact).  Returns (particle-turns, seconds)."""
     import numpy as np
     from oracle import oracle_ctypes as O
-    per_thread, passes = (1024, 6) if name != "fodo_ring" else (2000, 60)
     bl, Po, rows, _ = workload(name, per_thread * threads)
     shards = [np.ascontiguousarray(rows[k * per_thread:(k + 1) * per_thread]) for k in range(threads)]
     efforts = [0] * threads
@@ -83,17 +106,27 @@ def cpu_baseline(name, threads):
         t.start()
     for t in ts:
         t.join()
-    dt = time.perf_counter() - t0
-    total = sum(efforts)
-    model = "?"
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "?")
-    except OSError:
-        pass
+    return sum(efforts), time.perf_counter() - t0
+
+
+def cpu_baseline(name, threads):
+    """The CPU checker timed on this host on bounded samples of the same
+    workload: all `threads` cores (the Pelegant-equivalent static split) and
+    one core (the serial, elegant-equivalent run).  About 10 s of CPU work
+    each for the storage rings, the FODO ring's first 30 passes in full."""
+    if name == "fodo_ring":
+        per_thread, passes, serial = 10000 // threads, 30, (10000, 10)
+    else:
+        per_thread, passes, serial = 2048, 16, (2048, 8)
+    total, dt = _cpu_run(name, threads, per_thread, passes)
+    stotal, sdt = _cpu_run(name, 1, *serial)
+    model = _cpu_model()
     return {"value": total / dt, "unit": "particle-turns/s", "cores": threads, "kind": "port",
             "sample": "%d threads x %d particles x %d passes of the same workload (%d particle-turns, %.1f s) "
-                      "on %s, nproc %d" % (threads, per_thread, passes, total, dt, model, os.cpu_count() or 0)}
+                      "on %s, nproc %d" % (threads, per_thread, passes, total, dt, model, os.cpu_count() or 0),
+            "serial": {"value": stotal / sdt, "unit": "particle-turns/s", "cores": 1,
+                       "sample": "1 thread x %d particles x %d passes (%d particle-turns, %.1f s)"
+                                 % (serial[0], serial[1], stotal, sdt)}}
 
 
 def traffic(kernel, workload):
@@ -200,6 +233,7 @@ def main():
     wall = time.perf_counter() - t0
     value = st.effort / wall
     flops_pt = Lt.flops_per_particle_turn(bl)
+    flops_item = Lt.flops_per_particle_turn_itemised(bl)
     achieved = flops_pt * st.effort / (st.track_ms * 1e-3) / 1e12 if st.track_ms > 0 else None
     line = {
         "metric": "particle-turns/s FP64 6D tracking",
@@ -217,9 +251,11 @@ def main():
         "config": {"workload": desc, "particles": n, "elements": len(bl), "passes_timed": args.steps,
                    "survivors": int(st.n_left), "kernel_launches": int(st.kernel_launches)},
         "roofline": {
-            "bound": "mfma",
-            "note": "FP64 vector ALU bound (no MFMA used); MI355X FP64 peak 78.6 TFLOP/s; algorithmic FLOPs "
-                    "%d per particle-turn" % flops_pt,
+            "bound": "fp64_valu",
+            "note": "FP64 vector ALU bound (no MFMA: nothing here is a dense contraction); MI355X FP64 vector "
+                    "peak 78.6 TFLOP/s; algorithmic FLOPs %d per particle-turn by the per-element formulas "
+                    "(+,-,*,/,sqrt 1, libm call 20); itemised count of the routines as written %d"
+                    % (flops_pt, flops_item),
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -228,15 +264,23 @@ def main():
             "kernel_ms_per_launch": st.track_ms / max(st.kernel_launches, 1),
         },
     }
+    if args.survivors:
+        import numpy as np
+        lp, _ = tr.loss_records()
+        lost = np.sort(lp[lp >= 0])
+        passes = np.arange(args.warmup, args.warmup + args.steps)
+        # survivors after pass k: particles not lost on any pass <= k
+        line["survivors_after_pass"] = (n - np.searchsorted(lost, passes, side="right")).tolist()
     if args.workload == "storage_ring_wake":
         line["collective_ms"] = st.collective_ms
         line["collective_GBps"] = (Lt.collective_bytes_per_particle() * 2 * st.effort / max(st.collective_ms, 1e-9)
                                    / 1e6)
     tr.close()
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cores()
         line["cpu_baseline"] = cpu_baseline(args.workload, threads)
         line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+        line["speedup_vs_serial_cpu"] = value / line["cpu_baseline"]["serial"]["value"]
     print(json.dumps(line))
 
 

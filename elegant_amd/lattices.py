@@ -230,15 +230,71 @@ def storage_ring_workload(n=1_000_000, seed=2024, lattice_seed=1):
 LIBM = 20  # one sin/cos/asin/tan/log call
 
 
+def _multipole_flops_alg(order, slices, misaligned, tilted):
+    """Algorithmic FLOPs of multipole_tracking2 with the 4th-order
+    integrator (multipole.c:693-1410) by the per-element convention:
+    26 + S (243 + 15 m) + 15 for S slices and multipole order m (KQUAD
+    m = 1: 299, KSEXT 314, KOCT 329 at one slice), +24 for a DX/DY/DZ
+    offset and +24 for a TILT roll (entry and exit transforms)."""
+    f = 26 + slices * (243 + 15 * order) + 15
+    if misaligned:
+        f += 24
+    if tilted:
+        f += 24
+    return f
+
+
+def _csbend_flops_alg(slices):
+    """Algorithmic FLOPs of track_through_csbend + integrate_csbend_ordn
+    (csbend.c:665-1693), 4th order: 113 + 277 S arithmetic and 16 S + 4 libm
+    calls (asin, sin, cos, cos per drift substep; the edges' tan) at LIBM
+    each; S = 16 gives 9,745."""
+    return 113 + 277 * slices + (16 * slices + 4) * LIBM
+
+
+def _rfca_flops_alg():
+    """RFCA, one kick (simple_rfca.c:565-941): 85 arithmetic + 3 libm = 145."""
+    return 85 + 3 * LIBM
+
+
+def flops_per_particle_turn(beamline):
+    """Algorithmic FLOPs per surviving particle per pass: +, -, *, /, sqrt
+    count 1, each libm call LIBM = 20, integer work and compares 0, by the
+    per-element formulas of the reference routines' operation counts: DRIF
+    10, EDRIFT 11 (exactDrift, csbend.c:3767-3790), multipoles and CSBEND as
+    above, RFCA 145, WAKE/ZLONGIT 45 per particle (time, bin index and the
+    energy kick; the O(bins) convolution or FFT is left out)."""
+    total = 0
+    for e in beamline.elements:
+        t = e.type
+        if t == A.EH_T_DRIF:
+            total += 10
+        elif t == A.EH_T_EDRIFT:
+            total += 11
+        elif t in (A.EH_T_KQUAD, A.EH_T_KSEXT, A.EH_T_KOCT):
+            p = getattr(e.p, A.PARAM_STRUCT[t][0])
+            order = {A.EH_T_KQUAD: 1, A.EH_T_KSEXT: 2, A.EH_T_KOCT: 3}[t]
+            nk = p.n_kicks
+            S = max(1, math.ceil(nk / p.integration_order)) if nk > 0 else p.nSlices
+            total += _multipole_flops_alg(order, S, bool(p.dx or p.dy or p.dz), bool(p.tilt))
+        elif t == A.EH_T_CSBEND:
+            b = e.p.csbend
+            total += _csbend_flops_alg(b.nSlices) if b.angle else 11
+        elif t == A.EH_T_RFCA:
+            total += _rfca_flops_alg() if e.p.rfca.volt else 11
+        elif t in (A.EH_T_WAKE, A.EH_T_ZLONGIT):
+            total += 45
+    return total
+
+
 def _multipole_flops(order, slices, misaligned, tilted):
-    """multipole_tracking2 + integrate_kick_multipole_ordn (multipole.c:693-1410),
-    4th-order integrator: 4 drifts and 3 kicks per slice.  Counted: entry
-    slopes->momenta (11) and momenta->slopes (9); per drift substep dsh, x, y,
-    s (with sqrt) and dzLoss (13); per kick the power arrays, the field sums
-    and q updates (14/19/24 for order 1/2/3), momenta->slopes (9) and the
-    q/(1+dp)*(1+dp) round trip with its second conversion (17); aperture
-    offsets x+dx, y+dy (2 per check); exit conversion and s update (10).
-    The F2/dsFactor/sigmaDelta2 terms only feed sigmaDelta2 and are not counted."""
+    """An itemised count of the same routine as written (multipole.c:
+    693-1410), 4th-order integrator: entry slopes->momenta (11) and
+    momenta->slopes (9); per drift substep dsh, x, y, s (with sqrt) and
+    dzLoss (13); per kick the power arrays, the field sums and q updates
+    (14/19/24 for order 1/2/3), momenta->slopes (9) and the q/(1+dp)*(1+dp)
+    round trip with its second conversion (17); aperture offsets x+dx, y+dy
+    (2 per check); exit conversion and s update (10)."""
     kick = {1: 14, 2: 19, 3: 24}[order] + 9 + 17
     per_slice = 4 * 13 + 3 * kick + 2
     f = 20 + slices * per_slice + 2 + 10
@@ -250,26 +306,20 @@ def _multipole_flops(order, slices, misaligned, tilted):
 
 
 def _csbend_flops(slices, edge1, edge2):
-    """track_through_csbend + integrate_csbend_ordn (csbend.c:665-1693), pure
-    dipole, 4th order: entry transform 25, canonical conversion 9, per drift
-    substep 109 (asin, sin, cos, cos = 4 libm calls + 29 arithmetic), per kick
-    21 (field kick 12, round trip 8, ds 1), exit conversion 10, velocity
-    rescaling 15, exit transform 29; each edge (edge order 1) 31 incl. one tan."""
+    """Itemised count of track_through_csbend + integrate_csbend_ordn
+    (csbend.c:665-1693), pure dipole, 4th order: entry transform 25,
+    canonical conversion 9, per drift substep 109 (asin, sin, cos, cos = 4
+    libm calls + 29 arithmetic), per kick 21 (field kick 12, round trip 8,
+    ds 1), exit conversion 10, velocity rescaling 15, exit transform 29;
+    each edge (edge order 1) 31 incl. one tan."""
     per_slice = 4 * (29 + 4 * LIBM) + 3 * 21
     return 25 + 9 + slices * per_slice + 2 + 10 + 15 + 29 + 31 * (edge1 + edge2)
 
 
-def _rfca_flops():
-    """RFCA with L=0 (kick method, simple_rfca.c:565-640): time of flight 10,
-    phase 5, one sin, add_to_particle_energy 28, misalignment 4, gamma1 and
-    inverseF 4."""
-    return 10 + 5 + LIBM + 2 + 28 + 4 + 4
-
-
-def flops_per_particle_turn(beamline):
-    """Algorithmic FLOPs per surviving particle per pass: +, -, *, /, sqrt
-    count 1, each libm call 20 (LIBM); only operations the coordinates depend
-    on are counted."""
+def flops_per_particle_turn_itemised(beamline):
+    """The itemised count of the operations each routine performs as written
+    (reported beside the algorithmic figure; the convention is the same:
+    +, -, *, /, sqrt 1, libm LIBM)."""
     total = 0
     for e in beamline.elements:
         t = e.type
@@ -283,9 +333,9 @@ def flops_per_particle_turn(beamline):
             total += _multipole_flops(order, S, bool(p.dx or p.dy or p.dz), bool(p.tilt))
         elif t == A.EH_T_CSBEND:
             b = e.p.csbend
-            total += _csbend_flops(b.nSlices, b.edge1_effects, b.edge2_effects)
+            total += _csbend_flops(b.nSlices, b.edge1_effects, b.edge2_effects) if b.angle else 12
         elif t == A.EH_T_RFCA:
-            total += _rfca_flops()
+            total += 10 + 5 + LIBM + 2 + 28 + 4 + 4
         elif t == A.EH_T_EDRIFT:
             total += 12
     return total

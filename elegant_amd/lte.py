@@ -158,13 +158,17 @@ class Lattice:
 
     def parse(self, text):
         for st in _statements(text):
-            if st.startswith("USE"):
-                self.use = _split_top(st, ",")[1] if "," in st else None
-                continue
-            if st.startswith("RETURN"):
-                break
+            # a statement with a "label:" prefix defines an element or line;
+            # only one without it can be the USE or RETURN command, as
+            # tell_type decides from the token before the colon
+            # (mad_parse.c:669-713)
             m = re.match(r'^"?([^:"]+)"?\s*:\s*(.*)$', st)
             if not m:
+                if re.match(r"^USE\s*(,|$)", st):
+                    self.use = _split_top(st, ",")[1] if "," in st else None
+                    continue
+                if re.match(r"^RETURN\s*$", st):
+                    break
                 raise LatticeError("cannot parse %r" % st)
             label, body = m.group(1).strip(), m.group(2).strip()
             lm = re.match(r"^LINE\s*=\s*\((.*)\)\s*$", body)

@@ -387,3 +387,74 @@ def test_everything_lost_and_invalid_inputs(tracker):
     nc = o.do_tracking(c, bl, Po, 3)
     assert ng == nc == 0 and eg == o.effort()
     assert_rows_identical(g, c, "all lost")
+
+
+@pytest.mark.parametrize("end1,end2,fid", [(0, 0, "tref"), (1, 1, "tref"), (1, 0, "tmean")])
+def test_simple_rf_cavity_matrix_method(tracker, end1, end2, fid):
+    """RFCA with L > 0 and N_KICKS = 0: the matrix method (simple_rfca.c:
+    403-407, 824-941), with and without the end-focus kicks, on rows of
+    every kind (its own removeInvalidParticles is the only cull), bit for
+    bit against the checker, signs of zeros included"""
+    kw = dict(L=0.6, VOLT=4e6, PHASE=150.0, FREQ=352e6, N_KICKS=0, END1_FOCUS=end1, END2_FOCUS=end2)
+    if fid == "tref":
+        kw["T_REFERENCE"] = 2e-8
+    rf = E.RFCA(**kw).p.rfca
+    r = csbend_edge_rows(1000, 37) if fid == "tref" else beam(1500, 38, halo=0.0)
+    if fid != "tref":
+        r[:4, 5] = -1.5
+    g = r.copy()
+    c = r.copy()
+    ng, ph = tracker.simple_rf_cavity(g, rf, PO, 9.0)
+    o = O.Oracle("emath")
+    ptr, keep = rows_ptr(c)
+    pc = C.c_double(PO)
+    nc = o.L.or_simple_rf_cavity(o.ctx, 0, ptr, c.shape[0], C.byref(rf), C.byref(pc), 9.0)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "rfca matrix end %d%d %s" % (end1, end2, fid))
+    ids = g[:ng, 6].astype(int) - 1  # rows were given with IDs 1..n in order
+    assert np.nanmax(np.abs(g[:ng, 5] - r[ids, 5])) > 1e-5  # the cavity accelerated
+
+
+@pytest.mark.parametrize("aperture", [False, True])
+def test_csbend_zero_angle_is_the_exact_drift(tracker, aperture):
+    """CSBEND with ANGLE = 0 is exactDrift (csbend.c:801-803, 3767-3790),
+    through the element entry point, on rows of every kind"""
+    b = E.CSBEND(L=0.7, ANGLE=0.0, N_KICKS=16).p.csbend
+    ap = A.EH_MAXAMP(0.05, 0.04, 0, 2, 2, 0) if aperture else None
+    r = csbend_edge_rows(1000, 13)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.track_through_csbend(g, b, PO, 3.0, ap)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0,
+                                                 C.byref(ap) if ap is not None else None)
+    assert ng == nc == r.shape[0]
+    assert_rows_bitwise(g[:, :7], c[:, :7], "csbend angle 0")
+
+
+def test_ring_with_exact_drifts_and_matrix_cavity(tracker):
+    """do_tracking over the storage ring with every drift an EDRIFT, one
+    zero-angle CSBEND per cell and the RF cavity on the matrix method
+    (L = 0.5, N_KICKS = 0, TMEAN fiducial): rows, order, losses and effort
+    identical to the checker over three passes, with a lossy halo"""
+    bl, Po = Lt.storage_ring(seed=4)
+    els = []
+    for e in bl.elements:
+        if e.type == A.EH_T_DRIF:
+            els.append(E.EDRIFT(L=e.p.drift.length))
+        elif e.type == A.EH_T_RFCA:
+            els.append(E.RFCA("RF", L=0.5, VOLT=6e6, FREQ=352e6, PHASE=180.0, N_KICKS=0))
+        else:
+            els.append(e)
+            if e.type == A.EH_T_KOCT:
+                els.append(E.CSBEND(L=0.05, ANGLE=0.0))
+    ring = E.Beamline(els)
+    rows = Lt.matched_beam(bl, 1200, 1e-9, 1e-11, 1e-3, 10e-12, 21, Po)
+    rows[:400, 0] += 0.0148
+    g = rows.copy()
+    c = rows.copy()
+    ng, eg = tracker.do_tracking(g, ring, Po, 3)
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, ring, Po, 3)
+    assert ng == nc and eg == o.effort() and 0 < nc < rows.shape[0]
+    assert_rows_identical(g, c, "ring with EDRIFT and matrix RFCA")

@@ -131,3 +131,25 @@ R: LINE=("ID3Extru", D, "extru", D)
     rows[:, 7] = -1
     n = O.Oracle("glibc").do_tracking(rows, E.Beamline(bl.elements[:2]), 1e4, 1)
     assert n == 1 and rows[0, 6] == 1
+
+
+def test_labels_starting_with_use_or_return_are_definitions():
+    """Only a statement without a "label:" prefix is the USE or RETURN
+    command (tell_type reads the token before the colon, mad_parse.c:669-713):
+    elements named USERDRIFT or RETURNQ are ordinary definitions."""
+    text = """
+    userdrift: drif, l=1.0
+    returnq: kquad, l=0.3, k1=0.7
+    usecell: line=(userdrift, returnq)
+    use, usecell
+    return
+    late: drif, l=9
+    """
+    lat = lte.Lattice().parse(text)
+    assert lat.elements["USERDRIFT"] == ("DRIF", {"L": 1.0})
+    assert lat.elements["RETURNQ"] == ("KQUAD", {"L": 0.3, "K1": 0.7})
+    assert lat.use == "USECELL"
+    assert "LATE" not in lat.elements            # parsing stopped at RETURN
+    assert [n for n, _ in lat.expand()] == ["USERDRIFT", "RETURNQ"]
+    with pytest.raises(lte.LatticeError, match="cannot parse"):
+        lte.Lattice().parse("usex, ring")

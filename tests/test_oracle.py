@@ -161,6 +161,98 @@ def test_rfca_zero_length_energy_gain():
     assert abs(r[0, 5] - (math.sqrt(g * g - 1) - PO) / PO) < 1e-14
 
 
+def _rf_point_time(pt, L, tref, Po=PO):
+    """arrival time at the cavity centre, s = beta c t (simple_rfca.c:848-850)"""
+    P = Po * (1 + pt[5])
+    beta = P / math.sqrt(P * P + 1)
+    return (pt[4] + L / 2 * math.sqrt(1 + pt[1] ** 2 + pt[3] ** 2)) / (Lt.C_MKS * beta) - tref
+
+
+@pytest.mark.parametrize("mode", ["glibc", "emath"])
+def test_rfca_matrix_method_is_an_accelerating_drift(mode):
+    """RFCA with L > 0 and N_KICKS = 0 takes the matrix method
+    (simple_rfca.c:403-407, 824-941): the gain dgamma = V sin(phi) is spread
+    evenly along the cavity and there is no transverse force.  So P x' is
+    conserved while P rises linearly from P to P1 = sqrt((gamma+dgamma)^2 - 1):
+    x' leaves as x' P/P1, x advances by x' * integral_0^L P/P(z) dz (done here
+    by numerical quadrature at 40 digits), delta = (P1 - Po)/Po, and s = beta c t keeps the
+    arrival time while the path (half the cavity at each slope) is covered."""
+    import mpmath
+    mpmath.mp.dps = 40
+    V, L, freq, tref, ph = 4e6, 0.6, 352e6, 1.7e-9, 40.0
+    rf = E.RFCA(L=L, VOLT=V, PHASE=ph, FREQ=freq, T_REFERENCE=tref, N_KICKS=0)
+    pts = [[1e-3, 2e-4, -5e-4, -1e-4, 0.2, 1e-3], [-2e-3, -3e-4, 1e-3, 2e-4, 0.5, -2e-3],
+           [0.0, 1e-3, 0.0, -7e-4, 0.01, 0.0]]
+    r, n, _ = track([rf], rows_of(pts), mode=mode)
+    assert n == len(pts)
+    for pt, out in zip(pts, r):
+        P = PO * (1 + pt[5])
+        g = math.sqrt(P * P + 1)
+        phi = 2 * math.pi * freq * _rf_point_time(pt, L, tref) + math.radians(ph)
+        dg = V / (1e6 * Lt.ME_MEV) * math.sin(phi)
+        P1 = math.sqrt((g + dg) ** 2 - 1)
+        assert abs(P1 - P) > 1e-6 * P  # a real acceleration, not the L*x' limit
+        reach = float(mpmath.quad(lambda z: P / (P + (P1 - P) * z / L), [0, L]))
+        # the reference forms log(1 + u), u = dP/P, from the rounded 1 + u: a
+        # relative error up to 2^-53/|u| in the reach, against O(u) for a
+        # wrong formula (P/P1 for P/dP, a missing log, ...)
+        u = (P1 - P) / P
+        tol = 2.0 ** -52 / abs(u) + 1e-15
+        for c in (0, 2):
+            if pt[c + 1]:
+                assert abs((out[c] - pt[c]) / pt[c + 1] - reach) <= tol * reach
+            else:
+                assert out[c] == pt[c]
+            assert abs(out[c + 1] - pt[c + 1] * P / P1) <= 1e-15 * abs(pt[c + 1])
+        assert abs(out[5] - (P1 - PO) / PO) <= 2e-16
+        path = L / 2 * (math.sqrt(1 + pt[1] ** 2 + pt[3] ** 2) +
+                        math.sqrt(1 + (pt[1] * P / P1) ** 2 + (pt[3] * P / P1) ** 2))
+        beta0, beta1 = P / g, P1 / math.sqrt(P1 * P1 + 1)
+        assert abs(out[4] - (pt[4] + path) * beta1 / beta0) <= 1e-14 * abs(out[4])
+
+
+def test_rfca_matrix_method_at_vanishing_voltage_is_a_drift():
+    """With |dP/P| <= 1e-14 the matrix method takes R12 = L (simple_rfca.c:
+    921-924): a cavity with a vanishing voltage moves x and y exactly as the
+    exact drift does, and leaves slopes, delta and s unchanged to rounding
+    (delta to the rounding of P = Po(1 + delta) it is re-formed from)."""
+    pts = [[1e-3, 2e-4, -5e-4, -1e-4, 0.2, 1e-3], [-2e-3, -3e-4, 1e-3, 2e-4, 0.5, -2e-3]]
+    L = 0.8
+    rf = E.RFCA(L=L, VOLT=1e-9, PHASE=90.0, FREQ=352e6, T_REFERENCE=0.0, N_KICKS=0)
+    r, n, _ = track([rf], rows_of(pts))
+    d, nd, _ = track([E.EDRIFT(L=L)], rows_of(pts))
+    assert n == nd == 2
+    assert (r[:, [0, 2]] == d[:, [0, 2]]).all()
+    assert np.allclose(r[:, [1, 3]], d[:, [1, 3]], rtol=1e-15, atol=0)
+    assert np.allclose(r[:, 4], d[:, 4], rtol=1e-15, atol=0)
+    # delta is re-formed as (P + dP - Po)/Po: a rounding of P, not of delta
+    assert np.abs(r[:, 5] - d[:, 5]).max() <= 2 * np.spacing(PO) / PO
+
+
+def test_exact_drift_path_length():
+    """exactDrift (csbend.c:3767-3790), for EDRIFT and for a CSBEND with
+    ANGLE = 0 (csbend.c:801-803): x and y advance by the slopes times L and s
+    by the length of the straight path, sqrt(L^2 + (x'L)^2 + (y'L)^2),
+    checked here in exact rational arithmetic to within an ulp."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    pts = [[1e-3, 2e-4, -5e-4, -1e-4, 0.2, 1e-3], [0.0, 0.3, 0.0, -0.2, 1.5, 0.0],
+           [-1e-2, 1e-7, 2e-3, 0.0, 0.0, -2e-2]]
+    L = 1.3
+    for elems in ([E.EDRIFT(L=L)], [E.CSBEND(L=L, ANGLE=0.0)]):
+        r, n, _ = track(elems, rows_of(pts))
+        assert n == len(pts)
+        for pt, out in zip(pts, r):
+            assert out[0] == pt[0] + pt[1] * L and out[2] == pt[2] + pt[3] * L
+            dL = Decimal(L)
+            exact = (dL * dL + (Decimal(pt[1]) * dL) ** 2 + (Decimal(pt[3]) * dL) ** 2).sqrt()
+            assert abs(Decimal(out[4]) - (Decimal(pt[4]) + exact)) <= Decimal(np.spacing(out[4])) * 2
+            assert out[1] == pt[1] and out[3] == pt[3] and out[5] == pt[5]
+    e, _, _ = track([E.EDRIFT(L=L)], rows_of(pts))
+    b, _, _ = track([E.CSBEND(L=L, ANGLE=0.0)], rows_of(pts))
+    assert (e.view(np.int64) == b.view(np.int64)).all()
+
+
 def test_kahan_plus_against_exact_sum():
     L = O.lib("glibc")
     rng = np.random.default_rng(4)
@@ -174,6 +266,63 @@ def test_kahan_plus_against_exact_sum():
         naive += x
     assert abs(Fraction(s) - exact) <= abs(Fraction(naive) - exact)
     assert abs(float(Fraction(s) - exact)) <= np.spacing(s)
+
+
+def test_kahan_plus_carries_the_error_into_the_next_addend():
+    """KahanPlus (kahan.c:27-36) adds the running error to the next addend
+    *before* that addend meets the sum: Y = b + error, sum = oldSum + Y,
+    error = Y - (sum - oldSum).  On [-5/512, -5/32, 1.25*2^-56, -1/16] this
+    rounds the carried -0.75*2^-56 into Y = -1/16 - 2^-56, and the last sum
+    -117/512 + 2^-56 is a tie that goes to the even -117/512: the result is
+    -117/512, one ulp from the correctly rounded sum -117/512 + 2^-55, with
+    the dropped 2^-56 left in the error.
+    Plausible restatement mistakes give other results on the same sequence:
+    plain summation, the error applied once at the end (Neumaier's or a
+    separate running correction) and the carry with its sign flipped."""
+    L = O.lib("glibc")
+    v = [-5 / 512, -5 / 32, 1.25 * 2.0 ** -56, -1 / 16]
+    s, err = 0.0, C.c_double(0.0)
+    for x in v:
+        s = L.or_kahan_plus(s, x, C.byref(err))
+    assert s == -117 / 512
+    # error = Y - (sum - oldSum) = (-1/16 - 2^-56) - (-1/16 - 2^-55): the 2^-56 the tie dropped
+    assert err.value == 2.0 ** -56
+    exact = float(sum(Fraction(x) for x in v))
+    assert exact == -117 / 512 + 2.0 ** -55
+
+    def naive(v):
+        t = 0.0
+        for x in v:
+            t += x
+        return t
+
+    def end_corrected(v):  # errors summed apart and added once at the end
+        t, c = 0.0, 0.0
+        for x in v:
+            u = t + x
+            c += x - (u - t)
+            t = u
+        return t + c
+
+    def neumaier(v):
+        t, c = 0.0, 0.0
+        for x in v:
+            u = t + x
+            c += ((t - u) + x) if abs(t) >= abs(x) else ((x - u) + t)
+            t = u
+        return t + c
+
+    def sign_flipped(v):
+        t, e = 0.0, 0.0
+        for x in v:
+            y = x - e
+            u = t + y
+            e = y - (u - t)
+            t = u
+        return t
+
+    for alt in (naive, end_corrected, neumaier, sign_flipped):
+        assert alt(v) != s, alt.__name__
 
 
 def test_fixed_point_mean_is_exact_and_order_free():
