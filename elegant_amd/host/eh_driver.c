@@ -11,11 +11,13 @@
  * an order-dependent reduction (the RFCA fiducial mean, simple_rfca.c:179-210)
  * needs it.
  */
+#define _POSIX_C_SOURCE 200809L
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include "eh_consts.h"
 #include "eh_host.h"
@@ -49,6 +51,18 @@ struct eh_ctx {
   int64_t pass_offset0; /* first pass of the run (CHARGE acts on it) */
   int64_t n_original;
   int32_t chunk_passes;
+  /* columns of a particle row (totalPropertiesPerParticle, elegant.c:906-912):
+   * rows move whole, as swapParticles moves them (elegant.c:2663-2670) */
+  int32_t row_props;
+  /* columns 8.. of the uploaded rows (bunch, weight, ...), carried with
+   * their particle to eh_download */
+  double *extra;
+  int32_t extra_w;
+  /* stopping conditions of the pass loop (do_tracking.c:494-520) */
+  int64_t particle_limit;
+  char *interrupt_path;
+  double interrupt_mtime;
+  volatile const int32_t *interrupt_flag;
 };
 
 static int fail(eh_ctx *c, int code, const char *m) {
@@ -111,7 +125,15 @@ int eh_ctx_create(int device, eh_ctx **out) {
     return rc;
   }
   c->chunk_passes = 64;
+  c->row_props = 8;
   *out = c;
+  return EH_OK;
+}
+
+int eh_set_particle_properties(eh_ctx *c, int32_t nprops) {
+  if (!c || nprops < 8 || nprops > 64)
+    return fail(c, EH_ERR_INVALID, "eh_set_particle_properties: 8 <= nprops <= 64");
+  c->row_props = nprops;
   return EH_OK;
 }
 
@@ -125,6 +147,9 @@ static void free_particles(eh_ctx *c) {
   free(c->h_code);
   free(c->h_s);
   free(c->h_dp);
+  free(c->extra);
+  c->extra = NULL;
+  c->extra_w = 0;
   c->ids = NULL;
   c->order = c->pos = c->scratch = NULL;
   c->applied = NULL;
@@ -139,6 +164,7 @@ int eh_ctx_destroy(eh_ctx *c) {
   free_particles(c);
   free(c->elems);
   free(c->prog);
+  free(c->interrupt_path);
   ehd_destroy(c->dev);
   free(c);
   return EH_OK;
@@ -247,6 +273,16 @@ int eh_upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cul
   soa = (double *)malloc(sizeof(double) * 6 * (n + 1));
   if (!soa)
     return EH_ERR_NOMEM;
+  if (nprops > 8) {
+    c->extra_w = nprops - 8;
+    c->extra = (double *)malloc(sizeof(double) * c->extra_w * (n + 1));
+    if (!c->extra) {
+      free(soa);
+      return EH_ERR_NOMEM;
+    }
+    for (int64_t i = 0; i < n; i++)
+      memcpy(c->extra + c->extra_w * i, coord[i] + 8, sizeof(double) * c->extra_w);
+  }
   {
     int64_t nlost = 0;
     for (int64_t i = 0; i < n; i++) {
@@ -486,9 +522,72 @@ static void charge_elements(eh_ctx *c, int64_t pass) {
   }
 }
 
+/* ---- stopping conditions of do_tracking's pass loop (do_tracking.c:494-520) ---- */
+
+int eh_set_particle_limit(eh_ctx *c, int64_t limit) {
+  if (!c)
+    return EH_ERR_INVALID;
+  c->particle_limit = limit > 0 ? limit : 0;
+  return EH_OK;
+}
+
+static double file_mtime(const char *path) {
+  struct stat st;
+  if (stat(path, &st) != 0)
+    return -1;
+  return (double)st.st_mtim.tv_sec + 1e-9 * (double)st.st_mtim.tv_nsec;
+}
+
+int eh_set_interrupt_file(eh_ctx *c, const char *path) {
+  if (!c)
+    return EH_ERR_INVALID;
+  free(c->interrupt_path);
+  c->interrupt_path = NULL;
+  c->interrupt_mtime = 0;
+  if (path && path[0]) {
+    size_t len = strlen(path);
+    c->interrupt_path = (char *)malloc(len + 1);
+    if (!c->interrupt_path)
+      return EH_ERR_NOMEM;
+    memcpy(c->interrupt_path, path, len + 1);
+    /* a file already there counts only once it is modified again (elegant.c:1133-1137) */
+    double m = file_mtime(path);
+    c->interrupt_mtime = m > 0 ? m : 0;
+  }
+  return EH_OK;
+}
+
+int eh_set_interrupt_flag(eh_ctx *c, volatile const int32_t *flag) {
+  if (!c)
+    return EH_ERR_INVALID;
+  c->interrupt_flag = flag;
+  return EH_OK;
+}
+
+static int interrupt_requested(const eh_ctx *c) {
+  if (c->interrupt_flag && *c->interrupt_flag)
+    return 1;
+  if (c->interrupt_path) {
+    double m = file_mtime(c->interrupt_path);
+    if (m >= 0 && m > c->interrupt_mtime)
+      return 1;
+  }
+  return 0;
+}
+
+/* particles not lost before pass q (h_pass current); rows removed by the
+ * initial NaN cull (code -1) were lost before the first pass */
+static int64_t alive_at(const eh_ctx *c, int64_t q) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] < 0 || (c->h_pass[i] >= q && c->h_code[i] != -1))
+      k++;
+  return k;
+}
+
 int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_stats *stats) {
   int rc;
-  int64_t n_alive0 = 0;
+  int64_t n_alive0 = 0, done_end;
   if (!c || !P_central || npasses < 0)
     return fail(c, EH_ERR_INVALID, "eh_track: bad arguments");
   if (!c->loaded)
@@ -511,12 +610,50 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
     int64_t pass = pass0, end = pass0 + npasses;
     while (pass < end) {
       int32_t b = next_barrier(c, 0);
+      if (interrupt_requested(c))
+        break;
+      if (c->particle_limit > 0) {
+        rc = refresh_losses(c);
+        if (rc)
+          return rc;
+        if (alive_at(c, pass) < c->particle_limit)
+          break;
+      }
       charge_elements(c, pass);
       if (b >= c->prog->nops) {
         /* no host step in this pass: run many passes in one launch */
         int64_t chunk = end - pass;
         if (chunk > c->chunk_passes)
           chunk = c->chunk_passes;
+        if (c->particle_limit > 0) {
+          /* the limit may be crossed inside the launch: keep the state at its
+           * start, and if a later pass of it would not have begun, go back
+           * and run only the passes before that one */
+          rc = ehd_state_snapshot(c->dev, 0);
+          if (rc)
+            return devfail(c, rc);
+          rc = ehd_track(c->dev, pass, (int32_t)chunk, 0, c->prog->nops);
+          if (rc)
+            return devfail(c, rc);
+          rc = refresh_losses(c);
+          if (rc)
+            return rc;
+          int64_t stop = -1;
+          for (int64_t q = pass + 1; q < pass + chunk && stop < 0; q++)
+            if (alive_at(c, q) < c->particle_limit)
+              stop = q;
+          if (stop >= 0) {
+            rc = ehd_state_snapshot(c->dev, 1);
+            if (!rc)
+              rc = ehd_track(c->dev, pass, (int32_t)(stop - pass), 0, c->prog->nops);
+            if (rc)
+              return devfail(c, rc);
+            pass = stop;
+            break;
+          }
+          pass += chunk;
+          continue;
+        }
         rc = ehd_track(c->dev, pass, (int32_t)chunk, 0, c->prog->nops);
         if (rc)
           return devfail(c, rc);
@@ -563,6 +700,7 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
         pass++;
       }
     }
+    done_end = pass;
   }
   rc = refresh_losses(c);
   if (rc)
@@ -574,7 +712,7 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
       if (lp >= 0 && lp < pass0)
         continue;
       if (lp < 0) {
-        effort += npasses;
+        effort += done_end - pass0;
         left++;
       } else
         effort += lp - pass0;
@@ -582,7 +720,7 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
     memset(stats, 0, sizeof(*stats));
     stats->effort = effort;
     stats->n_left = left;
-    stats->passes_done = npasses;
+    stats->passes_done = done_end - pass0;
     ehd_timing(c->dev, &stats->kernel_launches, &stats->track_ms);
     ehd_collective_timing(c->dev, &left, &stats->collective_ms);
   }
@@ -624,6 +762,8 @@ int eh_download(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int64_t *n
       r[k] = soa[k * n + i];
     r[6] = c->ids[i];
     r[7] = c->h_pass[i] >= 0 ? (double)c->h_pass[i] : -1.0;
+    for (int k = 8; k < nprops && k - 8 < c->extra_w; k++)
+      r[k] = c->extra[c->extra_w * i + k - 8];
   }
   if (nLeft)
     *nLeft = c->n_alive_ref;
@@ -632,24 +772,39 @@ int eh_download(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int64_t *n
 }
 
 int64_t eh_do_tracking(eh_ctx *c, double **coord, int64_t nOriginal, int64_t *effort, const eh_element *beamline,
-                       int64_t n_elems, double *P_central, int64_t n_passes, int64_t passOffset) {
+                       int64_t n_elems, double *P_central, double **accepted, int64_t n_passes, int64_t passOffset) {
   eh_stats st;
   int64_t nLeft = 0;
   int rc;
-  if (!c || !P_central)
-    return EH_ERR_INVALID;
+  const int64_t W = c ? c->row_props : 0;
+  double *initial = NULL;
+  if (!c || !P_central || nOriginal < 0 || (nOriginal > 0 && (!coord || !beamline)))
+    return fail(c, EH_ERR_INVALID, "eh_do_tracking: bad arguments");
   rc = eh_load_program(c, beamline, n_elems, *P_central);
   if (rc)
     return rc;
-  rc = eh_upload(c, coord, nOriginal, 8);
-  if (rc)
-    return rc;
-  c->pass_offset0 = passOffset;
-  c->n_original = nOriginal;
-  rc = eh_track(c, passOffset, n_passes, P_central, &st);
-  if (rc)
-    return rc;
-  rc = eh_download(c, coord, nOriginal, 8, &nLeft);
+  if (accepted && nOriginal > 0) {
+    /* accepted starts as a copy of the rows (do_tracking.c:360-361) and is
+     * then permuted with them, so it ends holding each particle's initial
+     * row in that particle's final row */
+    initial = (double *)malloc(sizeof(double) * W * nOriginal);
+    if (!initial)
+      return EH_ERR_NOMEM;
+    for (int64_t i = 0; i < nOriginal; i++)
+      memcpy(initial + W * i, coord[i], sizeof(double) * W);
+  }
+  rc = eh_upload(c, coord, nOriginal, (int32_t)W);
+  if (!rc) {
+    c->pass_offset0 = passOffset;
+    c->n_original = nOriginal;
+    rc = eh_track(c, passOffset, n_passes, P_central, &st);
+  }
+  if (!rc)
+    rc = eh_download(c, coord, nOriginal, (int32_t)W, &nLeft);
+  if (!rc && initial)
+    for (int64_t j = 0; j < nOriginal; j++)
+      memcpy(accepted[j], initial + W * c->order[j], sizeof(double) * W);
+  free(initial);
   if (rc)
     return rc;
   if (effort)
@@ -659,10 +814,55 @@ int64_t eh_do_tracking(eh_ctx *c, double **coord, int64_t nOriginal, int64_t *ef
 
 /* ------------------------------------------------ element entry points */
 
+/* The rows of a single-element call back to the caller in the reference's
+ * order.  Whole rows move, as swapParticles moves row_props doubles
+ * (elegant.c:2663-2670): the element's columns 0..6 come from the device,
+ * the other columns keep their values and follow their particle, and the
+ * accepted rows, if given, take the same permutation (multipole.c:1075-1076,
+ * limit_amplitudes.c:263-264 and 308-309).  Rows were uploaded in their
+ * order, so slot i is row i. */
+static int writeback_rows(eh_ctx *c, double **rows, int64_t n, double **accepted, int64_t *nLeft) {
+  const int64_t W = c->row_props;
+  double **tmp = (double **)malloc(sizeof(double *) * n);
+  double *buf = (double *)malloc(sizeof(double) * 8 * n);
+  double *keep = (double *)malloc(sizeof(double) * W * n);
+  double *acc = accepted ? (double *)malloc(sizeof(double) * W * n) : NULL;
+  int rc;
+  if (!tmp || !buf || !keep || (accepted && !acc)) {
+    free(tmp);
+    free(buf);
+    free(keep);
+    free(acc);
+    return EH_ERR_NOMEM;
+  }
+  for (int64_t i = 0; i < n; i++)
+    tmp[i] = buf + 8 * i;
+  rc = eh_download(c, tmp, n, 8, nLeft);
+  if (!rc) {
+    for (int64_t i = 0; i < n; i++) {
+      memcpy(keep + W * i, rows[i], sizeof(double) * W);
+      if (acc)
+        memcpy(acc + W * i, accepted[i], sizeof(double) * W);
+    }
+    for (int64_t j = 0; j < n; j++) {
+      const int64_t src = c->order[j];
+      memcpy(rows[j], tmp[j], sizeof(double) * 7);
+      memcpy(rows[j] + 7, keep + W * src + 7, sizeof(double) * (W - 7));
+      if (acc)
+        memcpy(accepted[j], acc + W * src, sizeof(double) * W);
+    }
+  }
+  free(tmp);
+  free(buf);
+  free(keep);
+  free(acc);
+  return rc;
+}
+
 /* run a one-element program on host rows and compact them like the
  * reference element routine (lost rows to the tail) */
-static int64_t run_single(eh_ctx *c, double **rows, int64_t n, const eh_element *e, double Po, double z_start,
-                          const EH_MAXAMP *maxamp, int64_t pass) {
+static int64_t run_single(eh_ctx *c, double **rows, int64_t n, double **accepted, const eh_element *e, double Po,
+                          double z_start, const EH_MAXAMP *maxamp, int64_t pass) {
   eh_element bl[2];
   int nb = 0, rc;
   int64_t nLeft = 0;
@@ -698,46 +898,28 @@ static int64_t run_single(eh_ctx *c, double **rows, int64_t n, const eh_element 
   rc = ehd_track(c->dev, pass, 1, 0, 1);
   if (rc)
     return devfail(c, rc);
-  {
-    double **tmp = (double **)malloc(sizeof(double *) * n);
-    double *buf = (double *)malloc(sizeof(double) * 8 * n);
-    if (!tmp || !buf) {
-      free(tmp);
-      free(buf);
-      return EH_ERR_NOMEM;
-    }
-    for (int64_t i = 0; i < n; i++)
-      tmp[i] = buf + 8 * i;
-    rc = eh_download(c, tmp, n, 8, &nLeft);
-    if (!rc)
-      for (int64_t i = 0; i < n; i++) {
-        int ncopy = 7;
-        memcpy(rows[i], tmp[i], sizeof(double) * ncopy);
-      }
-    free(tmp);
-    free(buf);
-    if (rc)
-      return rc;
-  }
+  rc = writeback_rows(c, rows, n, accepted, &nLeft);
+  if (rc)
+    return rc;
   (void)st;
   return nLeft;
 }
 
 int64_t eh_multipole_tracking2(eh_ctx *c, double **particle, int64_t n_part, const eh_element *elem, double Po,
-                               double z_start, const EH_MAXAMP *maxamp) {
+                               double **accepted, double z_start, const EH_MAXAMP *maxamp) {
   if (!c || !elem || (elem->type != EH_T_KQUAD && elem->type != EH_T_KSEXT && elem->type != EH_T_KOCT))
     return fail(c, EH_ERR_INVALID, "eh_multipole_tracking2: element must be KQUAD, KSEXT or KOCT");
-  return run_single(c, particle, n_part, elem, Po, z_start, maxamp, 0);
+  return run_single(c, particle, n_part, accepted, elem, Po, z_start, maxamp, 0);
 }
 
 int64_t eh_track_through_csbend(eh_ctx *c, double **part, int64_t n_part, const EH_CSBEND *csbend, double Po,
-                                double z_start, const EH_MAXAMP *maxamp) {
+                                double **accepted, double z_start, const EH_MAXAMP *maxamp) {
   eh_element e;
   if (!c || !csbend)
     return EH_ERR_INVALID;
   eh_element_defaults(&e, EH_T_CSBEND);
   e.p.csbend = *csbend;
-  return run_single(c, part, n_part, &e, Po, z_start, maxamp, 0);
+  return run_single(c, part, n_part, accepted, &e, Po, z_start, maxamp, 0);
 }
 
 int64_t eh_track_particles_drift(eh_ctx *c, double **part, int64_t n_part, const EH_DRIFT *drift) {
@@ -746,11 +928,11 @@ int64_t eh_track_particles_drift(eh_ctx *c, double **part, int64_t n_part, const
     return EH_ERR_INVALID;
   eh_element_defaults(&e, EH_T_DRIF);
   e.p.drift = *drift;
-  return run_single(c, part, n_part, &e, 1.0, 0.0, NULL, 0);
+  return run_single(c, part, n_part, NULL, &e, 1.0, 0.0, NULL, 0);
 }
 
-int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA *rfca, double *P_central,
-                            double zEnd, double *phase_fiducial) {
+int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA *rfca, double **accepted,
+                            double *P_central, double zEnd, double *phase_fiducial) {
   eh_element e;
   int64_t r;
   if (!c || !rfca || !P_central || !phase_fiducial)
@@ -808,25 +990,9 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     rc = ehd_track(c->dev, 0, 1, 0, 1);
     if (rc)
       return devfail(c, rc);
-    {
-      double **tmp = (double **)malloc(sizeof(double *) * np);
-      double *buf = (double *)malloc(sizeof(double) * 8 * np);
-      if (!tmp || !buf) {
-        free(tmp);
-        free(buf);
-        return EH_ERR_NOMEM;
-      }
-      for (int64_t i = 0; i < np; i++)
-        tmp[i] = buf + 8 * i;
-      rc = eh_download(c, tmp, np, 8, &nLeft);
-      if (!rc)
-        for (int64_t i = 0; i < np; i++)
-          memcpy(part[i], tmp[i], sizeof(double) * 7);
-      free(tmp);
-      free(buf);
-      if (rc)
-        return rc;
-    }
+    rc = writeback_rows(c, part, np, accepted, &nLeft);
+    if (rc)
+      return rc;
     r = nLeft;
   }
   return r;
@@ -835,8 +1001,8 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
 /* one MAXAMP check on its own, at z: a zero-length step whose only work is
  * the check do_tracking makes after an element (do_tracking.c:2461-2470)
  * with the given aperture */
-static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, const EH_MAXAMP *m, double z, double Po,
-                            int32_t extrapolate_z) {
+static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, double **accepted, const EH_MAXAMP *m, double z,
+                            double Po, int32_t extrapolate_z) {
   eh_element bl[2];
   int rc;
   int64_t nLeft = 0;
@@ -865,30 +1031,14 @@ static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, const EH_MAXA
   rc = ehd_track(c->dev, 0, 1, 0, 1);
   if (rc)
     return devfail(c, rc);
-  {
-    double **tmp = (double **)malloc(sizeof(double *) * np);
-    double *buf = (double *)malloc(sizeof(double) * 8 * np);
-    if (!tmp || !buf) {
-      free(tmp);
-      free(buf);
-      return EH_ERR_NOMEM;
-    }
-    for (int64_t i = 0; i < np; i++)
-      tmp[i] = buf + 8 * i;
-    rc = eh_download(c, tmp, np, 8, &nLeft);
-    if (!rc)
-      for (int64_t i = 0; i < np; i++)
-        memcpy(coord[i], tmp[i], sizeof(double) * 7);
-    free(tmp);
-    free(buf);
-    if (rc)
-      return rc;
-  }
+  rc = writeback_rows(c, coord, np, accepted, &nLeft);
+  if (rc)
+    return rc;
   return nLeft;
 }
 
-int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
-                            int32_t extrapolate_z, int32_t openCode) {
+int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double **accepted,
+                            double z, double Po, int32_t extrapolate_z, int32_t openCode) {
   EH_MAXAMP m;
   if (!c || openCode < 0 || openCode > 4)
     return EH_ERR_INVALID;
@@ -900,11 +1050,12 @@ int64_t eh_limit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax,
   m.y_max = ymax;
   m.exponent = 2;
   m.openSide = openCode;
-  return run_aperture(c, coord, np, &m, z, Po, extrapolate_z);
+  return run_aperture(c, coord, np, accepted, &m, z, Po, extrapolate_z);
 }
 
-int64_t eh_elimit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double z, double Po,
-                             int32_t extrapolate_z, int32_t openCode, int32_t exponent, int32_t yexponent) {
+int64_t eh_elimit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax, int64_t np, double **accepted,
+                             double z, double Po, int32_t extrapolate_z, int32_t openCode, int32_t exponent,
+                             int32_t yexponent) {
   EH_MAXAMP m;
   if (!c || openCode < 0 || openCode > 4)
     return EH_ERR_INVALID;
@@ -915,7 +1066,7 @@ int64_t eh_elimit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax
    * check at all (limit_amplitudes.c:523-528) */
   if (xmax <= 0 || ymax <= 0) {
     if (xmax > 0 || ymax > 0)
-      return eh_limit_amplitudes(c, coord, xmax, ymax, np, z, Po, extrapolate_z, openCode);
+      return eh_limit_amplitudes(c, coord, xmax, ymax, np, accepted, z, Po, extrapolate_z, openCode);
     return np;
   }
   if (np <= 0)
@@ -927,7 +1078,7 @@ int64_t eh_elimit_amplitudes(eh_ctx *c, double **coord, double xmax, double ymax
   m.exponent = exponent;
   m.yExponent = yexponent;
   m.openSide = openCode;
-  return run_aperture(c, coord, np, &m, z, Po, extrapolate_z);
+  return run_aperture(c, coord, np, accepted, &m, z, Po, extrapolate_z);
 }
 
 /* debugging/measurement helpers (not in the public header) */

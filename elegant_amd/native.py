@@ -42,18 +42,22 @@ SIGNATURES = [
     ("eh_upload", C.c_int, [P, PP, C.c_int64, C.c_int32]),
     ("eh_track", C.c_int, [P, C.c_int64, C.c_int64, PD, C.POINTER(A.eh_stats)]),
     ("eh_download", C.c_int, [P, PP, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
-    ("eh_do_tracking", C.c_int64, [P, PP, C.c_int64, C.POINTER(C.c_int64), PE, C.c_int64, PD, C.c_int64,
+    ("eh_set_particle_properties", C.c_int, [P, C.c_int32]),
+    ("eh_set_particle_limit", C.c_int, [P, C.c_int64]),
+    ("eh_set_interrupt_file", C.c_int, [P, C.c_char_p]),
+    ("eh_set_interrupt_flag", C.c_int, [P, C.POINTER(C.c_int32)]),
+    ("eh_do_tracking", C.c_int64, [P, PP, C.c_int64, C.POINTER(C.c_int64), PE, C.c_int64, PD, PP, C.c_int64,
                                    C.c_int64]),
-    ("eh_multipole_tracking2", C.c_int64, [P, PP, C.c_int64, PE, C.c_double, C.c_double,
+    ("eh_multipole_tracking2", C.c_int64, [P, PP, C.c_int64, PE, C.c_double, PP, C.c_double,
                                            C.POINTER(A.EH_MAXAMP)]),
-    ("eh_track_through_csbend", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_CSBEND), C.c_double, C.c_double,
+    ("eh_track_through_csbend", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_CSBEND), C.c_double, PP, C.c_double,
                                             C.POINTER(A.EH_MAXAMP)]),
     ("eh_rfca_fiducial", C.c_int, [P, C.c_int64, PD]),
-    ("eh_simple_rf_cavity", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_RFCA), PD, C.c_double, PD]),
+    ("eh_simple_rf_cavity", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_RFCA), PP, PD, C.c_double, PD]),
     ("eh_track_particles_drift", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_DRIFT)]),
-    ("eh_limit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+    ("eh_limit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, PP, C.c_double, C.c_double,
                                         C.c_int32, C.c_int32]),
-    ("eh_elimit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+    ("eh_elimit_amplitudes", C.c_int64, [P, PP, C.c_double, C.c_double, C.c_int64, PP, C.c_double, C.c_double,
                                          C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("eh_track_through_wake", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_WAKE), C.c_double, C.c_int64,
                                           C.c_double]),
@@ -126,6 +130,15 @@ def check(ctx, rc):
 
 def _dptr(a):
     return a.ctypes.data_as(PD)
+
+
+def _acc_ptr(accepted, rows):
+    """row pointers of an accepted array shaped like rows (or NULL)"""
+    if accepted is None:
+        return None, None
+    if accepted.dtype != np.float64 or not accepted.flags.c_contiguous or accepted.shape != rows.shape:
+        raise ValueError("accepted must be C-contiguous float64 shaped like the rows")
+    return A.rows_pointer(accepted)
 
 
 class Batch:
@@ -259,17 +272,35 @@ class Tracker:
         check(self._ctx, lib().eh_download(self._ctx, ptr, self.n, 8, C.byref(nleft)))
         return out, int(nleft.value)
 
-    def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0):
-        """do_tracking on host rows (n, >=8), in place; returns (nLeft, effort)."""
+    def set_particle_properties(self, nprops):
+        """columns of every particle row (elegant's totalPropertiesPerParticle)"""
+        check(self._ctx, lib().eh_set_particle_properties(self._ctx, int(nprops)))
+
+    def set_particle_limit(self, limit):
+        check(self._ctx, lib().eh_set_particle_limit(self._ctx, int(limit)))
+
+    def set_interrupt_file(self, path):
+        check(self._ctx, lib().eh_set_interrupt_file(self._ctx, path.encode() if path else None))
+
+    def set_interrupt_flag(self, flag):
+        """flag: a ctypes c_int32 kept alive by the caller (None turns it off)"""
+        self._flag = flag
+        check(self._ctx, lib().eh_set_interrupt_flag(self._ctx, C.byref(flag) if flag is not None else None))
+
+    def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0, accepted=None):
+        """do_tracking on host rows (n, >=8), in place; returns (nLeft, effort).
+        accepted: None or an array shaped like rows, filled with each
+        particle's initial row at its final row."""
         if rows.dtype != np.float64 or not rows.flags.c_contiguous or rows.shape[1] < 8:
             raise ValueError("rows must be C-contiguous float64 (n, >=8)")
         ptr, keep = A.rows_pointer(rows)
+        aptr, akeep = _acc_ptr(accepted, rows)
         eff = C.c_int64(0)
         pc = C.c_double(Po)
         self._beamline = beamline
         self.Po = float(Po)
         r = lib().eh_do_tracking(self._ctx, ptr, rows.shape[0], C.byref(eff), beamline.pointer(), len(beamline),
-                                 C.byref(pc), n_passes, pass_offset)
+                                 C.byref(pc), aptr, n_passes, pass_offset)
         check(self._ctx, r)
         self.n = rows.shape[0]
         return int(r), int(eff.value)
@@ -311,18 +342,21 @@ class Tracker:
                                                    _dptr(h), _dptr(v), C.byref(nb), C.byref(tmin)))
         return h[:nb.value], v[:nb.value], tmin.value
 
-    # element-by-element entry points (rows: float64 (n, 8), mutated in place)
-    def multipole_tracking2(self, rows, elem, Po, z_start, maxamp=None):
+    # element-by-element entry points (rows: float64 (n, 8), mutated in place;
+    # accepted: None or an array shaped like rows, permuted with them)
+    def multipole_tracking2(self, rows, elem, Po, z_start, maxamp=None, accepted=None):
         ptr, keep = A.rows_pointer(rows)
+        aptr, akeep = _acc_ptr(accepted, rows)
         mp = C.byref(maxamp) if maxamp is not None else None
         return check(self._ctx, lib().eh_multipole_tracking2(self._ctx, ptr, rows.shape[0], C.byref(elem), Po,
-                                                             z_start, mp))
+                                                             aptr, z_start, mp))
 
-    def track_through_csbend(self, rows, csbend, Po, z_start, maxamp=None):
+    def track_through_csbend(self, rows, csbend, Po, z_start, maxamp=None, accepted=None):
         ptr, keep = A.rows_pointer(rows)
+        aptr, akeep = _acc_ptr(accepted, rows)
         mp = C.byref(maxamp) if maxamp is not None else None
         return check(self._ctx, lib().eh_track_through_csbend(self._ctx, ptr, rows.shape[0], C.byref(csbend), Po,
-                                                              z_start, mp))
+                                                              aptr, z_start, mp))
 
     def rfca_fiducial(self, elem_index):
         """the fiducial phase the last fused run fixed for the RFCA at elem_index (NaN if unset)"""
@@ -330,26 +364,30 @@ class Tracker:
         check(self._ctx, lib().eh_rfca_fiducial(self._ctx, elem_index, C.byref(ph)))
         return ph.value
 
-    def simple_rf_cavity(self, rows, rfca, Po, zEnd, phase_fiducial=float("nan")):
+    def simple_rf_cavity(self, rows, rfca, Po, zEnd, phase_fiducial=float("nan"), accepted=None):
         ptr, keep = A.rows_pointer(rows)
+        aptr, akeep = _acc_ptr(accepted, rows)
         pc = C.c_double(Po)
         ph = C.c_double(phase_fiducial)
-        r = check(self._ctx, lib().eh_simple_rf_cavity(self._ctx, ptr, rows.shape[0], C.byref(rfca), C.byref(pc),
-                                                       zEnd, C.byref(ph)))
+        r = check(self._ctx, lib().eh_simple_rf_cavity(self._ctx, ptr, rows.shape[0], C.byref(rfca), aptr,
+                                                       C.byref(pc), zEnd, C.byref(ph)))
         return r, ph.value
 
     def track_particles_drift(self, rows, drift):
         ptr, keep = A.rows_pointer(rows)
         return check(self._ctx, lib().eh_track_particles_drift(self._ctx, ptr, rows.shape[0], C.byref(drift)))
 
-    def limit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0):
+    def limit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0, accepted=None):
         ptr, keep = A.rows_pointer(rows)
-        return check(self._ctx, lib().eh_limit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], z, Po,
+        aptr, akeep = _acc_ptr(accepted, rows)
+        return check(self._ctx, lib().eh_limit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], aptr, z, Po,
                                                           extrapolate_z, open_code))
 
-    def elimit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0, exponent=2, yexponent=0):
+    def elimit_amplitudes(self, rows, xmax, ymax, z, Po, extrapolate_z, open_code=0, exponent=2, yexponent=0,
+                          accepted=None):
         ptr, keep = A.rows_pointer(rows)
-        return check(self._ctx, lib().eh_elimit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], z, Po,
+        aptr, akeep = _acc_ptr(accepted, rows)
+        return check(self._ctx, lib().eh_elimit_amplitudes(self._ctx, ptr, xmax, ymax, rows.shape[0], aptr, z, Po,
                                                            extrapolate_z, open_code, exponent, yexponent))
 
     def track_through_wake(self, rows, wake, Po, i_pass, macro_charge):

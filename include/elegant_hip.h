@@ -212,40 +212,78 @@ const char *eh_last_error(const eh_ctx *ctx);
 /* ---- lattice: flattened beamline, all defaults resolved by the caller ---- */
 int eh_load_program(eh_ctx *ctx, const eh_element *elems, int64_t n_elems, double Po);
 
+/* ---- particle rows ----
+ * Columns per particle row, elegant's totalPropertiesPerParticle
+ * (elegant.c:906-912): 6 coordinates, ID, pass of loss, bunch, weight and,
+ * with global loss coordinates, three more.  Rows (and accepted rows) are
+ * permuted whole, as swapParticles moves them (elegant.c:2663-2670): the
+ * library writes columns 0..7 and carries every other column with its
+ * particle.  Default 8; 8 <= nprops <= 64. */
+int eh_set_particle_properties(eh_ctx *ctx, int32_t nprops);
+
 /* ---- particle state ---- */
-/* Copy AoS rows (>= 7 columns: x, x', y, y', s, delta, ID) to device SoA. */
+/* Copy AoS rows (>= 7 columns: x, x', y, y', s, delta, ID) to device SoA;
+ * columns 8..nprops-1 are kept on the host and follow their particle. */
 int eh_upload(eh_ctx *ctx, double **coord, int64_t n, int32_t nprops);
 /* Track passes [pass0, pass0+npasses) through the loaded program (the pass
  * and element loops of do_tracking.c:488,816).  *P_central is read (and
- * would be updated by CHANGE_P0 elements, which are not supported). */
+ * would be updated by CHANGE_P0 elements, which are not supported).  The
+ * pass loop stops early, before a pass, when a stopping condition below
+ * holds (do_tracking.c:494-520); stats->passes_done then counts the passes
+ * completed. */
 int eh_track(eh_ctx *ctx, int64_t pass0, int64_t npasses, double *P_central, eh_stats *stats);
 /* Write the particles back in elegant's row order: survivors first in the
  * order swap-to-tail compaction leaves them, then lost rows with s = loss
- * position, delta = Po(1+delta) and column 7 = pass of loss.  *nLeft gets
- * the survivor count.  nprops >= 8. */
+ * position, delta = Po(1+delta) and column 7 = pass of loss; columns 8 and
+ * up (as far as both this nprops and the upload's reach) are the uploaded
+ * ones of the same particle.  *nLeft gets the survivor count.  nprops >= 8. */
 int eh_download(eh_ctx *ctx, double **coord, int64_t n, int32_t nprops, int64_t *nLeft);
+
+/* ---- stopping conditions of the pass loop (do_tracking.c:494-520) ----
+ * run_setup's stop_tracking_particle_limit (elegant.c:1131): stop before a
+ * pass that would start with fewer than `limit` particles (<= 0: off).
+ * Exact to the pass: a multi-pass launch that crosses the limit is rolled
+ * back to its start and rerun up to the stopping pass. */
+int eh_set_particle_limit(eh_ctx *ctx, int64_t limit);
+/* track's interrupt_file (elegant.c:1133-1137): stop before a pass once the
+ * file exists with a modification time later than it had when this was
+ * called (or at all, if it did not exist).  NULL or "" turns it off.
+ * Checked at pass boundaries between kernel launches (every pass when a
+ * pass needs the host, else every chunk of fused passes). */
+int eh_set_interrupt_file(eh_ctx *ctx, const char *path);
+/* The same stop through a word the caller sets to nonzero (from another
+ * thread or a signal handler); NULL turns it off.  No counterpart in the
+ * reference, which polls the file only. */
+int eh_set_interrupt_flag(eh_ctx *ctx, volatile const int32_t *flag);
 
 /* do_tracking (do_tracking.c:144-3241) for a flattened beamline: upload,
  * track n_passes starting at passOffset, download.  Returns nLeft >= 0 or
- * a negative error code.  *effort accumulates survivors per pass. */
+ * a negative error code.  *effort accumulates survivors per pass.  accepted
+ * (NULL or nOriginal rows of the same width) starts as a copy of the rows
+ * (do_tracking.c:360-361) and is permuted with them, so it ends holding
+ * each particle's initial row at that particle's final row. */
 int64_t eh_do_tracking(eh_ctx *ctx, double **coord, int64_t nOriginal, int64_t *effort,
-                       const eh_element *beamline, int64_t n_elems, double *P_central,
+                       const eh_element *beamline, int64_t n_elems, double *P_central, double **accepted,
                        int64_t n_passes, int64_t passOffset);
 
 /* ---- element-by-element entry points (host-row API of the reference) ----
  * Each uploads the rows, runs one element on the GPU and writes the rows
- * back with the reference's compaction.  The aperture is the MAXAMP in force
- * (NULL for none).  They return nLeft >= 0 or a negative error code.  Like
- * the routines they replace, they leave NaN and infinite coordinates to the
- * caller: do_tracking's culls (before the first element and
+ * back with the reference's compaction: whole rows of the width set by
+ * eh_set_particle_properties move, and `accepted` (NULL, or n rows of the
+ * same width) is permuted with them, as the reference swaps accepted[i]
+ * with particle[i] (multipole.c:1075-1076, limit_amplitudes.c:263-264,
+ * 308-309).  The aperture is the MAXAMP in force (NULL for none).  They
+ * return nLeft >= 0 or a negative error code.  Like the routines they
+ * replace, they leave NaN and infinite coordinates to the caller:
+ * do_tracking's culls (before the first element and
  * limit_amplitudes(DBL_MAX) after each one) belong to eh_do_tracking and
  * eh_track; eh_limit_amplitudes removes such rows at its own z. */
 /* multipole_tracking2 (multipole.c:693-1117) for KQUAD/KSEXT/KOCT */
 int64_t eh_multipole_tracking2(eh_ctx *ctx, double **particle, int64_t n_part, const eh_element *elem,
-                               double Po, double z_start, const EH_MAXAMP *maxamp);
+                               double Po, double **accepted, double z_start, const EH_MAXAMP *maxamp);
 /* track_through_csbend (csbend.c:665-1409) */
 int64_t eh_track_through_csbend(eh_ctx *ctx, double **part, int64_t n_part, const EH_CSBEND *csbend,
-                                double Po, double z_start, const EH_MAXAMP *maxamp);
+                                double Po, double **accepted, double z_start, const EH_MAXAMP *maxamp);
 /* The fiducial phase that eh_do_tracking / eh_track fixed for the RFCA at
  * beamline index elem_index on its first pass (rfca->phase_fiducial once
  * fiducial_seen is set, simple_rfca.c:466-471); NaN while unset.  An
@@ -256,19 +294,19 @@ int eh_rfca_fiducial(eh_ctx *ctx, int64_t elem_index, double *phase_fiducial);
 /* simple_rf_cavity (simple_rfca.c:264-293); *phase_fiducial carries the
  * element's fiducial phase between calls (NaN = not yet set, as
  * rfca->fiducial_seen == 0) */
-int64_t eh_simple_rf_cavity(eh_ctx *ctx, double **part, int64_t np, const EH_RFCA *rfca, double *P_central,
-                            double zEnd, double *phase_fiducial);
+int64_t eh_simple_rf_cavity(eh_ctx *ctx, double **part, int64_t np, const EH_RFCA *rfca, double **accepted,
+                            double *P_central, double zEnd, double *phase_fiducial);
 /* track_particles with the order-2 drift matrix (matrix.c:294-468) */
 int64_t eh_track_particles_drift(eh_ctx *ctx, double **part, int64_t n_part, const EH_DRIFT *drift);
 /* limit_amplitudes (limit_amplitudes.c:193-272); openCode as EH_MAXAMP.openSide */
-int64_t eh_limit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np, double z,
-                            double Po, int32_t extrapolate_z, int32_t openCode);
+int64_t eh_limit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np,
+                            double **accepted, double z, double Po, int32_t extrapolate_z, int32_t openCode);
 /* elimit_amplitudes (limit_amplitudes.c:472-598): the (super-)elliptical
  * aperture; lost rows get delta = sqrt((Po(1+delta))^2 + 1) as there.
  * Odd or < 2 exponents give EH_ERR_INVALID (the reference exits). */
-int64_t eh_elimit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np, double z,
-                             double Po, int32_t extrapolate_z, int32_t openCode, int32_t exponent,
-                             int32_t yexponent);
+int64_t eh_elimit_amplitudes(eh_ctx *ctx, double **coord, double xmax, double ymax, int64_t np,
+                             double **accepted, double z, double Po, int32_t extrapolate_z, int32_t openCode,
+                             int32_t exponent, int32_t yexponent);
 /* track_through_wake (wake.c:23-303); macroParticleCharge as set by CHARGE */
 int64_t eh_track_through_wake(eh_ctx *ctx, double **part, int64_t np, const EH_WAKE *wake, double Po,
                               int64_t i_pass, double macroParticleCharge);

@@ -1803,6 +1803,9 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
   nToTrack = nLeft;
   for (i_pass = passOffset; i_pass < n_passes + passOffset; i_pass++) {
     double z = 0, last_z = 0;
+    /* stop_tracking_particle_limit (do_tracking.c:500-520) */
+    if (c->particle_limit > 0 && nToTrack < c->particle_limit)
+      break;
     if (!nToTrack)
       break;
     for (long ie = 0; ie < n_elems && nToTrack; ie++) {
@@ -1908,6 +1911,8 @@ void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial) {
   c->phase_fiducial[elem_index] = phase_fiducial;
   c->fiducial_seen[elem_index] = 1;
 }
+
+void or_set_particle_limit(or_ctx *c, long limit) { c->particle_limit = limit; }
 
 void or_set_macro_charge(or_ctx *c, double q) {
   c->macroParticleCharge = q;

@@ -43,6 +43,7 @@ typedef struct {
   int zl_initialized[OR_MAX_ELEMS];  /* ZLONGIT bin settings (zlongit.c:583-632) */
   long zl_n_bins[OR_MAX_ELEMS];
   double zl_bin_size[OR_MAX_ELEMS];
+  long particle_limit; /* stop_tracking_particle_limit (do_tracking.c:500-520), 0 = off */
 } or_ctx;
 
 or_ctx *or_ctx_new(void);
@@ -79,6 +80,7 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
 int64_t or_last_hist(const or_ctx *c, double *hist, double *volt, int64_t cap, double *tmin, double *dt);
 double or_compute_average(const double *v, long n);
 void or_set_macro_charge(or_ctx *c, double q);
+void or_set_particle_limit(or_ctx *c, long limit);
 void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial);
 
 /* FFT used by ZLONGIT (packed real layout, see oracle.c) */

@@ -69,6 +69,7 @@ def lib(mode="glibc"):
         L.or_compute_average.argtypes = [PD, C.c_long]
         L.or_set_macro_charge.argtypes = [C.c_void_p, C.c_double]
         L.or_set_fiducial.argtypes = [C.c_void_p, C.c_long, C.c_double]
+        L.or_set_particle_limit.argtypes = [C.c_void_p, C.c_long]
         _libs[mode] = L
     return _libs[mode]
 
@@ -108,6 +109,10 @@ class Oracle:
 
     def set_macro_charge(self, q):
         self.L.or_set_macro_charge(self.ctx, q)
+
+    def set_particle_limit(self, limit):
+        """do_tracking stops before a pass that would start with fewer particles"""
+        self.L.or_set_particle_limit(self.ctx, int(limit))
 
     def set_fiducial(self, elem_index, phase_fiducial):
         """the RFCA at elem_index keeps this fiducial phase (as after its first pass)"""
