@@ -15,6 +15,8 @@ EH_ERR_NODEVICE = -6
 EH_ERR_LONGBEAM = -7
 EH_ERR_FIDUCIAL = -8
 
+EH_NOALLOC_WAKE_POINTS = 16384  # most wake points of eh_track_through_wake_noalloc
+
 EH_T_MARK = 1
 EH_T_DRIF = 2
 EH_T_EDRIFT = 3

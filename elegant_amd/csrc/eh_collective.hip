@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
   __shared__ unsigned long long smin[CB / 64], smax[CB / 64], scnt[CB / 64];
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
   for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
-    if (lp[i] < 0) {
+    if (!lp || lp[i] < 0) {
       unsigned long long k = order_key(arrival_time(s[i], dp[i], Po));
       kmin = k < kmin ? k : kmin;
       kmax = k > kmax ? k : kmax;
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, co
   eh_fsum a;
   eh_fsum_init(&a, e);
   for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB)
-    if (lp[i] < 0)
+    if (!lp || lp[i] < 0)
       eh_fsum_add(&a, arrival_time(s[i], dp[i], Po));
   sh[threadIdx.x] = a;
   __syncthreads();
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
     __syncthreads();
   }
   for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
-    if (lp[i] >= 0)
+    if (lp && lp[i] >= 0)
       continue;
     int64_t ib = bin_of(arrival_time(s[i], dp[i], Po), tmin, dt, round_half);
     if (ib >= 0 && ib <= nb - 1)
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, dou
   int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   const int32_t nb = plan->nb;
   const double tmin = plan->tmin;
-  if (nb <= 0 || plan->err || i >= n || lp[i] >= 0)
+  if (nb <= 0 || plan->err || i >= n || (lp && lp[i] >= 0))
     return;
   const double mc2 = 1e6 * EH_ME_MEV * EH_REL_SIGN;
   double S = s[i], D = dp[i];
@@ -680,40 +680,36 @@ static int64_t reduce_grid(int64_t n) {
   return g < 1 ? 1 : g;
 }
 
-/* One WAKE or ZLONGIT step, queued as one sequence of kernels with no host
- * round trip: reset, min/max, plan, [mean], histogram, convolution or FFT,
- * kick.  The host waits once at the end for the plan (error, bin layout). */
-int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist_out, double *volt_out,
-                        int64_t hist_cap, int64_t *nb_out, double *tmin_out) {
-  (void)hist_cap;
-  if (slot < 0 || slot >= EHD_MAX_COLL)
-    return EH_ERR_INVALID;
-  ehd_coll_slot *sl = &d->slot[slot];
-  const int64_t n = d->n;
+/* One WAKE or ZLONGIT step on the particle arrays crd[0..5] (alive: lp[i] < 0,
+ * or every particle when lp is NULL), queued as one sequence of kernels with
+ * no host round trip: reset, min/max, plan, [mean], histogram, convolution
+ * or FFT, kick.  work is the step's scratch (plan, histogram, voltage,
+ * partials) laid out for nb_max bins; the host waits once at the end for the
+ * plan (error, bin layout), which lands in *hp. */
+static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6], int32_t *lp, int32_t *lc,
+                       int64_t n, const double *W, int64_t wpoints, const ehd_coll_slot *sl, int64_t nb_max,
+                       void *work, coll_plan *hp) {
   const int64_t grid = (n + CB - 1) / CB;
   const int64_t rgrid = reduce_grid(n);
-  CHK(hipSetDevice(d->device));
-  const int64_t nb_max = c->kind == EH_OP_WAKE ? sl->wpoints : sl->nb; /* most bins this element can use */
   const int64_t nb_cap = nb_max + 8;
-  int rc = ensure_work(d, nb_cap, rgrid);
-  if (rc)
-    return rc;
-  coll_plan *plan = (coll_plan *)d->cwork;
-  unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
+  coll_plan *plan = (coll_plan *)work;
+  unsigned int *hist = (unsigned int *)((char *)work + PLAN_BYTES);
   double *volt = (double *)((char *)hist + hist_bytes(nb_cap));
   eh_fsum *parts = (eh_fsum *)((char *)volt + 8 * (nb_cap + 8));
   const int32_t kind = c->kind;
   const int32_t fixed = kind == EH_OP_WAKE ? c->n_bins_fixed : 0;
   const int32_t nclear = (int32_t)(nb_cap + 8);
+  /* without a loss-pass array nothing may be marked lost: no checks */
+  const int32_t ap_on = (!lp || (c->flags & EHD_COLL_NOCHECKS)) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0);
   CHK(hipEventRecord(d->cev0, d->stream));
   hipLaunchKernelGGL(coll_reset, dim3((unsigned)((nclear + CB - 1) / CB)), dim3(CB), 0, d->stream, plan, hist, nclear);
-  hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                     d->loss_pass, n, c->Po, (unsigned long long *)parts);
+  hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po,
+                     (unsigned long long *)parts);
   hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)parts, rgrid,
                      kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max);
   if (kind == EH_OP_ZLONGIT || fixed) {
-    hipLaunchKernelGGL(time_fsum, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                       d->loss_pass, n, c->Po, plan, parts);
+    hipLaunchKernelGGL(time_fsum, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po, plan,
+                       parts);
     hipLaunchKernelGGL(fsum_final, dim3(1), dim3(CB), 0, d->stream, parts, rgrid, plan, c->dt);
   }
   {
@@ -724,13 +720,12 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
       hgrid = 2048;
     /* room for one copy of the counts per wave, at most 48 KB of LDS */
     const int32_t lds_bins = nb_max * (CB / 64) < 12288 ? (int32_t)(nb_max * (CB / 64)) : 12288;
-    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, d->coord[4],
-                       d->coord[5], d->loss_pass, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins,
-                       hist);
+    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, crd[4], crd[5],
+                       lp, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins, hist);
   }
   if (kind == EH_OP_WAKE) {
-    hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CB - 1) / CB)), dim3(CB), 0, d->stream, hist,
-                       sl->W, sl->wpoints, plan, c->factor, volt);
+    hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CB - 1) / CB)), dim3(CB), 0, d->stream, hist, W,
+                       wpoints, plan, c->factor, volt);
   } else {
     int bits = 0;
     while ((1 << bits) < c->nb / 2)
@@ -739,22 +734,41 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
                        sl->ts, c->nb, bits, c->factor, plan, volt);
   }
   if (n > 0)
-    hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, d->coord[0], d->coord[1], d->coord[2],
-                       d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po, plan, c->dt, volt,
-                       kind, c->interpolate, c->z_end, c->ap_xmax, c->ap_ymax,
-                       (c->flags & EHD_COLL_NOCHECKS) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0), c->ap_code, c->pass,
-                       4 * c->op_index);
+    hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, crd[0], crd[1], crd[2], crd[3], crd[4],
+                       crd[5], lp, lc, n, c->Po, plan, c->dt, volt, kind, c->interpolate, c->z_end, c->ap_xmax,
+                       c->ap_ymax, ap_on, c->ap_code, c->pass, 4 * c->op_index);
   CHK(hipGetLastError());
   CHK(hipEventRecord(d->cev1, d->stream));
-  coll_plan hp;
-  CHK(hipMemcpyAsync(&hp, plan, sizeof(hp), hipMemcpyDeviceToHost, d->stream));
+  CHK(hipMemcpyAsync(hp, plan, sizeof(*hp), hipMemcpyDeviceToHost, d->stream));
   CHK(hipStreamSynchronize(d->stream));
   float ms = 0;
   CHK(hipEventElapsedTime(&ms, d->cev0, d->cev1));
   d->coll_ms += ms;
   d->coll_launches += 1;
+  return EH_OK;
+}
+
+int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist_out, double *volt_out,
+                        int64_t hist_cap, int64_t *nb_out, double *tmin_out) {
+  (void)hist_cap;
+  if (slot < 0 || slot >= EHD_MAX_COLL)
+    return EH_ERR_INVALID;
+  ehd_coll_slot *sl = &d->slot[slot];
+  const int64_t n = d->n;
+  CHK(hipSetDevice(d->device));
+  const int64_t nb_max = c->kind == EH_OP_WAKE ? sl->wpoints : sl->nb; /* most bins this element can use */
+  const int64_t nb_cap = nb_max + 8;
+  int rc = ensure_work(d, nb_cap, reduce_grid(n));
+  if (rc)
+    return rc;
+  coll_plan hp;
+  rc = coll_launch(d, c, d->coord, d->loss_pass, d->loss_code, n, sl->W, sl->wpoints, sl, nb_max, d->cwork, &hp);
+  if (rc)
+    return rc;
   if (hp.err)
     return hp.err;
+  unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
+  double *volt = (double *)((char *)hist + hist_bytes(nb_cap));
   const int32_t nb = hp.red[2] ? hp.nb : 0;
   if (nb > 0 && (hist_out || volt_out)) {
     unsigned int *h = (unsigned int *)malloc(sizeof(unsigned int) * nb);
@@ -774,6 +788,53 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
   if (tmin_out)
     *tmin_out = hp.tmin;
   return EH_OK;
+}
+
+/* ---- the WAKE step without allocation ----
+ * Scratch of the step (plan, histogram, voltage, partials) in a static
+ * device array instead of hipMalloc'ed memory.  Each GPU holds its own copy
+ * from module load until the process ends.  Host code never uses the
+ * array's name as an address (on the host it is only a stand-in): ehd_create
+ * resolves the device address once per context, with the device current,
+ * through hipGetSymbolAddress and keeps it in d->noalloc. */
+static __device__ __align__(256) unsigned char scratch[EHD_NOALLOC_BYTES];
+
+int ehd_noalloc_init(ehd_dev *d) {
+  void *ptr = NULL;
+  CHK(hipSetDevice(d->device));
+  CHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  d->noalloc = ptr;
+  return EH_OK;
+}
+
+int64_t ehd_noalloc_bins_max(void) { return EHD_NOALLOC_BINS; }
+
+int ehd_wake_step_noalloc(ehd_dev *d, const ehd_collective *c, double *const crd[6], int64_t n, const double *W,
+                          int64_t wpoints) {
+  if (c->kind != EH_OP_WAKE || !W || wpoints < 2 || n < 0)
+    return EH_ERR_INVALID;
+  if (wpoints > EHD_NOALLOC_BINS || n > ((int64_t)1 << 40)) {
+    snprintf(d->msg, sizeof(d->msg), "no-allocation WAKE step: %lld wake points exceed the scratch limit of %d",
+             (long long)wpoints, EHD_NOALLOC_BINS);
+    return EH_ERR_INVALID;
+  }
+  if (!d->noalloc) {
+    snprintf(d->msg, sizeof(d->msg), "no-allocation WAKE step: scratch address not resolved");
+    return EH_ERR_STATE;
+  }
+  const int64_t nb_cap = wpoints + 8;
+  const int64_t need = PLAN_BYTES + hist_bytes(nb_cap) + 8 * (nb_cap + 8) +
+                       (int64_t)sizeof(eh_fsum) * (reduce_grid(n) + 1) + 256;
+  if (need > EHD_NOALLOC_BYTES)
+    return EH_ERR_INVALID; /* not reached for wpoints <= EHD_NOALLOC_BINS; kept as a guard */
+  if (n == 0)
+    return EH_OK;
+  CHK(hipSetDevice(d->device));
+  coll_plan hp;
+  int rc = coll_launch(d, c, crd, NULL, NULL, n, W, wpoints, NULL, wpoints, d->noalloc, &hp);
+  if (rc)
+    return rc;
+  return hp.err ? hp.err : EH_OK;
 }
 
 int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count) {

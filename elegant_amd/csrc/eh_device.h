@@ -88,7 +88,19 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
 int ehd_collective_timing(ehd_dev *d, int64_t *launches, double *ms);
 /* exact min/max of the alive particles' arrival times and their count */
 int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count);
+/* The WAKE step without any allocation: device SoA particle arrays crd[0..5]
+ * of n particles (all alive, no checks after the step) and the Green
+ * function W[wpoints] on the device, owned by the caller; the step's scratch
+ * is the library's static per-GPU array (at most ehd_noalloc_bins_max()
+ * wake points).  Calls that overlap on one GPU share that array and race. */
+int ehd_noalloc_init(ehd_dev *d);
+int64_t ehd_noalloc_bins_max(void);
+int ehd_wake_step_noalloc(ehd_dev *d, const ehd_collective *c, double *const crd[6], int64_t n, const double *W,
+                          int64_t wpoints);
 #define EHD_COLL_NOCHECKS (1 << 30) /* element entry points: no do_tracking checks after the step */
+
+/* tests: op 0 allocate *ptr, 1 free it, 2 copy host -> *ptr, 3 copy *ptr -> host */
+int ehd_test_buffer(ehd_dev *d, int32_t op, void **ptr, void *host, int64_t bytes);
 
 /* deterministic math check: out[i] = f(in[i]) on the device, f = 0 sin, 1 cos, 2 asin, 3 log, 4 tan,
  * 5 sqrt, 6 division in[i]/in2[i] */

@@ -1186,6 +1186,13 @@ int ehd_create(int device, ehd_dev **out, char *msg, int msglen) {
     free(d);
     return EH_ERR_HIP;
   }
+  int rc = ehd_noalloc_init(d);
+  if (rc) {
+    if (msg && msglen > 0)
+      snprintf(msg, msglen, "%s", d->msg);
+    ehd_destroy(d);
+    return rc;
+  }
   *out = d;
   return EH_OK;
 }
@@ -1579,6 +1586,27 @@ int ehd_state_snapshot(ehd_dev *d, int restore) {
                            sizeof(int32_t) * d->n, hipMemcpyDeviceToDevice, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
   return EH_OK;
+}
+
+/* ---- device buffers for tests of the device-pointer entry points ---- */
+int ehd_test_buffer(ehd_dev *d, int32_t op, void **ptr, void *host, int64_t bytes) {
+  HIPCHK(d, hipSetDevice(d->device));
+  switch (op) {
+  case 0: /* allocate */
+    HIPCHK(d, hipMalloc(ptr, bytes > 0 ? bytes : 8));
+    return EH_OK;
+  case 1: /* free */
+    HIPCHK(d, hipFree(*ptr));
+    *ptr = NULL;
+    return EH_OK;
+  case 2: /* host -> device */
+    HIPCHK(d, hipMemcpy(*ptr, host, bytes, hipMemcpyHostToDevice));
+    return EH_OK;
+  case 3: /* device -> host */
+    HIPCHK(d, hipMemcpy(host, *ptr, bytes, hipMemcpyDeviceToHost));
+    return EH_OK;
+  }
+  return EH_ERR_INVALID;
 }
 
 /* ---- deterministic-math probe (used by tests to compare with the CPU) ---- */

@@ -10,6 +10,11 @@
 #include "eh_program.h"
 
 #define EHD_MAX_COLL 16
+/* the no-allocation WAKE step: most wake points (bins) and the size of its
+ * static scratch (plan 128 B + histogram and voltage for EHD_NOALLOC_BINS + 16
+ * bins + 1,025 fixed-point partials, rounded up) */
+#define EHD_NOALLOC_BINS 16384
+#define EHD_NOALLOC_BYTES (256 * 1024)
 
 typedef struct {
   double *W;          /* WAKE Green function */
@@ -34,6 +39,7 @@ struct ehd_dev {
   /* collective work buffers */
   ehd_coll_slot slot[EHD_MAX_COLL];
   void *cwork;       /* reductions, histogram, voltages */
+  void *noalloc;     /* device address of this GPU's static no-allocation scratch */
   int64_t cwork_bytes;
   /* batched variants */
   eh_vdesc *vdesc;

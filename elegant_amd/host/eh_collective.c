@@ -290,3 +290,55 @@ int64_t eh_track_through_zlongit(eh_ctx *c, double **part, int64_t np, const EH_
   rc = eh_collective_probe(c, part, np, &e, Po, i_pass, macroParticleCharge, NULL, NULL, NULL, NULL);
   return rc ? rc : np;
 }
+
+/* The WAKE step for callers that cannot allocate: track_through_wake's
+ * arithmetic (wake.c:23-303) on device-resident particles, with the step's
+ * scratch in the library's static per-GPU array.  Nothing here or below
+ * allocates, host or device. */
+int64_t eh_track_through_wake_noalloc(eh_ctx *c, double *const coord[6], int64_t np, const EH_WAKE *wake,
+                                      const double *W_device, double Po, int64_t i_pass,
+                                      double macroParticleCharge) {
+  ehd_collective cc;
+  double tmin = DBL_MAX, tmax = -DBL_MAX;
+  int rc;
+  if (!c || !wake || !coord || !W_device || np < 0)
+    return EH_ERR_INVALID;
+  for (int k = 0; k < 6; k++)
+    if (!coord[k])
+      return EH_ERR_INVALID;
+  if (wake->smoothing || wake->change_p0)
+    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE with SMOOTHING or CHANGE_P0");
+  if (wake->wakePoints < 2 || !wake->t)
+    return msgf(c, EH_ERR_INVALID, "WAKE: too little wake data");
+  if (wake->wakePoints > ehd_noalloc_bins_max())
+    return msgf(c, EH_ERR_INVALID, "no-allocation WAKE step: more wake points than its scratch holds");
+  for (int64_t i = 0; i < wake->wakePoints; i++) {
+    if (wake->t[i] < tmin)
+      tmin = wake->t[i];
+    if (wake->t[i] > tmax)
+      tmax = wake->t[i];
+  }
+  if (tmin >= tmax)
+    return msgf(c, EH_ERR_INVALID, "WAKE: zero or negative time span");
+  if (tmin != 0)
+    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE: acausal Green functions (t[0] != 0)");
+  if (np == 0)
+    return 0;
+  memset(&cc, 0, sizeof(cc));
+  cc.kind = EH_OP_WAKE;
+  cc.pass = i_pass;
+  cc.Po = Po;
+  cc.flags = EHD_COLL_NOCHECKS;
+  cc.dt = (tmax - tmin) / (wake->wakePoints - 1);
+  cc.wake_span = wake->t[wake->wakePoints - 1] - wake->t[0];
+  cc.interpolate = wake->interpolate;
+  cc.n_bins_fixed = wake->n_bins;
+  cc.allow_long_beam = wake->allowLongBeam;
+  cc.factor = macroParticleCharge * EH_REL_SIGN * wake->factor * ramp_factor((long)i_pass, wake->rampPasses);
+  rc = ehd_wake_step_noalloc(eh_ctx_dev(c), &cc, coord, np, W_device, wake->wakePoints);
+  if (rc == EH_ERR_LONGBEAM)
+    return msgf(c, rc, "bunch longer than the wake window (wake.c:134-140)");
+  if (rc)
+    return msgf(c, rc, ehd_message(eh_ctx_dev(c)));
+  return np;
+}

@@ -1096,6 +1096,13 @@ int eh_math_probe(eh_ctx *c, int32_t fn, const double *in, const double *in2, do
   return rc ? devfail(c, rc) : EH_OK;
 }
 
+/* device buffers for tests of the device-pointer entry points (not in the
+ * public header) */
+int eh_test_buffer(eh_ctx *c, int32_t op, void **ptr, void *host, int64_t bytes) {
+  int rc = ehd_test_buffer(c->dev, op, ptr, host, bytes);
+  return rc ? devfail(c, rc) : EH_OK;
+}
+
 /* accessors used by the collective step (eh_collective.c) */
 ehd_dev *eh_ctx_dev(eh_ctx *c) { return c->dev; }
 const eh_program *eh_ctx_prog(eh_ctx *c) { return c->prog; }

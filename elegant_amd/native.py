@@ -63,6 +63,8 @@ SIGNATURES = [
                                           C.c_double]),
     ("eh_track_through_zlongit", C.c_int64, [P, PP, C.c_int64, C.POINTER(A.EH_ZLONGIT), C.c_double,
                                              C.c_int64, C.c_double]),
+    ("eh_track_through_wake_noalloc", C.c_int64, [P, C.c_void_p * 6, C.c_int64, C.POINTER(A.EH_WAKE), C.c_void_p,
+                                                  C.c_double, C.c_int64, C.c_double]),
     ("eh_last_kernel_timing", C.c_int, [P, C.POINTER(C.c_int64), PD]),
     ("eh_batch_create", C.c_int, [P, C.POINTER(P)]),
     ("eh_batch_destroy", C.c_int, [P]),
@@ -97,6 +99,7 @@ SIGNATURES = [
 EXTRA = [
     ("eh_get_loss_records", C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("eh_math_probe", C.c_int, [P, C.c_int32, PD, PD, PD, C.c_int64]),
+    ("eh_test_buffer", C.c_int, [P, C.c_int32, C.POINTER(C.c_void_p), C.c_void_p, C.c_int64]),
     ("eh_set_chunk_passes", None, [P, C.c_int32]),
     ("eh_set_block", None, [P, C.c_int32]),
     ("eh_collective_probe", C.c_int, [P, PP, C.c_int64, PE, C.c_double, C.c_int64, C.c_double, PD, PD,
@@ -394,6 +397,33 @@ class Tracker:
         ptr, keep = A.rows_pointer(rows)
         return check(self._ctx, lib().eh_track_through_wake(self._ctx, ptr, rows.shape[0], C.byref(wake), Po,
                                                             i_pass, macro_charge))
+
+    def track_through_wake_noalloc(self, coord_ptrs, n, wake, W_ptr, Po, i_pass, macro_charge):
+        """the no-allocation WAKE step on device arrays: coord_ptrs are six
+        device addresses (x, x', y, y', s, delta), W_ptr the Green function's"""
+        arr = (C.c_void_p * 6)(*[C.c_void_p(int(p)) for p in coord_ptrs])
+        return check(self._ctx, lib().eh_track_through_wake_noalloc(self._ctx, arr, n, C.byref(wake),
+                                                                    C.c_void_p(int(W_ptr)), Po, i_pass,
+                                                                    macro_charge))
+
+    def device_array(self, host):
+        """a device copy of a float64 numpy array (tests of the device-pointer
+        entry points); returns its address; free with free_device_array"""
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        ptr = C.c_void_p()
+        check(self._ctx, lib().eh_test_buffer(self._ctx, 0, C.byref(ptr), None, h.nbytes))
+        check(self._ctx, lib().eh_test_buffer(self._ctx, 2, C.byref(ptr), h.ctypes.data_as(C.c_void_p), h.nbytes))
+        return ptr.value
+
+    def read_device_array(self, addr, n):
+        out = np.empty(n, dtype=np.float64)
+        ptr = C.c_void_p(addr)
+        check(self._ctx, lib().eh_test_buffer(self._ctx, 3, C.byref(ptr), out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def free_device_array(self, addr):
+        ptr = C.c_void_p(addr)
+        check(self._ctx, lib().eh_test_buffer(self._ctx, 1, C.byref(ptr), None, 0))
 
     def track_through_zlongit(self, rows, zl, Po, i_pass, macro_charge):
         ptr, keep = A.rows_pointer(rows)

@@ -313,6 +313,24 @@ int64_t eh_track_through_wake(eh_ctx *ctx, double **part, int64_t np, const EH_W
 /* track_through_zlongit (zlongit.c:56-557) */
 int64_t eh_track_through_zlongit(eh_ctx *ctx, double **part, int64_t np, const EH_ZLONGIT *zlongit, double Po,
                                  int64_t i_pass, double macroParticleCharge);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * eh_track_through_wake's step on device-resident particles, allocating
+ * nothing on the host or the device.  coord[0..5] are the caller's device
+ * arrays x, x', y, y', s, delta of np particles (all taken as alive; updated
+ * in place, nothing is lost, as track_through_wake loses nothing); W_device
+ * is the Green function W[wakePoints] on the device; wake->t stays a host
+ * array (its spacing and span are read here) and wake->W is not used.  The
+ * step's scratch is one static array per GPU (256 KiB, held until the
+ * process ends): at most EH_NOALLOC_WAKE_POINTS wake points, larger tables
+ * give EH_ERR_INVALID.  Calls that overlap on one GPU (from several
+ * contexts or threads) share that scratch and race; each call leaves it
+ * holding its own step and clears what it needs at its start.  Returns np
+ * or a negative error code. */
+#define EH_NOALLOC_WAKE_POINTS 16384
+int64_t eh_track_through_wake_noalloc(eh_ctx *ctx, double *const coord[6], int64_t np, const EH_WAKE *wake,
+                                      const double *W_device, double Po, int64_t i_pass,
+                                      double macroParticleCharge);
 
 /* ---- batched lattice variants (error-seed studies, dynamic-aperture grids) ----
  * No counterpart call in the reference, which runs each variant and each
