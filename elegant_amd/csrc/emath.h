@@ -11,21 +11,44 @@
  * and is compiled with -ffp-contract=off on both sides, so the same source
  * gives the same bits on the host and on the GPU.
  *
- * Methods (all public, none taken from a vendor library):
- *  - argument reduction modulo pi/2: Cody & Waite (1980) with pi/2 split into
- *    seven 24-bit pieces, accumulated in double-double (Knuth TwoSum), valid
- *    for |x| < 2^28; larger |x| returns NaN (documented domain limit);
- *  - sin/cos on [-pi/4, pi/4]: Taylor series to degree 17/16 (coefficients
- *    are 1/n! rounded to double), shortened to degree 9/8 for |x| < 2^-5,
- *    with the compensated 1 - z/2 step for cos;
- *  - asin: Taylor series of asin in x^2 on |x| <= 0.5 (5, 9 or 25 terms by
- *    band) and the reflection asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above;
+ * Correct rounding.  sin and cos for |x| >= 2^-5, tan for |x| >= 2^-27 and
+ * log everywhere return the double nearest to the exact value, for every
+ * argument, except where the exact value lies within 2^-100 (relative) of
+ * the midpoint between two doubles.  Method: Ziv's strategy (A. Ziv, "Fast
+ * evaluation of elementary mathematical functions with correctly rounded
+ * last bit", ACM TOMS 17, 1991): a fast evaluation into an unevaluated pair
+ * h + l with a stated error bound u; when h + l - u and h + l + u round to
+ * the same double that double is the result, else an accurate evaluation in
+ * double-double arithmetic (relative error below 2^-100) decides.
+ *  - argument reduction modulo pi/2: Cody & Waite (1980) with pi/2 split
+ *    into seven 24-bit pieces, accumulated in double-double (Knuth TwoSum),
+ *    for |x| < 2^28; beyond that, and in every accurate evaluation,
+ *    Payne & Hanek (1983) with the bits of 2/pi in a table: the product of
+ *    the 53-bit significand with the 256 bits of 2/pi that matter modulo 4,
+ *    in exact integer arithmetic (K. C. Ng, "Argument reduction for huge
+ *    arguments: good to the last bit", 1992);
+ *  - sin/cos of the reduced argument r: r = j pi/64 + d, |d| <= pi/128,
+ *    sin(r) = S_j cos d + C_j sin d and cos(r) = C_j cos d - S_j sin d with
+ *    S_j, C_j tabulated as double-double pairs and Taylor polynomials in d
+ *    (double precision in the fast path, double-double in the accurate one);
+ *    tan = sin/cos by a double-double quotient;
+ *  - log: x = 2^e m, m in [sqrt(2)/2, sqrt(2)], m r_i = 1 + u with
+ *    r_i = RN(128/i) for the i nearest 128 m (the product m r_i split exactly
+ *    by an FMA), log x = e ln2 - log r_i + log1p(u), -log r_i tabulated as a
+ *    pair and log1p(u) by its series (|u| < 2^-7.4);
+ *  - the small-argument bands, which the bend integrators use branch-free:
+ *    sin/cos for |x| < 2^-5 by Taylor series to degree 9/8 (coefficients
+ *    1/n! rounded to double) with the compensated 1 - z/2 step for cos, and
+ *    asin everywhere (Taylor series in x^2 on |x| <= 0.5 by band, and the
+ *    reflection asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above) are faithful
+ *    evaluations that round correctly except on a small fraction of
+ *    arguments (measured against 200-bit evaluations in tests/test_emath.py);
  *  - every polynomial by Horner's rule in fused multiply-adds (correctly
  *    rounded on both sides: the C library's fma on the host, v_fma_f64 on
- *    the device);
- *  - log: x = 2^e m, log(m) = 2 atanh(f/(2+f)) with f = m - 1.
- * Constants were derived from exact rational arithmetic (pi from Machin's
- * formula, ln 2 from sum 1/(k 2^k)) and are written as hex literals.
+ *    the device).
+ * Constants and tables were derived in exact integer arithmetic (pi from
+ * Machin's formula, logarithms from atanh series; gen_emath_tables.py writes
+ * emath_tables.h) and are written as hex literals.
  */
 #ifndef ELEGANT_AMD_EMATH_H
 #define ELEGANT_AMD_EMATH_H
@@ -34,9 +57,13 @@
 
 #ifdef __HIPCC__
 #define EM_FN static inline __host__ __device__
+#define EM_TABLE static const __device__
 #else
 #define EM_FN static inline
+#define EM_TABLE static const
 #endif
+
+#include "emath_tables.h"
 
 #define EM_PIO4 0x1.921fb54442d18p-1
 #define EM_PIO2_HI 0x1.921fb54442d18p+0
@@ -123,32 +150,15 @@ EM_FN int em_reduce_pio2(double x, double *rh, double *rl) {
   return (int)(long long)kd;
 }
 
-/* Polynomial cores, evaluated by Horner's rule in fused multiply-adds.  The
- * band |x| < 2^-5 keeps the series through x^9 (sin) and x^8 (cos): the
- * first dropped term is below 2^-64 relative, so both bands meet the same
- * accuracy while the narrow one costs half the operations. */
+/* The small-argument band |x| < 2^-5 (z = x*x < 2^-10): sin through x^9
+ * and cos through x^8; the first dropped term is below 2^-64 relative. */
 #define EM_S1 (-0x1.5555555555555p-3)
 #define EM_SMALL_Z 0x1.0p-10
 
-/* (sin(x) - x - S1 x^3) / x^5 as a polynomial in z = x*x; the narrow band
- * is its own function so that branch-free callers evaluate the same bits */
+/* (sin(x) - x - S1 x^3) / x^5 as a polynomial in z = x*x */
 EM_FN double em_sin_r_small(double z) {
   const double s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13, s4 = 0x1.71de3a556c734p-19;
   return __builtin_fma(z, __builtin_fma(z, em_k(s4), em_k(s3)), em_k(s2));
-}
-
-EM_FN double em_sin_r(double z) {
-  const double s2 = 0x1.1111111111111p-7, s3 = -0x1.a01a01a01a01ap-13, s4 = 0x1.71de3a556c734p-19,
-               s5 = -0x1.ae64567f544e4p-26, s6 = 0x1.6124613a86d09p-33, s7 = -0x1.ae7f3e733b81fp-41,
-               s8 = 0x1.952c77030ad4ap-49;
-  if (z < EM_SMALL_Z)
-    return em_sin_r_small(z);
-  double r = __builtin_fma(z, em_k(s8), em_k(s7));
-  r = __builtin_fma(z, r, em_k(s6));
-  r = __builtin_fma(z, r, em_k(s5));
-  r = __builtin_fma(z, r, em_k(s4));
-  r = __builtin_fma(z, r, em_k(s3));
-  return __builtin_fma(z, r, em_k(s2));
 }
 
 /* (cos(x) - 1 + z/2) / z^2 as a polynomial in z = x*x */
@@ -157,53 +167,7 @@ EM_FN double em_cos_r_small(double z) {
   return __builtin_fma(z, __builtin_fma(z, em_k(c4), em_k(c3)), em_k(c2));
 }
 
-EM_FN double em_cos_r(double z) {
-  const double c2 = 0x1.5555555555555p-5, c3 = -0x1.6c16c16c16c17p-10, c4 = 0x1.a01a01a01a01ap-16,
-               c5 = -0x1.27e4fb7789f5cp-22, c6 = 0x1.1eed8eff8d898p-29, c7 = -0x1.93974a8c07c9dp-37,
-               c8 = 0x1.ae7f3e733b81fp-45;
-  if (z < EM_SMALL_Z)
-    return em_cos_r_small(z);
-  double r = __builtin_fma(z, em_k(c8), em_k(c7));
-  r = __builtin_fma(z, r, em_k(c6));
-  r = __builtin_fma(z, r, em_k(c5));
-  r = __builtin_fma(z, r, em_k(c4));
-  r = __builtin_fma(z, r, em_k(c3));
-  return __builtin_fma(z, r, em_k(c2));
-}
-
-/* sin(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x| */
-EM_FN double em_sin_kernel(double x, double y) {
-  double z = x * x;
-  double v = z * x;
-  return x + ((y - 0.5 * z * y) + v * __builtin_fma(z, em_sin_r(z), em_k(EM_S1)));
-}
-
-/* cos(x + y) for |x| <= ~pi/4, |y| <= 2^-50 |x|; 1 - z/2 is split so that
- * its rounding error is carried into the tail */
-EM_FN double em_cos_kernel(double x, double y) {
-  double z = x * x;
-  double r = z * z * em_cos_r(z);
-  double hz = 0.5 * z;
-  double w = 1.0 - hz;
-  return w + (((1.0 - w) - hz) + (r - x * y));
-}
-
-/* the same kernels with y = 0, for arguments that need no reduction */
-EM_FN double em_sin_kernel0(double x) {
-  double z = x * x;
-  return x + (z * x) * __builtin_fma(z, em_sin_r(z), em_k(EM_S1));
-}
-
-EM_FN double em_cos_kernel0(double x) {
-  double z = x * x;
-  double r = z * z * em_cos_r(z);
-  double hz = 0.5 * z;
-  double w = 1.0 - hz;
-  return w + (((1.0 - w) - hz) + r);
-}
-
-/* the kernels on the narrow band, z = x*x < 2^-10: em_sin(x) for x != 0 and
- * em_cos(x) there, term for term */
+/* em_sin(x) and em_cos(x) on the band, z = x*x < 2^-10 (x != 0 for sin) */
 EM_FN double em_sin_small(double x, double z) {
   return x + (z * x) * __builtin_fma(z, em_sin_r_small(z), em_k(EM_S1));
 }
@@ -215,76 +179,351 @@ EM_FN double em_cos_small(double x, double z) {
   return w + (((1.0 - w) - hz) + r);
 }
 
-EM_FN void em_sincos(double x, double *sp, double *cp) {
-  double rh, rl, s, c;
+/* ---- double-double arithmetic ---- */
+/* a + b = s + e exactly, |a| >= |b| or a = 0 (Dekker's Fast2Sum) */
+EM_FN void em_fast2sum(double a, double b, double *s, double *e) {
+  double t = a + b;
+  *e = b - (t - a);
+  *s = t;
+}
+
+/* a + b = s + e exactly (Knuth's TwoSum) */
+EM_FN void em_two_sum(double a, double b, double *s, double *e) {
+  double t = a + b;
+  double bb = t - a;
+  *e = (a - (t - bb)) + (b - bb);
+  *s = t;
+}
+
+/* a b = p + e exactly */
+EM_FN void em_two_prod(double a, double b, double *p, double *e) {
+  double t = a * b;
+  *e = __builtin_fma(a, b, -t);
+  *p = t;
+}
+
+EM_FN void em_dd_mul(double ah, double al, double bh, double bl, double *rh, double *rl) {
+  double p, e;
+  em_two_prod(ah, bh, &p, &e);
+  e += ah * bl + al * bh;
+  em_fast2sum(p, e, rh, rl);
+}
+
+EM_FN void em_dd_add(double ah, double al, double bh, double bl, double *rh, double *rl) {
+  double s, e;
+  em_two_sum(ah, bh, &s, &e);
+  e += al + bl;
+  em_fast2sum(s, e, rh, rl);
+}
+
+/* Ziv's rounding test: 1 when every value within u of h + l (h = RN(h + l))
+ * rounds to h.  u is enlarged by 2^-10 of itself, which covers the rounding
+ * of l +- u, so the two floating-point sums decide for the exact interval. */
+EM_FN int em_rounds_to(double h, double l, double u) {
+  u += u * 0x1.0p-10;
+  return h == h + (l + u) && h == h + (l - u);
+}
+
+/* ---- Payne-Hanek reduction ---- */
+/* word wi of the 320-bit little-endian integer p, zero outside [0, 10); a
+ * select chain rather than an indexed load, so that p stays in registers */
+EM_FN uint32_t em_word(const uint32_t *p, int wi) {
+  uint32_t v = 0;
+  for (int k = 0; k < 10; k++)
+    v = k == wi ? p[k] : v;
+  return v;
+}
+
+/* bits [pos, pos + 32) of p, pos >= 0 */
+EM_FN uint32_t em_bits32(const uint32_t *p, int pos) {
+  int wi = pos >> 5, sh = pos & 31;
+  uint32_t v = em_word(p, wi) >> sh;
+  if (sh)
+    v |= em_word(p, wi + 1) << (32 - sh);
+  return v;
+}
+
+/* bits [pos, pos + 48) of p */
+EM_FN uint64_t em_bits48(const uint32_t *p, int pos) {
+  return (((uint64_t)em_bits32(p, pos + 32) & 0xffffULL) << 32) | em_bits32(p, pos);
+}
+
+/* x = k pi/2 + (rh + rl) for finite |x| >= 2^-5: k mod 4 is returned, the
+ * pair has a relative error below 2^-103 and |rh| <= pi/4 (1 + 2^-50).
+ * |x| = M 2^E with M a 53-bit integer; bits b_i (weight 2^-i) of 2/pi with
+ * i <= E - 2 contribute multiples of 4 to x 2/pi and are skipped, the next
+ * 256 bits are multiplied by M exactly (32-bit limbs), the truncation error
+ * is below 2^-200, and the integer part of the product is k, its fraction
+ * (rounded to nearest k) times pi/2 is r. */
+EM_FN int em_reduce_ph(double x, double *rh, double *rl) {
+  uint64_t u = em_bits(x);
+  int neg = (int)(u >> 63);
+  int E = (int)((u >> 52) & 0x7ff) - 1075;
+  uint64_t M = (u & 0x000fffffffffffffULL) | 0x0010000000000000ULL;
+  int i0 = E - 1 > 1 ? E - 1 : 1;
+  int q = (i0 - 1) >> 5, sh = (i0 - 1) & 31;
+  uint32_t w[8], p[10], m[2];
+  for (int k = 0; k < 8; k++) {
+    uint32_t a = em_two_over_pi_bits[q + 7 - k], b = em_two_over_pi_bits[q + 8 - k];
+    w[k] = sh ? (a << sh) | (b >> (32 - sh)) : a;
+  }
+  m[0] = (uint32_t)M;
+  m[1] = (uint32_t)(M >> 32);
+  for (int k = 0; k < 10; k++)
+    p[k] = 0;
+  for (int a = 0; a < 2; a++) {
+    uint64_t carry = 0;
+    for (int b = 0; b < 8; b++) {
+      uint64_t t = (uint64_t)m[a] * w[b] + p[a + b] + carry;
+      p[a + b] = (uint32_t)t;
+      carry = t >> 32;
+    }
+    p[a + 8] = (uint32_t)carry;
+  }
+  int F = i0 + 255 - E; /* binary point of x 2/pi in p */
+  int k = (int)(em_bits32(p, F) & 3);
+  uint64_t g1 = em_bits48(p, F - 48);
+  double d1 = (double)g1 * 0x1.0p-48, d2 = (double)em_bits48(p, F - 96) * 0x1.0p-96;
+  double d3 = (double)em_bits48(p, F - 144) * 0x1.0p-144, d4 = (double)em_bits48(p, F - 192) * 0x1.0p-192;
+  if (g1 >> 47) { /* fraction >= 1/2: round k up */
+    d1 -= 1.0;
+    k += 1;
+  }
+  double h, l, l2;
+  em_two_sum(d1, d2, &h, &l);
+  em_two_sum(h, d3 + d4, &h, &l2);
+  l += l2;
+  em_fast2sum(h, l, &h, &l);
+  em_dd_mul(h, l, EM_PIO2_1, EM_PIO2_2, rh, rl);
+  if (neg) {
+    *rh = -*rh;
+    *rl = -*rl;
+    k = -k;
+  }
+  return k & 3;
+}
+
+/* reduced argument of a finite |x| >= 2^-5 for the fast evaluations */
+EM_FN int em_reduce_fast(double x, double *rh, double *rl) {
+  double ax = em_fabs(x);
+  if (ax <= EM_PIO4) {
+    *rh = x;
+    *rl = 0.0;
+    return 0;
+  }
+  if (ax < 0x1.0p+28)
+    return em_reduce_pio2(x, rh, rl) & 3;
+  return em_reduce_ph(x, rh, rl);
+}
+
+/* ---- sin and cos of a reduced argument ---- */
+/* r = rh + rl = j pi/64 + d: j, and d as a pair (dh, dl) */
+EM_FN int em_split_pio64(double rh, double rl, double *dh, double *dl) {
+  double jd = (rh * EM_64_OVER_PI + EM_SHIFT) - EM_SHIFT;
+  double d = rh - jd * EM_PIO64_A; /* exact: j A has 53 bits and lies within a factor 2 of rh */
+  double t, te;
+  em_two_prod(jd, EM_PIO64_B, &t, &te);
+  em_two_sum(d, -t, dh, dl);
+  *dl += (rl - te) - jd * EM_PIO64_C;
+  return (int)jd;
+}
+
+EM_FN void em_table_sc(int j, double *Sh, double *Sl, double *Ch, double *Cl) {
+  int aj = j < 0 ? -j : j;
+  *Sh = em_sin_tab_hi[aj];
+  *Sl = em_sin_tab_lo[aj];
+  *Ch = em_cos_tab_hi[aj];
+  *Cl = em_cos_tab_lo[aj];
+  if (j < 0) {
+    *Sh = -*Sh;
+    *Sl = -*Sl;
+  }
+}
+
+/* fast: sin r and cos r as pairs, |r| <= pi/4 (1 + 2^-50); relative error
+ * below 2^-60 plus an absolute 2^-101 from the reduced argument.  The
+ * dominant error is the rounding of S_j (cos d - 1), below 2^-62 of the
+ * result; the table and the polynomials (through d^9 and d^8, |d| <= pi/128)
+ * contribute below 2^-75. */
+EM_FN void em_sincos_fast(double rh, double rl, double *sh, double *sl, double *ch, double *cl) {
+  double dh, dl, Sh, Sl, Ch, Cl, p, pe, s, se, tail;
+  int j = em_split_pio64(rh, rl, &dh, &dl);
+  em_two_sum(dh, dl, &dh, &dl);
+  double z = dh * dh;
+  double ps = (dh * z) * __builtin_fma(z, __builtin_fma(z, __builtin_fma(z, em_k(0x1.71de3a556c734p-19),
+                                                                            em_k(-0x1.a01a01a01a01ap-13)),
+                                                        em_k(0x1.1111111111111p-7)),
+                                       em_k(-0x1.5555555555555p-3));
+  double pc = z * __builtin_fma(z, __builtin_fma(z, __builtin_fma(z, em_k(0x1.a01a01a01a01ap-16),
+                                                                 em_k(-0x1.6c16c16c16c17p-10)),
+                                                 em_k(0x1.5555555555555p-5)),
+                                -0.5) -
+              dh * dl;
+  em_table_sc(j, &Sh, &Sl, &Ch, &Cl);
+  /* sin(a + d) = S + C d + (S (cos d - 1) + C (sin d - d)) */
+  em_two_prod(Ch, dh, &p, &pe);
+  em_two_sum(Sh, p, &s, &se);
+  tail = se + pe + Sl + Cl * dh + Sh * pc + Ch * (dl + ps);
+  em_fast2sum(s, tail, sh, sl);
+  /* cos(a + d) = C - S d + (C (cos d - 1) - S (sin d - d)) */
+  em_two_prod(Sh, dh, &p, &pe);
+  em_two_sum(Ch, -p, &s, &se);
+  tail = se - pe + Cl - Sl * dh + Ch * pc - Sh * (dl + ps);
+  em_fast2sum(s, tail, ch, cl);
+}
+
+/* accurate: the same in double-double, relative error below 2^-100
+ * (series of sin d - d and cos d - 1 through d^13 and d^12 with 1/n! as
+ * pairs; the first dropped terms are below 2^-112 relative) */
+EM_FN void em_sincos_acc(double rh, double rl, double *sh, double *sl, double *ch, double *cl) {
+  double dh, dl, zh, zl, qh, ql, ph, pl, th, tl, Sh, Sl, Ch, Cl, ah, al, bh, bl;
+  int j = em_split_pio64(rh, rl, &dh, &dl);
+  em_two_sum(dh, dl, &dh, &dl);
+  em_dd_mul(dh, dl, dh, dl, &zh, &zl);
+  /* Q(z) = sum_{k=1..6} (-1)^k z^(k-1) / (2k+1)!, P(z) = sum_{k=1..6} (-1)^k z^(k-1) / (2k)! */
+  qh = em_inv_fact[13][0];
+  ql = em_inv_fact[13][1];
+  ph = em_inv_fact[12][0];
+  pl = em_inv_fact[12][1];
+  for (int k = 5; k >= 1; k--) {
+    double sg = (k & 1) ? -1.0 : 1.0;
+    em_dd_mul(qh, ql, zh, zl, &qh, &ql);
+    em_dd_add(qh, ql, sg * em_inv_fact[2 * k + 1][0], sg * em_inv_fact[2 * k + 1][1], &qh, &ql);
+    em_dd_mul(ph, pl, zh, zl, &ph, &pl);
+    em_dd_add(ph, pl, sg * em_inv_fact[2 * k][0], sg * em_inv_fact[2 * k][1], &ph, &pl);
+  }
+  /* sin d = d + d z Q, cos d - 1 = z P */
+  em_dd_mul(qh, ql, zh, zl, &th, &tl);
+  em_dd_mul(th, tl, dh, dl, &th, &tl);
+  em_dd_add(dh, dl, th, tl, &qh, &ql); /* q = sin d */
+  em_dd_mul(ph, pl, zh, zl, &ph, &pl); /* p = cos d - 1 */
+  em_table_sc(j, &Sh, &Sl, &Ch, &Cl);
+  em_dd_mul(Sh, Sl, ph, pl, &ah, &al);
+  em_dd_mul(Ch, Cl, qh, ql, &bh, &bl);
+  em_dd_add(Sh, Sl, bh, bl, &th, &tl);
+  em_dd_add(th, tl, ah, al, sh, sl);
+  em_dd_mul(Ch, Cl, ph, pl, &ah, &al);
+  em_dd_mul(Sh, Sl, qh, ql, &bh, &bl);
+  em_dd_add(Ch, Cl, -bh, -bl, &th, &tl);
+  em_dd_add(th, tl, ah, al, ch, cl);
+}
+
+/* the accurate evaluation of a finite |x| >= 2^-5: sin (fn 0), cos (fn 1)
+ * or tan (fn 2), correctly rounded unless within 2^-100 of a midpoint */
+EM_FN double em_trig_acc(double x, int fn) {
+  double rh, rl, sh, sl, ch, cl;
   int k;
-  if (x == 0.0) {
-    *sp = x;
-    *cp = 1.0;
-    return;
-  }
   if (em_fabs(x) <= EM_PIO4) {
-    *sp = em_sin_kernel0(x);
-    *cp = em_cos_kernel0(x);
-    return;
+    rh = x;
+    rl = 0.0;
+    k = 0;
+  } else
+    k = em_reduce_ph(x, &rh, &rl);
+  em_sincos_acc(rh, rl, &sh, &sl, &ch, &cl);
+  if (fn == 2) {
+    /* tan = sin r / cos r (k even) or -cos r / sin r (k odd), pair quotient */
+    double nh = (k & 1) ? -ch : sh, nl = (k & 1) ? -cl : sl;
+    double dh = (k & 1) ? sh : ch, dl = (k & 1) ? sl : cl;
+    double q = nh / dh;
+    double r = __builtin_fma(-q, dh, nh); /* exact remainder */
+    r = (r + nl) - q * dl;
+    double h, l;
+    em_fast2sum(q, r / dh, &h, &l);
+    return h;
   }
-  k = em_reduce_pio2(x, &rh, &rl);
-  if (k < 0 && rh != rh) {
-    *sp = *cp = em_nan();
-    return;
+  int qd = (k + fn) & 3; /* cos x = sin(x + pi/2) */
+  double h = (qd & 1) ? ch : sh;
+  return (qd & 2) ? -h : h;
+}
+
+/* sin (fn 0) or cos (fn 1) of a finite |x| >= 2^-5, correctly rounded */
+EM_FN double em_sincos_cr1(double x, int fn) {
+  double rh, rl, sh, sl, ch, cl;
+  int k = em_reduce_fast(x, &rh, &rl);
+  em_sincos_fast(rh, rl, &sh, &sl, &ch, &cl);
+  int qd = (k + fn) & 3;
+  double h = (qd & 1) ? ch : sh, l = (qd & 1) ? cl : sl;
+  if (qd & 2) {
+    h = -h;
+    l = -l;
   }
-  s = em_sin_kernel(rh, rl);
-  c = em_cos_kernel(rh, rl);
-  switch (k & 3) {
-  case 0:
-    *sp = s;
-    *cp = c;
-    break;
-  case 1:
-    *sp = c;
-    *cp = -s;
-    break;
-  case 2:
-    *sp = -s;
-    *cp = -c;
-    break;
-  default:
-    *sp = -c;
-    *cp = s;
-    break;
-  }
+  if (em_rounds_to(h, l, 0x1.0p-60 * em_fabs(h) + 0x1.0p-101))
+    return h;
+  return em_trig_acc(x, fn);
 }
 
 EM_FN double em_sin(double x) {
-  double rh, rl, v;
-  int k;
-  if (x == 0.0)
-    return x;
-  if (em_fabs(x) <= EM_PIO4)
-    return em_sin_kernel0(x);
-  k = em_reduce_pio2(x, &rh, &rl);
-  if (k < 0 && rh != rh)
+  double ax = em_fabs(x);
+  if (ax < 0x1.0p-5)
+    return x == 0.0 ? x : em_sin_small(x, x * x);
+  if (!(ax <= 0x1.fffffffffffffp+1023))
     return em_nan();
-  v = (k & 1) ? em_cos_kernel(rh, rl) : em_sin_kernel(rh, rl);
-  return (k & 2) ? -v : v;
+  return em_sincos_cr1(x, 0);
 }
 
 EM_FN double em_cos(double x) {
-  double rh, rl, v;
-  int k;
-  if (em_fabs(x) <= EM_PIO4)
-    return x == 0.0 ? 1.0 : em_cos_kernel0(x);
-  k = em_reduce_pio2(x, &rh, &rl);
-  if (k < 0 && rh != rh)
+  double ax = em_fabs(x);
+  if (ax < 0x1.0p-5)
+    return x == 0.0 ? 1.0 : em_cos_small(x, x * x);
+  if (!(ax <= 0x1.fffffffffffffp+1023))
     return em_nan();
-  v = (k & 1) ? em_sin_kernel(rh, rl) : em_cos_kernel(rh, rl);
-  return ((k + 1) & 2) ? -v : v;
+  return em_sincos_cr1(x, 1);
 }
 
+EM_FN void em_sincos(double x, double *sp, double *cp) {
+  double ax = em_fabs(x);
+  if (ax < 0x1.0p-5) {
+    double z = x * x;
+    *sp = x == 0.0 ? x : em_sin_small(x, z);
+    *cp = x == 0.0 ? 1.0 : em_cos_small(x, z);
+    return;
+  }
+  if (!(ax <= 0x1.fffffffffffffp+1023)) {
+    *sp = *cp = em_nan();
+    return;
+  }
+  double rh, rl, sh, sl, ch, cl;
+  int k = em_reduce_fast(x, &rh, &rl);
+  em_sincos_fast(rh, rl, &sh, &sl, &ch, &cl);
+  const double u0 = 0x1.0p-101;
+  /* sin x: quadrant k; cos x: quadrant k + 1 */
+  for (int fn = 0; fn < 2; fn++) {
+    int qd = (k + fn) & 3;
+    double h = (qd & 1) ? ch : sh, l = (qd & 1) ? cl : sl;
+    if (qd & 2) {
+      h = -h;
+      l = -l;
+    }
+    double v = em_rounds_to(h, l, 0x1.0p-60 * em_fabs(h) + u0) ? h : em_trig_acc(x, fn);
+    if (fn == 0)
+      *sp = v;
+    else
+      *cp = v;
+  }
+}
+
+/* tan x: x itself below 2^-27 (x^3/3 is below half an ulp of x there);
+ * else sin r / cos r or -cos r / sin r from the fast pairs by a pair
+ * quotient, whose relative error is below 2^-59 plus 2^-101 / |sin r|. */
 EM_FN double em_tan(double x) {
-  double s, c;
-  em_sincos(x, &s, &c);
-  return s / c;
+  double ax = em_fabs(x);
+  if (ax < 0x1.0p-27)
+    return x;
+  if (!(ax <= 0x1.fffffffffffffp+1023))
+    return em_nan();
+  double rh, rl, sh, sl, ch, cl;
+  int k = em_reduce_fast(x, &rh, &rl);
+  em_sincos_fast(rh, rl, &sh, &sl, &ch, &cl);
+  double nh = (k & 1) ? -ch : sh, nl = (k & 1) ? -cl : sl;
+  double dh = (k & 1) ? sh : ch, dl = (k & 1) ? sl : cl;
+  double q = nh / dh;
+  double r = __builtin_fma(-q, dh, nh);
+  r = (r + nl) - q * dl;
+  double h, l;
+  em_fast2sum(q, r / dh, &h, &l);
+  if (em_rounds_to(h, l, em_fabs(h) * (0x1.0p-59 + 0x1.0p-101 / em_fabs(sh))))
+    return h;
+  return em_trig_acc(x, 2);
 }
 
 /* the five-term band of em_asin_poly_small */
@@ -353,14 +592,32 @@ EM_FN double em_asin(double x) {
   return x < 0 ? -r : r;
 }
 
+/* log x = e ln2 - log r_i + log1p(u) with u = m r_i - 1 as an exact pair.
+ * Fast: the large terms summed exactly (TwoSum), log1p's u^2/2 as an exact
+ * product, u^3 (1/3 - u/4 + ... + u^8/11) in double (next term below 2^-80
+ * of u); relative error below 2^-66.  Accurate: log1p(u) by its series
+ * through u^15 in double-double and ln 2 to 148 bits. */
+EM_FN double em_log_acc(int e, double uh, double ul, int i) {
+  double ph = em_inv_int[15][0], pl = em_inv_int[15][1], th, tl, ah, al;
+  for (int k = 14; k >= 1; k--) {
+    double sg = (k & 1) ? 1.0 : -1.0;
+    em_dd_mul(ph, pl, uh, ul, &ph, &pl);
+    em_dd_add(ph, pl, sg * em_inv_int[k][0], sg * em_inv_int[k][1], &ph, &pl);
+  }
+  em_dd_mul(ph, pl, uh, ul, &th, &tl); /* log1p(u) */
+  double ed = (double)e, bh, bl;
+  em_two_prod(ed, EM_LN2_2, &bh, &bl);
+  bl += ed * EM_LN2_3;
+  em_dd_add(ed * EM_LN2_1, 0.0, bh, bl, &ah, &al); /* e ln 2 */
+  em_dd_add(ah, al, em_log_hi[i], em_log_lo[i], &ah, &al);
+  em_dd_add(ah, al, th, tl, &ah, &al);
+  return ah;
+}
+
 EM_FN double em_log(double x) {
-  const double l1 = 0x1.5555555555555p-1, l2 = 0x1.999999999999ap-2, l3 = 0x1.2492492492492p-2,
-               l4 = 0x1.c71c71c71c71cp-3, l5 = 0x1.745d1745d1746p-3, l6 = 0x1.3b13b13b13b14p-3,
-               l7 = 0x1.1111111111111p-3, l8 = 0x1.e1e1e1e1e1e1ep-4, l9 = 0x1.af286bca1af28p-4,
-               l10 = 0x1.8618618618618p-4, l11 = 0x1.642c8590b2164p-4, l12 = 0x1.47ae147ae147bp-4;
   uint64_t u;
-  int e;
-  double m, f, s, z, r, hfsq, ed;
+  int e, i;
+  double m, r, p, pe, uh, ul, q, qe, ed, s, t1, t2, t3, poly, small, h, l;
   if (x != x)
     return x;
   if (x < 0.0)
@@ -381,13 +638,31 @@ EM_FN double em_log(double x) {
     m *= 0.5;
     e += 1;
   }
-  f = m - 1.0; /* exact */
-  s = f / (2.0 + f);
-  z = s * s;
-  r = z * (l1 + z * (l2 + z * (l3 + z * (l4 + z * (l5 + z * (l6 + z * (l7 + z * (l8 + z * (l9 + z * (l10 + z * (l11 + z * l12)))))))))));
-  hfsq = 0.5 * f * f;
+  i = (int)(m * 128.0 + 0.5) - 91; /* 0..90 */
+  r = em_log_r[i];
+  em_two_prod(m, r, &p, &pe);
+  em_two_sum(p - 1.0, pe, &uh, &ul); /* p - 1 is exact (p in [0.99, 1.01]) */
+  em_two_prod(uh, uh, &q, &qe);
   ed = (double)e;
-  return ed * EM_LN2_HI + (f - (hfsq - (s * (hfsq + r) + ed * EM_LN2_LO)));
+  em_two_sum(ed * EM_LN2_1, em_log_hi[i], &s, &t1); /* e EM_LN2_1 is exact */
+  em_two_sum(s, uh, &s, &t2);
+  em_two_sum(s, -0.5 * q, &s, &t3);
+  /* u^3 (1/3 - u/4 + u^2/5 - ... + u^8/11) */
+  poly = __builtin_fma(uh, em_k(0x1.745d1745d1746p-4), em_k(-0x1.999999999999ap-4));
+  poly = __builtin_fma(uh, poly, em_k(0x1.c71c71c71c71cp-4));
+  poly = __builtin_fma(uh, poly, em_k(-0x1.0p-3));
+  poly = __builtin_fma(uh, poly, em_k(0x1.2492492492492p-3));
+  poly = __builtin_fma(uh, poly, em_k(-0x1.5555555555555p-3));
+  poly = __builtin_fma(uh, poly, em_k(0x1.999999999999ap-3));
+  poly = __builtin_fma(uh, poly, em_k(-0x1.0p-2));
+  poly = __builtin_fma(uh, poly, em_k(0x1.5555555555555p-2));
+  poly *= uh * q;
+  /* u^2/2 = q/2 + qe/2 + uh ul (+ ul^2/2, below 2^-120 of u) */
+  small = t1 + t2 + t3 + (ed * EM_LN2_2 + em_log_lo[i]) + ul - 0.5 * qe - uh * ul + poly;
+  em_fast2sum(s, small, &h, &l);
+  if (em_rounds_to(h, l, 0x1.0p-63 * em_fabs(h)))
+    return h;
+  return em_log_acc(e, uh, ul, i);
 }
 
 /* ---- division and square root without range scaling ----
