@@ -1912,6 +1912,13 @@ void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial) {
   c->fiducial_seen[elem_index] = 1;
 }
 
+/* the fiducial phase fixed so far for the RFCA at elem_index (NaN if none) */
+double or_get_fiducial(const or_ctx *c, long elem_index) {
+  if (elem_index < 0 || elem_index >= OR_MAX_ELEMS || !c->fiducial_seen[elem_index])
+    return NAN;
+  return c->phase_fiducial[elem_index];
+}
+
 void or_set_particle_limit(or_ctx *c, long limit) { c->particle_limit = limit; }
 
 void or_set_macro_charge(or_ctx *c, double q) {

@@ -82,6 +82,7 @@ double or_compute_average(const double *v, long n);
 void or_set_macro_charge(or_ctx *c, double q);
 void or_set_particle_limit(or_ctx *c, long limit);
 void or_set_fiducial(or_ctx *c, long elem_index, double phase_fiducial);
+double or_get_fiducial(const or_ctx *c, long elem_index);
 
 /* FFT used by ZLONGIT (packed real layout, see oracle.c) */
 void or_real_fft(double *data, long n, int inverse);

@@ -69,6 +69,8 @@ def lib(mode="glibc"):
         L.or_compute_average.argtypes = [PD, C.c_long]
         L.or_set_macro_charge.argtypes = [C.c_void_p, C.c_double]
         L.or_set_fiducial.argtypes = [C.c_void_p, C.c_long, C.c_double]
+        L.or_get_fiducial.argtypes = [C.c_void_p, C.c_long]
+        L.or_get_fiducial.restype = C.c_double
         L.or_set_particle_limit.argtypes = [C.c_void_p, C.c_long]
         _libs[mode] = L
     return _libs[mode]
@@ -117,6 +119,10 @@ class Oracle:
     def set_fiducial(self, elem_index, phase_fiducial):
         """the RFCA at elem_index keeps this fiducial phase (as after its first pass)"""
         self.L.or_set_fiducial(self.ctx, elem_index, phase_fiducial)
+
+    def fiducial(self, elem_index):
+        """the fiducial phase this context fixed for the RFCA at elem_index (NaN if none)"""
+        return float(self.L.or_get_fiducial(self.ctx, elem_index))
 
     def wake(self, rows, wake, Po, i_pass):
         ptr, keep = A.rows_pointer(rows)
