@@ -9,7 +9,7 @@
  * per-particle arithmetic.  The kernel reads records with wave-uniform
  * indices, which the compiler turns into scalar-cache loads.
  *
- * Layout: 16 int32 + 88 doubles = 768 bytes per record.
+ * Layout: 16 int32 + 96 doubles = 832 bytes per record.
  */
 #ifndef ELEGANT_AMD_EH_PROGRAM_H
 #define ELEGANT_AMD_EH_PROGRAM_H
@@ -53,6 +53,7 @@
 #define EH_F_FAST (1 << 20)    /* CSBEND: radii, substep angles and kick lengths admit the branch-free common path */
 #define EH_F_ALONE (1 << 21)   /* single-element call: no limit_amplitudes cull after the element */
 #define EH_F_DFAST (1 << 22)   /* DRIF: 2^-300 <= |L| <= 2^300, the short form of the order-2 drift applies */
+#define EH_F_FIELD (1 << 23)   /* CSBEND: multipole field expansion (nfield coefficients at EH_D_FIELD) */
 
 /* Aperture shapes other than the closed rectangle, packed in one int32:
  * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the
@@ -75,14 +76,15 @@ typedef struct {
   int32_t op, elem, flags, nslices;
   int32_t nsub, nkick_orders, order0, order1;
   int32_t order2, skew0, skew1, skew2;
-  int32_t nkicks_rf, ap_code, in_code, pad2; /* ap_code, in_code: EH_AP_* shape of the two checks, 0 = rectangle */
+  int32_t nkicks_rf, ap_code, in_code; /* ap_code, in_code: EH_AP_* shape of the two checks, 0 = rectangle */
+  int32_t nfield;                       /* CSBEND: expansion order + 1 of the field sums (EH_F_FIELD) */
   /* common */
   double z_start, z_end, Po;
   double ap_xmax, ap_ymax;      /* post-element rectangle (0 = open in that plane) */
   double in_xmax, in_ymax;      /* in-element rectangle */
   double in_cs0, in_cs1;        /* cos/sin of the aperture's reverse tilt */
   /* element specific, indexed by the EH_D_* names below */
-  double d[79];
+  double d[87];
 } eh_op;
 
 /* ---- MULT ---- */
@@ -128,6 +130,8 @@ typedef struct {
 #define EH_D_RHOA_SGN 52  /* +1 or -1: the sign of rho_actual (t / rho_actual for t = +-0 or NaN) */
 #define EH_D_RRHO0 53     /* 1 / rho0 rounded to nearest: the quotients by rho0 use em_div */
 #define EH_D_RRHOA 54     /* 1 / rho_actual rounded to nearest */
+#define EH_D_FIELD 55     /* [<= 28]: field coefficients in the order the sums read them (eh_bend_field.h) */
+#define EH_D_FIELD_MAX 28 /* expansion order 6: K1..K4 and XKICK */
 
 /* ---- RFCA ---- */
 #define EH_D_VOLT 0       /* volt / (1e6 me_mev relSign) / nKicks */

@@ -584,6 +584,35 @@ DEV int track_multipole(particle &p, const eh_op &op) {
  * slice; per-substep coefficients are scalar loads of the element record.  Returns true if the particle is lost; X, QX, Y, QY, dist
  * and dz_lost then hold their values at the loss, as the reference leaves
  * them. */
+/* computeCSBENDFields (csbend.c:100-141), normal multipoles: the
+ * coefficients in the order eh_bend_field_flatten lays them out, each term
+ * (F x^i) y^j with the powers of y built by repeated multiplication as the
+ * reference's table of powers is */
+DEV void bend_fields(const eh_op &op, double x, double y, double &Fx, double &Fy) {
+  const int n1 = op.nfield;
+  double sumFx = 0, sumFy = 0, xt = 1;
+  int k = 0;
+  for (int i = 0; i < n1; i++) {
+    double p = y; /* y^1 */
+    for (int j = 1; j < n1 - i; j += 2) {
+      sumFx += opc(&op.d[EH_D_FIELD + k]) * xt * p;
+      k++;
+      p *= y;
+      p *= y;
+    }
+    p = 1.0; /* y^0 */
+    for (int j = 0; j < n1 - i; j += 2) {
+      sumFy += opc(&op.d[EH_D_FIELD + k]) * xt * p;
+      k++;
+      p *= y;
+      p *= y;
+    }
+    xt *= x;
+  }
+  Fx = sumFx;
+  Fy = sumFy;
+}
+
 template <bool SH>
 DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double DP, double &dist, double &dz_lost,
                      const eh_op &op) {
@@ -628,10 +657,17 @@ DEV bool csbend_body(double &X, double &QX, double &Y, double &QY, const double 
         break; /* kickFrac[last] == 0 */
       double ds = opc(&op.d[EH_D_BDS + j]);
       double g = 1 + em_div(X, rho0, opc(&op.d[EH_D_RRHO0]), ok_rho);
-      QX += em_div(-ds * g * 1.0, rhoa, opc(&op.d[EH_D_RRHOA]), ok_rho);
-      /* Fx = 0: ds*g*0.0 is +-0 or NaN, and dividing it by rho_actual only
-       * sets its sign, which the multiplication by sign(rho_actual) does */
-      QY += ds * g * 0.0 * rsgn;
+      if (op.flags & EH_F_FIELD) {
+        double Fx, Fy;
+        bend_fields(op, X, Y, Fx, Fy);
+        QX += em_div(-ds * g * Fy, rhoa, opc(&op.d[EH_D_RRHOA]), ok_rho);
+        QY += em_div(ds * g * Fx, rhoa, opc(&op.d[EH_D_RRHOA]), ok_rho);
+      } else {
+        QX += em_div(-ds * g * 1.0, rhoa, opc(&op.d[EH_D_RRHOA]), ok_rho);
+        /* Fx = 0: ds*g*0.0 is +-0 or NaN, and dividing it by rho_actual only
+         * sets its sign, which the multiplication by sign(rho_actual) does */
+        QY += ds * g * 0.0 * rsgn;
+      }
       QX = em_div(QX, opd, r_opd, ok_opd);
       QY = em_div(QY, opd, r_opd, ok_opd);
       QX *= opd;

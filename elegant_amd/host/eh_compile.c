@@ -12,6 +12,7 @@
 
 #include "eh_consts.h"
 #include "eh_host.h"
+#include "eh_bend_field.h"
 #include "emath.h"
 
 static double sqr(double x) { return x * x; }
@@ -440,11 +441,9 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
     op->d[0] = cb->length;
     return EH_OK;
   }
-  if (cb->k1 || cb->k2 || cb->k3 || cb->k4 || cb->xKick || cb->yKick || cb->etilt || cb->malignMethod ||
-      cb->referenceCorrection || cb->expandHamiltonian || cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 ||
-      cb->h2 || cb->fseQuadrupole) {
-    snprintf(msg, msglen, "%s: CSBEND supports pure dipoles without ETILT, radiation, H1/H2 or reference correction",
-             e->name);
+  if (cb->yKick || cb->etilt || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian ||
+      cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 || cb->h2) {
+    snprintf(msg, msglen, "%s: CSBEND without YKICK, ETILT, radiation, H1/H2 or reference correction", e->name);
     return EH_ERR_UNSUPPORTED;
   }
   if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1) ||
@@ -473,6 +472,36 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
   if (fabs(rho0) > 1e6) {
     snprintf(msg, msglen, "%s: CSBEND with radius > 1e6", e->name);
     return EH_ERR_UNSUPPORTED;
+  }
+  {
+    /* the multipole content (csbend.c:812-883): b_n = K_n rho0 with the
+     * signed radius, the FSE adjustments of b_1 and b_2, b_0 = XKICK/ANGLE,
+     * and for a negative angle the odd b_n change sign */
+    double b[9], F[EH_D_FIELD_MAX], Fx[EH_FIELD_MAX1][EH_FIELD_MAX1], Fy[EH_FIELD_MAX1][EH_FIELD_MAX1];
+    const double rho_s = cb->length / cb->angle;
+    int order1;
+    memset(b, 0, sizeof(b));
+    b[1] = cb->k1 * rho_s;
+    b[2] = cb->k2 * rho_s;
+    b[3] = cb->k3 * rho_s;
+    b[4] = cb->k4 * rho_s;
+    b[1] *= (1 + cb->fse + cb->fseQuadrupole) / (1 + cb->fse + cb->fseDipole);
+    b[2] *= (1 + cb->fse) / (1 + cb->fse + cb->fseDipole);
+    b[0] = cb->xKick / cb->angle;
+    if (cb->angle < 0)
+      for (int k = 1; k < 9; k += 2)
+        b[k] *= -1;
+    order1 = eh_bend_field_setup(b, 1 / rho0, cb->nonlinear, 0, Fx, Fy);
+    if (order1 < 0 || eh_bend_field_count(order1) > EH_D_FIELD_MAX) {
+      snprintf(msg, msglen, "%s: CSBEND field expansion beyond order 6", e->name);
+      return EH_ERR_UNSUPPORTED;
+    }
+    if (order1 > 0) {
+      int k = eh_bend_field_flatten(order1, Fx, Fy, F);
+      memcpy(&op->d[EH_D_FIELD], F, sizeof(double) * k);
+      op->nfield = order1;
+      op->flags |= EH_F_FIELD;
+    }
   }
   fse = cb->fse + cb->fseDipole;
   h = 1 / rho0;
@@ -549,7 +578,7 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
   /* the branch-free common path of the kernel needs positive radii within
    * the shared-reciprocal range, substep angles that are 0 or moderate and
    * below 2^-5, and kick lengths that are 0 or moderate */
-  if (rho0 > 0 && rho_actual > 0 && em_recip_ok(rho0) && em_recip_ok(rho_actual)) {
+  if (rho0 > 0 && rho_actual > 0 && em_recip_ok(rho0) && em_recip_ok(rho_actual) && !(op->flags & EH_F_FIELD)) {
     int fast = 1;
     for (int j = 0; j < nsub; j++) {
       double a = fabs(op->d[EH_D_BDSHR + j]), k = fabs(op->d[EH_D_BDS + j]);

@@ -39,6 +39,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "eh_bend_field.h"
 #include "eh_consts.h"
 #include "eh_fixedsum.h"
 
@@ -865,15 +866,41 @@ typedef struct {
   double rho0, rho_actual, e1, e2, psi1, psi2, tan_e1, tan_e2, cos_ttilt, sin_ttilt;
   double dxi, dyi, dzi, dxf, dyf, dzf, e1_kick_limit, e2_kick_limit;
   int edge1, edge2;
+  int order1; /* expansion order + 1 of the field sums; 0: pure dipole */
+  double Fx[EH_FIELD_MAX1][EH_FIELD_MAX1], Fy[EH_FIELD_MAX1][EH_FIELD_MAX1];
 } bend_setup;
 
-/* set-up part of track_through_csbend for a pure dipole with edge order 1
- * (csbend.c:785-1090) */
+/* computeCSBENDFields (csbend.c:100-141, the TURBO_MODE >= 5 form), normal
+ * multipoles: for each power i of x, the odd powers j of y for Fx and the
+ * even ones for Fy, i + j below the expansion order + 1 */
+static void bend_fields(const bend_setup *b, double x, double y, double *Fx, double *Fy) {
+  double yp[EH_FIELD_MAX1], sumFx = 0, sumFy = 0, xt = 1;
+  long i, j;
+  if (!b->order1) {
+    *Fx = 0;
+    *Fy = 1;
+    return;
+  }
+  yp[0] = 1;
+  for (i = 1; i < b->order1; i++)
+    yp[i] = yp[i - 1] * y;
+  for (i = 0; i < b->order1; i++) {
+    for (j = 1; j < b->order1 - i; j += 2)
+      sumFx += b->Fx[i][j] * xt * yp[j];
+    for (j = 0; j < b->order1 - i; j += 2)
+      sumFy += b->Fy[i][j] * xt * yp[j];
+    xt *= x;
+  }
+  *Fx = sumFx;
+  *Fy = sumFy;
+}
+
+/* set-up part of track_through_csbend with edge order 1 (csbend.c:785-1090):
+ * the dipole with normal multipoles K1..K4 and XKICK */
 static int csbend_setup(const EH_CSBEND *cb, bend_setup *b) {
   double angle, e1, e2, tilt, etilt, ttilt, fse, h, Kg1, Kg2, fint1, fint2;
-  if (cb->k1 || cb->k2 || cb->k3 || cb->k4 || cb->xKick || cb->yKick || cb->etilt || cb->malignMethod ||
-      cb->referenceCorrection || cb->expandHamiltonian || cb->synch_rad || cb->isr || cb->fseCorrection ||
-      cb->h1 || cb->h2)
+  if (cb->yKick || cb->etilt || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian ||
+      cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 || cb->h2)
     return EH_ERR_UNSUPPORTED;
   if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1) ||
       cb->edge_order > 1)
@@ -898,6 +925,26 @@ static int csbend_setup(const EH_CSBEND *cb, bend_setup *b) {
   b->rho0 = cb->length / angle;
   if (fabs(b->rho0) > 1e6)
     return EH_ERR_UNSUPPORTED;
+  {
+    /* csbend.c:812-883: b_n = K_n rho0 (signed radius), FSE adjustments,
+     * b_0 = XKICK/ANGLE, odd b_n negated for a negative angle */
+    double bn[9];
+    const double rho_s = cb->length / cb->angle;
+    memset(bn, 0, sizeof(bn));
+    bn[1] = cb->k1 * rho_s;
+    bn[2] = cb->k2 * rho_s;
+    bn[3] = cb->k3 * rho_s;
+    bn[4] = cb->k4 * rho_s;
+    bn[1] *= (1 + cb->fse + cb->fseQuadrupole) / (1 + cb->fse + cb->fseDipole);
+    bn[2] *= (1 + cb->fse) / (1 + cb->fse + cb->fseDipole);
+    bn[0] = cb->xKick / cb->angle;
+    if (cb->angle < 0)
+      for (int k = 1; k < 9; k += 2)
+        bn[k] *= -1;
+    b->order1 = eh_bend_field_setup(bn, 1 / b->rho0, cb->nonlinear, 0, b->Fx, b->Fy);
+    if (b->order1 < 0)
+      return EH_ERR_UNSUPPORTED;
+  }
   fse = cb->fse + cb->fseDipole;
   h = 1 / b->rho0;
   if (fabs(fse + 1) < 1e-12)
@@ -951,13 +998,13 @@ static int csbend_setup(const EH_CSBEND *cb, bend_setup *b) {
   return EH_OK;
 }
 
-/* integrate_csbend_ordn (csbend.c:1434-1693) for Fx=0, Fy=1 (pure dipole),
- * with the rad_coef=0, isrConstant<0 round trip of the kick (csbend.c:1603-1624) */
+/* integrate_csbend_ordn (csbend.c:1434-1693) with the fields of bend_fields,
+ * and the rad_coef=0, isrConstant<0 round trip of the kick (csbend.c:1603-1624) */
 static int integrate_csbend(double *Qf, const double *Qi, double length, long n, double rho0, double rho_actual,
-                            double *dz_lost, const mult_ap *ap, int integration_order) {
+                            double *dz_lost, const mult_ap *ap, int integration_order, const bend_setup *fb) {
   long i;
   double f, phi, ds, dsh, dist, s;
-  double sine, cosi, tang, sin_phi, cos_phi, factor;
+  double sine, cosi, tang, sin_phi, cos_phi, factor, Fx, Fy;
   const double *driftFrac, *kickFrac;
   int nSubsteps;
   integrator_fracs(integration_order, &driftFrac, &kickFrac, &nSubsteps);
@@ -995,9 +1042,9 @@ static int integrate_csbend(double *Qf, const double *Qi, double length, long n,
       if (kickFrac[j] == 0)
         break;
       ds = s * kickFrac[j];
-      /* Fx = 0, Fy = 1 */
-      Qf[1] += -ds * (1 + Qf[0] / rho0) * 1.0 / rho_actual;
-      Qf[3] += ds * (1 + Qf[0] / rho0) * 0.0 / rho_actual;
+      bend_fields(fb, Qf[0], Qf[2], &Fx, &Fy);
+      Qf[1] += -ds * (1 + Qf[0] / rho0) * Fy / rho_actual;
+      Qf[3] += ds * (1 + Qf[0] / rho0) * Fx / rho_actual;
       Qf[1] /= (1 + Qf[5]);
       Qf[3] /= (1 + Qf[5]);
       Qf[1] *= (1 + Qf[5]);
@@ -1062,7 +1109,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *cb, do
       Qi[3] *= f;
     }
     particle_lost = !integrate_csbend(Qf, Qi, cb->length, cb->nSlices, b.rho0, b.rho_actual, &dz_lost, &ap,
-                                      cb->integration_order);
+                                      cb->integration_order, &b);
     {
       /* convertFromDipoleCanonicalCoordinates (csbend.c:1421-1429) */
       double f = 1 / sqrt(sqr(1 + Qf[5]) - sqr(Qf[1]) - sqr(Qf[3]));

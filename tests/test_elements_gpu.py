@@ -458,3 +458,28 @@ def test_ring_with_exact_drifts_and_matrix_cavity(tracker):
     nc = o.do_tracking(c, ring, Po, 3)
     assert ng == nc and eg == o.effort() and 0 < nc < rows.shape[0]
     assert_rows_identical(g, c, "ring with EDRIFT and matrix RFCA")
+
+
+@pytest.mark.parametrize("kw", [
+    dict(L=1.0, ANGLE=0.08, K1=0.7, N_KICKS=16, E1=0.02, E2=0.04),
+    dict(L=0.8, ANGLE=-0.785398163397, E1=-0.445, E2=-0.445, K2=0.1375, K3=100.0, HGAP=0.0225, FINT=0.41,
+         N_KICKS=10),
+    dict(L=0.5, ANGLE=0.05, K1=-0.3, K2=20.0, K3=-300.0, K4=5000.0, N_KICKS=8, FSE=1e-3, FSE_QUADRUPOLE=2e-3,
+         FSE_DIPOLE=-1e-3, XKICK=1e-4, DX=1e-4, TILT=2e-3),
+    dict(L=0.5, ANGLE=0.05, K2=20.0, NONLINEAR=0, N_KICKS=8),
+    dict(L=0.5, ANGLE=0.05, XKICK=2e-4, N_KICKS=8),
+])
+def test_csbend_with_multipoles(tracker, kw):
+    """CSBEND with normal multipoles K1..K4 and XKICK (the field expansion
+    of csbend.c:100-141, 186-660 in the kick of csbend.c:1586-1602): the
+    general bend code on rows of every kind, bit for bit against the
+    checker, signs of zeros included"""
+    b = E.CSBEND(**kw).p.csbend
+    r = csbend_edge_rows(1000, 14)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.track_through_csbend(g, b, PO, 3.0, None)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0, None)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "csbend with multipoles %r" % kw)

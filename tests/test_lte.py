@@ -153,3 +153,33 @@ def test_labels_starting_with_use_or_return_are_definitions():
     assert [n for n, _ in lat.expand()] == ["USERDRIFT", "RETURNQ"]
     with pytest.raises(lte.LatticeError, match="cannot parse"):
         lte.Lattice().parse("usex, ring")
+
+
+def par_ring():
+    """the reference's example deck (parTrack.lte) as elegant would track
+    it with QUAD/SEXT transmuted to KQUAD/KSEXT: its bends carry K2 and K3"""
+    lat = lte.load(os.path.join(GOLDEN, "parTrack.lte"))
+    return lat.beamline("PAR", transmute={"QUAD": "KQUAD", "SEXT": "KSEXT"},
+                        ignore=("MALIGN", "RECIRC", "RCOL", "WATCH")), Lt.momentum(450.0)
+
+
+def test_example_deck_tracks_on_the_checker():
+    """the deck's ring, with the bends' sextupole and octupole components,
+    tracks on the checker: the ideal particle stays near the axis (within
+    the orbit-closure error of the deck's ten-kick bends, a few 1e-8 per
+    turn) and a small beam survives 50 turns with bounded amplitudes"""
+    bl, Po = par_ring()
+    rows = np.zeros((64, 8))
+    rng = np.random.default_rng(1)
+    rows[1:, 0] = rng.normal(0, 2e-4, 63)
+    rows[1:, 2] = rng.normal(0, 1e-4, 63)
+    rows[1:, 5] = rng.normal(0, 2e-4, 63)
+    rows[:, 6] = np.arange(1, 65)
+    rows[:, 7] = -1
+    o = O.Oracle("emath")
+    r = rows.copy()
+    n = o.do_tracking(r, bl, Po, 50)
+    assert n == 64
+    ideal = r[r[:, 6] == 1][0]
+    assert np.abs(ideal[[0, 1, 2, 3, 5]]).max() < 1e-5
+    assert np.abs(r[:, 0]).max() < 5e-3

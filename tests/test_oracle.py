@@ -711,3 +711,118 @@ def test_open_side_inside_an_element():
         assert n == 1 and r[0, 6] == 1 and r[0, 0] == 0.012, ell
         lost = {int(row[6]): row for row in r[1:]}
         assert lost[2][4] == 0.0 and lost[3][4] == 0.0, ell
+
+
+# ------------------------------------------------ CSBEND multipole fields
+
+def _field_table():
+    """the committed eh_csbend_fields.h as exact polynomials:
+    {(field, i, j): {(p, m): Fraction}}"""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "elegant_amd", "csrc", "eh_csbend_fields.h")).read()
+    out = {}
+    for f, i, j, p, m, n, d in re.findall(r"\{(\d), (\d+), (\d+), (\d+), (\d+), \{0\}, (-?\d+)\.0, (\d+)\.0\}", text):
+        out.setdefault((int(f), int(i), int(j)), {})[(int(p), int(m))] = Fraction(int(n), int(d))
+    return out
+
+
+def test_bend_field_table_satisfies_maxwell():
+    """the tabulated field of a curved sector magnet (By = sum a_ij x^i y^j,
+    Bx = sum c_ij x^i y^j, midplane By = 1 + sum b_n x^n/n!) is curl-free,
+    dBx/dy = dBy/dx, and divergence-free in the bend's frame,
+    (1 + hx) dBx/dx + h Bx + (1 + hx) dBy/dy = 0: every coefficient of both
+    residuals through total degree 9, in exact arithmetic, for arbitrary
+    rational h and b_1..b_8 (a dropped term or a wrong sign fails)"""
+    tab = _field_table()
+    assert len(tab) > 50
+    h = Fraction(3, 7)
+    b = [Fraction(0)] + [Fraction((-1) ** k * (k + 2), 5 + k) for k in range(1, 9)]
+
+    def val(field, i, j):
+        if i < 0 or j < 0:
+            return Fraction(0)
+        if field == 0 and (i, j) == (1, 0):
+            return b[1]
+        if field == 1 and (i, j) == (0, 1):
+            return b[1]
+        poly = tab.get((field, i, j), {})
+        return sum((v * h ** p * b[m] for (p, m), v in poly.items()), Fraction(0))
+
+    a = lambda i, j: val(0, i, j)
+    c = lambda i, j: val(1, i, j)
+    for i in range(0, 9):
+        assert a(i, 0) == (b[i] / math.factorial(i) if i >= 1 else 0)
+    for deg in range(0, 10):
+        for i in range(deg + 1):
+            j = deg - i
+            curl = (j + 1) * c(i, j + 1) - (i + 1) * a(i + 1, j)
+            div = ((i + 1) * c(i + 1, j) + h * i * c(i, j) + h * c(i, j) + (j + 1) * a(i, j + 1)
+                   + h * (j + 1) * a(i - 1, j + 1))
+            assert curl == 0 and div == 0, (i, j, curl, div)
+
+
+def test_combined_function_bend_linear_map():
+    """CSBEND with K1: the combined-function sector bend, x'' = -(h^2 + K1) x
+    + h delta and y'' = K1 y (K1 > 0 focuses horizontally): cos/sin and
+    cosh/sinh blocks and the dispersion, from the tracked Jacobian"""
+    L, th, K1 = 1.0, 0.1, 0.5
+    h = th / L
+    J = jacobian6([E.CSBEND(L=L, ANGLE=th, K1=K1, N_KICKS=64, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)])
+    kx = math.sqrt(h * h + K1)
+    ky = math.sqrt(K1)
+    want = {(0, 0): math.cos(kx * L), (0, 1): math.sin(kx * L) / kx, (1, 0): -kx * math.sin(kx * L),
+            (1, 1): math.cos(kx * L), (0, 5): h * (1 - math.cos(kx * L)) / kx ** 2,
+            (1, 5): h * math.sin(kx * L) / kx,
+            (2, 2): math.cosh(ky * L), (2, 3): math.sinh(ky * L) / ky, (3, 2): ky * math.sinh(ky * L),
+            (3, 3): math.cosh(ky * L)}
+    for (i, j), v in want.items():
+        assert abs(J[i, j] - v) < 2e-6 * max(1.0, abs(v)), (i, j, J[i, j], v)
+
+
+@pytest.mark.parametrize("key,k,ctor,ref_key", [("K2", 40.0, E.KSEXT, "K2"), ("K3", 600.0, E.KOCT, "K3")])
+def test_bend_multipole_component_matches_the_multipole_element(key, k, ctor, ref_key):
+    """a CSBEND whose curvature is negligible (angle 1e-5) with a K2 or K3
+    component kicks as KSEXT or KOCT of the same length and strength: the
+    change of x' and y' caused by the component, at points off both axes"""
+    L = 0.2
+    pts = [[2e-3, 0, 1e-3, 0, 0, 0], [-1.5e-3, 1e-4, 0.5e-3, -2e-4, 0, 1e-3], [1e-3, 0, -2e-3, 0, 0, 0]]
+    bend = lambda kk: E.CSBEND(L=L, ANGLE=1e-5, N_KICKS=64, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0, **{key: kk})
+    rb, _, _ = track([bend(k)], rows_of(pts))
+    r0, _, _ = track([bend(0.0)], rows_of(pts))
+    rm, _, _ = track([ctor(L=L, N_KICKS=64, **{ref_key: k})], rows_of(pts))
+    rd, _, _ = track([E.EDRIFT(L=L)], rows_of(pts))
+    for col in (1, 3):
+        d_bend = rb[:, col] - r0[:, col]
+        d_mult = rm[:, col] - rd[:, col]
+        assert np.all(np.abs(d_bend - d_mult) <= 1e-4 * np.abs(d_mult) + 1e-15), (col, d_bend, d_mult)
+        assert np.all(np.abs(d_mult) > 1e-9)
+
+
+def test_bend_decapole_component_thin_limit():
+    """K4 in a short weak bend: the kick of the thin decapole, dx' = -K4 L
+    (x^4 - 6 x^2 y^2 + y^4)/24 and dy' = K4 L (4 x^3 y - 4 x y^3)/24, to a few
+    per cent (thick-lens terms)"""
+    L, K4 = 0.05, 2e5
+    pts = [[3e-3, 0, 0, 0, 0, 0], [2e-3, 0, 1.5e-3, 0, 0, 0], [0, 0, 3e-3, 0, 0, 0]]
+    bend = lambda kk: E.CSBEND(L=L, ANGLE=1e-5, N_KICKS=32, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0, K4=kk)
+    rb, _, _ = track([bend(K4)], rows_of(pts))
+    r0, _, _ = track([bend(0.0)], rows_of(pts))
+    scale = K4 * L * 3e-3 ** 4 / 24
+    for p, gb, g0 in zip(pts, rb, r0):
+        x, y = p[0], p[2]
+        wx = -K4 * L * (x ** 4 - 6 * x * x * y * y + y ** 4) / 24
+        wy = K4 * L * (4 * x ** 3 * y - 4 * x * y ** 3) / 24
+        assert abs((gb[1] - g0[1]) - wx) <= 0.03 * abs(wx) + 1e-3 * scale
+        assert abs((gb[3] - g0[3]) - wy) <= 0.03 * abs(wy) + 1e-3 * scale
+
+
+@pytest.mark.parametrize("elem", [
+    lambda: E.CSBEND(L=1.0, ANGLE=0.1, K1=0.6, K2=12.0, N_KICKS=16, E1=0.02, E2=0.03),
+    lambda: E.CSBEND(L=0.8, ANGLE=-0.3, K2=0.1375, K3=100.0, N_KICKS=12, E1=-0.2, E2=-0.2),
+])
+def test_bend_with_multipoles_is_symplectic(elem):
+    """the bend body with a multipole field, like the pure dipole, has a
+    symplectic transverse map at fixed delta"""
+    test_transverse_map_is_symplectic(elem)

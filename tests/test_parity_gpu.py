@@ -69,3 +69,26 @@ def test_ring_edge_rows_bitwise(tracker):
     g, ng, eg, c, nc, ec = run_both(tracker, bl, Po, rows, 2, "emath")
     assert ng == nc and eg == ec and 0 < nc < rows.shape[0]
     assert_rows_bitwise(g, c, "ring edges")
+
+
+def test_example_deck_bit_exact_vs_emath_checker(tracker):
+    """the reference's example deck (parTrack.lte: bends with K2 and K3 and
+    HGAP/FINT edges, quadrupoles and sextupoles transmuted to KQUAD/KSEXT)
+    over 50 turns, behind a MAXAMP of 10 x 5 mm so that a large-amplitude
+    halo is lost along the ring: rows, order, losses and effort identical to
+    the checker"""
+    from elegant_amd import elements as E
+    from tests.test_lte import par_ring
+    from tests.test_elements_gpu import beam
+    deck, Po = par_ring()
+    bl = E.Beamline([E.MAXAMP("AP", X_MAX=0.01, Y_MAX=0.005)] + list(deck.elements), keep=deck.keep)
+    rows = beam(2000, 21, halo=0.3, xshift=0.006)
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 50)
+    o = O.Oracle("emath")
+    c = rows.copy()
+    nc = o.do_tracking(c, bl, Po, 50)
+    print("parTrack deck: survivors", ng, "effort", eg)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 10
+    assert_rows_identical(g, c, "parTrack deck")
