@@ -56,9 +56,11 @@ int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk 
  * surviving particles, grouped by variant); chunk.first indexes perm */
 int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_chunk *chunks, int32_t nchunks);
 /* passes [pass0, pass0+npasses) for every variant, npasses split into launches of
- * passes_per_launch; ppl particles per lane (1 or 2); graph = capture and replay */
+ * passes_per_launch; ppl particles per lane (1 or 2); graph = capture and replay;
+ * device_compact: launches of 1, 2, 4 ... passes up to passes_per_launch with
+ * the work queue rebuilt from the survivors on the device between them */
 int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,
-                    int32_t block, int32_t use_graph, double *ms, int64_t *launches);
+                    int32_t block, int32_t use_graph, int32_t device_compact, double *ms, int64_t *launches);
 int ehd_state_snapshot(ehd_dev *d, int restore);
 
 /* ---- collective elements (WAKE / ZLONGIT) ---- */

@@ -1089,10 +1089,12 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
 template <int PPL, bool SH>
 __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op *__restrict__ ops,
                                                        const eh_vdesc *__restrict__ vars,
-                                                       const eh_chunk *__restrict__ chunks, int32_t nchunks,
+                                                       const eh_chunk *__restrict__ chunks,
+                                                       const int32_t *__restrict__ nchunks_p,
                                                        const int32_t *__restrict__ perm, int64_t pass0,
                                                        int32_t npasses, unsigned int *queue) {
   __shared__ int s_chunk;
+  const int32_t nchunks = *nchunks_p; /* device-resident: a queue rebuilt on the device sets it */
   for (;;) {
     if (threadIdx.x == 0)
       s_chunk = (int)atomicAdd(queue, 1u);
@@ -1243,7 +1245,8 @@ void ehd_destroy(ehd_dev *d) {
   (void)hipStreamSynchronize(d->stream);
   void *bufs[] = {d->coord[0], d->coord[1], d->coord[2], d->coord[3], d->coord[4], d->coord[5],
                   d->loss_pass, d->loss_code, d->ops, d->counter, d->vdesc, d->chunks, d->perm, d->queue,
-                  d->snap[0], d->snap[1], d->snap[2], d->snap[3], d->snap[4], d->snap[5], d->snap_lp, d->snap_lc};
+                  d->snap[0], d->snap[1], d->snap[2], d->snap[3], d->snap[4], d->snap[5], d->snap_lp, d->snap_lc,
+                  d->tile_sum};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
     if (bufs[k])
       (void)hipFree(bufs[k]);
@@ -1473,8 +1476,10 @@ int ehd_batch_setup(ehd_dev *d, const eh_vdesc *v, int32_t nvar, const eh_chunk 
     HIPCHK(d, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
     d->num_cu = cu > 0 ? cu : 256;
   }
+  d->dev_nchunks = (int32_t *)d->queue + 32;
   HIPCHK(d, hipMemcpyAsync(d->vdesc, v, sizeof(eh_vdesc) * nvar, hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipMemcpyAsync(d->chunks, chunks, sizeof(eh_chunk) * nchunks, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->dev_nchunks, &nchunks, sizeof(int32_t), hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
   d->nvar = nvar;
   d->nchunks = nchunks;
@@ -1498,10 +1503,207 @@ int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_c
     HIPCHK(d, hipMemcpyAsync(d->perm, perm, sizeof(int32_t) * nperm, hipMemcpyHostToDevice, d->stream));
   if (nchunks > 0)
     HIPCHK(d, hipMemcpyAsync(d->chunks, chunks, sizeof(eh_chunk) * nchunks, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->dev_nchunks, &nchunks, sizeof(int32_t), hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
   d->nchunks = nchunks;
   d->use_perm = 1;
   return EH_OK;
+}
+
+/* ---- the work queue rebuilt on the device ----
+ * The surviving particles, in index order (so grouped by variant, whose
+ * particle ranges are contiguous), are listed in perm, and each variant's
+ * list is cut into chunks of csize; the count lands in dev_nchunks.  Six
+ * small kernels, no host round trip and no memset, so the sequence can sit
+ * between the tracking launches of one captured graph. */
+#define CT 1024 /* particles per tile of the compaction */
+
+/* alive particles per tile */
+__global__ void __launch_bounds__(256) compact_count(const int32_t *__restrict__ lp, int64_t n, int32_t *tile_sum) {
+  __shared__ int32_t s[4];
+  const int64_t base = (int64_t)blockIdx.x * CT;
+  int32_t c = 0;
+  for (int k = threadIdx.x; k < CT; k += 256) {
+    const int64_t i = base + k;
+    c += (i < n && lp[i] < 0) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0)
+    s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    tile_sum[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+/* exclusive scan of the tile sums in place (one workgroup); tile_sum[nt] = total */
+__global__ void __launch_bounds__(256) compact_scan(int32_t *tile_sum, int32_t nt) {
+  __shared__ int32_t s[256];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0)
+    carry = 0;
+  __syncthreads();
+  for (int32_t base = 0; base < nt; base += 256) {
+    const int32_t i = base + threadIdx.x;
+    const int32_t v = i < nt ? tile_sum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int32_t t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    const int32_t incl = s[threadIdx.x];
+    const int32_t c0 = carry;
+    if (i < nt)
+      tile_sum[i] = c0 + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 255)
+      carry = c0 + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    tile_sum[nt] = carry;
+}
+
+/* perm[position] = index of each alive particle, in index order */
+__global__ void __launch_bounds__(256) compact_scatter(const int32_t *__restrict__ lp, int64_t n,
+                                                       const int32_t *__restrict__ tile_sum, int32_t *perm) {
+  __shared__ int32_t wsum[4];
+  const int64_t base = (int64_t)blockIdx.x * CT;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t off = tile_sum[blockIdx.x];
+  for (int k0 = 0; k0 < CT; k0 += 256) {
+    const int64_t i = base + k0 + threadIdx.x;
+    const bool a = i < n && lp[i] < 0;
+    const unsigned long long m = __ballot(a);
+    const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+    if (lane == 0)
+      wsum[w] = __popcll(m);
+    __syncthreads();
+    int32_t wbase = 0;
+    for (int q = 0; q < w; q++)
+      wbase += wsum[q];
+    const int32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (a)
+      perm[off + wbase + before] = (int32_t)i;
+    __syncthreads();
+    off += tot;
+  }
+}
+
+/* alive particles with index below k, by the workgroup: the tile's prefix
+ * plus a count over the start of the tile */
+DEV int32_t alive_below(const int32_t *__restrict__ lp, int64_t n, const int32_t *__restrict__ tile_sum, int64_t k,
+                        int32_t *red) {
+  const int64_t tile = k / CT, base = tile * CT;
+  int32_t c = 0;
+  for (int64_t i = base + threadIdx.x; i < k && i < n; i += 256)
+    c += lp[i] < 0 ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1)
+    c += __shfl_xor(c, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  return tile_sum[tile] + red[0] + red[1] + red[2] + red[3];
+}
+
+/* per variant (one workgroup each): vinfo[4v..] = start in perm, alive, chunks */
+__global__ void __launch_bounds__(256) compact_variants(const int32_t *__restrict__ lp, int64_t n,
+                                                        const int32_t *__restrict__ tile_sum,
+                                                        const eh_vdesc *__restrict__ vars, int32_t csize,
+                                                        int32_t *vinfo) {
+  __shared__ int32_t red[4];
+  const eh_vdesc vd = vars[blockIdx.x];
+  const int32_t p0 = alive_below(lp, n, tile_sum, vd.first, red);
+  const int32_t p1 = alive_below(lp, n, tile_sum, vd.first + vd.n, red);
+  if (threadIdx.x == 0) {
+    vinfo[4 * blockIdx.x] = p0;
+    vinfo[4 * blockIdx.x + 1] = p1 - p0;
+    vinfo[4 * blockIdx.x + 2] = (p1 - p0 + csize - 1) / csize;
+  }
+}
+
+/* first chunk of each variant (exclusive scan, one workgroup) and the count */
+__global__ void __launch_bounds__(256) compact_chunk_offsets(int32_t *vinfo, int32_t nvar, int32_t *nchunks_out) {
+  __shared__ int32_t s[256];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0)
+    carry = 0;
+  __syncthreads();
+  for (int32_t base = 0; base < nvar; base += 256) {
+    const int32_t v = base + threadIdx.x;
+    const int32_t c = v < nvar ? vinfo[4 * v + 2] : 0;
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int32_t t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    const int32_t incl = s[threadIdx.x];
+    const int32_t c0 = carry;
+    if (v < nvar)
+      vinfo[4 * v + 3] = c0 + incl - c;
+    __syncthreads();
+    if (threadIdx.x == 255)
+      carry = c0 + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *nchunks_out = carry;
+}
+
+/* the chunks of each variant (one workgroup each) */
+__global__ void __launch_bounds__(256) compact_write_chunks(const int32_t *__restrict__ vinfo, int32_t csize,
+                                                            eh_chunk *chunks) {
+  const int32_t v = blockIdx.x;
+  const int32_t start = vinfo[4 * v], alive = vinfo[4 * v + 1], nch = vinfo[4 * v + 2], c0 = vinfo[4 * v + 3];
+  for (int32_t c = threadIdx.x; c < nch; c += 256) {
+    eh_chunk ch;
+    ch.variant = v;
+    ch.first = (int64_t)start + (int64_t)c * csize;
+    ch.count = alive - c * csize < csize ? alive - c * csize : csize;
+    chunks[c0 + c] = ch;
+  }
+}
+
+static int compact_ensure(ehd_dev *d) {
+  const int64_t nt = (d->n + CT - 1) / CT;
+  if (d->perm_cap < d->n) {
+    if (d->perm)
+      HIPCHK(d, hipFree(d->perm));
+    d->perm = NULL;
+    d->perm_cap = 0;
+    HIPCHK(d, hipMalloc((void **)&d->perm, sizeof(int32_t) * (d->n > 0 ? d->n : 1)));
+    d->perm_cap = d->n > 0 ? d->n : 1;
+  }
+  if (d->tile_cap < nt + 1 || d->vinfo_cap < d->nvar) {
+    if (d->tile_sum)
+      HIPCHK(d, hipFree(d->tile_sum));
+    d->tile_sum = NULL;
+    d->tile_cap = d->vinfo_cap = 0;
+    HIPCHK(d, hipMalloc((void **)&d->tile_sum, sizeof(int32_t) * (nt + 1 + 4 * (int64_t)d->nvar)));
+    d->tile_cap = nt + 1;
+    d->vinfo_cap = d->nvar;
+  }
+  return EH_OK;
+}
+
+static void compact_launch(ehd_dev *d, int32_t csize) {
+  const int64_t nt = (d->n + CT - 1) / CT;
+  int32_t *vinfo = d->tile_sum + d->tile_cap;
+  hipLaunchKernelGGL(compact_count, dim3((unsigned)nt), dim3(256), 0, d->stream, d->loss_pass, d->n, d->tile_sum);
+  hipLaunchKernelGGL(compact_scan, dim3(1), dim3(256), 0, d->stream, d->tile_sum, (int32_t)nt);
+  hipLaunchKernelGGL(compact_scatter, dim3((unsigned)nt), dim3(256), 0, d->stream, d->loss_pass, d->n, d->tile_sum,
+                     d->perm);
+  hipLaunchKernelGGL(compact_variants, dim3((unsigned)d->nvar), dim3(256), 0, d->stream, d->loss_pass, d->n,
+                     d->tile_sum, d->vdesc, csize, vinfo);
+  hipLaunchKernelGGL(compact_chunk_offsets, dim3(1), dim3(256), 0, d->stream, vinfo, d->nvar, d->dev_nchunks);
+  hipLaunchKernelGGL(compact_write_chunks, dim3((unsigned)d->nvar), dim3(256), 0, d->stream, vinfo, csize, d->chunks);
 }
 
 static int batch_launch(ehd_dev *d, eh_state st, int64_t pass0, int32_t npasses, int32_t ppl, int32_t block,
@@ -1510,21 +1712,21 @@ static int batch_launch(ehd_dev *d, eh_state st, int64_t pass0, int32_t npasses,
   const int32_t *perm = d->use_perm ? d->perm : (const int32_t *)NULL;
   if (ppl == 2 && d->shaped)
     hipLaunchKernelGGL((eh_batch_kernel<2, true>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
-                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+                       d->chunks, d->dev_nchunks, perm, pass0, npasses, d->queue);
   else if (ppl == 2)
     hipLaunchKernelGGL((eh_batch_kernel<2, false>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
-                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+                       d->chunks, d->dev_nchunks, perm, pass0, npasses, d->queue);
   else if (d->shaped)
     hipLaunchKernelGGL((eh_batch_kernel<1, true>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
-                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+                       d->chunks, d->dev_nchunks, perm, pass0, npasses, d->queue);
   else
     hipLaunchKernelGGL((eh_batch_kernel<1, false>), dim3(grid), dim3(block), 0, d->stream, st, d->ops, d->vdesc,
-                       d->chunks, d->nchunks, perm, pass0, npasses, d->queue);
+                       d->chunks, d->dev_nchunks, perm, pass0, npasses, d->queue);
   return EH_OK;
 }
 
 int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,
-                    int32_t block, int32_t use_graph, double *ms_out, int64_t *launches_out) {
+                    int32_t block, int32_t use_graph, int32_t device_compact, double *ms_out, int64_t *launches_out) {
   if (npasses <= 0 || d->nchunks <= 0) {
     *ms_out = 0;
     *launches_out = 0;
@@ -1533,6 +1735,11 @@ int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_p
   if ((ppl != 1 && ppl != 2) || block < 64 || block > 256 || block % 64 || passes_per_launch < 1)
     return EH_ERR_INVALID;
   HIPCHK(d, hipSetDevice(d->device));
+  if (device_compact) {
+    int rc = compact_ensure(d);
+    if (rc)
+      return rc;
+  }
   eh_state st;
   st.x = d->coord[0];
   st.xp = d->coord[1];
@@ -1564,10 +1771,29 @@ int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_p
     HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
   else
     HIPCHK(d, hipEventRecord(d->ev0, d->stream));
-  for (int32_t done = 0; done < npasses; done += passes_per_launch) {
-    int32_t k = npasses - done < passes_per_launch ? npasses - done : passes_per_launch;
-    batch_launch(d, st, pass0 + done, k, ppl, block, grid);
-    nl++;
+  if (!device_compact) {
+    for (int32_t done = 0; done < npasses; done += passes_per_launch) {
+      int32_t k = npasses - done < passes_per_launch ? npasses - done : passes_per_launch;
+      batch_launch(d, st, pass0 + done, k, ppl, block, grid);
+      nl++;
+    }
+  } else {
+    /* launches of 1, 2, 4 ... passes up to passes_per_launch, the queue
+     * rebuilt from the survivors on the device after each */
+    int32_t done = 0, step = 1;
+    while (done < npasses) {
+      int32_t k = step < passes_per_launch ? step : passes_per_launch;
+      if (k > npasses - done)
+        k = npasses - done;
+      batch_launch(d, st, pass0 + done, k, ppl, block, grid);
+      nl++;
+      done += k;
+      step *= 2;
+      if (done < npasses) {
+        compact_launch(d, block * ppl);
+        d->use_perm = 1;
+      }
+    }
   }
   if (use_graph) {
     HIPCHK(d, hipStreamEndCapture(d->stream, &graph));

@@ -49,6 +49,10 @@ struct ehd_dev {
   int32_t *perm;     /* compacted particle indices the chunks refer to (NULL: identity) */
   int64_t perm_cap;
   int use_perm;
+  int32_t *dev_nchunks; /* the chunk count the batch kernel reads (inside the queue buffer) */
+  int32_t *tile_sum;    /* device compaction: tile prefix sums, then per-variant info */
+  int64_t tile_cap;
+  int32_t vinfo_cap;
   int shaped; /* the loaded program has an elliptical or open-sided MAXAMP */
   double *snap[6];
   int32_t *snap_lp, *snap_lc;

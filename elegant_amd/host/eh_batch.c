@@ -245,7 +245,7 @@ static int upload(eh_batch *b) {
 
 int eh_batch_set_tuning(eh_batch *b, const eh_batch_tuning *t) {
   if (!b || !t || (t->particles_per_lane != 1 && t->particles_per_lane != 2) || t->block < 64 || t->block > 256 ||
-      t->block % 64 || t->passes_per_launch < 1)
+      t->block % 64 || t->passes_per_launch < 1 || t->compact < 0 || t->compact > 2)
     return EH_ERR_INVALID;
   if (t->block * t->particles_per_lane != b->tune.block * b->tune.particles_per_lane && b->uploaded) {
     /* the chunk size changed: rebuild the queue from the current device state */
@@ -268,30 +268,26 @@ int eh_batch_get_tuning(const eh_batch *b, eh_batch_tuning *t) {
   return EH_OK;
 }
 
-int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats) {
-  ehd_dev *d;
+/* the passes of one track call in the tuned work-queue mode:
+ * compact 0: one fixed queue, launches of passes_per_launch (captured in a
+ *   graph with use_graph);
+ * compact 1: launches of 1, 2, 4 ... passes up to passes_per_launch, so the
+ *   queue can shed the many particles a grid loses in its first passes; after
+ *   each launch the host rebuilds the queue from the survivors once a quarter
+ *   of the scheduled lanes are idle;
+ * compact 2: the same schedule with the queue rebuilt on the device after
+ *   every launch, the whole sequence captured in one graph with use_graph. */
+static int run_passes(eh_batch *b, int64_t pass0, int64_t npasses, double *ms_out, int64_t *nl_out) {
+  ehd_dev *d = eh_ctx_dev(b->ctx);
   double ms = 0;
   int64_t nl = 0;
   int rc;
-  if (!b || npasses < 0 || npasses > 0x7fffffff)
-    return EH_ERR_INVALID;
-  d = eh_ctx_dev(b->ctx);
-  if (!b->uploaded) {
-    rc = upload(b);
-    if (rc)
-      return rc;
-  }
-  ehd_timing_reset(d);
-  if (!b->tune.compact) {
+  if (b->tune.compact != 1) {
     rc = ehd_batch_track(d, pass0, (int32_t)npasses, b->tune.passes_per_launch, b->tune.particles_per_lane,
-                         b->tune.block, b->tune.use_graph, &ms, &nl);
+                         b->tune.block, b->tune.use_graph, b->tune.compact == 2, &ms, &nl);
     if (rc)
       return bmsg(b, rc, ehd_message(d));
   } else {
-    /* launches of 1, 2, 4 ... passes up to passes_per_launch, so the queue can
-     * shed the many particles a grid loses in its first passes; after each
-     * launch the queue is rebuilt from the survivors once a quarter of the
-     * scheduled lanes are idle */
     int64_t done = 0, step = 1;
     while (done < npasses) {
       double ms1 = 0;
@@ -299,7 +295,7 @@ int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats)
       int64_t k = step < b->tune.passes_per_launch ? step : b->tune.passes_per_launch;
       if (k > npasses - done)
         k = npasses - done;
-      rc = ehd_batch_track(d, pass0 + done, (int32_t)k, (int32_t)k, b->tune.particles_per_lane, b->tune.block, 0,
+      rc = ehd_batch_track(d, pass0 + done, (int32_t)k, (int32_t)k, b->tune.particles_per_lane, b->tune.block, 0, 0,
                            &ms1, &nl1);
       if (rc)
         return bmsg(b, rc, ehd_message(d));
@@ -323,6 +319,28 @@ int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats)
       }
     }
   }
+  *ms_out = ms;
+  *nl_out = nl;
+  return EH_OK;
+}
+
+int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats) {
+  ehd_dev *d;
+  double ms = 0;
+  int64_t nl = 0;
+  int rc;
+  if (!b || npasses < 0 || npasses > 0x7fffffff)
+    return EH_ERR_INVALID;
+  d = eh_ctx_dev(b->ctx);
+  if (!b->uploaded) {
+    rc = upload(b);
+    if (rc)
+      return rc;
+  }
+  ehd_timing_reset(d);
+  rc = run_passes(b, pass0, npasses, &ms, &nl);
+  if (rc)
+    return rc;
   if (stats) {
     int64_t effort = 0, left = 0;
     rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);
@@ -348,9 +366,22 @@ int eh_batch_track(eh_batch *b, int64_t pass0, int64_t npasses, eh_stats *stats)
   return EH_OK;
 }
 
-/* Time candidate (particles per lane, workgroup size, passes per launch) on a
- * short sample of the loaded batch, restoring the state after each trial, and
- * keep the fastest. */
+static double wall_ms(void) {
+  struct timespec ts;
+  timespec_get(&ts, TIME_UTC);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* The autotuner, on samples of the loaded batch from its current state
+ * (restored after every trial):
+ *  1. particles per lane x workgroup size, on the fixed queue for
+ *     sample_passes passes (kernel time);
+ *  2. the work-queue mode and the passes fused per launch: the fixed queue
+ *     in a graph, the host-rebuilt queue, and the device-rebuilt queue in a
+ *     graph with launches of at most 8, 32 or P passes (P: about a second of
+ *     work per launch), over 4 x sample_passes passes, on the host's wall
+ *     clock (it includes queue rebuilds, copies and graph instantiation).
+ * The fastest of each stage is kept. */
 int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_out) {
   static const int ppl[2] = {1, 2};
   static const int blocks[3] = {64, 128, 256};
@@ -379,11 +410,15 @@ int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_
       t.particles_per_lane = ppl[a];
       t.block = blocks[c];
       t.passes_per_launch = (int32_t)sample_passes;
+      t.compact = 0;
+      t.use_graph = 0;
       rc = eh_batch_set_tuning(b, &t);
       if (!rc)
         rc = ehd_state_snapshot(d, 1);
       if (!rc)
-        rc = ehd_batch_track(d, 0, (int32_t)sample_passes, t.passes_per_launch, t.particles_per_lane, t.block, 0,
+        rc = build_queue(b, 0);
+      if (!rc)
+        rc = ehd_batch_track(d, 0, (int32_t)sample_passes, t.passes_per_launch, t.particles_per_lane, t.block, 0, 0,
                              &ms, &nl);
       if (rc)
         return bmsg(b, rc, ehd_message(d));
@@ -392,20 +427,53 @@ int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best_
         best = t;
       }
     }
-  /* passes per launch: long enough to amortize the launch and the queue tail,
-   * short enough to keep launches under ~1 s */
-  {
-    double per_pass = best_ms / sample_passes;
-    int32_t ppass = per_pass > 0 ? (int32_t)(1000.0 / per_pass) : 1024;
-    if (ppass < 1)
-      ppass = 1;
-    if (ppass > 1024)
-      ppass = 1024;
-    best.passes_per_launch = ppass;
-  }
-  best.use_graph = 1;
-  best.compact = 1;
   best.sample_ms = best_ms;
+  {
+    /* launches of about a second at most */
+    double per_pass = best_ms / sample_passes;
+    int32_t P = per_pass > 0 ? (int32_t)(1000.0 / per_pass) : 1024;
+    const int64_t long_passes = 4 * sample_passes;
+    struct {
+      int32_t compact, graph, cap;
+    } cand[5];
+    double best_wall = 1e300;
+    eh_batch_tuning pick = best;
+    if (P < 1)
+      P = 1;
+    if (P > 1024)
+      P = 1024;
+    cand[0].compact = 0, cand[0].graph = 1, cand[0].cap = P;
+    cand[1].compact = 1, cand[1].graph = 0, cand[1].cap = P;
+    cand[2].compact = 2, cand[2].graph = 1, cand[2].cap = 8 < P ? 8 : P;
+    cand[3].compact = 2, cand[3].graph = 1, cand[3].cap = 32 < P ? 32 : P;
+    cand[4].compact = 2, cand[4].graph = 1, cand[4].cap = P;
+    for (int k = 0; k < 5; k++) {
+      double ms = 0, w0;
+      int64_t nl = 0;
+      t = best;
+      t.compact = cand[k].compact;
+      t.use_graph = cand[k].graph;
+      t.passes_per_launch = cand[k].cap;
+      rc = eh_batch_set_tuning(b, &t);
+      if (!rc)
+        rc = ehd_state_snapshot(d, 1);
+      if (!rc)
+        rc = build_queue(b, 0);
+      if (rc)
+        return bmsg(b, rc, ehd_message(d));
+      w0 = wall_ms();
+      rc = run_passes(b, 0, long_passes, &ms, &nl);
+      if (rc)
+        return rc;
+      w0 = wall_ms() - w0;
+      if (w0 < best_wall) {
+        best_wall = w0;
+        pick = t;
+      }
+    }
+    best = pick;
+    best.sample_ms = best_ms;
+  }
   rc = eh_batch_set_tuning(b, &best);
   if (!rc)
     rc = ehd_state_snapshot(d, 1);

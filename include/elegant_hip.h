@@ -345,10 +345,12 @@ typedef struct {
   int32_t particles_per_lane; /* 1 or 2 */
   int32_t block;              /* workgroup size: 64, 128 or 256 */
   int32_t passes_per_launch;  /* passes fused in one kernel launch */
-  int32_t use_graph;          /* capture the launch sequence in a HIP graph */
-  int32_t compact;            /* between launches, rebuild the work queue from the surviving particles when
-                                 fewer than 3/4 of the scheduled lanes are alive (launches are then not
-                                 captured in a graph) */
+  int32_t use_graph;          /* capture the launch sequence in a HIP graph (compact 0 and 2) */
+  int32_t compact;            /* the work queue between launches: 0 fixed; 1 rebuilt by the host from the
+                                 surviving particles when fewer than 3/4 of the scheduled lanes are alive
+                                 (launches not captured); 2 rebuilt on the device after every launch, inside
+                                 the captured graph.  With 1 and 2 launches run 1, 2, 4 ... passes up to
+                                 passes_per_launch */
   int32_t pad;
   double sample_ms;           /* autotune: time of the best sample */
 } eh_batch_tuning;
@@ -358,7 +360,9 @@ int eh_batch_destroy(eh_batch *b);
 /* returns the new variant's index (>= 0) or a negative error code */
 int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord, int64_t n,
                          int32_t nprops);
-/* time the candidate tunings on sample_passes of the loaded batch (state restored) and keep the fastest */
+/* time the candidate tunings on samples of the loaded batch (state restored after each) and keep the
+ * fastest: particles per lane x workgroup size over sample_passes, then the work-queue mode and the
+ * passes fused per launch over 4 x sample_passes on the host's wall clock */
 int eh_batch_autotune(eh_batch *b, int64_t sample_passes, eh_batch_tuning *best);
 int eh_batch_set_tuning(eh_batch *b, const eh_batch_tuning *t);
 int eh_batch_get_tuning(const eh_batch *b, eh_batch_tuning *t);

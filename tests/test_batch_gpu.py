@@ -26,16 +26,17 @@ def variants():
 
 
 @pytest.mark.parametrize("tuning", [(1, 256, 64, 1, 0), (2, 64, 1, 0, 0), (2, 128, 2, 1, 0), (1, 64, 4, 1, 1),
-                                    (2, 64, 2, 0, 1)])
+                                    (2, 64, 2, 0, 1), (1, 128, 4, 1, 2), (2, 64, 2, 0, 2), (1, 256, 1, 1, 2)])
 def test_batch_bit_exact_per_variant(tracker, tuning):
-    """the last two tunings rebuild the queue from the survivors between
-    launches (1, 2, ... passes per launch)"""
+    """compact 1 rebuilds the queue from the survivors on the host between
+    launches (1, 2, ... passes per launch), compact 2 on the device inside
+    the captured graph (use_graph 1) or between plain launches"""
     vs = variants()
     b = native.Batch(tracker)
     for bl, Po, rows in vs:
         b.add_variant(bl, Po, rows)
     b.set_tuning(*tuning)
-    passes = 6 if tuning[4] else 3
+    passes = 7 if tuning[4] else 3
     st = b.track(0, passes)
     total_lost = 0
     for k, (bl, Po, rows) in enumerate(vs):
@@ -57,6 +58,9 @@ def test_autotune_keeps_results(tracker):
         b.add_variant(bl, Po, rows)
     t = b.autotune(1)
     assert t.particles_per_lane in (1, 2) and t.block in (64, 128, 256) and t.passes_per_launch >= 1
+    assert t.compact in (0, 1, 2)
+    print("autotune picked ppl %d block %d passes/launch %d graph %d compact %d" % (
+        t.particles_per_lane, t.block, t.passes_per_launch, t.use_graph, t.compact))
     b.track(0, 2)
     bl, Po, rows = vs[1]
     g, ng = b.download(1)
