@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=100, help="timed passes")
     ap.add_argument("--warmup", type=int, default=2, help="untimed passes (the first includes the RF fiducial)")
     ap.add_argument("--workload", default="storage_ring",
-                    choices=["storage_ring", "fodo_ring", "storage_ring_wake", "error_seeds", "learned_da"])
+                    choices=["storage_ring", "fodo_ring", "storage_ring_wake", "error_seeds", "learned_da",
+                             "mlp_train"])
     ap.add_argument("--variants", type=int, default=256, help="error_seeds / learned_da: lattice variants")
     ap.add_argument("--particles", type=int, default=0, help="override the workload's particle count")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -199,6 +200,32 @@ def learned_da(args):
             "report": rep}
 
 
+def mlp_train(args):
+    """Training throughput of the aperture network (19-256-256-256-1, FP32,
+    Adam, minibatches of 4096) on a synthetic training set of the learned
+    study's size (1.86 M samples), 6 epochs after a one-epoch warm-up."""
+    import numpy as np
+    from elegant_amd import learned, native
+    tr = native.Tracker(0)
+    rng = np.random.default_rng(0)
+    n = args.particles or 1_860_000
+    X = rng.normal(0, 1, (n, 19)).astype(np.float32)
+    y = (X[:, 0] ** 2 + X[:, 1] ** 2 < 1.0).astype(np.float32)
+    m = learned.MLP(tr, seed=2)
+    m.train(X[:65536], y[:65536], epochs=1, batch=4096)
+    t0 = time.perf_counter()
+    losses = m.train(X, y, epochs=6, batch=4096, lr=1e-3)
+    wall = time.perf_counter() - t0
+    m.close()
+    tr.close()
+    flop = 6 * n * 3 * 2 * (19 * 256 + 256 * 256 * 2 + 256)
+    return {"metric": "MLP training samples/s (aperture network)", "value": 6 * n / wall, "unit": "samples/s",
+            "n_gpus": 1, "steps": 6, "warmup": 1, "ms_per_step": wall * 1e3 / 6, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "19-256-256-256-1 perceptron, %d samples x 6 epochs, batch 4096" % n},
+            "tflops_3x_forward": flop / wall / 1e12, "final_loss": float(losses[-1])}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -210,6 +237,9 @@ def main():
         return
     if args.workload == "learned_da":
         print(json.dumps(learned_da(args)))
+        return
+    if args.workload == "mlp_train":
+        print(json.dumps(mlp_train(args)))
         return
     import torch
     from elegant_amd import lattices as Lt

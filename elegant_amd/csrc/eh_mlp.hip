@@ -32,18 +32,23 @@
 template <int TA, int TB>
 __global__ void __launch_bounds__(256) gemm_f32(const float *__restrict__ A, const float *__restrict__ B,
                                                 float *__restrict__ C, const float *__restrict__ bias, int M, int N,
-                                                int K, int lda, int ldb, int ldc, int relu, int accumulate) {
+                                                int K, int lda, int ldb, int ldc, int relu, int accumulate,
+                                                int kslice, int64_t c_slice) {
   __shared__ float As[TK][TM + 4];
   __shared__ float Bs[TK][TN + 4];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  /* split K: slice z covers [z kslice, (z+1) kslice) and writes its partial
+   * product to C + z c_slice */
+  const int kb = blockIdx.z * kslice, ke = kb + kslice < K ? kb + kslice : K;
+  C += blockIdx.z * c_slice;
   float acc[4][4] = {{0}};
-  for (int k0 = 0; k0 < K; k0 += TK) {
+  for (int k0 = kb; k0 < ke; k0 += TK) {
     for (int e = threadIdx.x; e < TK * TM; e += 256) {
       int kk = e / TM, mm = e % TM;
       int gm = m0 + mm, gk = k0 + kk;
       float v = 0.f;
-      if (gm < M && gk < K)
+      if (gm < M && gk < ke)
         v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
       As[kk][mm] = v;
     }
@@ -51,7 +56,7 @@ __global__ void __launch_bounds__(256) gemm_f32(const float *__restrict__ A, con
       int kk = e / TN, nn = e % TN;
       int gn = n0 + nn, gk = k0 + kk;
       float v = 0.f;
-      if (gn < N && gk < K)
+      if (gn < N && gk < ke)
         v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
       Bs[kk][nn] = v;
     }
@@ -94,22 +99,63 @@ __global__ void __launch_bounds__(256) gemm_f32(const float *__restrict__ A, con
   }
 }
 
+/* out[i] = sum over s of part[s][i], s in order (deterministic split-K) */
+__global__ void reduce_slices(const float *__restrict__ part, int S, int64_t count, float *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  float v = 0.f;
+  for (int s = 0; s < S; s++)
+    v += part[(int64_t)s * count + i];
+  out[i] = v;
+}
+
+/* partial column sums of dz[rows, cols]: slice blockIdx.y of rowsl rows,
+ * 64 columns per workgroup, 4 row lanes per column combined in LDS */
+__global__ void __launch_bounds__(256) col_sum_part(const float *__restrict__ dz, float *part, int rows, int cols,
+                                                    int rowsl) {
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), lane = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * rowsl, r1 = r0 + rowsl < rows ? r0 + rowsl : rows;
+  float v = 0.f;
+  if (c < cols)
+    for (int r = r0 + lane; r < r1; r += 4)
+      v += dz[(size_t)r * cols + c];
+  red[lane][threadIdx.x & 63] = v;
+  __syncthreads();
+  if (lane == 0 && c < cols)
+    part[(size_t)blockIdx.y * cols + c] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                                          red[3][threadIdx.x];
+}
+
+/* the minibatch from the resident training set: rows perm[0..B) of X, y */
+__global__ void gather_batch(const float *__restrict__ X, const float *__restrict__ y,
+                             const int32_t *__restrict__ perm, int B, int nin, float *xb, float *yb) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * nin)
+    return;
+  const int i = (int)(e / nin), k = (int)(e % nin);
+  const int32_t j = perm[i];
+  xb[e] = X[(size_t)j * nin + k];
+  if (k == 0)
+    yb[i] = y[j];
+}
+
+/* acc[0] += the batch loss (sum of the block partials in order) */
+__global__ void loss_accumulate(const float *__restrict__ part, int nb, float *acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  float s = 0.f;
+  for (int k = 0; k < nb; k++)
+    s += part[k];
+  acc[0] += s;
+}
+
 /* dZ *= (H > 0): ReLU derivative with the layer's output H */
 __global__ void relu_mask(float *dz, const float *h, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && !(h[i] > 0.f))
     dz[i] = 0.f;
-}
-
-/* column sums: db[j] = sum_i dz[i, j] (one thread per column) */
-__global__ void col_sum(const float *dz, float *db, int rows, int cols) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols)
-    return;
-  float s = 0.f;
-  for (int i = 0; i < rows; i++)
-    s += dz[(size_t)i * cols + j];
-  db[j] = s;
 }
 
 /* binary cross-entropy on logits, mean over the batch: loss partials and dz = (sigmoid(z) - y)/B */
@@ -169,6 +215,12 @@ struct ehd_mlp {
   float *labels, *loss_part;
   int32_t cap;
   int64_t step;
+  float *part;           /* split-K and column-sum partials */
+  int64_t part_cap;
+  float *Xd, *yd;        /* the resident training set */
+  int32_t *permd;        /* its sample order, epochs x n */
+  float *eloss;          /* per-epoch loss sums */
+  int64_t data_cap, y_cap, perm_cap, eloss_cap;
   char msg[256];
 };
 
@@ -186,18 +238,35 @@ static int mlp_fail(ehd_mlp *m, hipError_t e, const char *what) {
       return mlp_fail(m, e_, #call);                                                                                   \
   } while (0)
 
-static void launch_gemm(ehd_mlp *m, int ta, int tb, const float *A, const float *B, float *C, const float *bias, int M,
-                        int N, int K, int lda, int ldb, int ldc, int relu, int acc) {
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+/* S > 1 splits K into S slices whose partial products go to C + z c_slice */
+static void launch_gemm_s(ehd_mlp *m, int ta, int tb, const float *A, const float *B, float *C, const float *bias,
+                          int M, int N, int K, int lda, int ldb, int ldc, int relu, int acc, int S, int64_t c_slice) {
+  const int kslice = (K + S - 1) / S;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, S);
   if (!ta && tb)
     hipLaunchKernelGGL((gemm_f32<0, 1>), grid, dim3(256), 0, m->d->stream, A, B, C, bias, M, N, K, lda, ldb, ldc,
-                       relu, acc);
+                       relu, acc, kslice, c_slice);
   else if (!ta && !tb)
     hipLaunchKernelGGL((gemm_f32<0, 0>), grid, dim3(256), 0, m->d->stream, A, B, C, bias, M, N, K, lda, ldb, ldc,
-                       relu, acc);
+                       relu, acc, kslice, c_slice);
   else
     hipLaunchKernelGGL((gemm_f32<1, 0>), grid, dim3(256), 0, m->d->stream, A, B, C, bias, M, N, K, lda, ldb, ldc,
-                       relu, acc);
+                       relu, acc, kslice, c_slice);
+}
+
+static void launch_gemm(ehd_mlp *m, int ta, int tb, const float *A, const float *B, float *C, const float *bias, int M,
+                        int N, int K, int lda, int ldb, int ldc, int relu, int acc) {
+  launch_gemm_s(m, ta, tb, A, B, C, bias, M, N, K, lda, ldb, ldc, relu, acc, 1, 0);
+}
+
+/* K slices for a weight-gradient GEMM of M x N over K = batch: enough
+ * workgroups to fill the device (about 256), slices of at least 64 */
+static int ksplit(int M, int N, int K) {
+  int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  int S = 256 / tiles;
+  if (S > K / 64)
+    S = K / 64;
+  return S < 1 ? 1 : S;
 }
 
 static int ensure_batch(ehd_mlp *m, int32_t B) {
@@ -222,6 +291,24 @@ static int ensure_batch(ehd_mlp *m, int32_t B) {
   MCHK(hipMalloc((void **)&m->labels, sizeof(float) * B));
   MCHK(hipMalloc((void **)&m->loss_part, sizeof(float) * ((B + 255) / 256)));
   m->cap = B;
+  {
+    /* partials of the split-K weight gradients and of the column sums */
+    int64_t need = 0;
+    for (int l = 0; l < MLP_LAYERS; l++) {
+      int64_t w = (int64_t)ksplit(m->out_dim[l], m->in_dim[l], B) * m->out_dim[l] * m->in_dim[l];
+      int64_t c = (int64_t)((B + 63) / 64) * m->out_dim[l];
+      need = w > need ? w : need;
+      need = c > need ? c : need;
+    }
+    if (need > m->part_cap) {
+      if (m->part)
+        MCHK(hipFree(m->part));
+      m->part = NULL;
+      m->part_cap = 0;
+      MCHK(hipMalloc((void **)&m->part, sizeof(float) * need));
+      m->part_cap = need;
+    }
+  }
   return EH_OK;
 }
 
@@ -269,8 +356,8 @@ void ehd_mlp_destroy(ehd_mlp *m) {
     return;
   (void)hipSetDevice(m->d->device);
   (void)hipStreamSynchronize(m->d->stream);
-  void *bufs[4] = {m->params, m->grads, m->adam_m, m->adam_v};
-  for (int k = 0; k < 4; k++)
+  void *bufs[9] = {m->params, m->grads, m->adam_m, m->adam_v, m->part, m->Xd, m->yd, m->permd, m->eloss};
+  for (int k = 0; k < 9; k++)
     if (bufs[k])
       (void)hipFree(bufs[k]);
   for (int l = 0; l <= MLP_LAYERS; l++) {
@@ -294,22 +381,40 @@ static void forward(ehd_mlp *m, int B) {
   }
 }
 
-/* gradient of the mean BCE over the batch into m->grads; returns the loss */
-static int backward(ehd_mlp *m, int B, float *loss) {
+/* gradient of the mean BCE over the batch into m->grads.  The loss: into
+ * *loss (synchronizing) when loss is given, else added to the device word
+ * loss_acc when that is given.  Weight gradients dW = dZ^T X are split over
+ * K (the batch) so that even the 19- and 1-wide layers fill the device; the
+ * slices and the bias gradients' row slices are summed in a fixed order. */
+static int backward(ehd_mlp *m, int B, float *loss, float *loss_acc) {
   int nb = (B + 255) / 256;
   hipLaunchKernelGGL(bce_grad, dim3(nb), dim3(256), 0, m->d->stream, m->act[MLP_LAYERS], m->labels,
                      m->dact[MLP_LAYERS], m->loss_part, B);
+  if (loss_acc)
+    hipLaunchKernelGGL(loss_accumulate, dim3(1), dim3(64), 0, m->d->stream, m->loss_part, nb, loss_acc);
   for (int l = MLP_LAYERS - 1; l >= 0; l--) {
-    /* dW = dZ^T X, db = colsum(dZ) */
-    launch_gemm(m, 1, 0, m->dact[l + 1], m->act[l], m->grads + m->woff[l], NULL, m->out_dim[l], m->in_dim[l], B,
-                m->out_dim[l], m->in_dim[l], m->in_dim[l], 0, 0);
-    hipLaunchKernelGGL(col_sum, dim3((m->out_dim[l] + 255) / 256), dim3(256), 0, m->d->stream, m->dact[l + 1],
-                       m->grads + m->boff[l], B, m->out_dim[l]);
+    const int M = m->out_dim[l], N = m->in_dim[l];
+    const int S = ksplit(M, N, B);
+    const int64_t cnt = (int64_t)M * N;
+    /* dW = dZ^T X */
+    if (S > 1) {
+      launch_gemm_s(m, 1, 0, m->dact[l + 1], m->act[l], m->part, NULL, M, N, B, M, N, N, 0, 0, S, cnt);
+      hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m->d->stream, m->part, S,
+                         cnt, m->grads + m->woff[l]);
+    } else
+      launch_gemm(m, 1, 0, m->dact[l + 1], m->act[l], m->grads + m->woff[l], NULL, M, N, B, M, N, N, 0, 0);
+    /* db = column sums of dZ, in row slices of 64 */
+    {
+      const int rs = (B + 63) / 64;
+      hipLaunchKernelGGL(col_sum_part, dim3((unsigned)((M + 63) / 64), (unsigned)rs), dim3(256), 0, m->d->stream,
+                         m->dact[l + 1], m->part, B, M, 64);
+      hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, m->d->stream, m->part, rs,
+                         (int64_t)M, m->grads + m->boff[l]);
+    }
     if (l > 0) {
       /* dX = dZ W, then the ReLU mask of the previous layer's output */
-      launch_gemm(m, 0, 0, m->dact[l + 1], m->params + m->woff[l], m->dact[l], NULL, B, m->in_dim[l],
-                  m->out_dim[l], m->out_dim[l], m->in_dim[l], m->in_dim[l], 0, 0);
-      int64_t n = (int64_t)B * m->in_dim[l];
+      launch_gemm(m, 0, 0, m->dact[l + 1], m->params + m->woff[l], m->dact[l], NULL, B, N, M, M, N, N, 0, 0);
+      int64_t n = (int64_t)B * N;
       hipLaunchKernelGGL(relu_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->d->stream, m->dact[l],
                          m->act[l], n);
     }
@@ -337,7 +442,7 @@ int ehd_mlp_grad(ehd_mlp *m, const float *X, const float *y, int32_t B, float *g
   MCHK(hipMemcpyAsync(m->act[0], X, sizeof(float) * (size_t)B * m->nin, hipMemcpyHostToDevice, m->d->stream));
   MCHK(hipMemcpyAsync(m->labels, y, sizeof(float) * B, hipMemcpyHostToDevice, m->d->stream));
   forward(m, B);
-  if ((rc = backward(m, B, loss)))
+  if ((rc = backward(m, B, loss, NULL)))
     return rc;
   MCHK(hipGetLastError());
   MCHK(hipMemcpyAsync(grad, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToHost, m->d->stream));
@@ -345,54 +450,76 @@ int ehd_mlp_grad(ehd_mlp *m, const float *X, const float *y, int32_t B, float *g
   return EH_OK;
 }
 
+/* grow a device buffer to hold count elements of size sz */
+static int grow_dev(ehd_mlp *m, void **p, int64_t *cap, int64_t count, size_t sz) {
+  if (count <= *cap)
+    return EH_OK;
+  if (*p)
+    MCHK(hipFree(*p));
+  *p = NULL;
+  *cap = 0;
+  MCHK(hipMalloc(p, sz * (size_t)(count > 0 ? count : 1)));
+  *cap = count;
+  return EH_OK;
+}
+
 /* minibatch Adam over n samples (X row-major [n, nin]) for the given epochs;
- * the sample order per epoch is the given permutation table (epochs x n) */
+ * the sample order per epoch is the given permutation table (epochs x n).
+ * The training set and the order are copied to the device once; each step
+ * gathers its minibatch on the device, the losses are summed on the device,
+ * and the host waits once, at the end. */
 int ehd_mlp_train(ehd_mlp *m, const float *X, const float *y, int64_t n, const int64_t *perm, int32_t epochs,
                   int32_t B, float lr, float *losses) {
   int rc;
-  float *xb, *yb;
+  const int64_t nsteps = n / B;
   MCHK(hipSetDevice(m->d->device));
+  if (n > 0x7fffffff)
+    return EH_ERR_INVALID;
   if ((rc = ensure_batch(m, B)))
     return rc;
-  xb = (float *)malloc(sizeof(float) * (size_t)B * m->nin);
-  yb = (float *)malloc(sizeof(float) * B);
-  if (!xb || !yb) {
-    free(xb);
-    free(yb);
-    return EH_ERR_NOMEM;
+  if ((rc = grow_dev(m, (void **)&m->Xd, &m->data_cap, n * m->nin, sizeof(float))) ||
+      (rc = grow_dev(m, (void **)&m->yd, &m->y_cap, n, sizeof(float))) ||
+      (rc = grow_dev(m, (void **)&m->permd, &m->perm_cap, (int64_t)epochs * n, sizeof(int32_t))) ||
+      (rc = grow_dev(m, (void **)&m->eloss, &m->eloss_cap, epochs, sizeof(float))))
+    return rc;
+  {
+    int32_t *p32 = (int32_t *)malloc(sizeof(int32_t) * (size_t)((int64_t)epochs * n > 0 ? (int64_t)epochs * n : 1));
+    if (!p32)
+      return EH_ERR_NOMEM;
+    for (int64_t k = 0; k < (int64_t)epochs * n; k++)
+      p32[k] = (int32_t)perm[k];
+    MCHK(hipMemcpyAsync(m->Xd, X, sizeof(float) * (size_t)(n * m->nin), hipMemcpyHostToDevice, m->d->stream));
+    MCHK(hipMemcpyAsync(m->yd, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, m->d->stream));
+    MCHK(hipMemcpyAsync(m->permd, p32, sizeof(int32_t) * (size_t)((int64_t)epochs * n), hipMemcpyHostToDevice,
+                        m->d->stream));
+    MCHK(hipStreamSynchronize(m->d->stream));
+    free(p32);
   }
+  if (epochs > 0)
+    hipLaunchKernelGGL(reduce_slices, dim3((unsigned)((epochs + 255) / 256)), dim3(256), 0, m->d->stream, m->eloss,
+                       0, (int64_t)epochs, m->eloss); /* zero the per-epoch sums (a kernel, no memset) */
   for (int ep = 0; ep < epochs; ep++) {
-    double tot = 0;
-    int nbatch = 0;
-    for (int64_t s0 = 0; s0 + B <= n; s0 += B) {
-      for (int i = 0; i < B; i++) {
-        int64_t j = perm[(int64_t)ep * n + s0 + i];
-        memcpy(xb + (size_t)i * m->nin, X + (size_t)j * m->nin, sizeof(float) * m->nin);
-        yb[i] = y[j];
-      }
-      MCHK(hipMemcpyAsync(m->act[0], xb, sizeof(float) * (size_t)B * m->nin, hipMemcpyHostToDevice, m->d->stream));
-      MCHK(hipMemcpyAsync(m->labels, yb, sizeof(float) * B, hipMemcpyHostToDevice, m->d->stream));
+    for (int64_t st = 0; st < nsteps; st++) {
+      const int64_t e = (int64_t)B * m->nin;
+      hipLaunchKernelGGL(gather_batch, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, m->d->stream, m->Xd, m->yd,
+                         m->permd + (int64_t)ep * n + st * B, B, m->nin, m->act[0], m->labels);
       forward(m, B);
-      float l = 0;
-      if ((rc = backward(m, B, &l))) {
-        free(xb);
-        free(yb);
+      if ((rc = backward(m, B, NULL, m->eloss + ep)))
         return rc;
-      }
       m->step++;
       float c1 = 1.f - powf(0.9f, (float)m->step), c2 = 1.f - powf(0.999f, (float)m->step);
       hipLaunchKernelGGL(adam_k, dim3((unsigned)((m->nparams + 255) / 256)), dim3(256), 0, m->d->stream, m->params,
                          m->grads, m->adam_m, m->adam_v, m->nparams, lr, 0.9f, 0.999f, 1e-8f, c1, c2);
-      MCHK(hipGetLastError());
-      tot += l;
-      nbatch++;
     }
-    if (losses)
-      losses[ep] = nbatch ? (float)(tot / nbatch) : 0.f;
+  }
+  MCHK(hipGetLastError());
+  if (losses && epochs > 0) {
+    MCHK(hipMemcpyAsync(losses, m->eloss, sizeof(float) * epochs, hipMemcpyDeviceToHost, m->d->stream));
+    MCHK(hipStreamSynchronize(m->d->stream));
+    for (int ep = 0; ep < epochs; ep++)
+      losses[ep] = nsteps ? losses[ep] / (float)nsteps : 0.f;
   }
   MCHK(hipStreamSynchronize(m->d->stream));
-  free(xb);
-  free(yb);
   return EH_OK;
 }
 
