@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="passes per kernel launch (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
+    ap.add_argument("--tuning", default="", help="error_seeds: ppl,block,passes_per_launch,use_graph,compact "
+                                                 "instead of the autotuner's choice")
     return ap.parse_args()
 
 
@@ -156,7 +158,11 @@ def error_seeds(args):
     b = native.Batch(tr)
     for v in vs:
         b.add_variant(v.beamline, v.Po, studies.grid_rows(rays, studies.DELTAS, v.nx))
-    tun = b.autotune(4)
+    if args.tuning:
+        b.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        tun = b.tuning()
+    else:
+        tun = b.autotune(4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = b.track(0, args.steps)

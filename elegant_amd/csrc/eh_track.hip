@@ -1788,7 +1788,8 @@ int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_p
       batch_launch(d, st, pass0 + done, k, ppl, block, grid);
       nl++;
       done += k;
-      step *= 2;
+      if (step < passes_per_launch)
+        step *= 2;
       if (done < npasses) {
         compact_launch(d, block * ppl);
         d->use_perm = 1;

@@ -302,7 +302,8 @@ static int run_passes(eh_batch *b, int64_t pass0, int64_t npasses, double *ms_ou
       ms += ms1;
       nl += nl1;
       done += k;
-      step *= 2;
+      if (step < b->tune.passes_per_launch)
+        step *= 2;
       if (done >= npasses)
         break;
       rc = ehd_get_state(d, NULL, NULL, NULL, NULL, NULL, NULL, b->lp, b->lc, b->n);

@@ -100,3 +100,31 @@ def test_batch_with_shaped_apertures(tracker, ppl):
         assert ng == nc and ng < rows.shape[0]
         assert_rows_identical(g, c, "shaped variant %d" % k)
     b.close()
+
+
+@pytest.mark.parametrize("compact,graph", [(1, 0), (2, 1), (2, 0)])
+def test_batch_many_launches(tracker, compact, graph):
+    """a long run in launches of at most one pass (70 launches, the doubling
+    schedule capped at once), the queue rebuilt between them by the host or
+    on the device: every variant bit-identical to the checker"""
+    from elegant_amd import elements as E
+    bl, Po, rows = Lt.fodo_workload(n=600, seed=5)
+    rows[:200, 0] *= 200.0  # a halo, partly lost over the run
+    vs = [(bl, Po, rows[:350].copy()), (bl, Po, rows[350:].copy())]
+    b = native.Batch(tracker)
+    for v in vs:
+        b.add_variant(*v)
+    b.set_tuning(1, 64, 1, graph, compact)
+    passes = 70
+    st = b.track(0, passes)
+    assert st.kernel_launches == passes
+    lost = 0
+    for k, (bl_, Po_, r) in enumerate(vs):
+        g, ng = b.download(k)
+        c = r.copy()
+        nc = O.Oracle("emath").do_tracking(c, bl_, Po_, passes)
+        assert ng == nc
+        lost += r.shape[0] - ng
+        assert_rows_identical(g, c, "variant %d, %d launches" % (k, passes))
+    assert lost > 20
+    b.close()
