@@ -69,29 +69,22 @@ typedef struct {
   int32_t pad[3];
 } coll_plan;
 
-/* start of a step: reset the reductions and clear the histogram (a kernel,
- * not a memset, so the step can be captured in a graph) */
-__global__ void __launch_bounds__(CB) coll_reset(coll_plan *plan, unsigned int *hist, int32_t nclear) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    plan->red[0] = ~0ULL;
-    plan->red[1] = 0;
-    plan->red[2] = 0;
-    plan->red[3] = 0;
-    plan->nb = 0;
-    plan->err = 0;
-  }
-  for (int k = blockIdx.x * CB + threadIdx.x; k < nclear; k += gridDim.x * CB)
-    hist[k] = 0;
-}
-
 /* after the min/max pass: the decisions of wake.c:120-182 and
  * zlongit.c:195-236 that do not need the mean */
 __global__ void __launch_bounds__(CB) coll_plan_minmax(coll_plan *plan, const unsigned long long *part,
                                                        int64_t nparts, int32_t kind, int32_t n_fixed, int32_t zl_nb,
                                                        double dt, double wake_span, int32_t allow_long,
-                                                       int32_t nb_cap) {
+                                                       int32_t nb_cap, unsigned int *hist, int32_t nclear) {
   __shared__ unsigned long long smin[CB], smax[CB], scnt[CB];
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
+  /* the step starts here: clear the histogram (a kernel rather than a
+   * memset, so that the step can be captured in a graph) */
+  for (int k = threadIdx.x; k < nclear; k += CB)
+    hist[k] = 0;
+  if (threadIdx.x == 0) {
+    plan->nb = 0;
+    plan->err = 0;
+  }
   for (int64_t k = threadIdx.x; k < nparts; k += CB) {
     unsigned long long a = part[3 * k], b = part[3 * k + 1];
     kmin = a < kmin ? a : kmin;
@@ -322,19 +315,37 @@ __global__ void __launch_bounds__(CB) wake_convolve(const unsigned int *__restri
       int lo = ib - wp + 1 > t0 ? ib - wp + 1 : t0;
       int hi = ib < t1 ? ib : t1;
       int j = lo;
-      /* eight terms per step: the loads are independent, the additions stay
-       * in order; a zero count adds +-0, which leaves the sum (never -0)
+      /* eight terms per step, software-pipelined: the next eight pairs are
+       * loaded while the current eight are added, the additions stay in
+       * order; a zero count adds +-0, which leaves the sum (never -0)
        * unchanged, exactly as the reference's unconditional loop */
-      for (; j + 7 <= hi; j += 8) {
+      if (j + 7 <= hi) {
         double c[8], w[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           c[u] = (double)hs[j + u - t0];
           w[u] = ws[ib - (j + u) - w_lo];
         }
+        for (; j + 15 <= hi; j += 8) {
+          double c2[8], w2[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            c2[u] = (double)hs[j + 8 + u - t0];
+            w2[u] = ws[ib - (j + 8 + u) - w_lo];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            v += c[u] * w[u];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            c[u] = c2[u];
+            w[u] = w2[u];
+          }
+        }
 #pragma unroll
         for (int u = 0; u < 8; u++)
           v += c[u] * w[u];
+        j += 8;
       }
       for (; j <= hi; j++)
         v += (double)hs[j - t0] * ws[ib - j - w_lo];
@@ -702,22 +713,28 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   /* without a loss-pass array nothing may be marked lost: no checks */
   const int32_t ap_on = (!lp || (c->flags & EHD_COLL_NOCHECKS)) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0);
   CHK(hipEventRecord(d->cev0, d->stream));
-  hipLaunchKernelGGL(coll_reset, dim3((unsigned)((nclear + CB - 1) / CB)), dim3(CB), 0, d->stream, plan, hist, nclear);
   hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po,
                      (unsigned long long *)parts);
   hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)parts, rgrid,
-                     kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max);
+                     kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max, hist, nclear);
   if (kind == EH_OP_ZLONGIT || fixed) {
     hipLaunchKernelGGL(time_fsum, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po, plan,
                        parts);
     hipLaunchKernelGGL(fsum_final, dim3(1), dim3(CB), 0, d->stream, parts, rgrid, plan, c->dt);
   }
   {
-    int hgrid = (int)((n + CB * 8 - 1) / (CB * 8));
+    /* one workgroup per CU: each sees enough particles that its LDS counts
+     * aggregate many hits per bin before the one global atomic per bin */
+    if (!d->num_cu) {
+      int cu = 0;
+      CHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
+      d->num_cu = cu > 0 ? cu : 256;
+    }
+    int hgrid = (int)((n + CB - 1) / CB);
     if (hgrid < 1)
       hgrid = 1;
-    if (hgrid > 2048)
-      hgrid = 2048;
+    if (hgrid > d->num_cu)
+      hgrid = d->num_cu;
     /* room for one copy of the counts per wave, at most 48 KB of LDS */
     const int32_t lds_bins = nb_max * (CB / 64) < 12288 ? (int32_t)(nb_max * (CB / 64)) : 12288;
     hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, crd[4], crd[5],
@@ -847,11 +864,10 @@ int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *co
   coll_plan *plan = (coll_plan *)d->cwork;
   unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
   unsigned long long *mm = (unsigned long long *)((char *)hist + hist_bytes(64) + 8 * (64 + 8));
-  hipLaunchKernelGGL(coll_reset, dim3(1), dim3(CB), 0, d->stream, plan, hist, 0);
   hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
                      d->loss_pass, n, Po, mm);
   hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)mm, rgrid,
-                     -1, 0, 0, 1.0, 0.0, 0, 0);
+                     -1, 0, 0, 1.0, 0.0, 0, 0, hist, 0);
   CHK(hipGetLastError());
   coll_plan hp;
   CHK(hipMemcpyAsync(&hp, plan, sizeof(hp), hipMemcpyDeviceToHost, d->stream));
