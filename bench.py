@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="passes per kernel launch (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
+    ap.add_argument("--epochs", type=int, default=6, help="learned_da: training epochs of the network")
     ap.add_argument("--tuning", default="", help="error_seeds: ppl,block,passes_per_launch,use_graph,compact "
                                                  "instead of the autotuner's choice")
     return ap.parse_args()
@@ -192,7 +193,7 @@ def learned_da(args):
     from elegant_amd import learned, native
     half = args.variants // 2
     tr = native.Tracker(0)
-    rep = learned.study(tr, range(half), range(half, 2 * half), passes=args.steps,
+    rep = learned.study(tr, range(half), range(half, 2 * half), passes=args.steps, epochs=args.epochs,
                         log=lambda m: print(m, file=sys.stderr, flush=True))
     tr.close()
     best = max(("learned", "learned_ladder", "learned_sweep"), key=lambda k: rep[k]["maps_per_hour"])

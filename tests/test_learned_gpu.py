@@ -56,3 +56,24 @@ def test_reported_boundaries_are_tracked_pairs(tracker):
                 assert p < 0, (li, idx)
             else:
                 assert p >= 0, (li, idx)
+
+
+def test_network_gains_over_bisection(tracker):
+    """on a study large enough to train the network (24 training and 24
+    evaluation variants of the storage ring, 41 rays x 5 offsets, 256
+    passes) the network's guesses, confirmed by FP64 tracking, spend fewer
+    particle-turns than bisection on the same grids, agree with the plain
+    grid on more rays, and with the first-round ladder and the sweep of the
+    brackets settle every ray in two rounds, where bisection needs more"""
+    deltas = (-0.02, -0.01, 0.0, 0.01, 0.02)
+    rep = learned.study(tracker, range(300, 324), range(400, 424), passes=256, epochs=6, deltas=deltas)
+    g, l, b, sw = rep["plain_grid"], rep["learned"], rep["bisection"], rep["learned_sweep"]
+    print("plain sequential %d, learned %d (%d rounds, %.4f identical, area err %.4f), bisection %d (%d rounds, "
+          "%.4f identical, area err %.4f), sweep %d (%d rounds)" % (
+              g["effort_sequential"], l["effort"], l["rounds"], l["rays_identical"], l["area_rel_err_mean"],
+              b["effort"], b["rounds"], b["rays_identical"], b["area_rel_err_mean"], sw["effort"], sw["rounds"]))
+    assert l["effort"] < b["effort"]
+    assert l["rays_identical"] >= b["rays_identical"]
+    assert l["area_rel_err_mean"] <= b["area_rel_err_mean"]
+    assert sw["rounds"] <= 2 < b["rounds"]
+    assert l["effort"] < g["effort_sequential"] / 3
