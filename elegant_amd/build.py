@@ -14,7 +14,8 @@ OUT = os.path.join(HERE, "libelegant_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["eh_track.hip", "eh_collective.hip", "eh_mlp.hip"]
-C_SOURCES = ["eh_compile.c", "eh_driver.c", "eh_collective.c", "eh_batch.c", "eh_da.c", "eh_learn.c", "eh_gen.c"]
+C_SOURCES = ["eh_compile.c", "eh_driver.c", "eh_collective.c", "eh_batch.c", "eh_da.c", "eh_learn.c", "eh_gen.c",
+             "eh_runs.c"]
 
 # FP64 parity needs IEEE division/sqrt and no contraction on both sides.
 HIPCC_FLAGS = [

@@ -29,9 +29,19 @@ int ehd_get_state(ehd_dev *d, double *x, double *xp, double *y, double *yp, doub
 int ehd_get_sdp(ehd_dev *d, double *s, double *dp, int32_t *loss_pass, int64_t n);
 int64_t ehd_state_n(const ehd_dev *d);
 
-/* program */
+/* program: ehd_put_ops loads the flat record array the batch kernel reads
+ * (eh_vdesc offsets into it); ehd_put_program loads the tracking kernel's
+ * program, folded into runs of repeated cells (eh_fold_runs), and
+ * ehd_patch_op replaces one of its records by flat position */
 int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops);
+int ehd_put_program(ehd_dev *d, const eh_op *ops, int32_t nops);
 int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op);
+int32_t ehd_program_runs(const ehd_dev *d);
+
+/* host side (eh_runs.c): folds ops[0, n) into runs of repeated cells with
+ * bodies of at most max_len records; see there */
+int32_t eh_fold_runs(const eh_op *ops, int32_t n, int32_t max_len, eh_run *runs, int32_t *body, int32_t *nbody);
+#define EH_RUN_MAX_LEN 256
 
 /* run passes [pass0, pass0 + npasses): ops [op_first, nops) in the first pass,
  * [0, op_last) in the last, all ops in between.  Asynchronous on the

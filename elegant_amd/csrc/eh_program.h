@@ -144,6 +144,15 @@ typedef struct {
 #define EH_D_HALFLEN 7    /* length / 2 */
 #define EH_D_RFLEN_TOTAL 8 /* element length (VOLT=0 drift) */
 
+/* A run of the tracking kernel's program: records [first, first + len) of
+ * the run table's record array, executed reps times in a row, standing for
+ * the flat beamline's positions [flat, flat + len * reps).  Repeated cells
+ * whose records differ only in their positions (z_start, z_end, elem) share
+ * one body. */
+typedef struct {
+  int32_t first, len, reps, flat;
+} eh_run;
+
 /* batched variants: one element program and one particle range per variant,
  * processed as chunks pulled from a queue by a persistent kernel */
 typedef struct {

@@ -80,12 +80,12 @@ DEV void aperture_loss(particle &p, double xmax, double ymax, int is_out, bool e
  * elimit_amplitudes or the open-sided limit_amplitudes instead of the
  * rectangle. */
 template <bool SH>
-DEV int post_element_checks(particle &p, const eh_op &op) {
+DEV int post_element_checks(particle &p, const eh_op &op, const double *z) {
   if (op.flags & EH_F_ALONE)
     return 0;
   if (SH && (op.flags & EH_F_AP) && op.ap_code) {
     if (ap_maxamp_shaped(p.x, p.xp, p.y, p.yp, p.s, p.dp, op.ap_xmax, op.ap_ymax, op.ap_code,
-                         (op.flags & EH_F_EXTRAP) != 0, op.z_end, op.Po))
+                         (op.flags & EH_F_EXTRAP) != 0, opc(&z[1]), op.Po))
       return EH_STAGE_MAXAMP + 1;
   } else if (op.flags & EH_F_AP) {
     double xmax = op.ap_xmax, ymax = op.ap_ymax;
@@ -97,7 +97,7 @@ DEV int post_element_checks(particle &p, const eh_op &op) {
     if (__builtin_isinf(p.x) || __builtin_isinf(p.y) || __builtin_isnan(p.x) || __builtin_isnan(p.y))
       is_out += 4;
     if (is_out) {
-      aperture_loss(p, xmax, ymax, is_out, (op.flags & EH_F_EXTRAP) != 0, op.z_end, op.Po);
+      aperture_loss(p, xmax, ymax, is_out, (op.flags & EH_F_EXTRAP) != 0, opc(&z[1]), op.Po);
       return EH_STAGE_MAXAMP + 1;
     }
   }
@@ -111,7 +111,7 @@ DEV int post_element_checks(particle &p, const eh_op &op) {
     if (__builtin_isinf(p.x) || __builtin_isinf(p.y) || __builtin_isnan(p.x) || __builtin_isnan(p.y))
       is_out += 4;
     if (is_out) {
-      aperture_loss(p, big, big, is_out, false, op.z_end, op.Po);
+      aperture_loss(p, big, big, is_out, false, opc(&z[1]), op.Po);
       return EH_STAGE_NAN + 1;
     }
   }
@@ -496,7 +496,7 @@ DEV int multipole_entry(const particle &p, double &xp, double &yp, double &qx, d
  * integrate_kick_multipole_ordn (multipole.c:1122-1410), exit misalignment.
  * Returns 0 if alive, EH_STAGE_ELEMENT + 1 if lost (row frozen). */
 template <bool SH>
-DEV int track_multipole(particle &p, const eh_op &op) {
+DEV int track_multipole(particle &p, const eh_op &op, const double *z) {
   const int flags = op.flags;
   const bool expand = (flags & EH_F_EXPAND) != 0;
   if (flags & EH_F_OFFSET)
@@ -566,7 +566,7 @@ DEV int track_multipole(particle &p, const eh_op &op) {
       p.y = y;
     }
     /* lost == 2: row as received; lost == 3: row fully written */
-    p.s = op.z_start + dzLoss;
+    p.s = opc(&z[0]) + dzLoss;
     p.dp = op.Po * (1 + p.dp);
   }
   /* exit misalignment, applied to lost rows too (multipole.c:1103-1104) */
@@ -765,7 +765,7 @@ DEV bool csbend_body_fast(double &X, double &QX, double &Y, double &QY, const do
 }
 
 template <bool SH>
-DEV int track_csbend(particle &p, const eh_op &op) {
+DEV int track_csbend(particle &p, const eh_op &op, const double *z) {
   const int flags = op.flags;
   const double ct = op.d[EH_D_CT], st = op.d[EH_D_ST];
   const double rho0 = op.d[EH_D_RHO0], rhoa = op.d[EH_D_RHOA];
@@ -835,7 +835,7 @@ DEV int track_csbend(particle &p, const eh_op &op) {
     p.y = X * st + Y * ct;
     p.xp = QX * ct - QY * st;
     p.yp = QX * st + QY * ct;
-    p.s = op.z_start + dz_lost;
+    p.s = opc(&z[0]) + dz_lost;
     p.dp = op.Po * (1 + DP);
     return EH_STAGE_ELEMENT + 1;
   }
@@ -894,7 +894,7 @@ DEV void add_energy(particle &p, double tof, double Po, double dgamma) {
 
 /* trackRfCavityWithWakes for one particle, forward tracking, no wakes
  * (simple_rfca.c:557-984) followed by removeInvalidParticles (:975) */
-DEV int track_rfca(particle &p, const eh_op &op) {
+DEV int track_rfca(particle &p, const eh_op &op, const double *z) {
   const double Po = op.Po;
   const double volt = op.d[EH_D_VOLT], omega = op.d[EH_D_OMEGA], phase = op.d[EH_D_PHASE];
   const double length = op.d[EH_D_LENGTH], dtLight = op.d[EH_D_DTLIGHT];
@@ -984,15 +984,18 @@ DEV int track_rfca(particle &p, const eh_op &op) {
   /* removeInvalidParticles (limit_amplitudes.c:274-320) */
   if (__builtin_isnan(p.x) || __builtin_isnan(p.xp) || __builtin_isnan(p.y) || __builtin_isnan(p.yp) ||
       __builtin_isnan(p.s) || __builtin_isnan(p.dp) || p.dp <= -1) {
-    p.s = op.z_end;
+    p.s = opc(&z[1]);
     p.dp = Po * (1 + p.dp);
     return EH_STAGE_ELEMENT + 1;
   }
   return 0;
 }
 
+/* One element on one particle.  z points at the element's (z_start, z_end),
+ * which only the loss paths read: in the element record itself, or in the
+ * position table of a program whose repeated cells share one record. */
 template <bool SH>
-DEV int run_op(particle &p, const eh_op &op) {
+DEV int run_op(particle &p, const eh_op &op, const double *z) {
   int lost = 0;
   switch (op.op) {
   case EH_OP_DRIFT2:
@@ -1008,13 +1011,13 @@ DEV int run_op(particle &p, const eh_op &op) {
     exact_drift(p, op.d[0]);
     break;
   case EH_OP_MULT:
-    lost = track_multipole<SH>(p, op);
+    lost = track_multipole<SH>(p, op, z);
     break;
   case EH_OP_CSBEND:
-    lost = track_csbend<SH>(p, op);
+    lost = track_csbend<SH>(p, op, z);
     break;
   case EH_OP_RFCA:
-    lost = track_rfca(p, op);
+    lost = track_rfca(p, op, z);
     break;
   case EH_OP_RFCA_DRIFT:
     /* VOLT=0: a drift when the cavity has length, else nothing (simple_rfca.c:389-402) */
@@ -1029,7 +1032,7 @@ DEV int run_op(particle &p, const eh_op &op) {
   }
   if (lost)
     return lost;
-  return post_element_checks<SH>(p, op);
+  return post_element_checks<SH>(p, op, z);
 }
 
 /* Four waves per SIMD (up to 128 VGPRs): the branch-free bend body needs
@@ -1038,11 +1041,19 @@ DEV int run_op(particle &p, const eh_op &op) {
  * storage ring (29.0-29.3 M against 29.4 M particle-turns/s).
  * SH: built with the elliptical / open-sided MAXAMP checks; launched only for
  * programs that have such an aperture, so the rectangle-only instance keeps
- * the registers and schedule of the common case. */
+ * the registers and schedule of the common case.
+ * The beamline of nflat elements arrives as runs (eh_run): a body of
+ * records executed reps times in a row, so a ring of identical cells keeps
+ * one cell's records hot in the scalar cache instead of streaming every
+ * element's record through it each pass.  Loss codes and op_first/op_last
+ * count positions in the flat beamline; zpos holds (z_start, z_end) for
+ * each of them. */
 template <bool SH>
-__global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops, int32_t nops,
-                                                       int64_t pass0, int32_t npasses, int32_t op_first,
-                                                       int32_t op_last) {
+__global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_op *__restrict__ ops,
+                                                       const eh_run *__restrict__ runs, int32_t nruns,
+                                                       const double *__restrict__ zpos, int64_t pass0,
+                                                       int32_t npasses, int32_t op_first, int32_t op_last,
+                                                       int32_t nflat) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n)
     return;
@@ -1058,14 +1069,28 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
   int lost = 0, lost_op = 0;
   int64_t lost_pass = 0;
   for (int32_t ip = 0; ip < npasses && !lost; ip++) {
+    /* element positions [ob, oe) of the flat beamline in this pass */
     const int32_t ob = (ip == 0) ? op_first : 0;
-    const int32_t oe = (ip == npasses - 1) ? op_last : nops;
-    for (int32_t io = ob; io < oe; io++) {
-      lost = run_op<SH>(p, ops[io]);
-      if (lost) {
-        lost_op = io;
-        lost_pass = pass0 + ip;
+    const int32_t oe = (ip == npasses - 1) ? op_last : nflat;
+    for (int32_t r = 0; r < nruns && !lost; r++) {
+      const int32_t fs = runs[r].flat, len = runs[r].len, first = runs[r].first;
+      const int32_t fe = fs + len * runs[r].reps;
+      if (fe <= ob)
+        continue;
+      if (fs >= oe)
         break;
+      int32_t f = fs > ob ? fs : ob;
+      const int32_t f1 = fe < oe ? fe : oe;
+      int32_t k = (f - fs) % len;
+      for (; f < f1; f++) {
+        lost = run_op<SH>(p, ops[first + k], zpos + 2 * f);
+        if (lost) {
+          lost_op = f;
+          lost_pass = pass0 + ip;
+          break;
+        }
+        if (++k == len)
+          k = 0;
       }
     }
   }
@@ -1139,7 +1164,7 @@ __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op 
         for (int k = 0; k < PPL; k++) {
           if (!live[k])
             continue;
-          int l = run_op<SH>(p[k], op);
+          int l = run_op<SH>(p[k], op, &op.z_start);
           if (l) {
             live[k] = false;
             lost[k] = l;
@@ -1246,11 +1271,12 @@ void ehd_destroy(ehd_dev *d) {
   void *bufs[] = {d->coord[0], d->coord[1], d->coord[2], d->coord[3], d->coord[4], d->coord[5],
                   d->loss_pass, d->loss_code, d->ops, d->counter, d->vdesc, d->chunks, d->perm, d->queue,
                   d->snap[0], d->snap[1], d->snap[2], d->snap[3], d->snap[4], d->snap[5], d->snap_lp, d->snap_lc,
-                  d->tile_sum};
+                  d->tile_sum, d->tops, d->runs, d->zpos};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
     if (bufs[k])
       (void)hipFree(bufs[k]);
   ehd_free_collective(d);
+  free(d->host_prog);
   (void)hipEventDestroy(d->ev0);
   (void)hipEventDestroy(d->ev1);
   (void)hipEventDestroy(d->cev0);
@@ -1322,6 +1348,8 @@ int ehd_get_sdp(ehd_dev *d, double *s, double *dp, int32_t *loss_pass, int64_t n
   return ehd_get_state(d, NULL, NULL, NULL, NULL, s, dp, loss_pass, NULL, n);
 }
 
+static int tops_shaped(const eh_op *op) { return op->ap_code || op->in_code; }
+
 int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops) {
   HIPCHK(d, hipSetDevice(d->device));
   if (nops > d->ops_cap) {
@@ -1339,21 +1367,108 @@ int ehd_put_ops(ehd_dev *d, const eh_op *ops, int32_t nops) {
    * instance that evaluates those shapes */
   d->shaped = 0;
   for (int32_t k = 0; k < nops; k++)
-    if (ops[k].ap_code || ops[k].in_code)
+    if (tops_shaped(&ops[k]))
       d->shaped = 1;
   return EH_OK;
 }
 
-int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op) {
-  if (index < 0 || index >= d->nops)
-    return EH_ERR_INVALID;
-  HIPCHK(d, hipSetDevice(d->device));
-  HIPCHK(d, hipMemcpyAsync(d->ops + index, op, sizeof(eh_op), hipMemcpyHostToDevice, d->stream));
-  HIPCHK(d, hipStreamSynchronize(d->stream));
-  if (op->ap_code || op->in_code)
-    d->shaped = 1;
+/* grows a device buffer to hold n elements of size sz (contents dropped) */
+static int grow(ehd_dev *d, void **buf, int32_t *cap, int32_t n, size_t sz) {
+  if (n <= *cap)
+    return EH_OK;
+  if (*buf)
+    HIPCHK(d, hipFree(*buf));
+  *buf = NULL;
+  *cap = 0;
+  HIPCHK(d, hipMalloc(buf, sz * (size_t)(n > 0 ? n : 1)));
+  *cap = n;
   return EH_OK;
 }
+
+/* folds the host copy into runs and uploads bodies, runs and positions */
+static int upload_program(ehd_dev *d) {
+  const int32_t n = d->nflat;
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  eh_run *runs = (eh_run *)malloc(sizeof(eh_run) * m);
+  int32_t *body = (int32_t *)malloc(sizeof(int32_t) * m);
+  eh_op *tops = (eh_op *)malloc(sizeof(eh_op) * m);
+  double *zpos = (double *)malloc(sizeof(double) * 2 * m);
+  int32_t nb = 0, nr = -1;
+  int rc = EH_OK;
+  if (runs && body && tops && zpos)
+    nr = eh_fold_runs(d->host_prog, n, EH_RUN_MAX_LEN, runs, body, &nb);
+  if (nr < 0) {
+    rc = EH_ERR_NOMEM;
+    snprintf(d->msg, sizeof d->msg, "out of host memory folding the program");
+  }
+  if (!rc) {
+    for (int32_t j = 0; j < nb; j++)
+      tops[j] = d->host_prog[body[j]];
+    for (int32_t f = 0; f < n; f++) {
+      zpos[2 * f] = d->host_prog[f].z_start;
+      zpos[2 * f + 1] = d->host_prog[f].z_end;
+    }
+    rc = grow(d, (void **)&d->tops, &d->tops_cap, nb, sizeof(eh_op));
+  }
+  if (!rc)
+    rc = grow(d, (void **)&d->runs, &d->runs_cap, nr, sizeof(eh_run));
+  if (!rc)
+    rc = grow(d, (void **)&d->zpos, &d->zpos_cap, 2 * n, sizeof(double));
+  hipError_t e = hipSuccess;
+  if (!rc && nb > 0)
+    e = hipMemcpyAsync(d->tops, tops, sizeof(eh_op) * nb, hipMemcpyHostToDevice, d->stream);
+  if (!rc && !e && nr > 0)
+    e = hipMemcpyAsync(d->runs, runs, sizeof(eh_run) * nr, hipMemcpyHostToDevice, d->stream);
+  if (!rc && !e && n > 0)
+    e = hipMemcpyAsync(d->zpos, zpos, sizeof(double) * 2 * n, hipMemcpyHostToDevice, d->stream);
+  /* the host buffers are the copies' sources: wait before freeing them */
+  hipError_t es = hipStreamSynchronize(d->stream);
+  if (!rc && (e || es))
+    rc = hip_fail(d, e ? e : es, "uploading the tracking program");
+  free(runs);
+  free(body);
+  free(tops);
+  free(zpos);
+  if (rc) {
+    d->nruns = d->ntops = 0;
+    d->nflat = 0;
+    return rc;
+  }
+  d->nruns = nr;
+  d->ntops = nb;
+  d->tshaped = 0;
+  for (int32_t f = 0; f < n; f++)
+    if (tops_shaped(&d->host_prog[f]))
+      d->tshaped = 1;
+  return EH_OK;
+}
+
+int ehd_put_program(ehd_dev *d, const eh_op *ops, int32_t nops) {
+  if (nops < 0 || (nops > 0 && !ops))
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  if (nops > d->host_prog_cap) {
+    eh_op *h = (eh_op *)realloc(d->host_prog, sizeof(eh_op) * (size_t)nops);
+    if (!h)
+      return EH_ERR_NOMEM;
+    d->host_prog = h;
+    d->host_prog_cap = nops;
+  }
+  if (nops > 0)
+    memcpy(d->host_prog, ops, sizeof(eh_op) * (size_t)nops);
+  d->nflat = nops;
+  return upload_program(d);
+}
+
+int ehd_patch_op(ehd_dev *d, int32_t index, const eh_op *op) {
+  if (index < 0 || index >= d->nflat)
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  d->host_prog[index] = *op;
+  return upload_program(d);
+}
+
+int32_t ehd_program_runs(const ehd_dev *d) { return d->nruns; }
 
 /* 64, 128, 192 or 256 lanes per workgroup; 0 = chosen per launch */
 void ehd_set_block(ehd_dev *d, int32_t block) {
@@ -1364,7 +1479,7 @@ void ehd_set_block(ehd_dev *d, int32_t block) {
 int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int32_t op_last) {
   if (npasses <= 0 || d->n == 0)
     return EH_OK;
-  if (op_first < 0 || op_last > d->nops || op_first > d->nops || (npasses == 1 && op_first > op_last))
+  if (op_first < 0 || op_last > d->nflat || op_first > d->nflat || (npasses == 1 && op_first > op_last))
     return EH_ERR_INVALID;
   HIPCHK(d, hipSetDevice(d->device));
   eh_state st;
@@ -1392,12 +1507,12 @@ int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int3
   }
   const int64_t grid = (d->n + block - 1) / block;
   HIPCHK(d, hipEventRecord(d->ev0, d->stream));
-  if (d->shaped)
-    hipLaunchKernelGGL(eh_track_kernel<true>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops,
-                       pass0, npasses, op_first, op_last);
+  if (d->tshaped)
+    hipLaunchKernelGGL(eh_track_kernel<true>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->tops, d->runs,
+                       d->nruns, d->zpos, pass0, npasses, op_first, op_last, d->nflat);
   else
-    hipLaunchKernelGGL(eh_track_kernel<false>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->ops, d->nops,
-                       pass0, npasses, op_first, op_last);
+    hipLaunchKernelGGL(eh_track_kernel<false>, dim3((unsigned)grid), dim3(block), 0, d->stream, st, d->tops, d->runs,
+                       d->nruns, d->zpos, pass0, npasses, op_first, op_last, d->nflat);
   HIPCHK(d, hipGetLastError());
   HIPCHK(d, hipEventRecord(d->ev1, d->stream));
   HIPCHK(d, hipEventSynchronize(d->ev1));

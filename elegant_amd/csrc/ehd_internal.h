@@ -30,8 +30,16 @@ struct ehd_dev {
   double *coord[6];
   int32_t *loss_pass, *loss_code;
   int64_t n, cap;
-  eh_op *ops;
+  eh_op *ops;          /* flat records of the batch kernel's programs */
   int32_t nops, ops_cap;
+  /* the tracking kernel's program (ehd_put_program) */
+  eh_op *host_prog;    /* flat copy on the host: patches refold it */
+  int32_t nflat, host_prog_cap;
+  eh_op *tops;         /* run bodies */
+  eh_run *runs;
+  double *zpos;        /* (z_start, z_end) of each flat position */
+  int32_t ntops, nruns, tops_cap, runs_cap, zpos_cap;
+  int tshaped;         /* the tracking program has an elliptical or open-sided MAXAMP */
   int32_t block;
   unsigned long long *counter;
   double track_ms, coll_ms;

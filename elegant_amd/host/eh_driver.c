@@ -222,7 +222,7 @@ int eh_load_program(eh_ctx *c, const eh_element *elems, int64_t n_elems, double 
       eh_rfca_set_phase(op, rf, c->phase_fid[k]);
     }
   }
-  rc = ehd_put_ops(c->dev, c->prog->ops, c->prog->nops);
+  rc = ehd_put_program(c->dev, c->prog->ops, c->prog->nops);
   if (rc)
     return devfail(c, rc);
   c->loaded = 1;
@@ -889,7 +889,7 @@ static int64_t run_single(eh_ctx *c, double **rows, int64_t n, double **accepted
   c->prog->ops[0].z_end = z_start + eh_element_length(e);
   c->Po = Po;
   c->loaded = 1;
-  rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+  rc = ehd_put_program(c->dev, c->prog->ops, 1);
   if (rc)
     return devfail(c, rc);
   rc = eh_upload_rows(c, rows, n, 8, 0);
@@ -981,7 +981,7 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     c->prog->ops[0].z_end = zEnd;
     c->Po = *P_central;
     c->loaded = 1;
-    rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+    rc = ehd_put_program(c->dev, c->prog->ops, 1);
     if (rc)
       return devfail(c, rc);
     rc = eh_upload_rows(c, part, np, 8, 0);
@@ -1021,7 +1021,7 @@ static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, double **acce
   c->prog->nops = 1;
   c->Po = Po;
   c->loaded = 1;
-  rc = ehd_put_ops(c->dev, c->prog->ops, 1);
+  rc = ehd_put_program(c->dev, c->prog->ops, 1);
   if (rc)
     return devfail(c, rc);
   /* NaN and infinite rows are the check's own to remove, at z */
@@ -1102,6 +1102,44 @@ int eh_test_buffer(eh_ctx *c, int32_t op, void **ptr, void *host, int64_t bytes)
   int rc = ehd_test_buffer(c->dev, op, ptr, host, bytes);
   return rc ? devfail(c, rc) : EH_OK;
 }
+
+/* the runs the tracking kernel walks for a beamline: compiles it on the host
+ * and folds it (eh_runs.c) into out[4 k .. 4 k + 3] = first, len, reps,
+ * flat of run k, for at most cap runs; returns the number of runs or a
+ * negative EH_ERR_* code (not in the public header) */
+int32_t eh_test_fold(const eh_element *bl, int64_t n_elems, double Po, int32_t *out, int32_t cap) {
+  eh_program *prog = (eh_program *)malloc(sizeof(eh_program));
+  char msg[256];
+  int32_t nr = EH_ERR_NOMEM, nb = 0;
+  if (!prog)
+    return EH_ERR_NOMEM;
+  int rc = eh_compile(bl, n_elems, Po, prog, msg, sizeof msg);
+  if (rc) {
+    free(prog);
+    return rc;
+  }
+  const size_t m = (size_t)(prog->nops > 0 ? prog->nops : 1);
+  eh_run *runs = (eh_run *)malloc(sizeof(eh_run) * m);
+  int32_t *body = (int32_t *)malloc(sizeof(int32_t) * m);
+  if (runs && body)
+    nr = eh_fold_runs(prog->ops, prog->nops, EH_RUN_MAX_LEN, runs, body, &nb);
+  if (nr < 0)
+    nr = EH_ERR_NOMEM;
+  for (int32_t k = 0; k < nr && k < cap; k++) {
+    out[4 * k] = runs[k].first;
+    out[4 * k + 1] = runs[k].len;
+    out[4 * k + 2] = runs[k].reps;
+    out[4 * k + 3] = runs[k].flat;
+  }
+  free(runs);
+  free(body);
+  free(prog);
+  return nr;
+}
+
+/* how many runs the loaded tracking program has on the device (not in the
+ * public header) */
+int32_t eh_test_program_runs(eh_ctx *c) { return ehd_program_runs(c->dev); }
 
 /* accessors used by the collective step (eh_collective.c) */
 ehd_dev *eh_ctx_dev(eh_ctx *c) { return c->dev; }

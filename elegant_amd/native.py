@@ -100,6 +100,8 @@ EXTRA = [
     ("eh_get_loss_records", C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("eh_math_probe", C.c_int, [P, C.c_int32, PD, PD, PD, C.c_int64]),
     ("eh_test_buffer", C.c_int, [P, C.c_int32, C.POINTER(C.c_void_p), C.c_void_p, C.c_int64]),
+    ("eh_test_fold", C.c_int32, [PE, C.c_int64, C.c_double, C.POINTER(C.c_int32), C.c_int32]),
+    ("eh_test_program_runs", C.c_int32, [P]),
     ("eh_set_chunk_passes", None, [P, C.c_int32]),
     ("eh_set_block", None, [P, C.c_int32]),
     ("eh_collective_probe", C.c_int, [P, PP, C.c_int64, PE, C.c_double, C.c_int64, C.c_double, PD, PD,
@@ -220,6 +222,17 @@ def da_trim_area(x, y, dxf, dyf, full_plane=0):
     return a, x, y
 
 
+def program_runs(beamline, Po):
+    """The runs the tracking kernel walks for this beamline, as (first, len,
+    reps, flat) tuples: repeated cells fold into one body of records."""
+    cap = max(len(beamline), 1) * 2 + 16
+    out = (C.c_int32 * (4 * cap))()
+    nr = lib().eh_test_fold(beamline.pointer(), len(beamline), Po, out, cap)
+    if nr < 0:
+        raise EHError("eh_test_fold failed (%d)" % nr)
+    return [tuple(out[4 * k:4 * k + 4]) for k in range(min(nr, cap))]
+
+
 class Tracker:
     """One device context: a loaded beamline and a particle population.
 
@@ -314,6 +327,10 @@ class Tracker:
         check(self._ctx, lib().eh_get_loss_records(self._ctx, p.ctypes.data_as(C.POINTER(C.c_int32)),
                                                    c.ctypes.data_as(C.POINTER(C.c_int32))))
         return p, c
+
+    def program_runs(self):
+        """runs of the tracking program now on the device"""
+        return int(lib().eh_test_program_runs(self._ctx))
 
     def kernel_timing(self):
         n = C.c_int64()

@@ -92,3 +92,28 @@ def test_example_deck_bit_exact_vs_emath_checker(tracker):
     assert ng == nc and eg == o.effort()
     assert rows.shape[0] - ng > 10
     assert_rows_identical(g, c, "parTrack deck")
+
+
+def test_repeated_cells_with_cavities_refold(tracker):
+    """twelve identical cells, each with its own RF cavity whose TMEAN
+    fiducial is set on the first pass from the beam's mean arrival time
+    there: before the fiducials are known the cells fold into one body of
+    records; each fiducial then makes its cavity's record differ, and the
+    program refolds to the flat beamline.  Rows, order, losses and effort
+    identical to the checker, with losses at every kind of element"""
+    from elegant_amd import elements as E
+    from elegant_amd import native
+    els = [E.MAXAMP("AP", X_MAX=0.02, Y_MAX=0.02)]
+    for _ in range(12):
+        els += [E.DRIF("D1", L=2.0), E.KQUAD("QF", L=0.5, K1=1.2, N_KICKS=4), E.KSEXT("SF", L=0.2, K2=20.0),
+                E.DRIF("D2", L=2.0), E.KQUAD("QD", L=0.5, K1=-1.2, N_KICKS=4), E.KSEXT("SD", L=0.2, K2=-35.0),
+                E.RFCA("RF", L=0.0, VOLT=2e5, FREQ=352e6, PHASE=180.0)]
+    bl = E.Beamline(els)
+    _, Po, rows = lossy_fodo(1200, seed=33)
+    assert native.program_runs(bl, Po) == [(0, 1, 1, 0), (1, 7, 12, 1)]
+    g, ng, eg, c, nc, ec = run_both(tracker, bl, Po, rows, 8, "emath")
+    print("cells with cavities: survivors", ng, "lost", rows.shape[0] - ng, "runs", tracker.program_runs())
+    assert tracker.program_runs() == 1
+    assert ng == nc and eg == ec
+    assert rows.shape[0] - ng > 100
+    assert_rows_identical(g, c, "cells with cavities")
