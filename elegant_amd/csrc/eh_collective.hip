@@ -36,6 +36,7 @@
 
 #define DEV static __device__ __forceinline__
 #define CB 256 /* threads per workgroup for particle sweeps */
+#define SWEEP_U 4 /* particles per lane per step of the grid-stride sweeps */
 
 DEV double sqd(double v) { return v * v; }
 
@@ -154,7 +155,28 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
                                                   unsigned long long *red) {
   __shared__ unsigned long long smin[CB / 64], smax[CB / 64], scnt[CB / 64];
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
+  const int64_t stride = (int64_t)gridDim.x * CB;
+  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  /* SWEEP_U particles per lane per step, their loads issued together */
+  for (; i + (SWEEP_U - 1) * stride < n; i += SWEEP_U * stride) {
+    double S[SWEEP_U], D[SWEEP_U];
+    bool live[SWEEP_U];
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++) {
+      live[u] = !lp || lp[i + u * stride] < 0;
+      S[u] = s[i + u * stride];
+      D[u] = dp[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++)
+      if (live[u]) {
+        unsigned long long k = order_key(arrival_time(S[u], D[u], Po));
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+        cnt++;
+      }
+  }
+  for (; i < n; i += stride) {
     if (!lp || lp[i] < 0) {
       unsigned long long k = order_key(arrival_time(s[i], dp[i], Po));
       kmin = k < kmin ? k : kmin;
@@ -197,7 +219,23 @@ __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, co
   const int32_t e = plan->e;
   eh_fsum a;
   eh_fsum_init(&a, e);
-  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB)
+  const int64_t stride = (int64_t)gridDim.x * CB;
+  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  for (; i + (SWEEP_U - 1) * stride < n; i += SWEEP_U * stride) {
+    double S[SWEEP_U], D[SWEEP_U];
+    bool live[SWEEP_U];
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++) {
+      live[u] = !lp || lp[i + u * stride] < 0;
+      S[u] = s[i + u * stride];
+      D[u] = dp[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++)
+      if (live[u])
+        eh_fsum_add(&a, arrival_time(S[u], D[u], Po));
+  }
+  for (; i < n; i += stride)
     if (!lp || lp[i] < 0)
       eh_fsum_add(&a, arrival_time(s[i], dp[i], Po));
   sh[threadIdx.x] = a;
@@ -261,7 +299,26 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
       lh[k] = 0;
     __syncthreads();
   }
-  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
+  const int64_t stride = (int64_t)gridDim.x * CB;
+  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  for (; i + (SWEEP_U - 1) * stride < n; i += SWEEP_U * stride) {
+    double S[SWEEP_U], D[SWEEP_U];
+    bool live[SWEEP_U];
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++) {
+      live[u] = !lp || lp[i + u * stride] < 0;
+      S[u] = s[i + u * stride];
+      D[u] = dp[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < SWEEP_U; u++)
+      if (live[u]) {
+        int64_t ib = bin_of(arrival_time(S[u], D[u], Po), tmin, dt, round_half);
+        if (ib >= 0 && ib <= nb - 1)
+          atomicAdd(&h[ib], 1u);
+      }
+  }
+  for (; i < n; i += stride) {
     if (lp && lp[i] >= 0)
       continue;
     int64_t ib = bin_of(arrival_time(s[i], dp[i], Po), tmin, dt, round_half);
@@ -282,73 +339,115 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
 
 /* convolveArrays with di2 = 0 (wake.c:479-495): V[ib] = sum_{j} I[j] W[ib-j]
  * over j = max(0, ib - wpoints + 1) .. ib, summed in increasing j, then
- * V *= factor (wake.c:258-260).  One lane per output bin;
- * the counts and the window of W a workgroup needs are staged through LDS
- * in tiles of CONV_T counts, which keeps every lane's sum in the reference
- * order. */
+ * V *= factor (wake.c:258-260).  One lane per output bin.  The counts (as
+ * doubles) and the window of W a workgroup needs are staged through LDS in
+ * tiles of CONV_T counts, with zeros wherever W is outside [0, wpoints) and
+ * past the last count.  Every lane then runs the tile's whole index range
+ * j = t0 .. t1 in increasing j: a term outside the lane's own range is
+ * count x 0 = +0, and adding +0 leaves the sum unchanged (it starts at +0
+ * and is never -0), so each lane's sum is the reference's, and the index is
+ * the same in every lane, which makes the count reads broadcasts. */
 #define CONV_T 1024
-__global__ void __launch_bounds__(CB) wake_convolve(const unsigned int *__restrict__ hist,
-                                                    const double *__restrict__ W, int64_t wpoints,
-                                                    const coll_plan *plan, double factor, double *volt) {
-  __shared__ unsigned int hs[CONV_T];
-  __shared__ double ws[CONV_T + CB];
+#define CONV_B 64   /* output bins (lanes) per workgroup: one wave, so that a wave has its CU's LDS to itself */
+#define CONV_PAD 48 /* zeros below the W window: the last steps of a tile and the loads two
+                       steps ahead reach up to 39 below */
+#define CONV_HPAD 48 /* zeros past the counts, read only by the loads ahead */
+__global__ void __launch_bounds__(CONV_B) wake_convolve(const unsigned int *__restrict__ hist,
+                                                        const double *__restrict__ W, int64_t wpoints,
+                                                        const coll_plan *plan, double factor, double *volt) {
+  __shared__ double hs[CONV_T + CONV_HPAD];
+  __shared__ double ws[CONV_PAD + CONV_T + CONV_B];
   const int32_t nb = plan->nb;
-  const int b0 = blockIdx.x * CB;
+  const int b0 = blockIdx.x * CONV_B;
   if (plan->err || nb <= 0 || b0 > nb)
     return;
-  const int ib = b0 + (int)threadIdx.x;
+  const int tid = (int)threadIdx.x;
+  const int ib = b0 + tid;
   const int wp = (int)wpoints;
-  const int last = (b0 + CB - 1 < nb - 1) ? b0 + CB - 1 : nb - 1;
+  const int last = (b0 + CONV_B - 1 < nb - 1) ? b0 + CONV_B - 1 : nb - 1;
   const int first = b0 - wp + 1 > 0 ? b0 - wp + 1 : 0;
+  if (tid < CONV_PAD)
+    ws[tid] = 0.0;
+  if (tid < CONV_HPAD)
+    hs[CONV_T + tid] = 0.0; /* read only by the loads ahead */
+  /* staging of a tile [t0, t1]: every global load issued before the first
+   * is used, with indices clamped into the arrays so that no load is
+   * conditional (the values outside the ranges become zeros when stored);
+   * the next tile's loads are in flight while the current tile is summed */
+  unsigned int hv[CONV_T / CONV_B];
+  double wv[(CONV_T + CONV_B) / CONV_B];
+  auto stage_load = [&](int t0) {
+    const int t1 = (t0 + CONV_T - 1 < last) ? t0 + CONV_T - 1 : last;
+    const int len = t1 - t0 + 1, w_lo = b0 - t1;
+#pragma unroll
+    for (int r = 0; r < CONV_T / CONV_B; r++) {
+      const int k = tid + r * CONV_B;
+      hv[r] = hist[t0 + (k < len ? k : len - 1)];
+    }
+#pragma unroll
+    for (int r = 0; r < (CONV_T + CONV_B) / CONV_B; r++) {
+      const int w = w_lo + tid + r * CONV_B;
+      wv[r] = W[w < 0 ? 0 : (w < wp ? w : wp - 1)];
+    }
+  };
+  stage_load(first);
   double v = 0;
   for (int t0 = first; t0 <= last; t0 += CONV_T) {
     const int t1 = (t0 + CONV_T - 1 < last) ? t0 + CONV_T - 1 : last;
-    const int w_lo = b0 - t1; /* W index of ws[0] */
-    for (int k = threadIdx.x; k <= t1 - t0; k += CB)
-      hs[k] = hist[t0 + k];
-    for (int k = threadIdx.x; k < t1 - t0 + CB; k += CB) {
-      int w = w_lo + k;
-      ws[k] = (w >= 0 && w < wp) ? W[w] : 0.0;
+    const int len = t1 - t0 + 1;
+    const int lenp = (len + 23) / 24 * 24;
+    const int w_lo = b0 - t1; /* W index of ws[CONV_PAD] */
+#pragma unroll
+    for (int r = 0; r < CONV_T / CONV_B; r++) {
+      const int k = tid + r * CONV_B;
+      hs[k] = k < len ? (double)hv[r] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < (CONV_T + CONV_B) / CONV_B; r++) {
+      const int w = w_lo + tid + r * CONV_B;
+      ws[CONV_PAD + tid + r * CONV_B] = (w >= 0 && w < wp) ? wv[r] : 0.0;
     }
     __syncthreads();
-    if (ib < nb) {
-      int lo = ib - wp + 1 > t0 ? ib - wp + 1 : t0;
-      int hi = ib < t1 ? ib : t1;
-      int j = lo;
-      /* eight terms per step, software-pipelined: the next eight pairs are
-       * loaded while the current eight are added, the additions stay in
-       * order; a zero count adds +-0, which leaves the sum (never -0)
-       * unchanged, exactly as the reference's unconditional loop */
-      if (j + 7 <= hi) {
-        double c[8], w[8];
+    if (t0 + CONV_T <= last)
+      stage_load(t0 + CONV_T);
+    /* term t0 + q of lane tid uses W[ib - t0 - q] = ws[CONV_PAD + tid + len - 1 - q];
+     * 24 terms per step in three groups of eight, each group's operands
+     * loaded two groups before it is added (the loads ahead stay inside hs
+     * and ws, in their zero padding) */
+    const double *wq = ws + CONV_PAD + tid + len - 1;
+    double ca[8], wa[8], cb[8], wb[8], cc[8], wc[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          c[u] = (double)hs[j + u - t0];
-          w[u] = ws[ib - (j + u) - w_lo];
-        }
-        for (; j + 15 <= hi; j += 8) {
-          double c2[8], w2[8];
+    for (int u = 0; u < 8; u++) {
+      ca[u] = hs[u];
+      wa[u] = wq[-u];
+      cb[u] = hs[8 + u];
+      wb[u] = wq[-(8 + u)];
+    }
+    for (int q = 0; q < lenp; q += 24) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            c2[u] = (double)hs[j + 8 + u - t0];
-            w2[u] = ws[ib - (j + 8 + u) - w_lo];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; u++)
-            v += c[u] * w[u];
-#pragma unroll
-          for (int u = 0; u < 8; u++) {
-            c[u] = c2[u];
-            w[u] = w2[u];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-          v += c[u] * w[u];
-        j += 8;
+      for (int u = 0; u < 8; u++) {
+        cc[u] = hs[q + 16 + u];
+        wc[u] = wq[-(q + 16 + u)];
       }
-      for (; j <= hi; j++)
-        v += (double)hs[j - t0] * ws[ib - j - w_lo];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        v += ca[u] * wa[u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        ca[u] = hs[q + 24 + u];
+        wa[u] = wq[-(q + 24 + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        v += cb[u] * wb[u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        cb[u] = hs[q + 32 + u];
+        wb[u] = wq[-(q + 32 + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        v += cc[u] * wc[u];
     }
     __syncthreads();
   }
@@ -361,118 +460,215 @@ __global__ void __launch_bounds__(CB) wake_convolve(const unsigned int *__restri
 /* packed real FFT, V = Z I factor, inverse FFT (zlongit.c:373-394), for
  * nb = 2m <= 8192 in one workgroup.  Same algorithm and operation order as
  * or_real_fft in the CPU checker: m-point radix-2 DIT complex FFT of
- * (x[2j], x[2j+1]) plus the real split, twiddles cos/sin(2 pi k/nb). */
+ * (x[2j], x[2j+1]) plus the real split, twiddles cos/sin(2 pi k/nb).
+ * Everything stays in LDS: the points, the twiddle tables and, between the
+ * two transforms, the packed spectrum (slot k holds the reference's
+ * (V[2k-1], V[2k]), slot 0 holds (V[0], V[nb-1])).  Each LDS pass applies
+ * two consecutive radix-2 stages to four points held in registers; the
+ * butterflies and their operands are those of the stage-by-stage loop. */
 #define FFT_T 1024
+#define FFT_MAXM 4096
+/* LDS index of point i: one padding word after every 16 and every 256
+ * points, which spreads the strided accesses of the first passes over the
+ * banks (the stage loop's patterns need 3328 bank cycles per 16-lane group
+ * sum instead of 7680) */
+#define FP(i) ((i) + ((i) >> 4) + ((i) >> 8))
+#define FFT_PADM (FP(FFT_MAXM - 1) + 1)
+
+/* one radix-2 butterfly (a, b) -> (a + t, a - t), t = b w or b conj(w) */
+DEV void fft_bfly(double &ar, double &ai, double &br, double &bi, double c, double s, int inverse) {
+  double tr, ti;
+  if (!inverse) {
+    tr = br * c + bi * s;
+    ti = bi * c - br * s;
+  } else {
+    tr = br * c - bi * s;
+    ti = bi * c + br * s;
+  }
+  const double xr = ar, xi = ai;
+  ar = xr + tr;
+  ai = xi + ti;
+  br = xr - tr;
+  bi = xi - ti;
+}
+
+/* the butterfly stages of one m-point transform on (zr, zi), bit-reversed
+ * input; stage h has butterflies (g + j, g + j + h) with the twiddle of
+ * table index j * m / h, which wr/wi hold at h + j (each stage's twiddles
+ * contiguous, so that neighbouring lanes read neighbouring words); two
+ * stages per pass */
+DEV void fft_stages(double *zr, double *zi, const double *wr, const double *wi, int m, int bits, int inverse) {
+  const int tid = (int)threadIdx.x;
+  int lh = 0;
+  for (; lh + 1 < bits; lh += 2) {
+    const int h = 1 << lh;
+    for (int grp = tid; grp < m / 4; grp += FFT_T) {
+      const int j = grp & (h - 1);
+      const int i0 = ((grp >> lh) << (lh + 2)) + j, i1 = i0 + h, i2 = i1 + h, i3 = i2 + h;
+      double r0 = zr[FP(i0)], q0 = zi[FP(i0)], r1 = zr[FP(i1)], q1 = zi[FP(i1)];
+      double r2 = zr[FP(i2)], q2 = zi[FP(i2)], r3 = zr[FP(i3)], q3 = zi[FP(i3)];
+      const int ta = h + j, tb = 2 * h + j, tcix = 3 * h + j;
+      /* stage h: (i0, i1) and (i2, i3), both at twiddle index j m / h */
+      fft_bfly(r0, q0, r1, q1, wr[ta], wi[ta], inverse);
+      fft_bfly(r2, q2, r3, q3, wr[ta], wi[ta], inverse);
+      /* stage 2h: (i0, i2) at j m / 2h and (i1, i3) at (j + h) m / 2h */
+      fft_bfly(r0, q0, r2, q2, wr[tb], wi[tb], inverse);
+      fft_bfly(r1, q1, r3, q3, wr[tcix], wi[tcix], inverse);
+      zr[FP(i0)] = r0;
+      zi[FP(i0)] = q0;
+      zr[FP(i1)] = r1;
+      zi[FP(i1)] = q1;
+      zr[FP(i2)] = r2;
+      zi[FP(i2)] = q2;
+      zr[FP(i3)] = r3;
+      zi[FP(i3)] = q3;
+    }
+    __syncthreads();
+  }
+  if (lh < bits) {
+    /* an odd stage count: the last stage alone */
+    const int h = 1 << lh;
+    for (int b = tid; b < m / 2; b += FFT_T) {
+      const int j = b & (h - 1);
+      const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+      const int t = h + j;
+      double r0 = zr[FP(i0)], q0 = zi[FP(i0)], r1 = zr[FP(i1)], q1 = zi[FP(i1)];
+      fft_bfly(r0, q0, r1, q1, wr[t], wi[t], inverse);
+      zr[FP(i0)] = r0;
+      zi[FP(i0)] = q0;
+      zr[FP(i1)] = r1;
+      zi[FP(i1)] = q1;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(FFT_T) zlongit_fft(const unsigned int *__restrict__ hist,
-                                                     const double *__restrict__ Z, const double *__restrict__ tc,
-                                                     const double *__restrict__ ts, int32_t nb, int32_t bits,
+                                                     const double *__restrict__ Z, const double *__restrict__ tc_g,
+                                                     const double *__restrict__ ts_g, int32_t nb, int32_t bits,
                                                      double factor, const coll_plan *plan, double *volt) {
-  extern __shared__ double sm[];
+  __shared__ double zr[FFT_PADM], zi[FFT_PADM], wr[FFT_MAXM], wi[FFT_MAXM];
   if (plan->nb <= 0 || plan->err)
     return;
   const int m = nb / 2;
-  double *zr = sm, *zi = sm + m;
-  const int tid = threadIdx.x;
-  /* load with bit reversal: z[rev(j)] = (x[2j], x[2j+1]) */
-  for (int j = tid; j < m; j += FFT_T) {
-    int r = (int)(__brev((unsigned)j) >> (32 - bits));
-    zr[r] = (double)hist[2 * j];
-    zi[r] = (double)hist[2 * j + 1];
+  const int tid = (int)threadIdx.x;
+  constexpr int KPT = FFT_MAXM / FFT_T; /* points per thread in the pointwise phases */
+  /* Every global value the thread needs, loaded up front with indices
+   * clamped into range so that no load waits behind a branch: for its points
+   * k = tid + u FFT_T the stage-table entry (wr[h + j] = tc[j m / h] for
+   * k = h + j, j < h), the pair (x[2k], x[2k+1]) loaded into bit-reversed
+   * position, cos/sin(2 pi k / nb) of the real split, and Z[2k-1], Z[2k]
+   * (Z[0], Z[nb-1] for k = 0). */
+  double tck[KPT], tsk[KPT], zkr[KPT], zki[KPT];
+  {
+    double a[KPT], b[KPT];
+    unsigned int x0[KPT], x1[KPT];
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      const int kc = k < m ? k : m - 1; /* clamped, for the loads only */
+      const int k1 = kc < 1 ? 1 : kc;
+      const int lh = 31 - __clz(k1), j = k1 - (1 << lh);
+      a[u] = tc_g[j << (bits - lh)];
+      b[u] = ts_g[j << (bits - lh)];
+      x0[u] = hist[2 * kc];
+      x1[u] = hist[2 * kc + 1];
+      tck[u] = tc_g[kc];
+      tsk[u] = ts_g[kc];
+      zkr[u] = Z[kc == 0 ? 0 : 2 * kc - 1];
+      zki[u] = Z[kc == 0 ? nb - 1 : 2 * kc];
+    }
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      if (k >= 1 && k < m) {
+        wr[k] = a[u];
+        wi[k] = b[u];
+      }
+      if (k < m) {
+        const int r = (int)(__brev((unsigned)k) >> (32 - bits));
+        zr[FP(r)] = (double)x0[u];
+        zi[FP(r)] = (double)x1[u];
+      }
+    }
   }
   __syncthreads();
-  for (int dir = 0; dir < 2; dir++) {
-    for (int h = 1; h < m; h <<= 1) {
-      const int stride = nb / (2 * h);
-      for (int b = tid; b < m / 2; b += FFT_T) {
-        int g = (b / h) * 2 * h, j = b % h;
-        double c = tc[j * stride], s = ts[j * stride];
-        double br = zr[g + j + h], bi = zi[g + j + h];
-        double tr, ti;
-        if (dir == 0) {
-          tr = br * c + bi * s;
-          ti = bi * c - br * s;
-        } else {
-          tr = br * c - bi * s;
-          ti = bi * c + br * s;
-        }
-        double ar = zr[g + j], ai = zi[g + j];
-        zr[g + j] = ar + tr;
-        zi[g + j] = ai + ti;
-        zr[g + j + h] = ar - tr;
-        zi[g + j + h] = ai - ti;
-      }
-      __syncthreads();
-    }
-    if (dir == 1)
-      break;
-    /* real split -> packed spectrum in volt[], then V = Z I factor, then the
-     * inverse pre-processing back into (zr, zi) with bit reversal */
+  fft_stages(zr, zi, wr, wi, m, bits, 0);
+  {
+    /* real split -> packed spectrum (scaled by 1/nb), then V = Z I factor
+     * (zlongit.c:379-391), kept in registers until every thread has read
+     * the points it pairs with */
     const double scale = 1.0 / nb;
-    for (int k = tid; k <= m; k += FFT_T) {
+    double vr[KPT], vi[KPT];
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      if (k >= m)
+        break;
+      double pr, pi;
       if (k == 0) {
-        volt[0] = (zr[0] + zi[0]) * scale;
-        volt[nb - 1] = (zr[0] - zi[0]) * scale;
-      } else if (k < m) {
-        double ar = zr[k], ai = zi[k], br = zr[m - k], bi = -zi[m - k];
+        pr = (zr[FP(0)] + zi[FP(0)]) * scale; /* V[0] */
+        pi = (zr[FP(0)] - zi[FP(0)]) * scale; /* V[nb - 1] */
+        vr[u] = pr * zkr[u] * factor;          /* Z[0] */
+        vi[u] = pi * zki[u] * factor;          /* Z[nb - 1] */
+      } else {
+        double ar = zr[FP(k)], ai = zi[FP(k)], br = zr[FP(m - k)], bi = -zi[FP(m - k)];
         double er = (ar + br) * 0.5, ei = (ai + bi) * 0.5;
         double dr = ar - br, di = ai - bi;
         double o_r = di * 0.5, o_i = -dr * 0.5;
-        double c = tc[k], s = ts[k];
-        volt[2 * k - 1] = (er + (c * o_r + s * o_i)) * scale;
-        volt[2 * k] = (ei + (c * o_i - s * o_r)) * scale;
+        double c = tck[u], s = tsk[u];
+        pr = (er + (c * o_r + s * o_i)) * scale; /* V[2k - 1] */
+        pi = (ei + (c * o_i - s * o_r)) * scale; /* V[2k] */
+        vr[u] = (pr * zkr[u] - pi * zki[u]) * factor; /* zkr = Z[2k-1], zki = Z[2k] */
+        vi[u] = (pr * zki[u] + pi * zkr[u]) * factor;
       }
     }
     __syncthreads();
-    {
-      /* V(f) = Z(f) I(f) factor, zlongit.c:379-391 (Ifreq read before any write) */
-      double vr_[8], vi_[8];
-      int cnt = 0;
-      for (int k = tid; k < m; k += FFT_T, cnt++) {
-        if (k == 0) {
-          vr_[cnt] = volt[0] * Z[0] * factor;
-          vi_[cnt] = volt[nb - 1] * Z[nb - 1] * factor;
-        } else {
-          int iR = 2 * k - 1, iI = iR + 1;
-          vr_[cnt] = (volt[iR] * Z[iR] - volt[iI] * Z[iI]) * factor;
-          vi_[cnt] = (volt[iR] * Z[iI] + volt[iI] * Z[iR]) * factor;
-        }
-      }
-      __syncthreads();
-      cnt = 0;
-      for (int k = tid; k < m; k += FFT_T, cnt++) {
-        if (k == 0) {
-          volt[0] = vr_[cnt];
-          volt[nb - 1] = vi_[cnt];
-        } else {
-          volt[2 * k - 1] = vr_[cnt];
-          volt[2 * k] = vi_[cnt];
-        }
-      }
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      if (k >= m)
+        break;
+      zr[FP(k)] = vr[u];
+      zi[FP(k)] = vi[u];
     }
     __syncthreads();
-    for (int k = tid; k < m; k += FFT_T) {
-      double zrk, zik;
+    /* the inverse transform's input, Z_k = (X_k + conj X_{m-k}) +
+     * i w^-k (X_k - conj X_{m-k}), placed in bit-reversed order */
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      if (k >= m)
+        break;
       if (k == 0) {
-        double x0 = volt[0], xm = volt[nb - 1];
-        zrk = x0 + xm;
-        zik = x0 - xm;
+        double x0 = zr[FP(0)], xm = zi[FP(0)];
+        vr[u] = x0 + xm;
+        vi[u] = x0 - xm;
       } else {
-        double ar = volt[2 * k - 1], ai = volt[2 * k];
-        double br = volt[2 * (m - k) - 1], bi = -volt[2 * (m - k)];
+        double ar = zr[FP(k)], ai = zi[FP(k)];
+        double br = zr[FP(m - k)], bi = -zi[FP(m - k)];
         double sr = ar + br, si = ai + bi, dr = ar - br, di = ai - bi;
-        double c = tc[k], s = ts[k];
-        zrk = sr - (c * di + s * dr);
-        zik = si + (c * dr - s * di);
+        double c = tck[u], s = tsk[u];
+        vr[u] = sr - (c * di + s * dr);
+        vi[u] = si + (c * dr - s * di);
       }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KPT; u++) {
+      const int k = tid + u * FFT_T;
+      if (k >= m)
+        break;
       int r = (int)(__brev((unsigned)k) >> (32 - bits));
-      zr[r] = zrk;
-      zi[r] = zik;
+      zr[FP(r)] = vr[u];
+      zi[FP(r)] = vi[u];
     }
     __syncthreads();
   }
+  fft_stages(zr, zi, wr, wi, m, bits, 1);
   for (int j = tid; j < m; j += FFT_T) {
-    volt[2 * j] = zr[j];
-    volt[2 * j + 1] = zi[j];
+    volt[2 * j] = zr[FP(j)];
+    volt[2 * j + 1] = zi[FP(j)];
   }
   if (tid == 0)
     volt[nb] = 0;
@@ -724,7 +920,9 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   }
   {
     /* one workgroup per CU: each sees enough particles that its LDS counts
-     * aggregate many hits per bin before the one global atomic per bin */
+     * aggregate many hits per bin before the one global atomic per bin (four
+     * per CU measured slower: the merge's global atomics grow with the
+     * number of workgroups) */
     if (!d->num_cu) {
       int cu = 0;
       CHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
@@ -741,14 +939,14 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
                        lp, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins, hist);
   }
   if (kind == EH_OP_WAKE) {
-    hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CB - 1) / CB)), dim3(CB), 0, d->stream, hist, W,
-                       wpoints, plan, c->factor, volt);
+    hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CONV_B - 1) / CONV_B)), dim3(CONV_B), 0, d->stream,
+                       hist, W, wpoints, plan, c->factor, volt);
   } else {
     int bits = 0;
     while ((1 << bits) < c->nb / 2)
       bits++;
-    hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), sizeof(double) * c->nb, d->stream, hist, sl->Z, sl->tc,
-                       sl->ts, c->nb, bits, c->factor, plan, volt);
+    hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), 0, d->stream, hist, sl->Z, sl->tc, sl->ts, c->nb, bits,
+                       c->factor, plan, volt);
   }
   if (n > 0)
     hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, crd[0], crd[1], crd[2], crd[3], crd[4],
