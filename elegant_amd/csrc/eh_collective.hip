@@ -468,6 +468,7 @@ __global__ void __launch_bounds__(CONV_B) wake_convolve(const unsigned int *__re
  * butterflies and their operands are those of the stage-by-stage loop. */
 #define FFT_T 1024
 #define FFT_MAXM 4096
+#define FFT_LDS_BINS (2 * FFT_MAXM) /* larger transforms take the global-memory path */
 /* LDS index of point i: one padding word after every 16 and every 256
  * points, which spreads the strided accesses of the first passes over the
  * banks (the stage loop's patterns need 3328 bank cycles per 16-lane group
@@ -674,6 +675,159 @@ __global__ void __launch_bounds__(FFT_T) zlongit_fft(const unsigned int *__restr
     volt[nb] = 0;
 }
 
+/* The same transform for nb > 8192 (up to the 2^20 bins the bin optimiser
+ * allows), where the points do not fit in LDS: the points live in global
+ * memory (zr, zi, m doubles each, plus a second pair sr, si for the packed
+ * spectrum) and each kernel below is one step of zlongit_fft, with the same
+ * butterflies, twiddle indices and pointwise formulas; a step's kernel
+ * boundary orders it after the previous one.  FG_B threads per workgroup,
+ * one point, group or butterfly per thread. */
+#define FG_B 256
+
+__global__ void __launch_bounds__(FG_B) fftg_load(const unsigned int *__restrict__ hist, double *zr, double *zi,
+                                                  int m, int bits, const coll_plan *plan) {
+  const int k = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || k >= m)
+    return;
+  const int r = (int)(__brev((unsigned)k) >> (32 - bits));
+  zr[r] = (double)hist[2 * k];
+  zi[r] = (double)hist[2 * k + 1];
+}
+
+/* stages h = 2^lh and 2h on the groups of four points of fft_stages */
+__global__ void __launch_bounds__(FG_B) fftg_stage2(double *zr, double *zi, const double *__restrict__ tc,
+                                                    const double *__restrict__ ts, int m, int bits, int lh,
+                                                    int inverse, const coll_plan *plan) {
+  const int grp = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || grp >= m / 4)
+    return;
+  const int h = 1 << lh;
+  const int j = grp & (h - 1);
+  const int i0 = ((grp >> lh) << (lh + 2)) + j, i1 = i0 + h, i2 = i1 + h, i3 = i2 + h;
+  double r0 = zr[i0], q0 = zi[i0], r1 = zr[i1], q1 = zi[i1];
+  double r2 = zr[i2], q2 = zi[i2], r3 = zr[i3], q3 = zi[i3];
+  const int ta = j << (bits - lh), tb = j << (bits - lh - 1), tcix = (j + h) << (bits - lh - 1);
+  fft_bfly(r0, q0, r1, q1, tc[ta], ts[ta], inverse);
+  fft_bfly(r2, q2, r3, q3, tc[ta], ts[ta], inverse);
+  fft_bfly(r0, q0, r2, q2, tc[tb], ts[tb], inverse);
+  fft_bfly(r1, q1, r3, q3, tc[tcix], ts[tcix], inverse);
+  zr[i0] = r0;
+  zi[i0] = q0;
+  zr[i1] = r1;
+  zi[i1] = q1;
+  zr[i2] = r2;
+  zi[i2] = q2;
+  zr[i3] = r3;
+  zi[i3] = q3;
+}
+
+/* the last stage alone, for an odd stage count */
+__global__ void __launch_bounds__(FG_B) fftg_stage1(double *zr, double *zi, const double *__restrict__ tc,
+                                                    const double *__restrict__ ts, int m, int bits, int lh,
+                                                    int inverse, const coll_plan *plan) {
+  const int b = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || b >= m / 2)
+    return;
+  const int h = 1 << lh;
+  const int j = b & (h - 1);
+  const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+  const int t = j << (bits - lh);
+  double r0 = zr[i0], q0 = zi[i0], r1 = zr[i1], q1 = zi[i1];
+  fft_bfly(r0, q0, r1, q1, tc[t], ts[t], inverse);
+  zr[i0] = r0;
+  zi[i0] = q0;
+  zr[i1] = r1;
+  zi[i1] = q1;
+}
+
+/* real split -> packed spectrum, V = Z I factor (slot k of sr, si as in
+ * zlongit_fft) */
+__global__ void __launch_bounds__(FG_B) fftg_split(const double *__restrict__ zr, const double *__restrict__ zi,
+                                                   double *sr, double *si, const double *__restrict__ tc,
+                                                   const double *__restrict__ ts, const double *__restrict__ Z,
+                                                   int nb, double factor, const coll_plan *plan) {
+  const int m = nb / 2;
+  const int k = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || k >= m)
+    return;
+  const double scale = 1.0 / nb;
+  if (k == 0) {
+    const double pr = (zr[0] + zi[0]) * scale, pi = (zr[0] - zi[0]) * scale;
+    sr[0] = pr * Z[0] * factor;
+    si[0] = pi * Z[nb - 1] * factor;
+  } else {
+    double ar = zr[k], ai = zi[k], br = zr[m - k], bi = -zi[m - k];
+    double er = (ar + br) * 0.5, ei = (ai + bi) * 0.5;
+    double dr = ar - br, di = ai - bi;
+    double o_r = di * 0.5, o_i = -dr * 0.5;
+    double c = tc[k], s = ts[k];
+    const double pr = (er + (c * o_r + s * o_i)) * scale, pi = (ei + (c * o_i - s * o_r)) * scale;
+    const int iR = 2 * k - 1, iI = iR + 1;
+    sr[k] = (pr * Z[iR] - pi * Z[iI]) * factor;
+    si[k] = (pr * Z[iI] + pi * Z[iR]) * factor;
+  }
+}
+
+/* the inverse transform's input in bit-reversed order */
+__global__ void __launch_bounds__(FG_B) fftg_prep(const double *__restrict__ sr, const double *__restrict__ si,
+                                                  double *zr, double *zi, const double *__restrict__ tc,
+                                                  const double *__restrict__ ts, int m, int bits,
+                                                  const coll_plan *plan) {
+  const int k = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || k >= m)
+    return;
+  double zrk, zik;
+  if (k == 0) {
+    const double x0 = sr[0], xm = si[0];
+    zrk = x0 + xm;
+    zik = x0 - xm;
+  } else {
+    double ar = sr[k], ai = si[k];
+    double br = sr[m - k], bi = -si[m - k];
+    double sr_ = ar + br, si_ = ai + bi, dr = ar - br, di = ai - bi;
+    double c = tc[k], s = ts[k];
+    zrk = sr_ - (c * di + s * dr);
+    zik = si_ + (c * dr - s * di);
+  }
+  const int r = (int)(__brev((unsigned)k) >> (32 - bits));
+  zr[r] = zrk;
+  zi[r] = zik;
+}
+
+__global__ void __launch_bounds__(FG_B) fftg_out(const double *__restrict__ zr, const double *__restrict__ zi,
+                                                 double *volt, int m, const coll_plan *plan) {
+  const int j = blockIdx.x * FG_B + threadIdx.x;
+  if (plan->nb <= 0 || plan->err || j >= m)
+    return;
+  volt[2 * j] = zr[j];
+  volt[2 * j + 1] = zi[j];
+  if (j == 0)
+    volt[2 * m] = 0;
+}
+
+/* host side of the global-memory path: scratch holds zr, zi, sr, si */
+static void fftg_launch(hipStream_t st, const unsigned int *hist, const double *Z, const double *tc, const double *ts,
+                        int nb, int bits, double factor, const coll_plan *plan, double *volt, double *scratch) {
+  const int m = nb / 2;
+  double *zr = scratch, *zi = scratch + m, *sr = scratch + 2 * m, *si = scratch + 3 * m;
+  const unsigned gm = (unsigned)((m + FG_B - 1) / FG_B);
+  const unsigned g2 = (unsigned)((m / 2 + FG_B - 1) / FG_B);
+  const unsigned g4 = (unsigned)((m / 4 + FG_B - 1) / FG_B);
+  hipLaunchKernelGGL(fftg_load, dim3(gm), dim3(FG_B), 0, st, hist, zr, zi, m, bits, plan);
+  for (int inverse = 0; inverse < 2; inverse++) {
+    int lh = 0;
+    for (; lh + 1 < bits; lh += 2)
+      hipLaunchKernelGGL(fftg_stage2, dim3(g4), dim3(FG_B), 0, st, zr, zi, tc, ts, m, bits, lh, inverse, plan);
+    if (lh < bits)
+      hipLaunchKernelGGL(fftg_stage1, dim3(g2), dim3(FG_B), 0, st, zr, zi, tc, ts, m, bits, lh, inverse, plan);
+    if (!inverse) {
+      hipLaunchKernelGGL(fftg_split, dim3(gm), dim3(FG_B), 0, st, zr, zi, sr, si, tc, ts, Z, nb, factor, plan);
+      hipLaunchKernelGGL(fftg_prep, dim3(gm), dim3(FG_B), 0, st, sr, si, zr, zi, tc, ts, m, bits, plan);
+    }
+  }
+  hipLaunchKernelGGL(fftg_out, dim3(gm), dim3(FG_B), 0, st, zr, zi, volt, m, plan);
+}
+
 /* add_to_particle_energy (simple_rfca.c:987-1007) */
 DEV void add_energy(double &xp, double &yp, double &s, double &dp, double tof, double Po, double dgamma) {
   double P = Po * (1 + dp);
@@ -839,7 +993,7 @@ int ehd_put_wake(ehd_dev *d, int32_t slot, const double *W, int64_t npoints) {
 }
 
 int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc, const double *ts, int32_t nb) {
-  if (slot < 0 || slot >= EHD_MAX_COLL || nb < 4 || nb > 8192 || (nb & (nb - 1)))
+  if (slot < 0 || slot >= EHD_MAX_COLL || nb < 4 || nb > EHD_ZL_MAX_BINS || (nb & (nb - 1)))
     return EH_ERR_INVALID;
   ehd_coll_slot *sl = &d->slot[slot];
   CHK(hipSetDevice(d->device));
@@ -861,12 +1015,19 @@ int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc,
 }
 
 /* work area: [0,128) the step plan, then the histogram (uint32 x (cap+8)),
- * the voltage (double x (cap+8)), the fixed-sum partials */
+ * the voltage (double x (cap+8)), the fixed-sum partials, and for ZLONGIT
+ * beyond the LDS transform's 8192 bins the global transform's points
+ * (4 x nb/2 doubles, 256-byte aligned) */
 #define PLAN_BYTES 128
 static int64_t hist_bytes(int64_t nb_cap) { return ((4 * (nb_cap + 8) + 7) / 8) * 8; }
+static int64_t fft_offset(int64_t nb_cap, int64_t nparts) {
+  int64_t o = PLAN_BYTES + hist_bytes(nb_cap) + 8 * (nb_cap + 8) + (int64_t)sizeof(eh_fsum) * (nparts + 1);
+  return (o + 255) / 256 * 256;
+}
+static int64_t fft_bytes(int64_t nb) { return nb > FFT_LDS_BINS ? 16 * nb : 0; }
 
-static int ensure_work(ehd_dev *d, int64_t nb_cap, int64_t nparts) {
-  int64_t need = PLAN_BYTES + hist_bytes(nb_cap) + 8 * (nb_cap + 8) + (int64_t)sizeof(eh_fsum) * (nparts + 1) + 256;
+static int ensure_work(ehd_dev *d, int64_t nb_cap, int64_t nparts, int64_t nb_fft) {
+  int64_t need = fft_offset(nb_cap, nparts) + fft_bytes(nb_fft) + 256;
   if (need <= d->cwork_bytes)
     return EH_OK;
   if (d->cwork)
@@ -945,8 +1106,12 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
     int bits = 0;
     while ((1 << bits) < c->nb / 2)
       bits++;
-    hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), 0, d->stream, hist, sl->Z, sl->tc, sl->ts, c->nb, bits,
-                       c->factor, plan, volt);
+    if (c->nb <= FFT_LDS_BINS)
+      hipLaunchKernelGGL(zlongit_fft, dim3(1), dim3(FFT_T), 0, d->stream, hist, sl->Z, sl->tc, sl->ts, c->nb, bits,
+                         c->factor, plan, volt);
+    else
+      fftg_launch(d->stream, hist, sl->Z, sl->tc, sl->ts, c->nb, bits, c->factor, plan, volt,
+                  (double *)((char *)work + fft_offset(nb_cap, rgrid)));
   }
   if (n > 0)
     hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, crd[0], crd[1], crd[2], crd[3], crd[4],
@@ -973,7 +1138,7 @@ int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, doubl
   CHK(hipSetDevice(d->device));
   const int64_t nb_max = c->kind == EH_OP_WAKE ? sl->wpoints : sl->nb; /* most bins this element can use */
   const int64_t nb_cap = nb_max + 8;
-  int rc = ensure_work(d, nb_cap, reduce_grid(n));
+  int rc = ensure_work(d, nb_cap, reduce_grid(n), c->kind == EH_OP_ZLONGIT ? nb_max : 0);
   if (rc)
     return rc;
   coll_plan hp;
@@ -1056,7 +1221,7 @@ int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *co
   const int64_t n = d->n;
   const int64_t rgrid = reduce_grid(n);
   CHK(hipSetDevice(d->device));
-  int rc = ensure_work(d, 64, rgrid);
+  int rc = ensure_work(d, 64, rgrid, 0);
   if (rc)
     return rc;
   coll_plan *plan = (coll_plan *)d->cwork;

@@ -13,6 +13,7 @@
 /* the no-allocation WAKE step: most wake points (bins) and the size of its
  * static scratch (plan 128 B + histogram and voltage for EHD_NOALLOC_BINS + 16
  * bins + 1,025 fixed-point partials, rounded up) */
+#define EHD_ZL_MAX_BINS (1 << 20) /* ZLONGIT bins: the bin optimiser's default cap (ztransverse.c:672-759) */
 #define EHD_NOALLOC_BINS 16384
 #define EHD_NOALLOC_BYTES (256 * 1024)
 

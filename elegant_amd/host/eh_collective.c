@@ -122,8 +122,12 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po) {
   rc = optimize_bins(tmax - tmin, zl->freq, zl->Q, &bin_size, &n_bins, zl->max_n_bins);
   if (rc)
     return msgf(c, rc, "ZLONGIT: bin settings cannot cover the bunch");
-  if (n_bins > 8192 || (n_bins & (n_bins - 1)))
-    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: more than 8192 bins");
+  /* powers of two up to 2^20 bins, the optimiser's default cap: at most 8192
+   * take the one-workgroup LDS transform, more the global-memory one */
+  if (n_bins > (1L << 20))
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: more than 2^20 bins");
+  if (n_bins & (n_bins - 1))
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: a bin count that is not a power of 2");
   {
     double *Z = (double *)malloc(sizeof(double) * n_bins);
     double *tc = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));

@@ -51,6 +51,29 @@ def test_zlongit_step_bit_exact(tracker):
     assert_rows_identical(g, c, "zlongit kick")
 
 
+@pytest.mark.parametrize("nbins,bin_size", [(16384, 1e-12), (65536, 2e-13), (131072, 1e-13)])
+def test_zlongit_beyond_the_lds_transform_bit_exact(tracker, nbins, bin_size):
+    """ZLONGIT with more bins than the one-workgroup LDS transform holds
+    (8192): the global-memory transform of the packed m = nb/2 points, with
+    odd (m = 2^13, 2^15) and even (m = 2^16) stage counts, gives the
+    checker's histogram, voltages and kicked rows bit for bit"""
+    n = 300000
+    rows = wake_bunch(n, 20e-12, seed=17)
+    zl = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=nbins, BIN_SIZE=bin_size)
+    q = 5e-9 / n
+    g = rows.copy()
+    h, v, tmin = tracker.collective_probe(g, zl, PO, 0, q)
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    c = rows.copy()
+    o.zlongit(c, zl.p.zlongit, PO, 0)
+    hc, vc, tminc, dt = o.last_hist()
+    assert h.size == nbins and tmin == tminc
+    assert np.array_equal(h, hc) and h.sum() == n
+    assert np.array_equal(v, vc)
+    assert_rows_identical(g, c, "zlongit kick, %d bins" % nbins)
+
+
 def test_ring_with_wakes_bit_exact_vs_emath_checker(tracker):
     bl, Po = Lt.storage_ring_wake()
     rows = Lt.matched_beam(bl, 3000, 1e-9, 1e-11, 1e-3, 20e-12, 31, Po)
