@@ -40,7 +40,8 @@
  *    sin/cos for |x| < 2^-5 by Taylor series to degree 9/8 (coefficients
  *    1/n! rounded to double) with the compensated 1 - z/2 step for cos, and
  *    asin everywhere (Taylor series in x^2 on |x| <= 0.5 by band, and the
- *    reflection asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above) are faithful
+ *    reflection asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above, the reduction
+ *    fdlibm's e_asin.c also uses) are faithful
  *    evaluations that round correctly except on a small fraction of
  *    arguments (measured against 200-bit evaluations in tests/test_emath.py);
  *  - every polynomial by Horner's rule in fused multiply-adds (correctly
@@ -70,8 +71,6 @@
 #define EM_PIO2_LO 0x1.1a62633145c07p-54
 #define EM_TWO_OVER_PI 0x1.45f306dc9c883p-1
 #define EM_SHIFT 0x1.8p+52
-#define EM_LN2_HI 0x1.62e42fee00000p-1
-#define EM_LN2_LO 0x1.a39ef35600000p-33
 
 /* A polynomial coefficient at its point of use.  On the device the empty
  * volatile asm keeps each 64-bit constant a scalar-register operand
@@ -172,6 +171,9 @@ EM_FN double em_sin_small(double x, double z) {
   return x + (z * x) * __builtin_fma(z, em_sin_r_small(z), em_k(EM_S1));
 }
 
+/* The compensated form w + ((1 - w) - z/2 + r), w = 1 - z/2, recovers the
+ * rounding error of 1 - z/2; it is the one the cosine kernel of Sun's fdlibm
+ * (k_cos.c) uses. */
 EM_FN double em_cos_small(double x, double z) {
   double r = z * z * em_cos_r_small(z);
   double hz = 0.5 * z;
