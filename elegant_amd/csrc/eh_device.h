@@ -72,6 +72,8 @@ int ehd_batch_compact(ehd_dev *d, const int32_t *perm, int64_t nperm, const eh_c
 int ehd_batch_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t passes_per_launch, int32_t ppl,
                     int32_t block, int32_t use_graph, int32_t device_compact, double *ms, int64_t *launches);
 int ehd_state_snapshot(ehd_dev *d, int restore);
+/* delta = dp + delta dr on the particles not lost (CHANGE_P0) */
+int ehd_rescale_dp(ehd_dev *d, double dp, double dr);
 
 /* ---- collective elements (WAKE / ZLONGIT) ---- */
 typedef struct {

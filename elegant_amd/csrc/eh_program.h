@@ -28,6 +28,8 @@
 #define EH_OP_APERTURE 8   /* MAXAMP: only the checks that follow every element */
 #define EH_OP_WAKE 9       /* collective: handled outside the tracking kernel */
 #define EH_OP_ZLONGIT 10   /* collective: handled outside the tracking kernel */
+#define EH_OP_MATCH_P0 11  /* CHANGE_P0 after an RFCA: do_match_energy on the host (do_tracking.c:3308-3420);
+                              the kernel does nothing here */
 
 /* flags */
 #define EH_F_AP (1 << 0)       /* MAXAMP aperture after the element (do_tracking.c:2459-2467) */

@@ -1523,6 +1523,27 @@ int ehd_track(ehd_dev *d, int64_t pass0, int32_t npasses, int32_t op_first, int3
   return EH_OK;
 }
 
+/* do_match_energy's re-expression of delta against the new central
+ * momentum, delta = dp + delta dr, on the particles still in the beam
+ * (do_tracking.c:3408-3414) */
+__global__ void eh_rescale_dp_kernel(double *dp_, const int32_t *loss_pass, int64_t n, double dp, double dr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && loss_pass[i] < 0)
+    dp_[i] = dp + dp_[i] * dr;
+}
+
+int ehd_rescale_dp(ehd_dev *d, double dp, double dr) {
+  if (d->n == 0)
+    return EH_OK;
+  HIPCHK(d, hipSetDevice(d->device));
+  const int64_t grid = (d->n + 255) / 256;
+  hipLaunchKernelGGL(eh_rescale_dp_kernel, dim3((unsigned)grid), dim3(256), 0, d->stream, d->coord[5], d->loss_pass,
+                     d->n, dp, dr);
+  HIPCHK(d, hipGetLastError());
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
 __global__ void eh_count_alive_kernel(const int32_t *loss_pass, int64_t n, unsigned long long *out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int alive = (i < n && loss_pass[i] < 0) ? 1 : 0;

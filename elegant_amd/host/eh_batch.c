@@ -101,6 +101,8 @@ int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, 
     return rc;
   if (b->prog->n_coll)
     return bmsg(b, EH_ERR_UNSUPPORTED, "batched variants cannot contain WAKE/ZLONGIT");
+  if (b->prog->n_p0)
+    return bmsg(b, EH_ERR_UNSUPPORTED, "batched variants cannot contain RFCA with CHANGE_P0");
   /* RF cavities need a fiducial known without a reduction over particles */
   for (int k = 0; k < b->prog->n_rfca; k++) {
     const EH_RFCA *rf = &elems[b->prog->rfca_elem[k]].p.rfca;

@@ -599,10 +599,8 @@ static int compile_rfca(const eh_element *e, eh_op *op, char *msg, int msglen) {
   double length = rf->length, volt, omega;
   long nKicks;
   int matrixMethod = 0;
-  if (rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase || rf->bodyFocusModel || rf->standingWave ||
-      rf->Q) {
-    snprintf(msg, msglen, "%s: RFCA with CHANGE_T/CHANGE_P0/LINEARIZE/LOCK_PHASE/BODY_FOCUS_MODEL/STANDING_WAVE/Q",
-             e->name);
+  if (rf->change_t || rf->linearize || rf->lockPhase || rf->bodyFocusModel || rf->standingWave || rf->Q) {
+    snprintf(msg, msglen, "%s: RFCA with CHANGE_T/LINEARIZE/LOCK_PHASE/BODY_FOCUS_MODEL/STANDING_WAVE/Q", e->name);
     return EH_ERR_UNSUPPORTED;
   }
   if (rf->fiducialMode != EH_FID_TMEAN && rf->fiducialMode != EH_FID_LIGHT) {
@@ -659,6 +657,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
   prog->nops = 0;
   prog->n_rfca = 0;
   prog->n_coll = 0;
+  prog->n_p0 = 0;
   prog->Po = Po;
   for (i = 0; i < n_elems; i++) {
     const eh_element *e = &bl[i];
@@ -745,9 +744,39 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
       op.ap_xmax = x_max;
       op.ap_ymax = y_max;
     }
-    if (prog->nops >= EH_MAX_OPS) {
+    if (prog->nops + 3 > EH_MAX_OPS) {
       snprintf(msg, msglen, "beamline longer than %d elements", EH_MAX_OPS);
       return EH_ERR_UNSUPPORTED;
+    }
+    if (op.op == EH_OP_RFCA && e->p.rfca.change_p0) {
+      /* CHANGE_P0: do_match_energy runs inside the cavity routine, after its
+       * own removeInvalidParticles and before do_tracking's checks that
+       * follow the element (simple_rfca.c:975-982).  Three records: the
+       * cavity without those checks, the host's match step, and the checks
+       * alone at the cavity's end, all carrying the cavity's element index. */
+      eh_op step, post;
+      memset(&step, 0, sizeof(step));
+      step.op = EH_OP_MATCH_P0;
+      step.elem = op.elem;
+      step.z_start = step.z_end = op.z_end;
+      step.Po = Po;
+      step.flags = EH_F_ALONE;
+      post = step;
+      post.op = EH_OP_APERTURE;
+      post.flags = op.flags & EH_F_AP; /* the MAXAMP check if any, then the NaN cull */
+      post.ap_code = op.ap_code;
+      post.ap_xmax = op.ap_xmax;
+      post.ap_ymax = op.ap_ymax;
+      op.flags = (op.flags & ~EH_F_AP) | EH_F_ALONE;
+      if (prog->n_p0 >= EH_MAX_RFCA) {
+        snprintf(msg, msglen, "too many RFCA elements with CHANGE_P0");
+        return EH_ERR_UNSUPPORTED;
+      }
+      prog->ops[prog->nops++] = op;
+      prog->p0_op[prog->n_p0++] = (int32_t)prog->nops;
+      prog->ops[prog->nops++] = step;
+      prog->ops[prog->nops++] = post;
+      continue;
     }
     prog->ops[prog->nops++] = op;
   }

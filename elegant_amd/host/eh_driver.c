@@ -502,7 +502,58 @@ static int32_t next_barrier(const eh_ctx *c, int32_t from) {
   for (int k = 0; k < p->n_coll; k++)
     if (p->coll_op[k] >= from && p->coll_op[k] < best)
       best = p->coll_op[k];
+  for (int k = 0; k < p->n_p0; k++)
+    if (p->p0_op[k] >= from && p->p0_op[k] < best)
+      best = p->p0_op[k];
   return best;
+}
+
+/* the program for the current central momentum c->Po, with the RFCA phases
+ * fixed so far, compiled and uploaded (CHANGE_P0 moves the momentum every
+ * element after the cavity sees, field-specified multipole strengths
+ * included) */
+static int recompile_program(eh_ctx *c) {
+  int rc = eh_compile(c->elems, c->n_elems, c->Po, c->prog, c->msg, sizeof(c->msg));
+  if (rc)
+    return rc;
+  for (int k = 0; k < c->prog->n_rfca; k++)
+    if (c->fid_seen[k])
+      eh_rfca_set_phase(&c->prog->ops[c->prog->rfca_op[k]], &c->elems[c->prog->rfca_elem[k]].p.rfca,
+                        c->phase_fid[k]);
+  rc = ehd_put_program(c->dev, c->prog->ops, c->prog->nops);
+  return rc ? devfail(c, rc) : EH_OK;
+}
+
+/* do_match_energy after an RFCA with CHANGE_P0 (simple_rfca.c:977-982,
+ * do_tracking.c:3371-3417): the KahanPlus mean of P_central (1 + delta)
+ * over the particles left after the cavity's own losses, in reference row
+ * order; when it moved by more than 1e-14 relative, delta is re-expressed
+ * against it on the device and the central momentum becomes it (and the
+ * program is recompiled for it if recompile is set).  opi is the
+ * EH_OP_MATCH_P0 step. */
+static int match_energy(eh_ctx *c, int32_t opi, int64_t pass, int recompile) {
+  double P_average = 0, error = 0;
+  int rc = ehd_get_state(c->dev, NULL, NULL, NULL, NULL, NULL, c->h_dp, c->h_pass, c->h_code, c->n);
+  if (rc)
+    return devfail(c, rc);
+  rc = replay_until(c, pass, 4 * opi);
+  if (rc)
+    return rc;
+  if (c->n_alive_ref <= 0)
+    return EH_OK;
+  for (int64_t j = 0; j < c->n_alive_ref; j++)
+    P_average = kahan_plus(P_average, c->Po * (1 + c->h_dp[c->order[j]]), &error);
+  P_average /= c->n_alive_ref;
+  if (fabs(P_average - c->Po) / c->Po > 1e-14) {
+    const double dp = (c->Po - P_average) / P_average, dr = c->Po / P_average;
+    rc = ehd_rescale_dp(c->dev, dp, dr);
+    if (rc)
+      return devfail(c, rc);
+    c->Po = P_average;
+    if (recompile)
+      return recompile_program(c);
+  }
+  return EH_OK;
 }
 
 static void charge_elements(eh_ctx *c, int64_t pass) {
@@ -689,6 +740,11 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
                   return devfail(c, rc);
                 cur = cur + 1;
               }
+            } else if (op->op == EH_OP_MATCH_P0) {
+              rc = match_energy(c, b, pass, 1);
+              if (rc)
+                return rc;
+              cur = b + 1;
             } else {
               rc = eh_collective(c, op->nkicks_rf, pass, 0);
               if (rc)
@@ -725,6 +781,7 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
     ehd_collective_timing(c->dev, &left, &stats->collective_ms);
   }
   (void)n_alive0;
+  *P_central = c->Po; /* moved by CHANGE_P0 cavities, as do_tracking's *P_central */
   return EH_OK;
 }
 
@@ -990,6 +1047,13 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     rc = ehd_track(c->dev, 0, 1, 0, 1);
     if (rc)
       return devfail(c, rc);
+    if (c->prog->n_p0) {
+      /* CHANGE_P0: the match step that follows the cavity's own culls */
+      rc = match_energy(c, c->prog->p0_op[0], 0, 0);
+      if (rc)
+        return rc;
+      *P_central = c->Po;
+    }
     rc = writeback_rows(c, part, np, accepted, &nLeft);
     if (rc)
       return rc;

@@ -20,6 +20,7 @@ typedef struct {
   double Po;
   int32_t n_rfca, rfca_op[EH_MAX_RFCA], rfca_elem[EH_MAX_RFCA];
   int32_t n_coll, coll_op[EH_MAX_COLL], coll_elem[EH_MAX_COLL];
+  int32_t n_p0, p0_op[EH_MAX_RFCA]; /* EH_OP_MATCH_P0 steps (RFCA with CHANGE_P0) */
 } eh_program;
 
 int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *prog, char *msg, int msglen);

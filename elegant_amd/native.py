@@ -319,6 +319,7 @@ class Tracker:
                                  C.byref(pc), aptr, n_passes, pass_offset)
         check(self._ctx, r)
         self.n = rows.shape[0]
+        self.P_central = pc.value  # moved by CHANGE_P0 cavities
         return int(r), int(eff.value)
 
     def loss_records(self):
@@ -391,6 +392,7 @@ class Tracker:
         ph = C.c_double(phase_fiducial)
         r = check(self._ctx, lib().eh_simple_rf_cavity(self._ctx, ptr, rows.shape[0], C.byref(rfca), aptr,
                                                        C.byref(pc), zEnd, C.byref(ph)))
+        self.P_central = pc.value  # moved by CHANGE_P0
         return r, ph.value
 
     def track_particles_drift(self, rows, drift):

@@ -226,8 +226,10 @@ int eh_set_particle_properties(eh_ctx *ctx, int32_t nprops);
  * columns 8..nprops-1 are kept on the host and follow their particle. */
 int eh_upload(eh_ctx *ctx, double **coord, int64_t n, int32_t nprops);
 /* Track passes [pass0, pass0+npasses) through the loaded program (the pass
- * and element loops of do_tracking.c:488,816).  *P_central is read (and
- * would be updated by CHANGE_P0 elements, which are not supported).  The
+ * and element loops of do_tracking.c:488,816).  *P_central must equal the
+ * program's momentum on entry and receives it on return: an RFCA with
+ * CHANGE_P0 moves it every pass to the beam's mean momentum (do_match_energy,
+ * do_tracking.c:3371-3417), and every element after it uses the new value.  The
  * pass loop stops early, before a pass, when a stopping condition below
  * holds (do_tracking.c:494-520); stats->passes_done then counts the passes
  * completed. */
@@ -293,7 +295,9 @@ int64_t eh_track_through_csbend(eh_ctx *ctx, double **part, int64_t n_part, cons
 int eh_rfca_fiducial(eh_ctx *ctx, int64_t elem_index, double *phase_fiducial);
 /* simple_rf_cavity (simple_rfca.c:264-293); *phase_fiducial carries the
  * element's fiducial phase between calls (NaN = not yet set, as
- * rfca->fiducial_seen == 0) */
+ * rfca->fiducial_seen == 0).  With CHANGE_P0, *P_central becomes the mean
+ * momentum of the rows left and their deltas are re-expressed against it
+ * (simple_rfca.c:977-982). */
 int64_t eh_simple_rf_cavity(eh_ctx *ctx, double **part, int64_t np, const EH_RFCA *rfca, double **accepted,
                             double *P_central, double zEnd, double *phase_fiducial);
 /* track_particles with the order-2 drift matrix (matrix.c:294-468) */

@@ -1222,6 +1222,26 @@ static int find_fiducial_time(double **part, long np, double s0, double sOffset,
   return EH_ERR_UNSUPPORTED;
 }
 
+/* do_match_energy with change_beam = 0 (do_tracking.c:3308-3420, serial
+ * build with USE_KAHAN): the central momentum becomes the KahanPlus mean of
+ * P_central (1 + delta) over the np particles in row order, and every delta
+ * is re-expressed against it, unless the relative change is at most 1e-14 */
+void or_do_match_energy(double **coord, long np, double *P_central) {
+  double P_average = 0, error = 0, dp, dr;
+  if (!np)
+    return;
+  for (long ip = 0; ip < np; ip++)
+    P_average = or_kahan_plus(P_average, (*P_central) * (1 + coord[ip][5]), &error);
+  P_average /= np;
+  if (fabs(P_average - (*P_central)) / (*P_central) > 1e-14) {
+    dp = (*P_central - P_average) / P_average;
+    dr = (*P_central) / P_average;
+    for (long ip = 0; ip < np; ip++)
+      coord[ip][5] = dp + coord[ip][5] * dr;
+    *P_central = P_average;
+  }
+}
+
 /* simple_rf_cavity -> trackRfCavityWithWakes without wakes, forward tracking
  * (simple_rfca.c:264-984): kick method and matrix method */
 long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RFCA *rfca, double *P_central,
@@ -1230,8 +1250,8 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
   double P, gamma, dgamma = 0.0, phase, length, dtLight, volt, beta_i, t, t0, omega, tau;
   int matrixMethod = 0;
   double timeOffset = 0;
-  if (rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase || rfca->bodyFocusModel ||
-      rfca->standingWave || rfca->Q)
+  if (rfca->change_t || rfca->linearize || rfca->lockPhase || rfca->bodyFocusModel || rfca->standingWave ||
+      rfca->Q)
     return EH_ERR_UNSUPPORTED;
   if (np <= 0)
     return np;
@@ -1383,7 +1403,11 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
     part[ip][0] += rfca->dx;
     part[ip][2] += rfca->dy;
   }
-  return or_remove_invalid_particles(part, np, zEnd, *P_central);
+  np = or_remove_invalid_particles(part, np, zEnd, *P_central);
+  /* CHANGE_P0 (simple_rfca.c:977-982) */
+  if (rfca->change_p0)
+    or_do_match_energy(part, np, P_central);
+  return np;
 }
 
 /* ---------------------------------------------------------- WAKE/ZLONGIT */

@@ -53,6 +53,7 @@ def lib(mode="glibc"):
         L.or_add_to_particle_energy.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.or_kahan_plus.restype = C.c_double
         L.or_kahan_plus.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_do_match_energy.argtypes = [PP, C.c_long, C.POINTER(C.c_double)]
         L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
         L.or_simple_rf_cavity.restype = C.c_long
         L.or_simple_rf_cavity.argtypes = [C.c_void_p, C.c_long, PP, C.c_long, C.POINTER(A.EH_RFCA),
@@ -97,7 +98,15 @@ class Oracle:
                                   n_passes, pass_offset)
         if r < 0:
             raise RuntimeError("oracle do_tracking failed: %d" % r)
+        self.P_central = pc.value  # changed by CHANGE_P0 cavities
         return int(r)
+
+    def match_energy(self, rows, np_, Po):
+        """do_match_energy on rows[:np_] (in place); returns the new P_central"""
+        ptr, keep = A.rows_pointer(rows)
+        pc = C.c_double(Po)
+        self.L.or_do_match_energy(ptr, np_, C.byref(pc))
+        return pc.value
 
     def last_hist(self):
         PD = C.POINTER(C.c_double)

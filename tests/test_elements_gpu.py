@@ -173,6 +173,26 @@ def test_simple_rf_cavity_two_calls(tracker):
     assert_rows_identical(g2[:, :7], c2[:, :7], "rfca pass 1")
 
 
+def test_simple_rf_cavity_change_p0(tracker):
+    """CHANGE_P0 (simple_rfca.c:977-982, do_tracking.c:3371-3417): after the
+    cavity's own culls the central momentum becomes the KahanPlus mean
+    momentum of the rows left, and every delta is re-expressed against it;
+    rows and the new central momentum bit for bit against the checker"""
+    rf = E.RFCA(L=0.0, VOLT=5e6, PHASE=160.0, FREQ=352e6, CHANGE_P0=1).p.rfca
+    r = beam(2000, 3, halo=0.0)
+    r[:5, 0] = np.nan  # removed by the cavity's removeInvalidParticles before the mean
+    g = r.copy()
+    c = r.copy()
+    ng, ph = tracker.simple_rf_cavity(g, rf, PO, 7.0)
+    o = O.Oracle("emath")
+    ptr, keep = rows_ptr(c)
+    pc = C.c_double(PO)
+    nc = o.L.or_simple_rf_cavity(o.ctx, 0, ptr, c.shape[0], C.byref(rf), C.byref(pc), 7.0)
+    assert ng == nc == r.shape[0] - 5
+    assert tracker.P_central == pc.value and pc.value != PO
+    assert_rows_identical(g[:, :7], c[:, :7], "rfca change_p0")
+
+
 @pytest.mark.parametrize("volt", [5e6, 0.0])
 def test_simple_rf_cavity_edges(tracker, volt):
     """the cavity with rows of every kind: its own removeInvalidParticles

@@ -826,3 +826,65 @@ def test_bend_with_multipoles_is_symplectic(elem):
     """the bend body with a multipole field, like the pure dipole, has a
     symplectic transverse map at fixed delta"""
     test_transverse_map_is_symplectic(elem)
+
+
+def _match(rows, n, Po):
+    o = O.Oracle("glibc")
+    return o.match_energy(rows, n, Po)
+
+
+def test_match_energy_exact_mean_and_momentum_conservation():
+    """do_match_energy (do_tracking.c:3371-3417): with every P0 (1 + delta)
+    and every partial sum exact in binary, the KahanPlus mean is the exact
+    mean momentum; afterwards each particle keeps its momentum,
+    P0' (1 + delta') = P0 (1 + delta) to rounding, the rows past np are not
+    touched, and a second match finds nothing to change"""
+    rng = np.random.default_rng(8)
+    n, Po = 1024, 1024.0
+    rows = np.zeros((n + 16, 8))
+    k = rng.integers(-2 ** 20, 2 ** 20, n)
+    rows[:n, 5] = k * 2.0 ** -30 + 3 * 2.0 ** -10
+    rows[n:, 5] = 0.25
+    before = rows.copy()
+    P1 = _match(rows, n, Po)
+    exact = sum(Fraction(Po) * (1 + Fraction(d)) for d in before[:n, 5]) / n
+    assert Fraction(P1) == exact and P1 != Po
+    p_old = Po * (1 + before[:n, 5])
+    p_new = P1 * (1 + rows[:n, 5])
+    assert np.max(np.abs(p_new - p_old) / p_old) <= 4 * 2.0 ** -52
+    assert np.array_equal(rows[n:], before[n:])
+    assert abs(np.mean(p_new) - P1) / P1 < 1e-14
+    again = rows.copy()
+    assert _match(again, n, P1) == P1 and np.array_equal(again, rows)
+
+
+def test_match_energy_threshold_and_summation():
+    """no change when the mean moves by at most 1e-14 relative; otherwise
+    the mean of a long sum (2^19 momenta) is the compensated one: within 2
+    ulps of the exact mean where plain summation is off by many"""
+    Po = 3000.0
+    rows = np.zeros((4, 8))
+    rows[:, 5] = [2e-15, -1e-15, 3e-15, 0.0]  # mean momentum change 1e-15 relative
+    before = rows.copy()
+    assert _match(rows, 4, Po) == Po and np.array_equal(rows, before)
+    rng = np.random.default_rng(9)
+    n = 1 << 19
+    rows = np.zeros((n, 8))
+    rows[:, 5] = rng.uniform(-3e-3, 3e-3, n) + 1e-3
+    d = rows[:, 5].copy()
+    P1 = _match(rows, n, Po)
+    terms = [Po * (1 + x) for x in d]
+    exact = float(sum(Fraction(t) for t in terms) / n)
+    naive = 0.0
+    for t in terms:
+        naive += t
+    naive /= n
+    assert abs(P1 - exact) <= 2 * np.spacing(exact)  # measured: 0
+    assert abs(naive - exact) > 8 * np.spacing(exact)  # measured: 46 ulps
+
+
+def test_match_energy_empty_beam():
+    rows = np.zeros((3, 8))
+    rows[:, 5] = 0.01
+    before = rows.copy()
+    assert _match(rows, 0, 100.0) == 100.0 and np.array_equal(rows, before)

@@ -19,6 +19,7 @@ def run_both(tracker, bl, Po, rows, passes, mode):
     c = rows.copy()
     o = O.Oracle(mode)
     nc = o.do_tracking(c, bl, Po, passes)
+    assert tracker.P_central == o.P_central  # equal to Po unless CHANGE_P0 moved it
     return g, ng, eff_g, c, nc, o.effort()
 
 
@@ -117,3 +118,27 @@ def test_repeated_cells_with_cavities_refold(tracker):
     assert ng == nc and eg == ec
     assert rows.shape[0] - ng > 100
     assert_rows_identical(g, c, "cells with cavities")
+
+
+def test_ring_with_change_p0_bit_exact(tracker):
+    """a FODO ring whose quadrupoles are given by field and bore (B/BORE:
+    their strengths scale with the central momentum, multipole.c) and two
+    RF cavities with CHANGE_P0, each moving the central momentum to the
+    beam's mean every pass: rows, losses, effort and the final central
+    momentum identical to the checker over 12 passes of a lossy beam"""
+    from elegant_amd import elements as E
+    els = [E.MAXAMP("AP", X_MAX=0.02, Y_MAX=0.02)]
+    for cell in range(12):
+        els += [E.DRIF("D1", L=2.0), E.KQUAD("QF", L=0.5, B=0.6, BORE=0.025, N_KICKS=4),
+                E.KSEXT("SF", L=0.2, K2=20.0), E.DRIF("D2", L=2.0),
+                E.KQUAD("QD", L=0.5, B=-0.6, BORE=0.025, N_KICKS=4), E.KSEXT("SD", L=0.2, K2=-35.0)]
+        if cell in (0, 6):
+            els.append(E.RFCA("RF", L=0.0, VOLT=3e6, FREQ=352e6, PHASE=150.0, CHANGE_P0=1))
+    bl = E.Beamline(els)
+    _, Po, rows = lossy_fodo(1500, seed=37)
+    g, ng, eg, c, nc, ec = run_both(tracker, bl, Po, rows, 12, "emath")
+    print("change_p0 ring: survivors", ng, "lost", rows.shape[0] - ng, "P_central", Po, "->", tracker.P_central)
+    assert ng == nc and eg == ec
+    assert rows.shape[0] - ng > 100
+    assert tracker.P_central != Po
+    assert_rows_identical(g, c, "change_p0 ring")
