@@ -1,35 +1,29 @@
 """Seeded lattice errors for error-seed studies.
 
-Restates the Gaussian branch of perturbation() (error.c:496-503): each error
-is amplitude * z with z a unit Gaussian redrawn until |z| <= cutoff
-(gauss_rn_lim, default cutoff 3, error.nl:23).  The toolkit's generator
-(random_1_elegant / gauss_rn_lim, drand_oag.c) is not vendored, so the draws
-come from the project's SplitMix64 stream: seeds are reproducible here but do
-not reproduce elegant's sequences.
+The errors come from the library's restatement of elegant's error
+generator: perturbation() (error.c:496-534) with Gaussian errors cut at
+`cutoff` sigma (gauss_rn_lim), drawing from the DLARAN uniform stream of
+random_1_elegant (drand_oag.c:94-180) seeded once per variant
+(generators.Rng, host/eh_gen.c).  The errors are asserted as four
+error_element definitions in this order: FSE, then DX, then DY, then TILT,
+each over every KQUAD and KSEXT in beamline order, one draw per element
+occurrence (error.c, assert_errors).  The Gaussian deviate itself follows
+the published polar method that the SDDS toolkit's gauss_rn implements; the
+toolkit is not part of the reference, so the sequences are elegant's where
+the reference ships the code and parity-unpinned for the Gaussian step.
 """
 import numpy as np
 
 from . import _abi as A
 from . import elements as E
-from .lattices import Rng
+from . import generators as G
 
 QUAD_F = ("QF1", "QF2", "QF3", "QF4")
 QUAD_D = ("QD1", "QD2")
 SEXT_F = ("SF1", "SF2")
 SEXT_D = ("SD1", "SD2")
 FAMILIES = (QUAD_F, QUAD_D, SEXT_F, SEXT_D)
-
-
-def gauss_cut(rng, n, cutoff=3.0):
-    out = np.empty(n)
-    filled = 0
-    while filled < n:
-        z = rng.normal(2 * (n - filled) + 8)
-        z = z[np.abs(z) <= cutoff]
-        take = min(n - filled, z.size)
-        out[filled:filled + take] = z[:take]
-        filled += take
-    return out
+SEED_BASE = 0x5EED0000
 
 
 def perturb(beamline, seed, fse=1e-3, dxy=30e-6, tilt=100e-6, cutoff=3.0, rf_reference=0.0):
@@ -37,25 +31,27 @@ def perturb(beamline, seed, fse=1e-3, dxy=30e-6, tilt=100e-6, cutoff=3.0, rf_ref
     and KSEXT, plus the 16 error-summary features (per family F/D quadrupoles
     and F/D sextupoles: RMS of FSE, DX, DY, TILT).  RFCA elements get
     T_REFERENCE = rf_reference so the variant needs no fiducial pass."""
-    rng = Rng(0x5EED0000 + seed)
-    els = []
+    rng = G.Rng(SEED_BASE + seed)
+    els = [A.eh_element.from_buffer_copy(e) for e in beamline.elements]
+    mults = [c for c in els if c.type in (A.EH_T_KQUAD, A.EH_T_KSEXT)]
+
+    def par(c):
+        return c.p.kquad if c.type == A.EH_T_KQUAD else c.p.ksext
+
+    for field, amp in (("fse", fse), ("dx", dxy), ("dy", dxy), ("tilt", tilt)):
+        for c in mults:
+            p = par(c)
+            setattr(p, field, getattr(p, field) + rng.perturbation(amp, cutoff, G.GAUSSIAN))
     per_family = {k: [] for k in range(4)}
-    for e in beamline.elements:
-        c = A.eh_element.from_buffer_copy(e)
+    for c in els:
         if c.type in (A.EH_T_KQUAD, A.EH_T_KSEXT):
-            p = c.p.kquad if c.type == A.EH_T_KQUAD else c.p.ksext
-            z = gauss_cut(rng, 4, cutoff)
-            p.fse += fse * z[0]
-            p.dx += dxy * z[1]
-            p.dy += dxy * z[2]
-            p.tilt += tilt * z[3]
+            p = par(c)
             name = c.name.decode()
             for k, fam in enumerate(FAMILIES):
                 if name in fam:
                     per_family[k].append((p.fse, p.dx, p.dy, p.tilt))
         elif c.type == A.EH_T_RFCA and rf_reference is not None:
             c.p.rfca.tReference = rf_reference
-        els.append(c)
     feats = []
     for k in range(4):
         arr = np.array(per_family[k]) if per_family[k] else np.zeros((1, 4))

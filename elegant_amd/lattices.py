@@ -187,25 +187,19 @@ def storage_ring(seed=1, cells=40, x_max=0.015, volt=6e6, freq=352e6, p_mev=6000
 
 def matched_beam(beamline, n, emit_x, emit_y, sigma_delta, sigma_t, seed, Po):
     """6D Gaussian rows (n, 8): transverse planes matched to the linear
-    one-turn map at the start, delta and t uncorrelated, s = c beta t,
-    particleID = 1..n, column 7 (pass of loss) = -1."""
+    one-turn map at the start, delta and s = c beta t uncorrelated,
+    particleID = 1..n, column 7 (pass of loss) = -1.  Drawn by the library's
+    restatement of elegant's bunch generator (eh_gaussian_bunch: DLARAN
+    stream seeded with seed, gaussian_distribution per plane, the
+    normalized-coordinate transform; generate_bunch.c), every plane cut at
+    10 sigma."""
+    from . import generators
     Mx, My = linear_maps(beamline)
     bx, ax, _ = twiss_from_map(Mx)
     by, ay, _ = twiss_from_map(My)
-    rng = Rng(seed)
-    z = rng.normal(6 * n).reshape(6, n)
-    rows = np.zeros((n, 8), dtype=np.float64)
-    sx, sy = math.sqrt(emit_x * bx), math.sqrt(emit_y * by)
-    rows[:, 0] = sx * z[0]
-    rows[:, 1] = (-ax * z[0] + z[1]) * math.sqrt(emit_x / bx)
-    rows[:, 2] = sy * z[2]
-    rows[:, 3] = (-ay * z[2] + z[3]) * math.sqrt(emit_y / by)
     beta = Po / math.sqrt(Po * Po + 1)
-    rows[:, 4] = C_MKS * beta * sigma_t * z[4]
-    rows[:, 5] = sigma_delta * z[5]
-    rows[:, 6] = np.arange(1, n + 1, dtype=np.float64)
-    rows[:, 7] = -1.0
-    return rows
+    return generators.gaussian_bunch(n, seed, (emit_x, bx, ax, 10.0), (emit_y, by, ay, 10.0),
+                                     (C_MKS * beta * sigma_t, sigma_delta, 0.0, 10.0))
 
 
 def fodo_workload(n=10000, seed=12345):

@@ -51,6 +51,31 @@ def test_zlongit_step_bit_exact(tracker):
     assert_rows_identical(g, c, "zlongit kick")
 
 
+def test_wake_table_from_the_example_sdds_file(tracker):
+    """the wake file the reference ships with its examples
+    (tests/golden/wake_Sx_200um_6mm_xband.sdds: 31 points 0.667 ps apart),
+    read by the SDDS reader, as the WAKE Green function of a 1.5 ps bunch:
+    histogram, voltages and kicked rows bit for bit as the checker's"""
+    import os
+    from elegant_amd import sdds
+    t, W = sdds.wake_table(os.path.join(os.path.dirname(__file__), "golden", "wake_Sx_200um_6mm_xband.sdds"))
+    w = E.WAKE(E.WakeTable(t, W))
+    n = 200000
+    rows = wake_bunch(n, 1.5e-12, seed=19)
+    q = 1e-9 / n
+    g = rows.copy()
+    h, v, tmin = tracker.collective_probe(g, w, PO, 0, q)
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    c = rows.copy()
+    o.wake(c, w.p.wake, PO, 0)
+    hc, vc, tminc, dt = o.last_hist()
+    assert h.size == hc.size > 10 and tmin == tminc
+    assert np.array_equal(h, hc) and h.sum() == n
+    assert np.array_equal(v, vc) and np.abs(v).max() > 0
+    assert_rows_identical(g, c, "example wake file")
+
+
 @pytest.mark.parametrize("nbins,bin_size", [(16384, 1e-12), (65536, 2e-13), (131072, 1e-13)])
 def test_zlongit_beyond_the_lds_transform_bit_exact(tracker, nbins, bin_size):
     """ZLONGIT with more bins than the one-workgroup LDS transform holds
