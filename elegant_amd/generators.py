@@ -18,6 +18,12 @@ class Rng:
         self._r = A.eh_rng()
         native.check(None, native.lib().eh_rng_seed(C.byref(self._r), int(iseed)))
 
+    def copy(self):
+        """an independent generator at the same point of the stream"""
+        r = Rng.__new__(Rng)
+        r._r = A.eh_rng.from_buffer_copy(self._r)
+        return r
+
     def uniform(self, n=None):
         f = native.lib().eh_rng_uniform
         if n is None:
@@ -30,8 +36,13 @@ class Rng:
             return f(C.byref(self._r), mean, sigma, limit)
         return np.array([f(C.byref(self._r), mean, sigma, limit) for _ in range(n)])
 
-    def perturbation(self, amplitude, cutoff=3.0, error_type=GAUSSIAN):
-        return native.lib().eh_perturbation(C.byref(self._r), amplitude, cutoff, error_type)
+    def perturbation(self, amplitude, cutoff=3.0, error_type=GAUSSIAN, n=None):
+        if n is None:
+            return native.lib().eh_perturbation(C.byref(self._r), amplitude, cutoff, error_type)
+        out = np.zeros(n)
+        native.check(None, native.lib().eh_perturbations(C.byref(self._r), amplitude, cutoff, error_type,
+                                                         native._dptr(out), n))
+        return out
 
     def gaussian_distribution(self, n, s1, s2, limit, symmetrize=False, limit_invar=0.0, beta=0.0, halo=False):
         rows = np.zeros((n, 2))

@@ -103,6 +103,14 @@ double eh_perturbation(eh_rng *r, double amplitude, double cutoff, int32_t error
   }
 }
 
+int eh_perturbations(eh_rng *r, double amplitude, double cutoff, int32_t error_type, double *out, int64_t n) {
+  if (!r || (!out && n > 0) || n < 0 || error_type < 0 || error_type > 2)
+    return EH_ERR_INVALID;
+  for (int64_t i = 0; i < n; i++)
+    out[i] = eh_perturbation(r, amplitude, cutoff, error_type);
+  return EH_OK;
+}
+
 /* gaussian_distribution() without Halton sequences (generate_bunch.c:629-657):
  * pairs (u, u') with sigmas s1, s2, kept when inside the cut (|u| <= s1 limit
  * and |u'| <= s2 limit, or u^2 + (beta u')^2 <= limit_invar when that is

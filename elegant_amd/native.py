@@ -90,6 +90,7 @@ SIGNATURES = [
     ("eh_gauss_rn", C.c_double, [C.POINTER(A.eh_rng)]),
     ("eh_gauss_rn_lim", C.c_double, [C.POINTER(A.eh_rng), C.c_double, C.c_double, C.c_double]),
     ("eh_perturbation", C.c_double, [C.POINTER(A.eh_rng), C.c_double, C.c_double, C.c_int32]),
+    ("eh_perturbations", C.c_int, [C.POINTER(A.eh_rng), C.c_double, C.c_double, C.c_int32, PD, C.c_int64]),
     ("eh_gaussian_distribution", C.c_int, [C.POINTER(A.eh_rng), PP, C.c_int64, C.c_int32, C.c_double, C.c_double,
                                            C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32]),
     ("eh_transform_from_normalized", None, [PP, C.c_int64, C.c_int32, C.c_double, C.c_double]),

@@ -421,6 +421,10 @@ double eh_gauss_rn(eh_rng *r);
 double eh_gauss_rn_lim(eh_rng *r, double mean, double sigma, double limit);
 /* perturbation() (error.c:496-534): error_type 0 uniform, 1 Gaussian with cutoff, 2 plus-or-minus */
 double eh_perturbation(eh_rng *r, double amplitude, double cutoff, int32_t error_type);
+/* n successive perturbation() values into out[0..n) (the calls an error
+ * definition makes over n element occurrences); no counterpart of its own in
+ * the reference, which calls perturbation() once per element */
+int eh_perturbations(eh_rng *r, double amplitude, double cutoff, int32_t error_type, double *out, int64_t n);
 /* gaussian_distribution() (generate_bunch.c:557-658, no Halton sequences) into columns offset, offset+1 */
 int eh_gaussian_distribution(eh_rng *r, double **particle, int64_t n, int32_t offset, double s1, double s2,
                              int32_t symmetrize, double limit, double limit_invar, double beta, int32_t halo);

@@ -7,6 +7,7 @@ import math
 
 import numpy as np
 
+from elegant_amd import _abi as A
 from elegant_amd import generators as G
 
 A48 = 33952834046453
@@ -76,3 +77,39 @@ def test_gaussian_bunch_has_the_requested_twiss_and_emittance():
         assert abs(-S[0, 1] / emit - a) < 0.03 * max(1, abs(a))
     assert abs(rows[:, 4].std() / 3e-3 - 1) < 0.02 and abs(rows[:, 5].std() / 1e-3 - 1) < 0.02
     assert np.array_equal(rows[:, 6], np.arange(1, n + 1)) and np.all(rows[:, 7] == -1)
+
+
+def test_perturbations_vector_matches_successive_calls():
+    from elegant_amd import generators as G
+    a, b = G.Rng(987654321), G.Rng(987654321)
+    for kind in (G.UNIFORM, G.GAUSSIAN, G.PLUS_OR_MINUS):
+        x = [a.perturbation(2.5e-4, 2.0, kind) for _ in range(101)]
+        y = b.perturbation(2.5e-4, 2.0, kind, n=101)
+        assert np.array_equal(np.array(x), y)
+
+
+def test_error_variants_are_steps_of_one_run():
+    """variant k carries the draws that follow those of variants 0..k-1 on
+    the run's stream: FSE, DX, DY, TILT in turn over every multipole, each
+    a Gaussian perturbation cut at 3 sigma"""
+    from elegant_amd import errors
+    from elegant_amd import generators as G
+    from elegant_amd import lattices as Lt
+    bl, Po = Lt.storage_ring(seed=1)
+    mults = [e for e in bl.elements if e.type in (A.EH_T_KQUAD, A.EH_T_KSEXT)]
+    r = G.Rng(errors.RUN_SEED)
+    r.perturbation(1.0, 3.0, G.GAUSSIAN, n=2 * 4 * len(mults))  # steps 0 and 1
+    z = r.perturbation(1.0, 3.0, G.GAUSSIAN, n=4 * len(mults)).reshape(4, len(mults))
+    v, feats = errors.perturb(bl, 2)
+    vm = [e for e in v.elements if e.type in (A.EH_T_KQUAD, A.EH_T_KSEXT)]
+    for j, (e0, e1) in enumerate(zip(mults, vm)):
+        p0 = e0.p.kquad if e0.type == A.EH_T_KQUAD else e0.p.ksext
+        p1 = e1.p.kquad if e1.type == A.EH_T_KQUAD else e1.p.ksext
+        assert p1.fse == p0.fse + 1e-3 * z[0, j] and p1.dx == p0.dx + 30e-6 * z[1, j]
+        assert p1.dy == p0.dy + 30e-6 * z[2, j] and p1.tilt == p0.tilt + 100e-6 * z[3, j]
+    assert np.max(np.abs(z)) <= 3.0
+    # distinct, reproducible, uncorrelated from one step to the next
+    fs = np.array([errors.perturb(bl, k)[1] for k in range(64)])
+    assert len({tuple(f) for f in fs}) == 64
+    assert np.array_equal(errors.perturb(bl, 5)[1], fs[5])
+    assert abs(np.corrcoef(fs[:-1, 0], fs[1:, 0])[0, 1]) < 0.4
