@@ -9,7 +9,7 @@
  * per-particle arithmetic.  The kernel reads records with wave-uniform
  * indices, which the compiler turns into scalar-cache loads.
  *
- * Layout: 16 int32 + 96 doubles = 832 bytes per record.
+ * Layout: 16 int32 + 97 doubles = 840 bytes per record.
  */
 #ifndef ELEGANT_AMD_EH_PROGRAM_H
 #define ELEGANT_AMD_EH_PROGRAM_H
@@ -56,6 +56,7 @@
 #define EH_F_ALONE (1 << 21)   /* single-element call: no limit_amplitudes cull after the element */
 #define EH_F_DFAST (1 << 22)   /* DRIF: 2^-300 <= |L| <= 2^300, the short form of the order-2 drift applies */
 #define EH_F_FIELD (1 << 23)   /* CSBEND: multipole field expansion (nfield coefficients at EH_D_FIELD) */
+#define EH_F_ETILT (1 << 24)   /* CSBEND: error tilt, exit offsets at EH_D_ET */
 
 /* Aperture shapes other than the closed rectangle, packed in one int32:
  * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the
@@ -86,7 +87,7 @@ typedef struct {
   double in_xmax, in_ymax;      /* in-element rectangle */
   double in_cs0, in_cs1;        /* cos/sin of the aperture's reverse tilt */
   /* element specific, indexed by the EH_D_* names below */
-  double d[87];
+  double d[88];
 } eh_op;
 
 /* ---- MULT ---- */
@@ -134,6 +135,7 @@ typedef struct {
 #define EH_D_RRHOA 54     /* 1 / rho_actual rounded to nearest */
 #define EH_D_FIELD 55     /* [<= 28]: field coefficients in the order the sums read them (eh_bend_field.h) */
 #define EH_D_FIELD_MAX 28 /* expansion order 6: K1..K4 and XKICK */
+#define EH_D_ET 83        /* [5]: ETILT exit offsets of x, x', y, y', s (eh_bend_etilt.h) */
 
 /* ---- RFCA ---- */
 #define EH_D_VOLT 0       /* volt / (1e6 me_mev relSign) / nKicks */

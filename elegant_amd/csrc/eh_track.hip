@@ -861,11 +861,20 @@ DEV int track_csbend(particle &p, const eh_op &op, const double *z) {
     double te = em_tan(op.d[EH_D_E2] - op.d[EH_D_PSI2] / (1 + dp));
     yp -= te / rho * y;
   }
-  p.x = x * ct - y * st + 0.0;
-  p.y = x * st + y * ct + 0.0;
-  p.xp = xp * ct - yp * st + 0.0;
-  p.yp = xp * st + yp * ct + 0.0;
-  p.s = s + 0.0;
+  /* the error tilt's exit offsets, zero without one (csbend.c:1359-1364) */
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0, o4 = 0.0;
+  if (flags & EH_F_ETILT) {
+    o0 = opc(&op.d[EH_D_ET]);
+    o1 = opc(&op.d[EH_D_ET + 1]);
+    o2 = opc(&op.d[EH_D_ET + 2]);
+    o3 = opc(&op.d[EH_D_ET + 3]);
+    o4 = opc(&op.d[EH_D_ET + 4]);
+  }
+  p.x = x * ct - y * st + o0;
+  p.y = x * st + y * ct + o2;
+  p.xp = xp * ct - yp * st + o1;
+  p.yp = xp * st + yp * ct + o3;
+  p.s = s + o4;
   p.dp = dp;
   p.x += op.d[EH_D_DXF] + op.d[EH_D_DZF] * p.xp;
   p.y += op.d[EH_D_DYF] + op.d[EH_D_DZF] * p.yp;

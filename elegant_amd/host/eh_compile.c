@@ -12,6 +12,7 @@
 
 #include "eh_consts.h"
 #include "eh_host.h"
+#include "eh_bend_etilt.h"
 #include "eh_bend_field.h"
 #include "emath.h"
 
@@ -441,9 +442,12 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
     op->d[0] = cb->length;
     return EH_OK;
   }
-  if (cb->yKick || cb->etilt || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian ||
-      cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 || cb->h2) {
-    snprintf(msg, msglen, "%s: CSBEND without YKICK, ETILT, radiation, H1/H2 or reference correction", e->name);
+  /* H1/H2 (pole-face curvature) enter only the second-order edge map and
+   * the K. Hwang fringe model (csbend.c:1150-1166, 1311-1325), which the
+   * EDGE_ORDER check below excludes: at EDGE_ORDER <= 1 they change nothing */
+  if (cb->yKick || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian || cb->synch_rad ||
+      cb->isr || cb->fseCorrection) {
+    snprintf(msg, msglen, "%s: CSBEND without YKICK, MALIGN_METHOD, radiation or reference correction", e->name);
     return EH_ERR_UNSUPPORTED;
   }
   if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1) ||
@@ -537,6 +541,11 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
     op->d[EH_D_ST] = 0;
   } else
     rotation_cs(ttilt, &op->d[EH_D_CT], &op->d[EH_D_ST]);
+  if (etilt) {
+    /* the error tilt's exit offsets (computeEtiltCentroidOffset, csbend.c:1075) */
+    eh_etilt_offsets(rho0, angle, etilt, tilt, &op->d[EH_D_ET]);
+    op->flags |= EH_F_ETILT;
+  }
   op->op = EH_OP_CSBEND;
   op->nslices = cb->nSlices;
   op->nsub = nsub;

@@ -39,6 +39,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "eh_bend_etilt.h"
 #include "eh_bend_field.h"
 #include "eh_consts.h"
 #include "eh_fixedsum.h"
@@ -865,6 +866,7 @@ long or_multipole_tracking2(double **particle, long n_part, const eh_element *el
 typedef struct {
   double rho0, rho_actual, e1, e2, psi1, psi2, tan_e1, tan_e2, cos_ttilt, sin_ttilt;
   double dxi, dyi, dzi, dxf, dyf, dzf, e1_kick_limit, e2_kick_limit;
+  double det[5]; /* the error tilt's exit offsets (computeEtiltCentroidOffset) */
   int edge1, edge2;
   int order1; /* expansion order + 1 of the field sums; 0: pure dipole */
   double Fx[EH_FIELD_MAX1][EH_FIELD_MAX1], Fy[EH_FIELD_MAX1][EH_FIELD_MAX1];
@@ -899,8 +901,9 @@ static void bend_fields(const bend_setup *b, double x, double y, double *Fx, dou
  * the dipole with normal multipoles K1..K4 and XKICK */
 static int csbend_setup(const EH_CSBEND *cb, bend_setup *b) {
   double angle, e1, e2, tilt, etilt, ttilt, fse, h, Kg1, Kg2, fint1, fint2;
-  if (cb->yKick || cb->etilt || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian ||
-      cb->synch_rad || cb->isr || cb->fseCorrection || cb->h1 || cb->h2)
+  /* H1/H2 act only in the second-order edge map (edge_order > 1, refused below) */
+  if (cb->yKick || cb->malignMethod || cb->referenceCorrection || cb->expandHamiltonian || cb->synch_rad ||
+      cb->isr || cb->fseCorrection)
     return EH_ERR_UNSUPPORTED;
   if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1) ||
       cb->edge_order > 1)
@@ -987,6 +990,7 @@ static int csbend_setup(const EH_CSBEND *cb, bend_setup *b) {
     b->cos_ttilt = cos(ttilt);
     b->sin_ttilt = sin(ttilt);
   }
+  eh_etilt_offsets(b->rho0, angle, etilt, tilt, b->det);
   b->dxi = -cb->dx;
   b->dzi = cb->dz;
   b->dyi = -cb->dy;
@@ -1154,11 +1158,11 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *cb, do
       xp += delta_xp;
       yp -= P_TAN(b.e2 - b.psi2 / (1 + dp)) / rho * y;
     }
-    coord[0] = x * b.cos_ttilt - y * b.sin_ttilt + 0.0;
-    coord[2] = x * b.sin_ttilt + y * b.cos_ttilt + 0.0;
-    coord[1] = xp * b.cos_ttilt - yp * b.sin_ttilt + 0.0;
-    coord[3] = xp * b.sin_ttilt + yp * b.cos_ttilt + 0.0;
-    coord[4] = s + 0.0;
+    coord[0] = x * b.cos_ttilt - y * b.sin_ttilt + b.det[0];
+    coord[2] = x * b.sin_ttilt + y * b.cos_ttilt + b.det[2];
+    coord[1] = xp * b.cos_ttilt - yp * b.sin_ttilt + b.det[1];
+    coord[3] = xp * b.sin_ttilt + yp * b.cos_ttilt + b.det[3];
+    coord[4] = s + b.det[4];
     coord[5] = dp;
     coord[0] += b.dxf + b.dzf * coord[1];
     coord[2] += b.dyf + b.dzf * coord[3];
@@ -1955,6 +1959,11 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
     c->effort += nToTrack;
   }
   return nToTrack;
+}
+
+/* computeEtiltCentroidOffset with the rotation by -tilt (test access) */
+void or_etilt_offsets(double rho0, double angle, double etilt, double tilt, double *out) {
+  eh_etilt_offsets(rho0, angle, etilt, tilt, out);
 }
 
 /* last WAKE/ZLONGIT histogram and voltage (test access) */

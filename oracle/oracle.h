@@ -72,6 +72,7 @@ long or_remove_invalid_particles(double **coord, long np, double z, double Po);
 void or_add_to_particle_energy(double *coord, double timeOfFlight, double Po, double dgamma);
 double or_kahan_plus(double oldSum, double b, double *error);
 void or_do_match_energy(double **coord, long np, double *P_central);
+void or_etilt_offsets(double rho0, double angle, double etilt, double tilt, double *out);
 
 /* do_tracking pass/element loop over a flattened beamline.  Returns nLeft,
  * or a negative EH_ERR_* code.  Lost rows carry the pass of loss in column 7. */

@@ -54,6 +54,7 @@ def lib(mode="glibc"):
         L.or_kahan_plus.restype = C.c_double
         L.or_kahan_plus.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_do_match_energy.argtypes = [PP, C.c_long, C.POINTER(C.c_double)]
+        L.or_etilt_offsets.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
         L.or_simple_rf_cavity.restype = C.c_long
         L.or_simple_rf_cavity.argtypes = [C.c_void_p, C.c_long, PP, C.c_long, C.POINTER(A.EH_RFCA),

@@ -481,6 +481,36 @@ def test_ring_with_exact_drifts_and_matrix_cavity(tracker):
 
 
 @pytest.mark.parametrize("kw", [
+    dict(L=1.0, ANGLE=0.1, N_KICKS=8, ETILT=1e-3),
+    dict(L=1.0, ANGLE=-0.1, N_KICKS=8, ETILT=-2e-3, TILT=0.3, E1=0.05, E2=0.02),
+    dict(L=0.8, ANGLE=0.2, N_KICKS=4, ETILT=5e-4, TILT=1.5707963267948966, K1=0.4, HGAP=0.01, FINT=0.5),
+    dict(L=1.0, ANGLE=0.15, N_KICKS=8, H1=0.3, H2=-0.2, E1=0.04, E2=0.04),
+])
+def test_csbend_error_tilt_and_pole_face_curvature(tracker, kw):
+    """ETILT: the tilt + error tilt rotation and the exit offsets of
+    computeEtiltCentroidOffset (csbend.c:1054-1075, 1359-1364), for both
+    signs of the angle, with TILT (a vertical bend among them), edges and a
+    gradient; H1/H2, which the first-order edges do not use: bit for bit
+    against the checker on rows of every kind, signs of zeros included"""
+    b = E.CSBEND(**kw).p.csbend
+    r = csbend_edge_rows(1000, 15)
+    g = r.copy()
+    c = r.copy()
+    ng = tracker.track_through_csbend(g, b, PO, 3.0, None)
+    ptr, keep = rows_ptr(c)
+    nc = O.lib("emath").or_track_through_csbend(ptr, c.shape[0], C.byref(b), PO, 3.0, None)
+    assert_same_survivors(r, g, ng, c, nc)
+    assert_rows_bitwise(g[:, :7], c[:, :7], "csbend %r" % kw)
+    if "H1" in kw:
+        plain = dict(kw)
+        plain.pop("H1")
+        plain.pop("H2")
+        h = r.copy()
+        tracker.track_through_csbend(h, E.CSBEND(**plain).p.csbend, PO, 3.0, None)
+        assert_rows_bitwise(g[:, :7], h[:, :7], "H1/H2 change nothing at EDGE_ORDER=1")
+
+
+@pytest.mark.parametrize("kw", [
     dict(L=1.0, ANGLE=0.08, K1=0.7, N_KICKS=16, E1=0.02, E2=0.04),
     dict(L=0.8, ANGLE=-0.785398163397, E1=-0.445, E2=-0.445, K2=0.1375, K3=100.0, HGAP=0.0225, FINT=0.41,
          N_KICKS=10),

@@ -888,3 +888,43 @@ def test_match_energy_empty_beam():
     rows[:, 5] = 0.01
     before = rows.copy()
     assert _match(rows, 0, 100.0) == 100.0 and np.array_equal(rows, before)
+
+
+def _etilt_geometry(rho0, angle, etilt, tilt):
+    """where the orbit of a sector bend rotated by etilt about its entrance
+    axis crosses the nominal exit plane, in the nominal exit frame (x, x',
+    y, y', distance along the orbit to the plane), then turned by the bend's
+    tilt into the lab frame; bend toward -x, rotation right-handed about z"""
+    th = angle
+    P = np.array([-(1 - math.cos(th)) * rho0, 0.0, rho0 * math.sin(th)])
+    D = np.array([-math.sin(th), 0.0, math.cos(th)])
+    R = np.array([[math.cos(etilt), -math.sin(etilt), 0], [math.sin(etilt), math.cos(etilt), 0], [0, 0, 1]])
+    Pt, Dt = R @ P, R @ D
+    t = np.dot(P - Pt, D) / np.dot(Dt, D)
+    Q = Pt + t * Dt
+    ze, ye = D, np.array([0.0, 1.0, 0.0])
+    xe = np.cross(ye, ze)
+    x, y = np.dot(Q - P, xe), np.dot(Q - P, ye)
+    xp, yp = np.dot(Dt, xe) / np.dot(Dt, ze), np.dot(Dt, ye) / np.dot(Dt, ze)
+    c, s = math.cos(tilt), math.sin(tilt)
+    return np.array([x * c - y * s, xp * c - yp * s, x * s + y * c, xp * s + yp * c, t])
+
+
+@pytest.mark.parametrize("rho0,angle,etilt,tilt", [
+    (5.0, 0.3, 0.01, 0.0), (20.0, 0.05, 1e-4, 0.0), (100.0, 0.02, -0.003, 0.0), (2.0, 0.6, 2e-3, 0.3),
+    (8.0, 0.25, -5e-4, math.pi / 2), (1.0, 1.2, 1e-3, -1.0)])
+def test_etilt_offsets_against_geometry(rho0, angle, etilt, tilt):
+    """computeEtiltCentroidOffset (csbend.c:4723-4776) with the rotation by
+    -tilt: the offsets agree with the exact crossing of the rotated orbit
+    and the nominal exit plane (for the small error tilts the formula is
+    written for: the reference takes the unnormalised direction component as
+    y', exact to first order), and vanish without an error tilt"""
+    L = O.lib("glibc")
+    out = (C.c_double * 5)()
+    L.or_etilt_offsets(rho0, angle, etilt, tilt, out)
+    got = np.array(out[:])
+    want = _etilt_geometry(rho0, angle, etilt, tilt)
+    scale = np.maximum(np.abs(want), 1e-6 * np.max(np.abs(want)))
+    assert np.max(np.abs(got - want) / scale) < 1e-6, (got, want)
+    L.or_etilt_offsets(rho0, angle, 0.0, tilt, out)
+    assert list(out) == [0.0] * 5
