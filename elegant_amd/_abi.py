@@ -100,14 +100,15 @@ class EH_RFCA(C.Structure):
 class EH_WAKE(C.Structure):
     _fields_ = [("t", C.POINTER(D)), ("W", C.POINTER(D)), ("wakePoints", L), ("factor", D)] + [
         (n, I) for n in ["n_bins", "interpolate", "rampPasses", "allowLongBeam", "smoothing",
-                         "change_p0", "bunchedBeamMode", "pad"]]
+                         "change_p0", "bunchedBeamMode", "startBunch", "endBunch", "pad"]]
 
 
 class EH_ZLONGIT(C.Structure):
     _fields_ = _fields(
         ["Ra", "Rs", "Q", "freq", "bin_size", "factor"],
         ["broad_band", "n_bins", "max_n_bins", "interpolate", "area_weight", "startOnPass",
-         "rampPasses", "reverseTimeOrder", "smoothing", "allowLongBeam", "bunchedBeamMode", "pad"])
+         "rampPasses", "reverseTimeOrder", "smoothing", "allowLongBeam", "bunchedBeamMode", "startBunch",
+         "endBunch", "pad"])
 
 
 class EH_MAXAMP(C.Structure):

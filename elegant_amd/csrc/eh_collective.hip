@@ -75,7 +75,8 @@ typedef struct {
 __global__ void __launch_bounds__(CB) coll_plan_minmax(coll_plan *plan, const unsigned long long *part,
                                                        int64_t nparts, int32_t kind, int32_t n_fixed, int32_t zl_nb,
                                                        double dt, double wake_span, int32_t allow_long,
-                                                       int32_t nb_cap, unsigned int *hist, int32_t nclear) {
+                                                       int32_t nb_cap, unsigned int *hist, int32_t nclear,
+                                                       int32_t mean_given, double mean) {
   __shared__ unsigned long long smin[CB], smax[CB], scnt[CB];
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
   /* the step starts here: clear the histogram (a kernel rather than a
@@ -130,7 +131,11 @@ __global__ void __launch_bounds__(CB) coll_plan_minmax(coll_plan *plan, const un
       return;
     }
     if (n_fixed) {
-      p->nb = n_fixed; /* tmin set from the mean */
+      p->nb = n_fixed; /* tmin set from the mean: the fixed-point one, or the host's */
+      if (mean_given) {
+        p->tmean = mean;
+        p->tmin = mean - dt * p->nb / 2.0;
+      }
     } else {
       int32_t nb = (int32_t)(int64_t)((tmax - tmin) / dt + 3);
       p->tmin = tmin - dt;
@@ -150,9 +155,16 @@ __global__ void __launch_bounds__(CB) coll_plan_minmax(coll_plan *plan, const un
 /* per-workgroup (min key, max key, alive count) triples in red[3 k ..];
  * grid-stride, so there are at most a thousand of them for the planning
  * kernel to combine (no global atomics) */
+/* a particle takes part in the step when it is alive (lp < 0, or no lp)
+ * and, for one bunch of a bunched beam, carries that bunch index */
+DEV bool in_step(const int32_t *__restrict__ lp, const int32_t *__restrict__ bunch, int32_t target, int64_t i) {
+  return (!lp || lp[i] < 0) && (!bunch || bunch[i] == target);
+}
+
 __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, const double *__restrict__ dp,
                                                   const int32_t *__restrict__ lp, int64_t n, double Po,
-                                                  unsigned long long *red) {
+                                                  unsigned long long *red, const int32_t *__restrict__ bunch,
+                                                  int32_t target) {
   __shared__ unsigned long long smin[CB / 64], smax[CB / 64], scnt[CB / 64];
   unsigned long long kmin = ~0ULL, kmax = 0, cnt = 0;
   const int64_t stride = (int64_t)gridDim.x * CB;
@@ -163,7 +175,7 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
     bool live[SWEEP_U];
 #pragma unroll
     for (int u = 0; u < SWEEP_U; u++) {
-      live[u] = !lp || lp[i + u * stride] < 0;
+      live[u] = in_step(lp, bunch, target, i + u * stride);
       S[u] = s[i + u * stride];
       D[u] = dp[i + u * stride];
     }
@@ -177,7 +189,7 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
       }
   }
   for (; i < n; i += stride) {
-    if (!lp || lp[i] < 0) {
+    if (in_step(lp, bunch, target, i)) {
       unsigned long long k = order_key(arrival_time(s[i], dp[i], Po));
       kmin = k < kmin ? k : kmin;
       kmax = k > kmax ? k : kmax;
@@ -212,7 +224,8 @@ __global__ void __launch_bounds__(CB) time_minmax(const double *__restrict__ s, 
 /* per-workgroup fixed-point partial sums of the arrival times */
 __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, const double *__restrict__ dp,
                                                 const int32_t *__restrict__ lp, int64_t n, double Po,
-                                                const coll_plan *plan, eh_fsum *partial) {
+                                                const coll_plan *plan, eh_fsum *partial,
+                                                const int32_t *__restrict__ bunch, int32_t target) {
   __shared__ eh_fsum sh[CB];
   if (plan->nb <= 0 || plan->err)
     return;
@@ -226,7 +239,7 @@ __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, co
     bool live[SWEEP_U];
 #pragma unroll
     for (int u = 0; u < SWEEP_U; u++) {
-      live[u] = !lp || lp[i + u * stride] < 0;
+      live[u] = in_step(lp, bunch, target, i + u * stride);
       S[u] = s[i + u * stride];
       D[u] = dp[i + u * stride];
     }
@@ -236,7 +249,7 @@ __global__ void __launch_bounds__(CB) time_fsum(const double *__restrict__ s, co
         eh_fsum_add(&a, arrival_time(S[u], D[u], Po));
   }
   for (; i < n; i += stride)
-    if (!lp || lp[i] < 0)
+    if (in_step(lp, bunch, target, i))
       eh_fsum_add(&a, arrival_time(s[i], dp[i], Po));
   sh[threadIdx.x] = a;
   __syncthreads();
@@ -283,7 +296,8 @@ DEV int64_t bin_of(double t, double tmin, double dt, int round_half) {
 __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, const double *__restrict__ dp,
                                                 const int32_t *__restrict__ lp, int64_t n, double Po,
                                                 const coll_plan *plan, double dt, int32_t round_half,
-                                                int32_t lds_bins, unsigned int *hist) {
+                                                int32_t lds_bins, unsigned int *hist, const int32_t *__restrict__ bunch,
+                                                int32_t target) {
   extern __shared__ unsigned int lh[];
   const int32_t nb = plan->nb;
   const double tmin = plan->tmin;
@@ -306,7 +320,7 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
     bool live[SWEEP_U];
 #pragma unroll
     for (int u = 0; u < SWEEP_U; u++) {
-      live[u] = !lp || lp[i + u * stride] < 0;
+      live[u] = in_step(lp, bunch, target, i + u * stride);
       S[u] = s[i + u * stride];
       D[u] = dp[i + u * stride];
     }
@@ -319,7 +333,7 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
       }
   }
   for (; i < n; i += stride) {
-    if (lp && lp[i] >= 0)
+    if (!in_step(lp, bunch, target, i))
       continue;
     int64_t ib = bin_of(arrival_time(s[i], dp[i], Po), tmin, dt, round_half);
     if (ib >= 0 && ib <= nb - 1)
@@ -846,23 +860,35 @@ DEV void add_energy(double &xp, double &yp, double &s, double &dp, double tof, d
 }
 
 /* applyLongitudinalWakeKicks (wake.c:305-329) / the ZLONGIT kick loop
- * (zlongit.c:454-486), then the post-element checks of do_tracking */
+ * (zlongit.c:454-486), then the post-element checks of do_tracking.  kind < 0:
+ * the checks alone (a bunch the element leaves out), plan and volt unused. */
 __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, double *yp, double *s, double *dp,
                                            int32_t *lp, int32_t *lc, int64_t n, double Po, const coll_plan *plan,
                                            double dt, const double *__restrict__ volt, int32_t kind,
                                            int32_t interpolate, double z_end, double ap_x, double ap_y,
-                                           int32_t ap_on, int32_t ap_code, int64_t pass, int32_t code_base) {
+                                           int32_t ap_on, int32_t ap_code, int64_t pass, int32_t code_base,
+                                           const int32_t *__restrict__ bunch, int32_t target) {
   int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
-  const int32_t nb = plan->nb;
-  const double tmin = plan->tmin;
-  if (nb <= 0 || plan->err || i >= n || (lp && lp[i] >= 0))
+  if (i >= n || !in_step(lp, bunch, target, i))
     return;
+  int32_t nb = 0;
+  double tmin = 0;
+  if (kind >= 0) {
+    nb = plan->nb;
+    tmin = plan->tmin;
+    if (nb <= 0 || plan->err)
+      return;
+  }
   const double mc2 = 1e6 * EH_ME_MEV * EH_REL_SIGN;
   double S = s[i], D = dp[i];
-  double t = arrival_time(S, D, Po);
-  int64_t ib = bin_of(t, tmin, dt, kind == EH_OP_WAKE);
   double XP = xp[i], YP = yp[i];
   bool changed = false;
+  double t = 0;
+  int64_t ib = -1;
+  if (kind >= 0) {
+    t = arrival_time(S, D, Po);
+    ib = bin_of(t, tmin, dt, kind == EH_OP_WAKE);
+  }
   if (ib >= 0 && ib <= nb - 1) {
     double dgam;
     bool skip = false;
@@ -976,6 +1002,10 @@ void ehd_free_collective(ehd_dev *d) {
     (void)hipFree(d->cwork);
   d->cwork = NULL;
   d->cwork_bytes = 0;
+  if (d->bunch)
+    (void)hipFree(d->bunch);
+  d->bunch = NULL;
+  d->bunch_cap = 0;
 }
 
 int ehd_put_wake(ehd_dev *d, int32_t slot, const double *W, int64_t npoints) {
@@ -1069,14 +1099,19 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   const int32_t nclear = (int32_t)(nb_cap + 8);
   /* without a loss-pass array nothing may be marked lost: no checks */
   const int32_t ap_on = (!lp || (c->flags & EHD_COLL_NOCHECKS)) ? -1 : ((c->flags & EH_F_AP) ? 1 : 0);
+  /* one bunch of a bunched beam: only its particles take part */
+  const int32_t *bunch = c->use_bunch ? d->bunch : NULL;
+  const int32_t target = c->bunch_target;
+  const int32_t mean_given = kind == EH_OP_WAKE && fixed && c->mean_given;
   CHK(hipEventRecord(d->cev0, d->stream));
   hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po,
-                     (unsigned long long *)parts);
+                     (unsigned long long *)parts, bunch, target);
   hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)parts, rgrid,
-                     kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max, hist, nclear);
-  if (kind == EH_OP_ZLONGIT || fixed) {
+                     kind, fixed, c->nb, c->dt, c->wake_span, c->allow_long_beam, (int32_t)nb_max, hist, nclear,
+                     mean_given, c->mean);
+  if ((kind == EH_OP_ZLONGIT || fixed) && !mean_given) {
     hipLaunchKernelGGL(time_fsum, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, crd[4], crd[5], lp, n, c->Po, plan,
-                       parts);
+                       parts, bunch, target);
     hipLaunchKernelGGL(fsum_final, dim3(1), dim3(CB), 0, d->stream, parts, rgrid, plan, c->dt);
   }
   {
@@ -1097,7 +1132,7 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
     /* room for one copy of the counts per wave, at most 48 KB of LDS */
     const int32_t lds_bins = nb_max * (CB / 64) < 12288 ? (int32_t)(nb_max * (CB / 64)) : 12288;
     hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, crd[4], crd[5],
-                       lp, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins, hist);
+                       lp, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins, hist, bunch, target);
   }
   if (kind == EH_OP_WAKE) {
     hipLaunchKernelGGL(wake_convolve, dim3((unsigned)((nb_max + 1 + CONV_B - 1) / CONV_B)), dim3(CONV_B), 0, d->stream,
@@ -1116,7 +1151,7 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   if (n > 0)
     hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, crd[0], crd[1], crd[2], crd[3], crd[4],
                        crd[5], lp, lc, n, c->Po, plan, c->dt, volt, kind, c->interpolate, c->z_end, c->ap_xmax,
-                       c->ap_ymax, ap_on, c->ap_code, c->pass, 4 * c->op_index);
+                       c->ap_ymax, ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, target);
   CHK(hipGetLastError());
   CHK(hipEventRecord(d->cev1, d->stream));
   CHK(hipMemcpyAsync(hp, plan, sizeof(*hp), hipMemcpyDeviceToHost, d->stream));
@@ -1125,6 +1160,24 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   CHK(hipEventElapsedTime(&ms, d->cev0, d->cev1));
   d->coll_ms += ms;
   d->coll_launches += 1;
+  return EH_OK;
+}
+
+/* the post-element checks alone for the particles of one bunch that a
+ * WAKE/ZLONGIT leaves out (START_BUNCH/END_BUNCH) */
+int ehd_collective_checks(ehd_dev *d, const ehd_collective *c) {
+  const int64_t n = d->n;
+  CHK(hipSetDevice(d->device));
+  if (n <= 0 || !d->loss_pass || (c->flags & EHD_COLL_NOCHECKS))
+    return EH_OK;
+  const int32_t ap_on = (c->flags & EH_F_AP) ? 1 : 0;
+  const int32_t *bunch = c->use_bunch ? d->bunch : NULL;
+  hipLaunchKernelGGL(kick, dim3((unsigned)((n + CB - 1) / CB)), dim3(CB), 0, d->stream, d->coord[0], d->coord[1],
+                     d->coord[2], d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po,
+                     (const coll_plan *)NULL, 0.0, (const double *)NULL, -1, 0, c->z_end, c->ap_xmax, c->ap_ymax,
+                     ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, c->bunch_target);
+  CHK(hipGetLastError());
+  CHK(hipStreamSynchronize(d->stream));
   return EH_OK;
 }
 
@@ -1217,7 +1270,8 @@ int ehd_wake_step_noalloc(ehd_dev *d, const ehd_collective *c, double *const crd
   return hp.err ? hp.err : EH_OK;
 }
 
-int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count) {
+int ehd_time_span(ehd_dev *d, double Po, int32_t use_bunch, int32_t target, double *tmin, double *tmax,
+                  int64_t *count) {
   const int64_t n = d->n;
   const int64_t rgrid = reduce_grid(n);
   CHK(hipSetDevice(d->device));
@@ -1228,9 +1282,9 @@ int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *co
   unsigned int *hist = (unsigned int *)((char *)d->cwork + PLAN_BYTES);
   unsigned long long *mm = (unsigned long long *)((char *)hist + hist_bytes(64) + 8 * (64 + 8));
   hipLaunchKernelGGL(time_minmax, dim3((unsigned)rgrid), dim3(CB), 0, d->stream, d->coord[4], d->coord[5],
-                     d->loss_pass, n, Po, mm);
+                     d->loss_pass, n, Po, mm, use_bunch ? d->bunch : NULL, target);
   hipLaunchKernelGGL(coll_plan_minmax, dim3(1), dim3(CB), 0, d->stream, plan, (const unsigned long long *)mm, rgrid,
-                     -1, 0, 0, 1.0, 0.0, 0, 0, hist, 0);
+                     -1, 0, 0, 1.0, 0.0, 0, 0, hist, 0, 0, 0.0);
   CHK(hipGetLastError());
   coll_plan hp;
   CHK(hipMemcpyAsync(&hp, plan, sizeof(hp), hipMemcpyDeviceToHost, d->stream));
@@ -1238,6 +1292,25 @@ int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *co
   *count = (int64_t)hp.red[2];
   *tmin = key_value(hp.red[0]);
   *tmax = key_value(hp.red[1]);
+  return EH_OK;
+}
+
+/* the bunch index of every particle (column 8 of the rows), for the
+ * collective steps of a bunched beam */
+int ehd_put_bunch(ehd_dev *d, const int32_t *bunch, int64_t n) {
+  if (n != d->n || (n > 0 && !bunch))
+    return EH_ERR_INVALID;
+  CHK(hipSetDevice(d->device));
+  if (n > d->bunch_cap) {
+    if (d->bunch)
+      CHK(hipFree(d->bunch));
+    d->bunch = NULL;
+    d->bunch_cap = 0;
+    CHK(hipMalloc((void **)&d->bunch, sizeof(int32_t) * (size_t)n));
+    d->bunch_cap = n;
+  }
+  if (n > 0)
+    CHK(hipMemcpy(d->bunch, bunch, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
   return EH_OK;
 }
 

@@ -89,6 +89,10 @@ typedef struct {
   double wake_span;     /* WAKE: t[last] - t[0] (long-beam check) */
   int32_t allow_long_beam;
   int32_t ap_code;      /* EH_AP_* shape of the MAXAMP check, 0 = rectangle */
+  int32_t use_bunch;    /* one bunch of a bunched beam: the particles with bunch index bunch_target */
+  int32_t bunch_target;
+  int32_t mean_given;   /* WAKE with N_BINS: the bucket's mean time from the host (a plain sum) */
+  double mean;
 } ehd_collective;
 
 /* WAKE Green function samples; ZLONGIT impedance Z[nb] and twiddles */
@@ -100,8 +104,12 @@ int ehd_put_zlongit(ehd_dev *d, int32_t slot, const double *Z, const double *tc,
 int ehd_collective_step(ehd_dev *d, int32_t slot, const ehd_collective *c, double *hist, double *volt,
                         int64_t hist_cap, int64_t *nb_out, double *tmin_out);
 int ehd_collective_timing(ehd_dev *d, int64_t *launches, double *ms);
+/* the post-element checks alone, for the bunch a step leaves out */
+int ehd_collective_checks(ehd_dev *d, const ehd_collective *c);
 /* exact min/max of the alive particles' arrival times and their count */
-int ehd_time_span(ehd_dev *d, double Po, double *tmin, double *tmax, int64_t *count);
+int ehd_time_span(ehd_dev *d, double Po, int32_t use_bunch, int32_t target, double *tmin, double *tmax,
+                  int64_t *count);
+int ehd_put_bunch(ehd_dev *d, const int32_t *bunch, int64_t n);
 /* The WAKE step without any allocation: device SoA particle arrays crd[0..5]
  * of n particles (all alive, no checks after the step) and the Green
  * function W[wpoints] on the device, owned by the caller; the step's scratch

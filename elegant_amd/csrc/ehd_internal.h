@@ -48,6 +48,8 @@ struct ehd_dev {
   /* collective work buffers */
   ehd_coll_slot slot[EHD_MAX_COLL];
   void *cwork;       /* reductions, histogram, voltages */
+  int32_t *bunch;    /* bunch index per particle (bunched beams) */
+  int64_t bunch_cap;
   void *noalloc;     /* device address of this GPU's static no-allocation scratch */
   int64_t cwork_bytes;
   /* batched variants */

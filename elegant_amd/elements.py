@@ -51,13 +51,14 @@ _NAMES = {
                   "FIDUCIAL": "fiducialMode"},
     A.EH_T_WAKE: {"FACTOR": "factor", "N_BINS": "n_bins", "INTERPOLATE": "interpolate",
                   "RAMP_PASSES": "rampPasses", "ALLOW_LONG_BEAM": "allowLongBeam", "SMOOTHING": "smoothing",
-                  "CHANGE_P0": "change_p0", "BUNCHED_BEAM_MODE": "bunchedBeamMode"},
+                  "CHANGE_P0": "change_p0", "BUNCHED_BEAM_MODE": "bunchedBeamMode", "START_BUNCH": "startBunch",
+                  "END_BUNCH": "endBunch"},
     A.EH_T_ZLONGIT: {"RA": "Ra", "RS": "Rs", "Q": "Q", "FREQ": "freq", "BIN_SIZE": "bin_size", "FACTOR": "factor",
                      "BROAD_BAND": "broad_band", "N_BINS": "n_bins", "MAX_N_BINS": "max_n_bins",
                      "INTERPOLATE": "interpolate", "AREA_WEIGHT": "area_weight", "START_ON_PASS": "startOnPass",
                      "RAMP_PASSES": "rampPasses", "REVERSE_TIME_ORDER": "reverseTimeOrder",
                      "SMOOTHING": "smoothing", "ALLOW_LONG_BEAM": "allowLongBeam",
-                     "BUNCHED_BEAM_MODE": "bunchedBeamMode"},
+                     "BUNCHED_BEAM_MODE": "bunchedBeamMode", "START_BUNCH": "startBunch", "END_BUNCH": "endBunch"},
     A.EH_T_MAXAMP: {"X_MAX": "x_max", "Y_MAX": "y_max", "ELLIPTICAL": "elliptical", "EXPONENT": "exponent",
                     "YEXPONENT": "yExponent", "OPEN_SIDE": "openSide"},
     A.EH_T_CHARGE: {"TOTAL": "charge", "PER_PARTICLE": "chargePerParticle"},
@@ -73,8 +74,8 @@ _DEFAULTS = {
     A.EH_T_CSBEND: {"fint": 0.5, "fint1": -1.0, "fint2": -1.0, "nSlices": 4, "etiltSign": 1, "nonlinear": 1,
                     "edge1_effects": 1, "edge2_effects": 1, "edge_order": 1, "integration_order": 4},
     A.EH_T_RFCA: {"freq": 500e6, "tReference": -1.0, "fiducialMode": A.EH_FID_TMEAN},
-    A.EH_T_WAKE: {"factor": 1.0, "bunchedBeamMode": 1},
-    A.EH_T_ZLONGIT: {"n_bins": 128, "factor": 1.0, "bunchedBeamMode": 1},
+    A.EH_T_WAKE: {"factor": 1.0, "bunchedBeamMode": 1, "startBunch": -1, "endBunch": -1},
+    A.EH_T_ZLONGIT: {"n_bins": 128, "factor": 1.0, "bunchedBeamMode": 1, "startBunch": -1, "endBunch": -1},
     A.EH_T_MAXAMP: {"exponent": 2},
 }
 

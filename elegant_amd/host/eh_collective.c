@@ -23,6 +23,11 @@ char *eh_ctx_msg(eh_ctx *c);
 int *eh_ctx_coll_loaded(eh_ctx *c);
 int32_t *eh_ctx_zl_nbins(eh_ctx *c);
 double *eh_ctx_zl_binsize(eh_ctx *c);
+int32_t eh_ctx_row_props(eh_ctx *c);
+int eh_ctx_buckets(eh_ctx *c, long *ib_min);
+int64_t eh_ctx_bucket_count(const eh_ctx *c, long b);
+int eh_ctx_bunch_to_device(eh_ctx *c);
+int eh_ctx_bucket_mean(eh_ctx *c, int64_t pass, int32_t code_lim, long b, double Po, double *mean);
 const char *ehd_message(const ehd_dev *d);
 int eh_upload(eh_ctx *c, double **coord, int64_t n, int32_t nprops);
 int eh_upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cull);
@@ -101,7 +106,7 @@ static int wake_setup(eh_ctx *c, int slot, const EH_WAKE *w, double *dt, double 
 }
 
 /* set_up_zlongit, broad-band branch (zlongit.c:587-632), done once per element */
-static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po) {
+static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po, int32_t use_bunch, int32_t target) {
   ehd_dev *d = eh_ctx_dev(c);
   double tmin, tmax, bin_size, df, Ra;
   long n_bins, nfreq;
@@ -113,7 +118,7 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po) {
     return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: broad-band resonator without AREA_WEIGHT/SMOOTHING/REVERSE only");
   if (zl->bin_size <= 0 || (zl->Ra && zl->Rs) || zl->n_bins % 2 != 0)
     return msgf(c, EH_ERR_INVALID, "ZLONGIT: BIN_SIZE, RA/RS or N_BINS invalid");
-  rc = ehd_time_span(d, Po, &tmin, &tmax, &count);
+  rc = ehd_time_span(d, Po, use_bunch, target, &tmin, &tmax, &count);
   if (rc)
     return msgf(c, rc, ehd_message(d));
   Ra = zl->Ra ? zl->Ra : 2 * zl->Rs;
@@ -166,12 +171,32 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po) {
   return EH_OK;
 }
 
+/* START_BUNCH/END_BUNCH (wake.c:84-88, zlongit.c:142-146): bucket b takes no
+ * wake or impedance */
+static int bucket_left_out(int32_t bunched, int32_t start, int32_t end, long b) {
+  return bunched && ((start >= 0 && b < start) || (end >= 0 && b > end));
+}
+
+/* One WAKE/ZLONGIT element: the whole beam as one bucket, or with a bunched
+ * beam (BUNCHED_BEAM_MODE and id slots per bunch) one step per bucket of the
+ * bunch index in column 8, in bucket order (index_bunch_assignments,
+ * lrwake.c:17-233; wake.c:84-117; zlongit.c:142-195).  A bucket's step sees
+ * only its own particles; its post-element checks are those of the
+ * element, particle by particle, so checking each bucket after its kick
+ * marks the same losses with the same codes as checking all at the end. */
 static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op *op, int op_index, int64_t pass,
                           double macro, int flags, double *hist, double *volt, int64_t *nb, double *tmin) {
   ehd_collective cc;
-  int rc;
+  ehd_dev *d = eh_ctx_dev(c);
+  const int is_wake = e->type == EH_T_WAKE;
+  const int32_t bunched = is_wake ? e->p.wake.bunchedBeamMode : e->p.zlongit.bunchedBeamMode;
+  const int32_t b_start = is_wake ? e->p.wake.startBunch : e->p.zlongit.startBunch;
+  const int32_t b_end = is_wake ? e->p.wake.endBunch : e->p.zlongit.endBunch;
+  long ib_min = 0;
+  int nbk = 1, rc;
+  long i_pass = (long)pass;
   memset(&cc, 0, sizeof(cc));
-  cc.kind = e->type == EH_T_WAKE ? EH_OP_WAKE : EH_OP_ZLONGIT;
+  cc.kind = is_wake ? EH_OP_WAKE : EH_OP_ZLONGIT;
   cc.op_index = op_index;
   cc.pass = pass;
   cc.Po = op->Po;
@@ -180,7 +205,7 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
   cc.ap_ymax = op->ap_ymax;
   cc.flags = (op->flags & EH_F_AP) | flags;
   cc.ap_code = op->ap_code;
-  if (e->type == EH_T_WAKE) {
+  if (is_wake) {
     const EH_WAKE *w = &e->p.wake;
     double dt, span;
     rc = wake_setup(c, slot, w, &dt, &span);
@@ -195,24 +220,62 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
     cc.factor = macro * EH_REL_SIGN * w->factor * ramp_factor((long)pass, w->rampPasses);
   } else {
     const EH_ZLONGIT *zl = &e->p.zlongit;
-    long i_pass = (long)pass - zl->startOnPass;
+    i_pass = (long)pass - zl->startOnPass;
     if (i_pass < 0 || zl->factor == 0)
       return EH_OK;
-    rc = zlongit_setup(c, slot, zl, op->Po);
-    if (rc)
-      return rc;
-    cc.nb = eh_ctx_zl_nbins(c)[slot];
-    cc.dt = eh_ctx_zl_binsize(c)[slot];
     cc.interpolate = zl->interpolate;
     cc.allow_long_beam = zl->allowLongBeam;
-    /* factor = macroParticleCharge * particleRelSign / dt * FACTOR * rampFactor (zlongit.c:377) */
-    cc.factor = macro * EH_REL_SIGN / cc.dt * zl->factor * ramp_factor(i_pass, zl->rampPasses);
   }
-  rc = ehd_collective_step(eh_ctx_dev(c), slot, &cc, hist, volt, 0, nb, tmin);
-  if (rc == EH_ERR_LONGBEAM)
-    return msgf(c, rc, "bunch longer than the wake/impedance window (wake.c:134-140, zlongit.c:219-236)");
-  if (rc)
-    return msgf(c, rc, ehd_message(eh_ctx_dev(c)));
+  if (bunched) {
+    nbk = eh_ctx_buckets(c, &ib_min);
+    if (nbk < 0)
+      return nbk;
+    if (nbk > 1) {
+      rc = eh_ctx_bunch_to_device(c);
+      if (rc)
+        return rc;
+    }
+  }
+  for (long b = 0; b < nbk; b++) {
+    if (nbk > 1) {
+      if (!eh_ctx_bucket_count(c, ib_min + b))
+        continue;
+      cc.use_bunch = 1;
+      cc.bunch_target = (int32_t)(ib_min + b);
+    }
+    if (bucket_left_out(bunched, b_start, b_end, b)) {
+      /* no step, but the element's checks still apply to the bucket */
+      rc = ehd_collective_checks(d, &cc);
+      if (rc)
+        return msgf(c, rc, ehd_message(d));
+      continue;
+    }
+    if (is_wake) {
+      cc.mean_given = 0;
+      if (nbk > 1 && cc.n_bins_fixed) {
+        /* a bucket's mean time is a plain sum over its rows (wake.c:110-116) */
+        rc = eh_ctx_bucket_mean(c, pass, 4 * op_index, ib_min + b, cc.Po, &cc.mean);
+        if (rc)
+          return rc;
+        cc.mean_given = 1;
+      }
+    } else {
+      const EH_ZLONGIT *zl = &e->p.zlongit;
+      /* the bin settings follow the first bucket that takes the impedance */
+      rc = zlongit_setup(c, slot, zl, op->Po, cc.use_bunch, cc.bunch_target);
+      if (rc)
+        return rc;
+      cc.nb = eh_ctx_zl_nbins(c)[slot];
+      cc.dt = eh_ctx_zl_binsize(c)[slot];
+      /* factor = macroParticleCharge * particleRelSign / dt * FACTOR * rampFactor (zlongit.c:377) */
+      cc.factor = macro * EH_REL_SIGN / cc.dt * zl->factor * ramp_factor(i_pass, zl->rampPasses);
+    }
+    rc = ehd_collective_step(d, slot, &cc, hist, volt, 0, nb, tmin);
+    if (rc == EH_ERR_LONGBEAM)
+      return msgf(c, rc, "bunch longer than the wake/impedance window (wake.c:134-140, zlongit.c:219-236)");
+    if (rc)
+      return msgf(c, rc, ehd_message(d));
+  }
   return EH_OK;
 }
 
@@ -240,8 +303,9 @@ int eh_collective_probe(eh_ctx *c, double **part, int64_t np, const eh_element *
   if (rc)
     return rc;
   memset(eh_ctx_coll_loaded(c), 0, sizeof(int) * EH_MAX_COLL);
-  /* the element routines leave NaN rows to their caller: no initial cull */
-  rc = eh_upload_rows(c, part, np, 8, 0);
+  /* the element routines leave NaN rows to their caller: no initial cull;
+   * whole rows, so that a bunched beam's bunch index (column 8) is seen */
+  rc = eh_upload_rows(c, part, np, eh_ctx_row_props(c), 0);
   if (rc)
     return rc;
   rc = collective_run(c, 0, e, &prog->ops[0], 0, i_pass, macro, EHD_COLL_NOCHECKS, hist, volt, nb, tmin);
@@ -328,6 +392,9 @@ int64_t eh_track_through_wake_noalloc(eh_ctx *c, double *const coord[6], int64_t
     return msgf(c, EH_ERR_UNSUPPORTED, "WAKE: acausal Green functions (t[0] != 0)");
   if (np == 0)
     return 0;
+  /* the particles carry no bunch index here: one bucket, bucket 0 */
+  if (bucket_left_out(wake->bunchedBeamMode, wake->startBunch, wake->endBunch, 0))
+    return np;
   memset(&cc, 0, sizeof(cc));
   cc.kind = EH_OP_WAKE;
   cc.pass = i_pass;

@@ -95,11 +95,13 @@ int eh_element_defaults(eh_element *e, int32_t type) {
   case EH_T_WAKE: /* track_data.c:2396-2414 */
     e->p.wake.factor = 1;
     e->p.wake.bunchedBeamMode = 1;
+    e->p.wake.startBunch = e->p.wake.endBunch = -1;
     break;
   case EH_T_ZLONGIT: /* track_data.c:2251-2283 (IS_LONG defaults from the integer field) */
     e->p.zlongit.n_bins = 128;
     e->p.zlongit.factor = 1;
     e->p.zlongit.bunchedBeamMode = 1;
+    e->p.zlongit.startBunch = e->p.zlongit.endBunch = -1;
     break;
   case EH_T_MAXAMP: /* track_data.c:990-997 */
     e->p.maxamp.exponent = 2;

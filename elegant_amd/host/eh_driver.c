@@ -13,6 +13,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -58,6 +59,10 @@ struct eh_ctx {
    * their particle to eh_download */
   double *extra;
   int32_t extra_w;
+  /* bunched beams: charge->idSlotsPerBunch (do_tracking.c:1341), and
+   * whether the device holds the bunch index of the particles uploaded */
+  int64_t id_slots_per_bunch;
+  int bunch_on_device;
   /* stopping conditions of the pass loop (do_tracking.c:494-520) */
   int64_t particle_limit;
   char *interrupt_path;
@@ -273,6 +278,7 @@ int eh_upload_rows(eh_ctx *c, double **coord, int64_t n, int32_t nprops, int cul
   soa = (double *)malloc(sizeof(double) * 6 * (n + 1));
   if (!soa)
     return EH_ERR_NOMEM;
+  c->bunch_on_device = 0;
   if (nprops > 8) {
     c->extra_w = nprops - 8;
     c->extra = (double *)malloc(sizeof(double) * c->extra_w * (n + 1));
@@ -1205,8 +1211,102 @@ int32_t eh_test_fold(const eh_element *bl, int64_t n_elems, double Po, int32_t *
  * public header) */
 int32_t eh_test_program_runs(eh_ctx *c) { return ehd_program_runs(c->dev); }
 
+/* ---- bunched beams (index_bunch_assignments, lrwake.c:17-233) ---- */
+
+int eh_set_id_slots_per_bunch(eh_ctx *c, int64_t id_slots_per_bunch) {
+  if (!c || id_slots_per_bunch < 0)
+    return EH_ERR_INVALID;
+  c->id_slots_per_bunch = id_slots_per_bunch;
+  return EH_OK;
+}
+
+/* the bunch index of particle slot i: column 8 of its row, as a long */
+static long bunch_of(const eh_ctx *c, int64_t i) { return (long)c->extra[(int64_t)c->extra_w * i]; }
+
+/* Buckets of the particles still in the beam: 0 without particles, 1 when
+ * the beam is not bunched (no id slots, or rows without a bunch column),
+ * else ibMax - ibMin + 1 with *ib_min set.  The loss state is refreshed
+ * from the device first. */
+int eh_ctx_buckets(eh_ctx *c, long *ib_min) {
+  long lo = LONG_MAX, hi = LONG_MIN;
+  int rc;
+  *ib_min = 0;
+  if (c->id_slots_per_bunch <= 0 || c->extra_w < 1)
+    return 1;
+  rc = refresh_losses(c);
+  if (rc)
+    return rc;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] < 0) {
+      long b = bunch_of(c, i);
+      if (b < lo)
+        lo = b;
+      if (b > hi)
+        hi = b;
+    }
+  if (lo == LONG_MAX)
+    return 0;
+  if (lo < INT32_MIN || hi > INT32_MAX || hi - lo >= INT32_MAX)
+    return fail(c, EH_ERR_UNSUPPORTED, "bunch indices beyond the 32-bit range");
+  *ib_min = lo;
+  return (int)(hi - lo + 1);
+}
+
+/* particles still in the beam with bunch index b */
+int64_t eh_ctx_bucket_count(const eh_ctx *c, long b) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < c->n; i++)
+    if (c->h_pass[i] < 0 && bunch_of(c, i) == b)
+      k++;
+  return k;
+}
+
+/* the device's copy of the bunch indices, uploaded once per particle upload */
+int eh_ctx_bunch_to_device(eh_ctx *c) {
+  if (c->bunch_on_device)
+    return EH_OK;
+  int32_t *b = (int32_t *)malloc(sizeof(int32_t) * (c->n > 0 ? c->n : 1));
+  if (!b)
+    return EH_ERR_NOMEM;
+  for (int64_t i = 0; i < c->n; i++)
+    b[i] = (int32_t)bunch_of(c, i);
+  int rc = ehd_put_bunch(c->dev, b, c->n);
+  free(b);
+  if (rc)
+    return devfail(c, rc);
+  c->bunch_on_device = 1;
+  return EH_OK;
+}
+
+/* WAKE with N_BINS over one bucket of a bunched beam: its mean arrival
+ * time is a plain sum in row order (wake.c:99-104), taken here over the
+ * bucket's particles in the reference's row order at the element (loss
+ * events replayed up to code_lim of this pass), with the kernels' time
+ * formula (computeTimeCoordinatesOnly, trwake.c:519-527) */
+int eh_ctx_bucket_mean(eh_ctx *c, int64_t pass, int32_t code_lim, long b, double Po, double *mean) {
+  double sum = 0;
+  int64_t k = 0;
+  int rc = ehd_get_state(c->dev, NULL, NULL, NULL, NULL, c->h_s, c->h_dp, c->h_pass, c->h_code, c->n);
+  if (rc)
+    return devfail(c, rc);
+  rc = replay_until(c, pass, code_lim);
+  if (rc)
+    return rc;
+  for (int64_t j = 0; j < c->n_alive_ref; j++) {
+    int64_t i = c->order[j];
+    if (bunch_of(c, i) != b)
+      continue;
+    double P = Po * (c->h_dp[i] + 1);
+    sum += c->h_s[i] * sqrt(P * P + 1) / (EH_C_MKS * P);
+    k++;
+  }
+  *mean = k ? sum / k : 0;
+  return EH_OK;
+}
+
 /* accessors used by the collective step (eh_collective.c) */
 ehd_dev *eh_ctx_dev(eh_ctx *c) { return c->dev; }
+int32_t eh_ctx_row_props(eh_ctx *c) { return c->row_props; }
 const eh_program *eh_ctx_prog(eh_ctx *c) { return c->prog; }
 const eh_element *eh_ctx_elems(eh_ctx *c) { return c->elems; }
 double eh_ctx_macro_charge(eh_ctx *c, int *seen) {

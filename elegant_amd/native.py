@@ -43,6 +43,7 @@ SIGNATURES = [
     ("eh_track", C.c_int, [P, C.c_int64, C.c_int64, PD, C.POINTER(A.eh_stats)]),
     ("eh_download", C.c_int, [P, PP, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
     ("eh_set_particle_properties", C.c_int, [P, C.c_int32]),
+    ("eh_set_id_slots_per_bunch", C.c_int, [P, C.c_int64]),
     ("eh_set_particle_limit", C.c_int, [P, C.c_int64]),
     ("eh_set_interrupt_file", C.c_int, [P, C.c_char_p]),
     ("eh_set_interrupt_flag", C.c_int, [P, C.POINTER(C.c_int32)]),
@@ -292,6 +293,12 @@ class Tracker:
     def set_particle_properties(self, nprops):
         """columns of every particle row (elegant's totalPropertiesPerParticle)"""
         check(self._ctx, lib().eh_set_particle_properties(self._ctx, int(nprops)))
+
+    def set_id_slots_per_bunch(self, n):
+        """charge->idSlotsPerBunch: with n > 0 and rows of at least 9
+        columns, WAKE/ZLONGIT in bunched-beam mode act per bunch index
+        (column 8)"""
+        check(self._ctx, lib().eh_set_id_slots_per_bunch(self._ctx, int(n)))
 
     def set_particle_limit(self, limit):
         check(self._ctx, lib().eh_set_particle_limit(self._ctx, int(limit)))

@@ -130,7 +130,11 @@ typedef struct {
   int64_t wakePoints;
   double factor;
   int32_t n_bins, interpolate, rampPasses, allowLongBeam;
-  int32_t smoothing, change_p0, bunchedBeamMode, pad;
+  int32_t smoothing, change_p0, bunchedBeamMode;
+  /* START_BUNCH/END_BUNCH: with a bunched beam, the buckets (counted from
+   * the lowest bunch index present) outside [startBunch, endBunch] take no
+   * wake (wake.c:84-88); -1, the reference's default, is no limit */
+  int32_t startBunch, endBunch, pad;
 } EH_WAKE;
 
 /* ZLONGIT (track.h:2836-2865; defaults track_data.c:2251-2283), broad-band
@@ -139,7 +143,8 @@ typedef struct {
   double Ra, Rs, Q, freq, bin_size, factor;
   int32_t broad_band, n_bins, max_n_bins, interpolate;
   int32_t area_weight, startOnPass, rampPasses, reverseTimeOrder;
-  int32_t smoothing, allowLongBeam, bunchedBeamMode, pad;
+  int32_t smoothing, allowLongBeam, bunchedBeamMode;
+  int32_t startBunch, endBunch, pad; /* as in EH_WAKE (zlongit.c:142-146) */
 } EH_ZLONGIT;
 
 /* MAXAMP (track.h:1955-1961; defaults track_data.c:990-997).  ELLIPTICAL
@@ -220,6 +225,14 @@ int eh_load_program(eh_ctx *ctx, const eh_element *elems, int64_t n_elems, doubl
  * library writes columns 0..7 and carries every other column with its
  * particle.  Default 8; 8 <= nprops <= 64. */
 int eh_set_particle_properties(eh_ctx *ctx, int32_t nprops);
+
+/* Bunched beams: charge->idSlotsPerBunch, the beam's id_slots_per_bunch
+ * (do_tracking.c:1341).  When it is positive and the rows carry the bunch
+ * index in column 8 (nprops >= 9), a WAKE or ZLONGIT with
+ * BUNCHED_BEAM_MODE acts on each bucket of particles with one bunch index
+ * on its own, in bucket order (index_bunch_assignments, lrwake.c:17-233).
+ * Default 0: the beam is one bucket. */
+int eh_set_id_slots_per_bunch(eh_ctx *ctx, int64_t id_slots_per_bunch);
 
 /* ---- particle state ---- */
 /* Copy AoS rows (>= 7 columns: x, x', y, y', s, delta, ID) to device SoA;

@@ -35,6 +35,7 @@
 #include "oracle.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -126,7 +127,11 @@ static void swap_rows(double *a, double *b, int nprops) {
 }
 
 /* rows carry at least 8 columns: x xp y yp s dp id lossPass */
-#define OR_NPROPS 8
+/* columns swapped with a row (elegant's totalPropertiesPerParticle): 8, or
+ * more when the rows carry the bunch index (column 8) and further properties */
+static int or_nprops = 8;
+#define OR_NPROPS or_nprops
+void or_set_row_width(long nprops) { or_nprops = nprops < 8 ? 8 : (int)nprops; }
 
 /* ---------------------------------------------------------------- drift */
 
@@ -1469,7 +1474,90 @@ static void keep_hist(or_ctx *c, const double *hist, const double *volt, long nb
 
 /* track_through_wake (wake.c:23-303) for one bunch, no smoothing, causal
  * Green function starting at t = 0 */
+/* index_bunch_assignments (lrwake.c:17-233): with idSlotsPerBunch > 0 the
+ * particles fall into buckets by the bunch index in column 8, bucket
+ * ib - ibMin; bucket[] lists each bucket's rows in row order after the
+ * first-row offsets first[b] (nbuckets + 1 entries).  Returns the number of
+ * buckets (1 without bunch slots, 0 without particles). */
+static long bunch_buckets(long id_slots, double **part, long np, long **first_out, long **rows_out) {
+  long ibMin = LONG_MAX, ibMax = LONG_MIN, nb;
+  *first_out = NULL;
+  *rows_out = NULL;
+  if (id_slots <= 0)
+    return 1;
+  for (long ip = 0; ip < np; ip++) {
+    long ib = (long)part[ip][8];
+    if (ib < ibMin)
+      ibMin = ib;
+    if (ib > ibMax)
+      ibMax = ib;
+  }
+  if (ibMin == LONG_MAX || ibMax == LONG_MIN)
+    return 0;
+  nb = ibMax - ibMin + 1;
+  long *first = (long *)calloc(nb + 1, sizeof(long));
+  long *rows = (long *)malloc(sizeof(long) * (np > 0 ? np : 1));
+  for (long ip = 0; ip < np; ip++)
+    first[(long)part[ip][8] - ibMin + 1]++;
+  for (long b = 0; b < nb; b++)
+    first[b + 1] += first[b];
+  {
+    long *fill = (long *)malloc(sizeof(long) * nb);
+    memcpy(fill, first, sizeof(long) * nb);
+    for (long ip = 0; ip < np; ip++)
+      rows[fill[(long)part[ip][8] - ibMin]++] = ip;
+    free(fill);
+  }
+  *first_out = first;
+  *rows_out = rows;
+  return nb;
+}
+
+static long wake_bucket(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass, int plain_mean);
+
+/* START_BUNCH/END_BUNCH (wake.c:84-88, zlongit.c:142-146): bucket b takes
+ * no wake or impedance */
+static int bucket_left_out(long bunched, long start, long end, long b) {
+  return bunched && ((start >= 0 && b < start) || (end >= 0 && b > end));
+}
+
+/* track_through_wake (wake.c:23-303): one bucket, or with a bunched beam one
+ * pass of the binning, convolution and kick per bucket on its own rows
+ * (wake.c:71-117, 285-295), the mean time of a bucket a plain sum in row
+ * order */
 long or_track_through_wake(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass) {
+  long *first, *rows, nbk;
+  if (np <= 0 || !(w->bunchedBeamMode && c->id_slots_per_bunch > 0))
+    return bucket_left_out(w->bunchedBeamMode, w->startBunch, w->endBunch, 0) ? np
+                                                                               : wake_bucket(c, part, np, w, Po, i_pass, 0);
+  nbk = bunch_buckets(c->id_slots_per_bunch, part, np, &first, &rows);
+  if (nbk == 1) {
+    free(first);
+    free(rows);
+    return bucket_left_out(w->bunchedBeamMode, w->startBunch, w->endBunch, 0) ? np
+                                                                               : wake_bucket(c, part, np, w, Po, i_pass, 0);
+  }
+  for (long b = 0; b < nbk; b++) {
+    long n = first[b + 1] - first[b];
+    if (bucket_left_out(w->bunchedBeamMode, w->startBunch, w->endBunch, b) || !n)
+      continue;
+    double **sub = (double **)malloc(sizeof(double *) * n);
+    for (long k = 0; k < n; k++)
+      sub[k] = part[rows[first[b] + k]];
+    long rc = wake_bucket(c, sub, n, w, Po, i_pass, 1);
+    free(sub);
+    if (rc < 0) {
+      free(first);
+      free(rows);
+      return rc;
+    }
+  }
+  free(first);
+  free(rows);
+  return np;
+}
+
+static long wake_bucket(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass, int plain_mean) {
   double tmin, tmax, dt, factor, rampFactor, wtmin = DBL_MAX, wtmax = -DBL_MAX, tmean = 0;
   long nb, ib, ip, n_binned;
   double *time, *Itime, *Vtime;
@@ -1494,8 +1582,14 @@ long or_track_through_wake(or_ctx *c, double **part, long np, const EH_WAKE *w, 
   time = (double *)malloc(sizeof(double) * np);
   pbin = (long *)malloc(sizeof(long) * np);
   time_coordinates(time, Po, part, np);
-  if (w->n_bins)
-    tmean = compute_average(time, np);
+  if (w->n_bins) {
+    if (plain_mean) {
+      for (ip = 0; ip < np; ip++)
+        tmean += time[ip];
+      tmean /= np;
+    } else
+      tmean = compute_average(time, np);
+  }
   tmax = -(tmin = DBL_MAX);
   find_min_max(&tmin, &tmax, time, np);
   if ((tmax - tmin) > (w->t[w->wakePoints - 1] - w->t[0]) && !w->allowLongBeam) {
@@ -1706,7 +1800,48 @@ static int optimize_bins(double timeSpan, double freq, double Q, double *binSize
 }
 
 /* track_through_zlongit (zlongit.c:56-557), broad-band resonator, one bunch */
+static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_ZLONGIT *zl, double Po,
+                           long i_pass);
+
+/* track_through_zlongit (zlongit.c:56-557), per bucket with a bunched beam
+ * (zlongit.c:125-195, 491-497); the bin settings are optimized on the first
+ * bucket binned */
 long or_track_through_zlongit(or_ctx *c, long ei, double **part, long np, const EH_ZLONGIT *zl, double Po, long i_pass) {
+  long *first, *rows, nbk;
+  if (np <= 0 || !(zl->bunchedBeamMode && c->id_slots_per_bunch > 0))
+    return bucket_left_out(zl->bunchedBeamMode, zl->startBunch, zl->endBunch, 0)
+               ? np
+               : zlongit_bucket(c, ei, part, np, zl, Po, i_pass);
+  nbk = bunch_buckets(c->id_slots_per_bunch, part, np, &first, &rows);
+  if (nbk == 1) {
+    free(first);
+    free(rows);
+    return bucket_left_out(zl->bunchedBeamMode, zl->startBunch, zl->endBunch, 0)
+               ? np
+               : zlongit_bucket(c, ei, part, np, zl, Po, i_pass);
+  }
+  for (long b = 0; b < nbk; b++) {
+    long n = first[b + 1] - first[b];
+    if (bucket_left_out(zl->bunchedBeamMode, zl->startBunch, zl->endBunch, b) || !n)
+      continue;
+    double **sub = (double **)malloc(sizeof(double *) * n);
+    for (long k = 0; k < n; k++)
+      sub[k] = part[rows[first[b] + k]];
+    long rc = zlongit_bucket(c, ei, sub, n, zl, Po, i_pass);
+    free(sub);
+    if (rc < 0) {
+      free(first);
+      free(rows);
+      return rc;
+    }
+  }
+  free(first);
+  free(rows);
+  return np;
+}
+
+static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_ZLONGIT *zl, double Po,
+                           long i_pass) {
   double tmin, tmax, tmean, dt, factor, rampFactor, Ra, bin_size, df;
   long nb, ib, ip, nfreq, n_bins;
   double *time, *Itime, *Vtime, *Z;
@@ -2000,6 +2135,9 @@ double or_get_fiducial(const or_ctx *c, long elem_index) {
 }
 
 void or_set_particle_limit(or_ctx *c, long limit) { c->particle_limit = limit; }
+
+/* charge->idSlotsPerBunch = beam->id_slots_per_bunch (do_tracking.c:1341) */
+void or_set_id_slots_per_bunch(or_ctx *c, long n) { c->id_slots_per_bunch = n; }
 
 void or_set_macro_charge(or_ctx *c, double q) {
   c->macroParticleCharge = q;

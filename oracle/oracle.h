@@ -44,6 +44,7 @@ typedef struct {
   long zl_n_bins[OR_MAX_ELEMS];
   double zl_bin_size[OR_MAX_ELEMS];
   long particle_limit; /* stop_tracking_particle_limit (do_tracking.c:500-520), 0 = off */
+  long id_slots_per_bunch; /* bunched beam: buckets by the bunch index in column 8 */
 } or_ctx;
 
 or_ctx *or_ctx_new(void);
@@ -73,6 +74,8 @@ void or_add_to_particle_energy(double *coord, double timeOfFlight, double Po, do
 double or_kahan_plus(double oldSum, double b, double *error);
 void or_do_match_energy(double **coord, long np, double *P_central);
 void or_etilt_offsets(double rho0, double angle, double etilt, double tilt, double *out);
+void or_set_row_width(long nprops);
+void or_set_id_slots_per_bunch(or_ctx *c, long n);
 
 /* do_tracking pass/element loop over a flattened beamline.  Returns nLeft,
  * or a negative EH_ERR_* code.  Lost rows carry the pass of loss in column 7. */

@@ -55,6 +55,8 @@ def lib(mode="glibc"):
         L.or_kahan_plus.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_do_match_energy.argtypes = [PP, C.c_long, C.POINTER(C.c_double)]
         L.or_etilt_offsets.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.or_set_row_width.argtypes = [C.c_long]
+        L.or_set_id_slots_per_bunch.argtypes = [C.c_void_p, C.c_long]
         L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
         L.or_simple_rf_cavity.restype = C.c_long
         L.or_simple_rf_cavity.argtypes = [C.c_void_p, C.c_long, PP, C.c_long, C.POINTER(A.EH_RFCA),
@@ -84,6 +86,23 @@ class Oracle:
     def __init__(self, mode="glibc"):
         self.L = lib(mode)
         self.ctx = self.L.or_ctx_new()
+        self.row_width = 8
+
+    def set_row_width(self, nprops):
+        """columns per particle row (totalPropertiesPerParticle) that move
+        with a particle; the rows passed in must have at least this many"""
+        self.row_width = int(nprops)
+
+    def set_id_slots_per_bunch(self, n):
+        """charge->idSlotsPerBunch: > 0 makes bunched-beam WAKE/ZLONGIT act
+        per bunch index (column 8)"""
+        self.L.or_set_id_slots_per_bunch(self.ctx, int(n))
+
+    def _rows(self, rows):
+        if rows.shape[1] < self.row_width:
+            raise ValueError("rows narrower than the row width %d" % self.row_width)
+        self.L.or_set_row_width(self.row_width)
+        return A.rows_pointer(rows)
 
     def __del__(self):
         try:
@@ -93,7 +112,7 @@ class Oracle:
 
     def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0):
         """rows: float64 (n, >=8) C-contiguous, modified in place.  Returns nLeft."""
-        ptr, keep = A.rows_pointer(rows)
+        ptr, keep = self._rows(rows)
         pc = C.c_double(Po)
         r = self.L.or_do_tracking(self.ctx, ptr, rows.shape[0], beamline.pointer(), len(beamline), C.byref(pc),
                                   n_passes, pass_offset)
@@ -135,14 +154,14 @@ class Oracle:
         return float(self.L.or_get_fiducial(self.ctx, elem_index))
 
     def wake(self, rows, wake, Po, i_pass):
-        ptr, keep = A.rows_pointer(rows)
+        ptr, keep = self._rows(rows)
         r = self.L.or_track_through_wake(self.ctx, ptr, rows.shape[0], C.byref(wake), Po, i_pass)
         if r < 0:
             raise RuntimeError("oracle wake failed: %d" % r)
         return r
 
     def zlongit(self, rows, zl, Po, i_pass, elem_index=0):
-        ptr, keep = A.rows_pointer(rows)
+        ptr, keep = self._rows(rows)
         r = self.L.or_track_through_zlongit(self.ctx, elem_index, ptr, rows.shape[0], C.byref(zl), Po, i_pass)
         if r < 0:
             raise RuntimeError("oracle zlongit failed: %d" % r)
