@@ -130,6 +130,10 @@ class eh_element(C.Structure):
     _fields_ = [("type", I), ("pad", I), ("name", C.c_char * 32), ("p", _P)]
 
 
+class eh_species(C.Structure):
+    _fields_ = [("mass", D), ("charge", D), ("massMV", D), ("radius", D), ("relSign", D)]
+
+
 class eh_stats(C.Structure):
     _fields_ = [("effort", L), ("n_left", L), ("passes_done", L), ("kernel_launches", L),
                 ("track_ms", D), ("collective_ms", D)]

@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, dou
                                            double dt, const double *__restrict__ volt, int32_t kind,
                                            int32_t interpolate, double z_end, double ap_x, double ap_y,
                                            int32_t ap_on, int32_t ap_code, int64_t pass, int32_t code_base,
-                                           const int32_t *__restrict__ bunch, int32_t target) {
+                                           const int32_t *__restrict__ bunch, int32_t target, double mc2) {
   int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n || !in_step(lp, bunch, target, i))
     return;
@@ -879,7 +879,6 @@ __global__ void __launch_bounds__(CB) kick(double *x, double *xp, double *y, dou
     if (nb <= 0 || plan->err)
       return;
   }
-  const double mc2 = 1e6 * EH_ME_MEV * EH_REL_SIGN;
   double S = s[i], D = dp[i];
   double XP = xp[i], YP = yp[i];
   bool changed = false;
@@ -1151,7 +1150,7 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
   if (n > 0)
     hipLaunchKernelGGL(kick, dim3((unsigned)grid), dim3(CB), 0, d->stream, crd[0], crd[1], crd[2], crd[3], crd[4],
                        crd[5], lp, lc, n, c->Po, plan, c->dt, volt, kind, c->interpolate, c->z_end, c->ap_xmax,
-                       c->ap_ymax, ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, target);
+                       c->ap_ymax, ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, target, c->mc2);
   CHK(hipGetLastError());
   CHK(hipEventRecord(d->cev1, d->stream));
   CHK(hipMemcpyAsync(hp, plan, sizeof(*hp), hipMemcpyDeviceToHost, d->stream));
@@ -1175,7 +1174,7 @@ int ehd_collective_checks(ehd_dev *d, const ehd_collective *c) {
   hipLaunchKernelGGL(kick, dim3((unsigned)((n + CB - 1) / CB)), dim3(CB), 0, d->stream, d->coord[0], d->coord[1],
                      d->coord[2], d->coord[3], d->coord[4], d->coord[5], d->loss_pass, d->loss_code, n, c->Po,
                      (const coll_plan *)NULL, 0.0, (const double *)NULL, -1, 0, c->z_end, c->ap_xmax, c->ap_ymax,
-                     ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, c->bunch_target);
+                     ap_on, c->ap_code, c->pass, 4 * c->op_index, bunch, c->bunch_target, c->mc2);
   CHK(hipGetLastError());
   CHK(hipStreamSynchronize(d->stream));
   return EH_OK;

@@ -93,6 +93,7 @@ typedef struct {
   int32_t bunch_target;
   int32_t mean_given;   /* WAKE with N_BINS: the bucket's mean time from the host (a plain sum) */
   double mean;
+  double mc2;           /* 1e6 particleMassMV particleRelSign: voltage to gamma (wake.c:320-322) */
 } ehd_collective;
 
 /* WAKE Green function samples; ZLONGIT impedance Z[nb] and twiddles */

@@ -96,7 +96,7 @@ int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, 
   int rc;
   if (!b || !elems || n_elems <= 0 || (!coord && n > 0) || n < 0 || nprops < 7 || !(Po > 0))
     return EH_ERR_INVALID;
-  rc = eh_compile(elems, n_elems, Po, b->prog, eh_ctx_msg(b->ctx), 512);
+  rc = eh_compile(elems, n_elems, Po, eh_ctx_species(b->ctx), b->prog, eh_ctx_msg(b->ctx), 512);
   if (rc)
     return rc;
   if (b->prog->n_coll)

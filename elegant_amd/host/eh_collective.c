@@ -197,6 +197,7 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
   long i_pass = (long)pass;
   memset(&cc, 0, sizeof(cc));
   cc.kind = is_wake ? EH_OP_WAKE : EH_OP_ZLONGIT;
+  cc.mc2 = 1e6 * eh_ctx_species(c)->massMV * eh_ctx_species(c)->relSign;
   cc.op_index = op_index;
   cc.pass = pass;
   cc.Po = op->Po;
@@ -217,7 +218,7 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
     cc.n_bins_fixed = w->n_bins;
     cc.allow_long_beam = w->allowLongBeam;
     /* factor = macroParticleCharge * particleRelSign * FACTOR * rampFactor (wake.c:258) */
-    cc.factor = macro * EH_REL_SIGN * w->factor * ramp_factor((long)pass, w->rampPasses);
+    cc.factor = macro * eh_ctx_species(c)->relSign * w->factor * ramp_factor((long)pass, w->rampPasses);
   } else {
     const EH_ZLONGIT *zl = &e->p.zlongit;
     i_pass = (long)pass - zl->startOnPass;
@@ -268,7 +269,7 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
       cc.nb = eh_ctx_zl_nbins(c)[slot];
       cc.dt = eh_ctx_zl_binsize(c)[slot];
       /* factor = macroParticleCharge * particleRelSign / dt * FACTOR * rampFactor (zlongit.c:377) */
-      cc.factor = macro * EH_REL_SIGN / cc.dt * zl->factor * ramp_factor(i_pass, zl->rampPasses);
+      cc.factor = macro * eh_ctx_species(c)->relSign / cc.dt * zl->factor * ramp_factor(i_pass, zl->rampPasses);
     }
     rc = ehd_collective_step(d, slot, &cc, hist, volt, 0, nb, tmin);
     if (rc == EH_ERR_LONGBEAM)
@@ -299,7 +300,7 @@ int eh_collective_probe(eh_ctx *c, double **part, int64_t np, const eh_element *
   int rc;
   int64_t nLeft;
   bl[0] = *e;
-  rc = eh_compile(bl, 1, Po, prog, eh_ctx_msg(c), 512);
+  rc = eh_compile(bl, 1, Po, eh_ctx_species(c), prog, eh_ctx_msg(c), 512);
   if (rc)
     return rc;
   memset(eh_ctx_coll_loaded(c), 0, sizeof(int) * EH_MAX_COLL);
@@ -397,6 +398,7 @@ int64_t eh_track_through_wake_noalloc(eh_ctx *c, double *const coord[6], int64_t
     return np;
   memset(&cc, 0, sizeof(cc));
   cc.kind = EH_OP_WAKE;
+  cc.mc2 = 1e6 * eh_ctx_species(c)->massMV * eh_ctx_species(c)->relSign;
   cc.pass = i_pass;
   cc.Po = Po;
   cc.flags = EHD_COLL_NOCHECKS;
@@ -405,7 +407,7 @@ int64_t eh_track_through_wake_noalloc(eh_ctx *c, double *const coord[6], int64_t
   cc.interpolate = wake->interpolate;
   cc.n_bins_fixed = wake->n_bins;
   cc.allow_long_beam = wake->allowLongBeam;
-  cc.factor = macroParticleCharge * EH_REL_SIGN * wake->factor * ramp_factor((long)i_pass, wake->rampPasses);
+  cc.factor = macroParticleCharge * eh_ctx_species(c)->relSign * wake->factor * ramp_factor((long)i_pass, wake->rampPasses);
   rc = ehd_wake_step_noalloc(eh_ctx_dev(c), &cc, coord, np, W_device, wake->wakePoints);
   if (rc == EH_ERR_LONGBEAM)
     return msgf(c, rc, "bunch longer than the wake window (wake.c:134-140)");

@@ -262,8 +262,8 @@ static int post_aperture(const EH_MAXAMP *m, double x_max, double y_max, int32_t
 }
 
 /* multipole_tracking2 set-up (multipole.c:765-1017) */
-static int compile_multipole(const eh_element *e, double Po, const EH_MAXAMP *maxamp, eh_op *op, char *msg,
-                             int msglen) {
+static int compile_multipole(const eh_element *e, double Po, const eh_species *sp, const EH_MAXAMP *maxamp,
+                             eh_op *op, char *msg, int msglen) {
   double KnL[3] = {0, 0, 0}, drift = 0, tilt = 0, dx = 0, dy = 0, dz = 0, xkick = 0, ykick = 0;
   long order[3] = {0, 0, 0}, n_kicks = 0, nSlices = 0, integ_order = 4;
   int skew[3] = {0, 0, 0}, expand = 0;
@@ -282,7 +282,7 @@ static int compile_multipole(const eh_element *e, double Po, const EH_MAXAMP *ma
     expand = q->expandHamiltonian;
     order[0] = 1;
     if (q->bore)
-      KnL[0] = q->B / q->bore * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      KnL[0] = q->B / q->bore * (sp->charge / (sp->mass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
     else
       KnL[0] = q->k1 * q->length * (1 + q->fse);
     drift = q->length;
@@ -306,7 +306,7 @@ static int compile_multipole(const eh_element *e, double Po, const EH_MAXAMP *ma
     expand = q->expandHamiltonian;
     order[0] = 2;
     if (q->bore)
-      KnL[0] = 2 * q->B / sqr(q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      KnL[0] = 2 * q->B / sqr(q->bore) * (sp->charge / (sp->mass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
     else
       KnL[0] = q->k2 * q->length * (1 + q->fse);
     drift = q->length;
@@ -339,7 +339,7 @@ static int compile_multipole(const eh_element *e, double Po, const EH_MAXAMP *ma
     expand = q->expandHamiltonian;
     order[0] = 3;
     if (q->bore)
-      KnL[0] = 6 * q->B / (q->bore * q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length *
+      KnL[0] = 6 * q->B / (q->bore * q->bore * q->bore) * (sp->charge / (sp->mass * EH_C_MKS * Po)) * q->length *
                (1 + q->fse);
     else
       KnL[0] = q->k3 * q->length * (1 + q->fse);
@@ -605,7 +605,7 @@ static int compile_csbend(const eh_element *e, const EH_MAXAMP *maxamp, eh_op *o
 
 /* trackRfCavityWithWakes set-up without wakes (simple_rfca.c:378-478); the
  * phase is completed by eh_rfca_set_phase once the fiducial is known */
-static int compile_rfca(const eh_element *e, eh_op *op, char *msg, int msglen) {
+static int compile_rfca(const eh_element *e, const eh_species *sp, eh_op *op, char *msg, int msglen) {
   const EH_RFCA *rf = &e->p.rfca;
   double length = rf->length, volt, omega;
   long nKicks;
@@ -624,7 +624,7 @@ static int compile_rfca(const eh_element *e, eh_op *op, char *msg, int msglen) {
     return EH_OK;
   }
   omega = EH_PIx2 * rf->freq;
-  volt = rf->volt / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+  volt = rf->volt / (1e6 * sp->massMV * sp->relSign);
   nKicks = length ? rf->nKicks : 1;
   if (nKicks <= 0) {
     matrixMethod = 1;
@@ -660,7 +660,8 @@ void eh_rfca_set_phase(eh_op *op, const EH_RFCA *rf, double phase_fiducial) {
 /* t0 from a TMEAN/LIGHT fiducialization (simple_rfca.c:412-471) -> phase_fiducial */
 double eh_rfca_phase_fiducial(const EH_RFCA *rf, double t0) { return -(EH_PIx2 * rf->freq) * t0; }
 
-int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *prog, char *msg, int msglen) {
+int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_species *sp, eh_program *prog, char *msg,
+               int msglen) {
   double z = 0, last_z;
   double x_max = 0, y_max = 0;
   const EH_MAXAMP *maxamp = NULL;
@@ -713,13 +714,13 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *pro
     case EH_T_KQUAD:
     case EH_T_KSEXT:
     case EH_T_KOCT:
-      rc = compile_multipole(e, Po, maxamp, &op, msg, msglen);
+      rc = compile_multipole(e, Po, sp, maxamp, &op, msg, msglen);
       break;
     case EH_T_CSBEND:
       rc = compile_csbend(e, maxamp, &op, msg, msglen);
       break;
     case EH_T_RFCA:
-      rc = compile_rfca(e, &op, msg, msglen);
+      rc = compile_rfca(e, sp, &op, msg, msglen);
       if (rc == EH_OK && op.op == EH_OP_RFCA) {
         if (prog->n_rfca >= EH_MAX_RFCA) {
           snprintf(msg, msglen, "too many RFCA elements");

@@ -63,6 +63,7 @@ struct eh_ctx {
    * whether the device holds the bunch index of the particles uploaded */
   int64_t id_slots_per_bunch;
   int bunch_on_device;
+  eh_species sp; /* the particle species (change_particle) */
   /* stopping conditions of the pass loop (do_tracking.c:494-520) */
   int64_t particle_limit;
   char *interrupt_path;
@@ -131,9 +132,70 @@ int eh_ctx_create(int device, eh_ctx **out) {
   }
   c->chunk_passes = 64;
   c->row_props = 8;
+  eh_species_electron(&c->sp);
   *out = c;
   return EH_OK;
 }
+
+/* ---- particle species (process_particle_command, elegant.c:2933-2991) ---- */
+
+void eh_species_electron(eh_species *sp) {
+  /* "ensure exact backward compatibility" (elegant.c:2946-2953) */
+  sp->mass = EH_ME_MKS;
+  sp->charge = EH_E_MKS;
+  sp->massMV = EH_ME_MEV;
+  sp->radius = EH_RE_MKS;
+  sp->relSign = 1;
+}
+
+int eh_species_named(const char *name, double mass_ratio, double charge_ratio, eh_species *sp) {
+  double charge; /* the table's charge of the type, electron = -e (elegant.c:2928-2929) */
+  if (!name || !sp)
+    return EH_ERR_INVALID;
+  if (!strcmp(name, "electron")) {
+    eh_species_electron(sp);
+    return EH_OK;
+  }
+  if (!strcmp(name, "positron")) {
+    /* "ensure exact reversal from electron" (elegant.c:2954-2961) */
+    eh_species_electron(sp);
+    sp->charge = -EH_E_MKS;
+    sp->relSign = -1;
+    return EH_OK;
+  }
+  if (!strcmp(name, "proton")) {
+    sp->mass = EH_MP_MKS;
+    charge = EH_E_MKS;
+  } else if (!strcmp(name, "muon")) {
+    sp->mass = EH_MMU_MKS;
+    charge = -EH_E_MKS;
+  } else if (!strcmp(name, "custom")) {
+    if (mass_ratio <= 0 || charge_ratio == 0)
+      return EH_ERR_INVALID; /* "Must have mass_ratio>0 and charge_ratio nonzero" */
+    sp->mass = EH_ME_MKS * mass_ratio;
+    charge = -EH_E_MKS * charge_ratio;
+  } else
+    return EH_ERR_INVALID;
+  /* the minus sign: a legacy of the electron-only code (elegant.c:2968-2977) */
+  sp->charge = -charge;
+  /* particleRelSign = -SIGN(particleCharge)/SIGN(charge[electron]), and the
+   * electron's table charge is negative: the sign of particleCharge */
+  sp->relSign = sp->charge > 0 ? 1.0 : -1.0;
+  sp->massMV = sp->mass * (EH_C_MKS * EH_C_MKS) / fabs(sp->charge) / 1e6;
+  sp->radius = (sp->charge * sp->charge) / (4 * EH_PI * EH_EPSILON_O * sp->mass * (EH_C_MKS * EH_C_MKS));
+  return EH_OK;
+}
+
+int eh_set_species(eh_ctx *c, const eh_species *sp) {
+  if (!c || !sp)
+    return EH_ERR_INVALID;
+  if (!(sp->mass > 0) || !(sp->massMV > 0) || sp->charge == 0 || (sp->relSign != 1 && sp->relSign != -1))
+    return fail(c, EH_ERR_INVALID, "eh_set_species: mass and massMV > 0, charge != 0, relSign +-1");
+  c->sp = *sp;
+  return EH_OK;
+}
+
+const eh_species *eh_ctx_species(const eh_ctx *c) { return &c->sp; }
 
 int eh_set_particle_properties(eh_ctx *c, int32_t nprops) {
   if (!c || nprops < 8 || nprops > 64)
@@ -192,7 +254,7 @@ int eh_load_program(eh_ctx *c, const eh_element *elems, int64_t n_elems, double 
   if (!c || (!elems && n_elems > 0) || n_elems < 0 || !(Po > 0))
     return fail(c, EH_ERR_INVALID, "eh_load_program: bad arguments");
   c->msg[0] = 0;
-  rc = eh_compile(elems, n_elems, Po, c->prog, c->msg, sizeof(c->msg));
+  rc = eh_compile(elems, n_elems, Po, &c->sp, c->prog, c->msg, sizeof(c->msg));
   if (rc)
     return rc;
   free(c->elems);
@@ -519,7 +581,7 @@ static int32_t next_barrier(const eh_ctx *c, int32_t from) {
  * element after the cavity sees, field-specified multipole strengths
  * included) */
 static int recompile_program(eh_ctx *c) {
-  int rc = eh_compile(c->elems, c->n_elems, c->Po, c->prog, c->msg, sizeof(c->msg));
+  int rc = eh_compile(c->elems, c->n_elems, c->Po, &c->sp, c->prog, c->msg, sizeof(c->msg));
   if (rc)
     return rc;
   for (int k = 0; k < c->prog->n_rfca; k++)
@@ -938,7 +1000,7 @@ static int64_t run_single(eh_ctx *c, double **rows, int64_t n, double **accepted
     nb++;
   }
   bl[nb++] = *e;
-  rc = eh_compile(bl, nb, Po, c->prog, c->msg, sizeof(c->msg));
+  rc = eh_compile(bl, nb, Po, &c->sp, c->prog, c->msg, sizeof(c->msg));
   if (rc)
     return rc;
   /* keep only the element itself, without the checks do_tracking adds after
@@ -1032,7 +1094,7 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     int rc;
     int64_t nLeft = 0;
     bl[0] = e;
-    rc = eh_compile(bl, 1, *P_central, c->prog, c->msg, sizeof(c->msg));
+    rc = eh_compile(bl, 1, *P_central, &c->sp, c->prog, c->msg, sizeof(c->msg));
     if (rc)
       return rc;
     if (c->prog->ops[0].op == EH_OP_RFCA)
@@ -1080,7 +1142,7 @@ static int64_t run_aperture(eh_ctx *c, double **coord, int64_t np, double **acce
   bl[0].p.maxamp = *m;
   eh_element_defaults(&bl[1], EH_T_DRIF);
   bl[1].p.drift.length = 0;
-  rc = eh_compile(bl, 2, Po, c->prog, c->msg, sizeof(c->msg));
+  rc = eh_compile(bl, 2, Po, &c->sp, c->prog, c->msg, sizeof(c->msg));
   if (rc)
     return rc;
   c->prog->ops[0] = c->prog->ops[1];
@@ -1181,9 +1243,11 @@ int32_t eh_test_fold(const eh_element *bl, int64_t n_elems, double Po, int32_t *
   eh_program *prog = (eh_program *)malloc(sizeof(eh_program));
   char msg[256];
   int32_t nr = EH_ERR_NOMEM, nb = 0;
+  eh_species sp;
   if (!prog)
     return EH_ERR_NOMEM;
-  int rc = eh_compile(bl, n_elems, Po, prog, msg, sizeof msg);
+  eh_species_electron(&sp);
+  int rc = eh_compile(bl, n_elems, Po, &sp, prog, msg, sizeof msg);
   if (rc) {
     free(prog);
     return rc;

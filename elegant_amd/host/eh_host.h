@@ -23,7 +23,11 @@ typedef struct {
   int32_t n_p0, p0_op[EH_MAX_RFCA]; /* EH_OP_MATCH_P0 steps (RFCA with CHANGE_P0) */
 } eh_program;
 
-int eh_compile(const eh_element *bl, int64_t n_elems, double Po, eh_program *prog, char *msg, int msglen);
+int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_species *sp, eh_program *prog, char *msg,
+               int msglen);
+const eh_species *eh_ctx_species(const eh_ctx *c);
+/* change_particle's electron (elegant.c:2946-2953) */
+void eh_species_electron(eh_species *sp);
 double eh_element_length(const eh_element *e);
 int eh_has_length(int32_t type);
 void eh_rfca_set_phase(eh_op *op, const EH_RFCA *rf, double phase_fiducial);

@@ -44,6 +44,8 @@ SIGNATURES = [
     ("eh_download", C.c_int, [P, PP, C.c_int64, C.c_int32, C.POINTER(C.c_int64)]),
     ("eh_set_particle_properties", C.c_int, [P, C.c_int32]),
     ("eh_set_id_slots_per_bunch", C.c_int, [P, C.c_int64]),
+    ("eh_species_named", C.c_int, [C.c_char_p, C.c_double, C.c_double, C.POINTER(A.eh_species)]),
+    ("eh_set_species", C.c_int, [P, C.POINTER(A.eh_species)]),
     ("eh_set_particle_limit", C.c_int, [P, C.c_int64]),
     ("eh_set_interrupt_file", C.c_int, [P, C.c_char_p]),
     ("eh_set_interrupt_flag", C.c_int, [P, C.POINTER(C.c_int32)]),
@@ -224,6 +226,16 @@ def da_trim_area(x, y, dxf, dyf, full_plane=0):
     return a, x, y
 
 
+def species(name, mass_ratio=0.0, charge_ratio=0.0):
+    """change_particle's species NAME ("electron", "positron", "proton",
+    "muon", or "custom" with MASS_RATIO and CHARGE_RATIO in electron units)"""
+    sp = A.eh_species()
+    rc = lib().eh_species_named(name.encode(), float(mass_ratio), float(charge_ratio), C.byref(sp))
+    if rc:
+        raise EHError(rc, "eh_species_named(%r)" % name)
+    return sp
+
+
 def program_runs(beamline, Po):
     """The runs the tracking kernel walks for this beamline, as (first, len,
     reps, flat) tuples: repeated cells fold into one body of records."""
@@ -231,7 +243,7 @@ def program_runs(beamline, Po):
     out = (C.c_int32 * (4 * cap))()
     nr = lib().eh_test_fold(beamline.pointer(), len(beamline), Po, out, cap)
     if nr < 0:
-        raise EHError("eh_test_fold failed (%d)" % nr)
+        raise EHError(nr, "eh_test_fold")
     return [tuple(out[4 * k:4 * k + 4]) for k in range(min(nr, cap))]
 
 
@@ -293,6 +305,10 @@ class Tracker:
     def set_particle_properties(self, nprops):
         """columns of every particle row (elegant's totalPropertiesPerParticle)"""
         check(self._ctx, lib().eh_set_particle_properties(self._ctx, int(nprops)))
+
+    def set_species(self, sp):
+        """the particle species (an eh_species, e.g. from species("proton"))"""
+        check(self._ctx, lib().eh_set_species(self._ctx, C.byref(sp)))
 
     def set_id_slots_per_bunch(self, n):
         """charge->idSlotsPerBunch: with n > 0 and rows of at least 9

@@ -214,6 +214,31 @@ const char *eh_strerror(int code);
 /* Message describing the last error on this context (empty if none). */
 const char *eh_last_error(const eh_ctx *ctx);
 
+/* ---- particle species (elegant's particleMass, particleCharge,
+ * particleMassMV, particleRadius, particleRelSign globals, set by the
+ * change_particle command, elegant.c:2933-2991) ----
+ * charge keeps elegant's sign convention: +e for electrons ("a legacy of
+ * time when this was an electron-only code"), relSign = +1 for electrons and
+ * -1 for positively charged particles.  A context starts with electrons.
+ * On the tracked path the species enters field-given multipole strengths
+ * (charge/mass), RF cavity voltages and wake/impedance kicks (massMV,
+ * relSign); radius only enters radiation, which is not tracked here. */
+typedef struct {
+  double mass;    /* kg */
+  double charge;  /* C */
+  double massMV;  /* rest energy over the charge, MV */
+  double radius;  /* classical radius, m */
+  double relSign;
+} eh_species;
+
+/* The species of change_particle's NAME ("electron", "positron", "proton",
+ * "muon" or "custom" with MASS_RATIO > 0 and CHARGE_RATIO != 0 in electron
+ * units), with process_particle_command's arithmetic (elegant.c:2933-2991). */
+int eh_species_named(const char *name, double mass_ratio, double charge_ratio, eh_species *out);
+/* The species every later program load, element entry point and batch
+ * variant of this context uses. */
+int eh_set_species(eh_ctx *ctx, const eh_species *species);
+
 /* ---- lattice: flattened beamline, all defaults resolved by the caller ---- */
 int eh_load_program(eh_ctx *ctx, const eh_element *elems, int64_t n_elems, double Po);
 

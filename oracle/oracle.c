@@ -129,6 +129,19 @@ static void swap_rows(double *a, double *b, int nprops) {
 /* rows carry at least 8 columns: x xp y yp s dp id lossPass */
 /* columns swapped with a row (elegant's totalPropertiesPerParticle): 8, or
  * more when the rows carry the bunch index (column 8) and further properties */
+/* the particle species: elegant's globals particleMass, particleCharge,
+ * particleMassMV, particleRadius and particleRelSign (abrat.c:182-186 set the
+ * electron's; change_particle, elegant.c:2933-2991, changes them) */
+static double particleMass = EH_ME_MKS, particleCharge = EH_E_MKS, particleMassMV = EH_ME_MEV,
+              particleRadius = EH_RE_MKS, particleRelSign = 1;
+void or_set_species(double mass, double charge, double massMV, double radius, double relSign) {
+  particleMass = mass;
+  particleCharge = charge;
+  particleMassMV = massMV;
+  particleRadius = radius;
+  particleRelSign = relSign;
+}
+
 static int or_nprops = 8;
 #define OR_NPROPS or_nprops
 void or_set_row_width(long nprops) { or_nprops = nprops < 8 ? 8 : (int)nprops; }
@@ -752,7 +765,7 @@ static int multipole_setup(const eh_element *elem, double Po, mult_setup *m) {
     m->expand = q->expandHamiltonian;
     m->order[0] = 1;
     if (q->bore)
-      m->KnL[0] = q->B / q->bore * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      m->KnL[0] = q->B / q->bore * (particleCharge / (particleMass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
     else
       m->KnL[0] = q->k1 * q->length * (1 + q->fse);
     m->drift = q->length;
@@ -774,7 +787,7 @@ static int multipole_setup(const eh_element *elem, double Po, mult_setup *m) {
     m->expand = q->expandHamiltonian;
     m->order[0] = 2;
     if (q->bore)
-      m->KnL[0] = 2 * q->B / sqr(q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      m->KnL[0] = 2 * q->B / sqr(q->bore) * (particleCharge / (particleMass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
     else
       m->KnL[0] = q->k2 * q->length * (1 + q->fse);
     m->drift = q->length;
@@ -805,7 +818,7 @@ static int multipole_setup(const eh_element *elem, double Po, mult_setup *m) {
     m->expand = q->expandHamiltonian;
     m->order[0] = 3;
     if (q->bore)
-      m->KnL[0] = 6 * q->B / (q->bore * q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * Po)) * q->length *
+      m->KnL[0] = 6 * q->B / (q->bore * q->bore * q->bore) * (particleCharge / (particleMass * EH_C_MKS * Po)) * q->length *
                   (1 + q->fse);
     else
       m->KnL[0] = q->k3 * q->length * (1 + q->fse);
@@ -1277,7 +1290,7 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
     return np;
   }
   omega = EH_PIx2 * rfca->freq;
-  volt = rfca->volt / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+  volt = rfca->volt / (1e6 * particleMassMV * particleRelSign);
   tau = 0;
   nKicks = length ? rfca->nKicks : 1;
   if (nKicks <= 0) {
@@ -1638,7 +1651,7 @@ static long wake_bucket(or_ctx *c, double **part, long np, const EH_WAKE *w, dou
     for (; ib1 < nb && ib2 >= 0; ib1++, ib2--)
       Vtime[ib] += Itime[ib1] * w->W[ib2];
   }
-  factor = c->macroParticleCharge * EH_REL_SIGN * w->factor * rampFactor;
+  factor = c->macroParticleCharge * particleRelSign * w->factor * rampFactor;
   for (ib = 0; ib < nb; ib++)
     Vtime[ib] *= factor;
   keep_hist(c, Itime, Vtime, nb, tmin, dt);
@@ -1652,9 +1665,9 @@ static long wake_bucket(or_ctx *c, double **part, long np, const EH_WAKE *w, dou
           ib--;
           dt1 += dt;
         }
-        dgam = (Vtime[ib] + (Vtime[ib + 1] - Vtime[ib]) / dt * dt1) / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+        dgam = (Vtime[ib] + (Vtime[ib + 1] - Vtime[ib]) / dt * dt1) / (1e6 * particleMassMV * particleRelSign);
       } else
-        dgam = Vtime[ib] / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+        dgam = Vtime[ib] / (1e6 * particleMassMV * particleRelSign);
       if (dgam)
         or_add_to_particle_energy(part[ip], time[ip], Po, -dgam);
     }
@@ -1920,7 +1933,7 @@ static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_
   }
   memcpy(Vtime, Itime, sizeof(double) * nb);
   or_real_fft(Vtime, nb, 0);
-  factor = c->macroParticleCharge * EH_REL_SIGN / dt * zl->factor * rampFactor;
+  factor = c->macroParticleCharge * particleRelSign / dt * zl->factor * rampFactor;
   {
     double *Ifreq = (double *)malloc(sizeof(double) * nb);
     memcpy(Ifreq, Vtime, sizeof(double) * nb);
@@ -1947,11 +1960,11 @@ static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_
         else
           ib += 1;
         if (ib < nb)
-          dgam = (Vtime[ib - 1] + (Vtime[ib] - Vtime[ib - 1]) / dt * dt1) / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+          dgam = (Vtime[ib - 1] + (Vtime[ib] - Vtime[ib - 1]) / dt * dt1) / (1e6 * particleMassMV * particleRelSign);
         else
           continue;
       } else
-        dgam = Vtime[ib] / (1e6 * EH_ME_MEV * EH_REL_SIGN);
+        dgam = Vtime[ib] / (1e6 * particleMassMV * particleRelSign);
       if (dgam)
         or_add_to_particle_energy(part[ip], time[ip], Po, -dgam);
     }

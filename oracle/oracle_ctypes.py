@@ -2,6 +2,7 @@
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
 never by the product package."""
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -56,6 +57,7 @@ def lib(mode="glibc"):
         L.or_do_match_energy.argtypes = [PP, C.c_long, C.POINTER(C.c_double)]
         L.or_etilt_offsets.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_set_row_width.argtypes = [C.c_long]
+        L.or_set_species.argtypes = [C.c_double] * 5
         L.or_set_id_slots_per_bunch.argtypes = [C.c_void_p, C.c_long]
         L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
         L.or_simple_rf_cavity.restype = C.c_long
@@ -78,6 +80,18 @@ def lib(mode="glibc"):
         L.or_set_particle_limit.argtypes = [C.c_void_p, C.c_long]
         _libs[mode] = L
     return _libs[mode]
+
+
+@contextlib.contextmanager
+def species(sp, mode="emath"):
+    """the oracle library's particle species (an eh_species) inside the
+    block; electrons again afterwards"""
+    L = lib(mode)
+    L.or_set_species(sp.mass, sp.charge, sp.massMV, sp.radius, sp.relSign)
+    try:
+        yield
+    finally:
+        L.or_set_species(9.1093837015e-31, 1.602176634e-19, 0.51099895000, 2.8179403262e-15, 1.0)
 
 
 class Oracle:
