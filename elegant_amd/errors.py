@@ -47,7 +47,10 @@ def perturb(beamline, seed, fse=1e-3, dxy=30e-6, tilt=100e-6, cutoff=3.0, rf_ref
     run (FSE, DX, DY and TILT on every KQUAD and KSEXT), plus the 16
     error-summary features (per family F/D quadrupoles and F/D sextupoles:
     RMS of FSE, DX, DY, TILT).  RFCA elements get T_REFERENCE = rf_reference
-    so the variant needs no fiducial pass."""
+    (None keeps their fiducial mode).  A study that tracks a variant's grid
+    in several rounds needs one phase per variant in every round: with the
+    cavity at the end of a ring of h RF wavelengths, T_REFERENCE = 0 is the
+    phase the on-orbit particle's TMEAN fiducial gives, modulo 2 pi."""
     els = [A.eh_element.from_buffer_copy(e) for e in beamline.elements]
     mults = [c for c in els if c.type in (A.EH_T_KQUAD, A.EH_T_KSEXT)]
     rng = _stream_at(run_seed, seed, len(mults), cutoff)

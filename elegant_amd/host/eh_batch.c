@@ -20,6 +20,9 @@ ehd_dev *eh_ctx_dev(eh_ctx *c);
 char *eh_ctx_msg(eh_ctx *c);
 const char *ehd_message(const ehd_dev *d);
 
+/* rows tried for a TMEAN fiducial before giving up */
+#define EH_FID_TRIES 1024
+
 struct eh_batch {
   eh_ctx *ctx;
   eh_program *prog; /* scratch for compiling one variant */
@@ -91,8 +94,58 @@ static int grow(void **p, int64_t *cap, int64_t need, size_t elem) {
   return EH_OK;
 }
 
+/* TMEAN fiducials of a variant's RF cavities.  elegant fixes a cavity's phase
+ * on the first beam that reaches it (simple_rfca.c:179-214, fiducial_seen)
+ * and keeps it for the rest of the step; an aperture search tracks its grid
+ * points one do_tracking call at a time (aperture_search.c:1080-1082), so
+ * the phase is that of the first point tracked, alone.  The batch tracks
+ * every row as such a call, so each row is tried alone through the
+ * variant's first pass on the context, in row order, until every cavity has
+ * a phase; phases[k] gets cavity k's (NaN where no tried row reached it).
+ * Uses the context's program and particle state, as the element entry
+ * points do. */
+static int batch_fiducials(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord,
+                           int64_t n, int32_t nprops, double *phases) {
+  eh_ctx *c = b->ctx;
+  const eh_program *p = b->prog;
+  int need = 0, rc;
+  for (int k = 0; k < p->n_rfca; k++) {
+    const EH_RFCA *rf = &elems[p->rfca_elem[k]].p.rfca;
+    const int tmean = rf->tReference == -1 && rf->fiducialMode == EH_FID_TMEAN;
+    phases[k] = NAN;
+    need += tmean;
+  }
+  for (int64_t r = 0; need && r < n && r < EH_FID_TRIES; r++) {
+    eh_stats st;
+    double pc = Po;
+    rc = eh_load_program(c, elems, n_elems, Po);
+    if (!rc)
+      rc = eh_upload(c, coord + r, 1, nprops);
+    if (!rc)
+      rc = eh_track(c, 0, 1, &pc, &st);
+    if (rc)
+      return rc;
+    for (int k = 0; k < p->n_rfca; k++) {
+      double ph;
+      if (!isnan(phases[k]))
+        continue;
+      rc = eh_rfca_fiducial(c, p->rfca_elem[k], &ph);
+      if (rc)
+        return rc;
+      if (!isnan(ph)) {
+        const EH_RFCA *rf = &elems[p->rfca_elem[k]].p.rfca;
+        phases[k] = ph;
+        need -= rf->tReference == -1 && rf->fiducialMode == EH_FID_TMEAN;
+      }
+    }
+  }
+  return need && n ? bmsg(b, EH_ERR_FIDUCIAL, "batched RFCA (TMEAN): no row tried reaches the cavity on the first pass")
+                   : EH_OK;
+}
+
 int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord, int64_t n,
                          int32_t nprops) {
+  double phases[EH_MAX_RFCA];
   int rc;
   if (!b || !elems || n_elems <= 0 || (!coord && n > 0) || n < 0 || nprops < 7 || !(Po > 0))
     return EH_ERR_INVALID;
@@ -103,22 +156,31 @@ int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, 
     return bmsg(b, EH_ERR_UNSUPPORTED, "batched variants cannot contain WAKE/ZLONGIT");
   if (b->prog->n_p0)
     return bmsg(b, EH_ERR_UNSUPPORTED, "batched variants cannot contain RFCA with CHANGE_P0");
-  /* RF cavities need a fiducial known without a reduction over particles */
+  /* the fiducial runs compile into the context's program, not b->prog */
+  rc = batch_fiducials(b, elems, n_elems, Po, coord, n, nprops, phases);
+  if (rc)
+    return rc;
   for (int k = 0; k < b->prog->n_rfca; k++) {
     const EH_RFCA *rf = &elems[b->prog->rfca_elem[k]].p.rfca;
     eh_op *op = &b->prog->ops[b->prog->rfca_op[k]];
     double t0;
+    if (!isnan(phases[k])) {
+      eh_rfca_set_phase(op, rf, phases[k]);
+      continue;
+    }
+    if (rf->tReference == -1 && rf->fiducialMode == EH_FID_TMEAN)
+      continue; /* a variant without particles: nothing reaches the cavity */
     if (rf->tReference != -1)
       t0 = rf->tReference;
-    else if (rf->fiducialMode == EH_FID_LIGHT) {
+    else {
+      /* FIDUCIAL=light (simple_rfca.c:429-431) */
       double length = rf->length;
       long nKicks = length ? rf->nKicks : 1;
       if (nKicks <= 0)
         nKicks = 1;
       length /= nKicks;
       t0 = ((op->z_end - rf->length) + length / 2) / 2.99792458e8;
-    } else
-      return bmsg(b, EH_ERR_UNSUPPORTED, "batched RFCA needs T_REFERENCE or FIDUCIAL=light");
+    }
     eh_rfca_set_phase(op, rf, eh_rfca_phase_fiducial(rf, t0));
   }
   {

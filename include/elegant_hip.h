@@ -399,7 +399,14 @@ typedef struct {
 
 int eh_batch_create(eh_ctx *ctx, eh_batch **out);
 int eh_batch_destroy(eh_batch *b);
-/* returns the new variant's index (>= 0) or a negative error code */
+/* Returns the new variant's index (>= 0) or a negative error code.  Every
+ * row is tracked as its own do_tracking call would track it (the aperture
+ * search's way, aperture_search.c:1080-1082): an RF cavity with the TMEAN
+ * fiducial takes its phase from the first row, in row order, that reaches it
+ * on the first pass when tracked alone (elegant's fiducial_seen,
+ * simple_rfca.c:179-214); those rows are tried here, on the context, whose
+ * program and particles they replace.  No CHANGE_P0 cavities, WAKE or
+ * ZLONGIT. */
 int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, double Po, double **coord, int64_t n,
                          int32_t nprops);
 /* time the candidate tunings on samples of the loaded batch (state restored after each) and keep the

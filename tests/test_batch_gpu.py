@@ -51,6 +51,49 @@ def test_batch_bit_exact_per_variant(tracker, tuning):
     b.close()
 
 
+def test_batch_tmean_fiducial_from_the_first_row_that_reaches_the_cavity(tracker):
+    """variants keeping the TMEAN fiducial: as in elegant's aperture search,
+    where the first grid point tracked alone fixes the cavity's phase, each
+    variant's phase comes from its first row that reaches the cavity on the
+    first pass (in one variant rows 0 and 1 are lost at the entrance
+    aperture, so row 2 sets it); rows bit-identical to the checker with that
+    phase"""
+    from elegant_amd import _abi as A
+    bl0, Po = Lt.storage_ring(seed=2)
+    vs = []
+    for seed, n in [(4, 500), (5, 400)]:
+        v, feats = errors.perturb(bl0, seed, rf_reference=None)
+        rows = Lt.matched_beam(v, n, 1e-9, 1e-11, 1e-3, 10e-12, 200 + seed, Po)
+        rows[0, :6] = 0.0  # the origin point first, as a grid scan starts
+        rows[n // 2:, 0] += 0.0135  # a halo in the later rows
+        if seed == 5:
+            rows[0:2, 0] = 0.03  # beyond the 15 mm aperture at the ring's start
+        vs.append((v, Po, rows))
+    b = native.Batch(tracker)
+    for bl, Po_, rows in vs:
+        b.add_variant(bl, Po_, rows)
+    b.track(0, 4)
+    for k, (bl, Po_, rows) in enumerate(vs):
+        ei = [i for i, e in enumerate(bl.elements) if e.type == A.EH_T_RFCA][0]
+        assert bl.elements[ei].p.rfca.tReference == -1
+        first = None
+        for r in range(rows.shape[0]):
+            o = O.Oracle("emath")
+            o.do_tracking(rows[r:r + 1].copy(), bl, Po_, 1)  # a copy: the slice is a view
+            if not np.isnan(o.fiducial(ei)):
+                first, ph = r, o.fiducial(ei)
+                break
+        assert first == (2 if k == 1 else 0)
+        g, ng = b.download(k)
+        c = rows.copy()
+        o = O.Oracle("emath")
+        o.set_fiducial(ei, ph)
+        nc = o.do_tracking(c, bl, Po_, 4)
+        assert ng == nc and rows.shape[0] - ng > 20
+        assert_rows_identical(g, c, "TMEAN variant %d" % k)
+    b.close()
+
+
 def test_autotune_keeps_results(tracker):
     vs = variants()
     b = native.Batch(tracker)
