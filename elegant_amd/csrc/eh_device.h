@@ -24,6 +24,7 @@ void ehd_destroy(ehd_dev *d);
 int ehd_alloc_state(ehd_dev *d, int64_t n);
 int ehd_put_state(ehd_dev *d, const double *x, const double *xp, const double *y, const double *yp,
                   const double *s, const double *dp, const int32_t *loss_pass, const int32_t *loss_code, int64_t n);
+int ehd_get_one(ehd_dev *d, int64_t i, double *s, double *dp, int32_t *loss_pass);
 int ehd_get_state(ehd_dev *d, double *x, double *xp, double *y, double *yp, double *s, double *dp,
                   int32_t *loss_pass, int32_t *loss_code, int64_t n);
 int ehd_get_sdp(ehd_dev *d, double *s, double *dp, int32_t *loss_pass, int64_t n);

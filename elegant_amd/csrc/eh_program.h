@@ -57,6 +57,8 @@
 #define EH_F_DFAST (1 << 22)   /* DRIF: 2^-300 <= |L| <= 2^300, the short form of the order-2 drift applies */
 #define EH_F_FIELD (1 << 23)   /* CSBEND: multipole field expansion (nfield coefficients at EH_D_FIELD) */
 #define EH_F_ETILT (1 << 24)   /* CSBEND: error tilt, exit offsets at EH_D_ET */
+#define EH_F_CHANGE_T (1 << 25) /* RFCA CHANGE_T: EH_D_TOFF whole RF periods taken out of t and s */
+#define EH_F_CHT_SELF (1 << 26) /* CHANGE_T with each particle as its own beam: the offset from its own time */
 
 /* Aperture shapes other than the closed rectangle, packed in one int32:
  * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the
@@ -147,6 +149,8 @@ typedef struct {
 #define EH_D_RFDY 6
 #define EH_D_HALFLEN 7    /* length / 2 */
 #define EH_D_RFLEN_TOTAL 8 /* element length (VOLT=0 drift) */
+#define EH_D_TOFF 9       /* CHANGE_T timeOffset (set by the host from the beam's first row) */
+#define EH_D_TO 10        /* RF period 2 pi / omega */
 
 /* A run of the tracking kernel's program: records [first, first + len) of
  * the run table's record array, executed reps times in a row, standing for

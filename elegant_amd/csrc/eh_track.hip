@@ -908,6 +908,18 @@ DEV int track_rfca(particle &p, const eh_op &op, const double *z) {
   const double volt = op.d[EH_D_VOLT], omega = op.d[EH_D_OMEGA], phase = op.d[EH_D_PHASE];
   const double length = op.d[EH_D_LENGTH], dtLight = op.d[EH_D_DTLIGHT];
   const int nKicks = op.nkicks_rf;
+  /* CHANGE_T (simple_rfca.c:488-497): whole RF periods in the arrival time
+   * of the beam's first row, which the host supplies, or of the particle
+   * itself when every particle is its own beam */
+  double timeOffset = op.d[EH_D_TOFF];
+  if (op.flags & EH_F_CHT_SELF) {
+    const double To = op.d[EH_D_TO];
+    double P = Po * (1 + p.dp);
+    double beta_i = P / em_sqrt(sq(P) + 1);
+    double t = p.s / (EH_C_MKS * beta_i);
+    timeOffset = t > (0.9 * To) ? (double)(int64_t)(t / To + 0.5) * To : 0.0;
+  }
+  const bool change_t = (op.flags & EH_F_CHANGE_T) && timeOffset != 0;
   p.x -= op.d[EH_D_RFDX];
   p.y -= op.d[EH_D_RFDY];
   if (!(op.flags & EH_F_MATRIX)) {
@@ -918,7 +930,9 @@ DEV int track_rfca(particle &p, const eh_op &op, const double *z) {
         double P = Po * (1 + p.dp);
         double gamma = em_sqrt(sq(P) + 1);
         double beta_i = P / gamma;
-        double t = (p.s + dc4) / (EH_C_MKS * beta_i) - 0.0;
+        double t = (p.s + dc4) / (EH_C_MKS * beta_i) - timeOffset;
+        if (ik == 0 && change_t)
+          p.s = t * EH_C_MKS * beta_i - dc4;
         double phi = omega * (t - 0 - ik * dtLight) + phase;
         double dgamma = volt * em_sin(phi) * 1;
         if (length) {
@@ -954,7 +968,9 @@ DEV int track_rfca(particle &p, const eh_op &op, const double *z) {
       double gamma = em_sqrt(sq(P) + 1);
       double beta_i = P / gamma;
       double ds1 = length / 2 * em_sqrt(1 + sq(p.xp) + sq(p.yp));
-      double t = (p.s + ds1) / (EH_C_MKS * beta_i) - 0.0;
+      double t = (p.s + ds1) / (EH_C_MKS * beta_i) - timeOffset;
+      if (change_t)
+        p.s = t * EH_C_MKS * beta_i - ds1;
       double sin_phase = em_sin(omega * (t - 0 - ik * dtLight) + phase);
       double dgamma = volt * 1 * sin_phase;
       if ((op.flags & EH_F_END1FOCUS) && length && ik == 0) {
@@ -1349,6 +1365,21 @@ int ehd_get_state(ehd_dev *d, double *x, double *xp, double *y, double *yp, doub
     HIPCHK(d, hipMemcpyAsync(loss_pass, d->loss_pass, sizeof(int32_t) * n, hipMemcpyDeviceToHost, d->stream));
   if (loss_code)
     HIPCHK(d, hipMemcpyAsync(loss_code, d->loss_code, sizeof(int32_t) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return EH_OK;
+}
+
+/* s, delta and loss pass of particle slot i alone (any may be NULL) */
+int ehd_get_one(ehd_dev *d, int64_t i, double *s, double *dp, int32_t *loss_pass) {
+  if (i < 0 || i >= d->n)
+    return EH_ERR_INVALID;
+  HIPCHK(d, hipSetDevice(d->device));
+  if (s)
+    HIPCHK(d, hipMemcpyAsync(s, d->coord[4] + i, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  if (dp)
+    HIPCHK(d, hipMemcpyAsync(dp, d->coord[5] + i, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  if (loss_pass)
+    HIPCHK(d, hipMemcpyAsync(loss_pass, d->loss_pass + i, sizeof(int32_t), hipMemcpyDeviceToHost, d->stream));
   HIPCHK(d, hipStreamSynchronize(d->stream));
   return EH_OK;
 }

@@ -164,6 +164,9 @@ int eh_batch_add_variant(eh_batch *b, const eh_element *elems, int64_t n_elems, 
     const EH_RFCA *rf = &elems[b->prog->rfca_elem[k]].p.rfca;
     eh_op *op = &b->prog->ops[b->prog->rfca_op[k]];
     double t0;
+    /* CHANGE_T: every row is its own beam, so its own first row */
+    if (op->flags & EH_F_CHANGE_T)
+      op->flags |= EH_F_CHT_SELF;
     if (!isnan(phases[k])) {
       eh_rfca_set_phase(op, rf, phases[k]);
       continue;

@@ -610,8 +610,8 @@ static int compile_rfca(const eh_element *e, const eh_species *sp, eh_op *op, ch
   double length = rf->length, volt, omega;
   long nKicks;
   int matrixMethod = 0;
-  if (rf->change_t || rf->linearize || rf->lockPhase || rf->bodyFocusModel || rf->standingWave || rf->Q) {
-    snprintf(msg, msglen, "%s: RFCA with CHANGE_T/LINEARIZE/LOCK_PHASE/BODY_FOCUS_MODEL/STANDING_WAVE/Q", e->name);
+  if (rf->linearize || rf->lockPhase || rf->bodyFocusModel || rf->standingWave || rf->Q) {
+    snprintf(msg, msglen, "%s: RFCA with LINEARIZE/LOCK_PHASE/BODY_FOCUS_MODEL/STANDING_WAVE/Q", e->name);
     return EH_ERR_UNSUPPORTED;
   }
   if (rf->fiducialMode != EH_FID_TMEAN && rf->fiducialMode != EH_FID_LIGHT) {
@@ -647,7 +647,29 @@ static int compile_rfca(const eh_element *e, const eh_species *sp, eh_op *op, ch
   op->d[EH_D_RFDX] = rf->dx;
   op->d[EH_D_RFDY] = rf->dy;
   op->d[EH_D_PHASE] = NAN; /* set by eh_rfca_set_phase */
+  /* To = PIx2 / omega (simple_rfca.c:473-478); the offset itself follows the
+   * beam's first row at every pass (eh_rfca_time_offset) */
+  op->d[EH_D_TO] = omega ? EH_PIx2 / omega : 0;
+  op->d[EH_D_TOFF] = 0;
+  if (rf->change_t && omega)
+    op->flags |= EH_F_CHANGE_T;
   return EH_OK;
+}
+
+/* timeOffset of CHANGE_T (simple_rfca.c:488-497) for a beam whose first row
+ * arrives with (s, delta): the whole RF periods in its arrival time, once
+ * it exceeds 0.9 of a period */
+double eh_rfca_time_offset(const eh_op *op, double Po, double s, double dp) {
+  const double To = op->d[EH_D_TO];
+  double P, beta_i, t;
+  if (!(op->flags & EH_F_CHANGE_T))
+    return 0;
+  P = Po * (1 + dp);
+  beta_i = P / sqrt(P * P + 1);
+  t = s / (EH_C_MKS * beta_i);
+  if (t > (0.9 * To))
+    return ((long)(t / To + 0.5)) * To;
+  return 0;
 }
 
 /* phase = phase_fiducial + PHASE*pi/180 (simple_rfca.c:466-478) */

@@ -560,12 +560,51 @@ int eh_rfca_fiducial(eh_ctx *c, int64_t elem_index, double *phase_fiducial) {
   return EH_ERR_INVALID;
 }
 
+/* CHANGE_T (simple_rfca.c:488-497): the cavity's time offset follows the
+ * beam's first row (part[0]) at every pass.  The first row in reference
+ * order moves only when the particle there is lost (swap compaction never
+ * touches row 0 otherwise), so one loss flag is read per pass and the loss
+ * events are replayed only when that particle has gone. */
+static int rfca_time_offset(eh_ctx *c, int k, int64_t pass) {
+  const int32_t opi = c->prog->rfca_op[k];
+  eh_op *op = &c->prog->ops[opi];
+  double s = 0, dp = 0, toff = 0;
+  int32_t lp = 0;
+  int rc;
+  if (c->n_alive_ref > 0) {
+    rc = ehd_get_one(c->dev, c->order[0], NULL, NULL, &lp);
+    if (rc)
+      return devfail(c, rc);
+  }
+  if (c->n_alive_ref <= 0 || lp >= 0) {
+    rc = refresh_losses(c);
+    if (!rc)
+      rc = replay_until(c, pass, 4 * opi);
+    if (rc)
+      return rc;
+  }
+  if (c->n_alive_ref > 0) {
+    rc = ehd_get_one(c->dev, c->order[0], &s, &dp, NULL);
+    if (rc)
+      return devfail(c, rc);
+    toff = eh_rfca_time_offset(op, c->Po, s, dp);
+  }
+  if (op->d[EH_D_TOFF] != toff) {
+    op->d[EH_D_TOFF] = toff;
+    rc = ehd_patch_op(c->dev, opi, op);
+    if (rc)
+      return devfail(c, rc);
+  }
+  return EH_OK;
+}
+
 /* index of the first op >= from that needs the host on this pass, or nops */
 static int32_t next_barrier(const eh_ctx *c, int32_t from) {
   const eh_program *p = c->prog;
   int32_t best = p->nops;
   for (int k = 0; k < p->n_rfca; k++)
-    if (!c->fid_seen[k] && p->rfca_op[k] >= from && p->rfca_op[k] < best)
+    if ((!c->fid_seen[k] || (p->ops[p->rfca_op[k]].flags & EH_F_CHANGE_T)) && p->rfca_op[k] >= from &&
+        p->rfca_op[k] < best)
       best = p->rfca_op[k];
   for (int k = 0; k < p->n_coll; k++)
     if (p->coll_op[k] >= from && p->coll_op[k] < best)
@@ -798,7 +837,11 @@ int eh_track(eh_ctx *c, int64_t pass0, int64_t npasses, double *P_central, eh_st
               for (k = 0; k < c->prog->n_rfca; k++)
                 if (c->prog->rfca_op[k] == b)
                   break;
-              rc = rfca_fiducial(c, k, pass);
+              rc = EH_OK;
+              if (!c->fid_seen[k])
+                rc = rfca_fiducial(c, k, pass);
+              if (!rc && (op->flags & EH_F_CHANGE_T))
+                rc = rfca_time_offset(c, k, pass);
               if (rc)
                 return rc;
               cur = b; /* the RFCA itself runs in the next launch */
@@ -1097,8 +1140,11 @@ int64_t eh_simple_rf_cavity(eh_ctx *c, double **part, int64_t np, const EH_RFCA 
     rc = eh_compile(bl, 1, *P_central, &c->sp, c->prog, c->msg, sizeof(c->msg));
     if (rc)
       return rc;
-    if (c->prog->ops[0].op == EH_OP_RFCA)
+    if (c->prog->ops[0].op == EH_OP_RFCA) {
       eh_rfca_set_phase(&c->prog->ops[0], rfca, *phase_fiducial);
+      /* CHANGE_T: the offset of the caller's first row (part[0]) */
+      c->prog->ops[0].d[EH_D_TOFF] = eh_rfca_time_offset(&c->prog->ops[0], *P_central, part[0][4], part[0][5]);
+    }
     /* the cavity's own removeInvalidParticles is the only cull, as in the
      * routine it replaces */
     c->prog->ops[0].flags &= ~EH_F_AP;

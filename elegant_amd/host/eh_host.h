@@ -31,6 +31,7 @@ void eh_species_electron(eh_species *sp);
 double eh_element_length(const eh_element *e);
 int eh_has_length(int32_t type);
 void eh_rfca_set_phase(eh_op *op, const EH_RFCA *rf, double phase_fiducial);
+double eh_rfca_time_offset(const eh_op *op, double Po, double s, double dp);
 double eh_rfca_phase_fiducial(const EH_RFCA *rf, double t0);
 
 /* swap-to-tail compaction of one loss event on the reference row order */

@@ -1271,9 +1271,8 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
   long ip, nKicks, ik;
   double P, gamma, dgamma = 0.0, phase, length, dtLight, volt, beta_i, t, t0, omega, tau;
   int matrixMethod = 0;
-  double timeOffset = 0;
-  if (rfca->change_t || rfca->linearize || rfca->lockPhase || rfca->bodyFocusModel || rfca->standingWave ||
-      rfca->Q)
+  double timeOffset = 0, To;
+  if (rfca->linearize || rfca->lockPhase || rfca->bodyFocusModel || rfca->standingWave || rfca->Q)
     return EH_ERR_UNSUPPORTED;
   if (np <= 0)
     return np;
@@ -1312,13 +1311,25 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
     c->fiducial_seen[ei] = 1;
   }
   phase = c->phase_fiducial[ei];
-  if (omega)
+  if (omega) {
     t0 = -c->phase_fiducial[ei] / omega;
-  else
-    t0 = 0;
+    To = EH_PIx2 / omega;
+  } else
+    t0 = To = 0;
   (void)t0;
   (void)tau;
   phase += rfca->phase * EH_PI / 180.0;
+
+  /* CHANGE_T: whole RF periods in the first row's arrival time
+   * (simple_rfca.c:488-497) */
+  if (omega && rfca->change_t && np != 0) {
+    double *coord = part[0];
+    P = *P_central * (1 + coord[5]);
+    beta_i = P / (gamma = sqrt(sqr(P) + 1));
+    t = coord[4] / (EH_C_MKS * beta_i);
+    if (omega != 0 && t > (0.9 * To) && rfca->change_t)
+      timeOffset = ((long)(t / To + 0.5)) * To;
+  }
 
   for (ip = 0; ip < np; ip++) {
     part[ip][0] -= rfca->dx;
@@ -1340,6 +1351,8 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
         P = *P_central * (1 + coord[5]);
         beta_i = P / (gamma = sqrt(sqr(P) + 1));
         t = (coord[4] + dc4) / (EH_C_MKS * beta_i) - timeOffset;
+        if (ik == 0 && timeOffset && rfca->change_t)
+          coord[4] = t * EH_C_MKS * beta_i - dc4;
         {
           double phi = omega * (t - 0 - ik * dtLight) + phase;
           dgamma = volt * P_SIN(phi) * 1;
@@ -1384,6 +1397,8 @@ long or_simple_rf_cavity(or_ctx *c, long ei, double **part, long np, const EH_RF
         beta_i = P / (gamma = sqrt(sqr(P) + 1));
         ds1 = length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
         t = (coord[4] + ds1) / (EH_C_MKS * beta_i) - timeOffset;
+        if (timeOffset && rfca->change_t)
+          coord[4] = t * EH_C_MKS * beta_i - ds1;
         {
           double sin_phase = P_SIN(omega * (t - 0 - ik * dtLight) + phase);
           dgamma = volt * 1 * sin_phase;
