@@ -59,6 +59,7 @@
 #define EH_F_ETILT (1 << 24)   /* CSBEND: error tilt, exit offsets at EH_D_ET */
 #define EH_F_CHANGE_T (1 << 25) /* RFCA CHANGE_T: EH_D_TOFF whole RF periods taken out of t and s */
 #define EH_F_CHT_SELF (1 << 26) /* CHANGE_T with each particle as its own beam: the offset from its own time */
+#define EH_F_LEND (1 << 27)    /* KQUAD LEFFECTIVE: exact drifts of EH_D_LEND around the body */
 
 /* Aperture shapes other than the closed rectangle, packed in one int32:
  * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the
@@ -106,6 +107,7 @@ typedef struct {
 #define EH_D_YK 48      /* [8]: -(ykick/n) * kickFrac[step] */
 #define EH_D_DRIFT 56   /* drift / n (nonzero test) */
 #define EH_D_TOTLEN 57  /* sum of all dsh in step order: dzLoss of a completed particle */
+#define EH_D_LEND 58    /* (L - LEFFECTIVE) / 2 */
 
 /* ---- CSBEND ---- */
 #define EH_D_RHO0 0

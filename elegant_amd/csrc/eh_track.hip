@@ -503,6 +503,9 @@ DEV int track_multipole(particle &p, const eh_op &op, const double *z) {
     offset(p, op.d[EH_D_DX], op.d[EH_D_DY], op.d[EH_D_DZ]);
   if (flags & EH_F_ROTATE)
     rotate(p, op.d[EH_D_COS_IN], op.d[EH_D_SIN_IN]);
+  /* KQUAD LEFFECTIVE: an exact drift before the body (multipole.c:1028-1030) */
+  if (flags & EH_F_LEND)
+    exact_drift(p, op.d[EH_D_LEND]);
 
   /* the row as the integrator receives it stays in p until the end */
   const double dp = p.dp;
@@ -569,7 +572,10 @@ DEV int track_multipole(particle &p, const eh_op &op, const double *z) {
     p.s = opc(&z[0]) + dzLoss;
     p.dp = op.Po * (1 + p.dp);
   }
-  /* exit misalignment, applied to lost rows too (multipole.c:1103-1104) */
+  /* the exit drift of LEFFECTIVE and the exit misalignment, applied to
+   * lost rows too (multipole.c:1098-1104) */
+  if (flags & EH_F_LEND)
+    exact_drift(p, op.d[EH_D_LEND]);
   if (flags & EH_F_ROTATE)
     rotate(p, op.d[EH_D_COS_OUT], op.d[EH_D_SIN_OUT]);
   if (flags & EH_F_OFFSET)

@@ -272,20 +272,28 @@ static int compile_multipole(const eh_element *e, double Po, const eh_species *s
   switch (e->type) {
   case EH_T_KQUAD: {
     const EH_KQUAD *q = &e->p.kquad;
-    if (q->synch_rad || q->isr || q->edge1_effects || q->edge2_effects || q->radial || q->lEffective > 0 ||
-        q->malignMethod) {
-      snprintf(msg, msglen, "%s: KQUAD with SYNCH_RAD/ISR/EDGE*_EFFECTS/RADIAL/LEFFECTIVE/MALIGN_METHOD", e->name);
+    if (q->synch_rad || q->isr || q->edge1_effects || q->edge2_effects || q->radial || q->malignMethod) {
+      snprintf(msg, msglen, "%s: KQUAD with SYNCH_RAD/ISR/EDGE*_EFFECTS/RADIAL/MALIGN_METHOD", e->name);
       return EH_ERR_UNSUPPORTED;
     }
+    double lEffective;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
     order[0] = 1;
+    /* LEFFECTIVE: the body takes the effective length, exact drifts the
+     * rest (multipole.c:766-777) */
+    if ((lEffective = q->lEffective) <= 0)
+      lEffective = q->length;
+    else {
+      op->flags |= EH_F_LEND;
+      op->d[EH_D_LEND] = (q->length - lEffective) / 2;
+    }
     if (q->bore)
-      KnL[0] = q->B / q->bore * (sp->charge / (sp->mass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      KnL[0] = q->B / q->bore * (sp->charge / (sp->mass * EH_C_MKS * Po)) * lEffective * (1 + q->fse);
     else
-      KnL[0] = q->k1 * q->length * (1 + q->fse);
-    drift = q->length;
+      KnL[0] = q->k1 * lEffective * (1 + q->fse);
+    drift = lEffective;
     tilt = q->tilt;
     dx = q->dx;
     dy = q->dy;

@@ -271,6 +271,8 @@ def flops_per_particle_turn(beamline):
             nk = p.n_kicks
             S = max(1, math.ceil(nk / p.integration_order)) if nk > 0 else p.nSlices
             total += _multipole_flops_alg(order, S, bool(p.dx or p.dy or p.dz), bool(p.tilt))
+            if t == A.EH_T_KQUAD and p.lEffective > 0:
+                total += 22  # the two exact end drifts of LEFFECTIVE
         elif t == A.EH_T_CSBEND:
             b = e.p.csbend
             total += _csbend_flops_alg(b.nSlices) if b.angle else 11

@@ -638,6 +638,8 @@ typedef struct {
   double dx, dy, dz, tilt, drift, xkick, ykick;
   long nSlices, integ_order;
   int expand;
+  int doEndDrift; /* KQUAD LEFFECTIVE: exact drifts of lEnd around the body (multipole.c:766-770) */
+  double lEnd;
 } mult_setup;
 
 /* integrate_kick_multipole_ordn (multipole.c:1122-1410) with no error,
@@ -757,18 +759,24 @@ static int multipole_setup(const eh_element *elem, double Po, mult_setup *m) {
   switch (elem->type) {
   case EH_T_KQUAD: {
     const EH_KQUAD *q = &elem->p.kquad;
-    if (q->synch_rad || q->isr || q->edge1_effects || q->edge2_effects || q->radial || q->lEffective > 0 ||
-        q->malignMethod)
+    double lEffective;
+    if (q->synch_rad || q->isr || q->edge1_effects || q->edge2_effects || q->radial || q->malignMethod)
       return EH_ERR_UNSUPPORTED;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     m->expand = q->expandHamiltonian;
     m->order[0] = 1;
+    if ((lEffective = q->lEffective) <= 0)
+      lEffective = q->length;
+    else {
+      m->lEnd = (q->length - lEffective) / 2;
+      m->doEndDrift = 1;
+    }
     if (q->bore)
-      m->KnL[0] = q->B / q->bore * (particleCharge / (particleMass * EH_C_MKS * Po)) * q->length * (1 + q->fse);
+      m->KnL[0] = q->B / q->bore * (particleCharge / (particleMass * EH_C_MKS * Po)) * lEffective * (1 + q->fse);
     else
-      m->KnL[0] = q->k1 * q->length * (1 + q->fse);
-    m->drift = q->length;
+      m->KnL[0] = q->k1 * lEffective * (1 + q->fse);
+    m->drift = lEffective;
     m->tilt = q->tilt;
     m->dx = q->dx;
     m->dy = q->dy;
@@ -863,6 +871,8 @@ long or_multipole_tracking2(double **particle, long n_part, const eh_element *el
   i_top = n_part - 1;
   setup_mult_ap(&ap, -m.tilt, maxamp);
   offset_for_misalignment(particle, n_part, m.dx, m.dy, m.dz, m.tilt, 1);
+  if (m.doEndDrift)
+    or_exact_drift(particle, n_part, m.lEnd);
   for (i_part = 0; i_part <= i_top; i_part++) {
     double *coord = particle[i_part];
     if (!integrate_kick_multipole(coord, &m, Po, &ap, &dzLoss)) {
@@ -874,6 +884,10 @@ long or_multipole_tracking2(double **particle, long n_part, const eh_element *el
       continue;
     }
   }
+  /* the exit drift, like the exit misalignment, covers the lost rows too
+   * (multipole.c:1098-1104) */
+  if (m.doEndDrift)
+    or_exact_drift(particle, n_part, m.lEnd);
   if (m.dx || m.dy || m.dz || m.tilt)
     offset_for_misalignment(particle, n_part, m.dx, m.dy, m.dz, m.tilt, 2);
   return i_top + 1;

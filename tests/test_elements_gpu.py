@@ -43,6 +43,7 @@ def rows_ptr(r):
     lambda: E.KQUAD(L=0.4, K1=1.8, N_KICKS=8, DX=2e-4, DY=-1e-4, TILT=3e-3),
     lambda: E.KSEXT(L=0.2, K2=300.0, N_KICKS=4, DX=1e-4),
     lambda: E.KOCT(L=0.1, K3=4e4, TILT=1e-3),
+    lambda: E.KQUAD(L=0.5, LEFFECTIVE=0.4, K1=1.8, N_KICKS=8, DX=2e-4, DY=-1e-4, TILT=3e-3),
 ])
 def test_multipole_tracking2(tracker, make):
     e = make()
@@ -63,6 +64,7 @@ def test_multipole_tracking2(tracker, make):
     (lambda: E.KSEXT(L=0.2, K2=300.0, N_KICKS=4, DX=1e-4), False),
     (lambda: E.KOCT(L=0.0, K3=4e4), False),
     (lambda: E.KQUAD(L=0.4, K1=1.8, N_KICKS=8, EXPAND_HAMILTONIAN=1), False),
+    (lambda: E.KQUAD(L=0.5, LEFFECTIVE=0.4, K1=1.8, N_KICKS=8, DX=2e-4, TILT=3e-3), True),
 ])
 def test_multipole_fast_path_edges(tracker, make, aperture):
     """multipole_tracking2 with rows of every kind in the same waves as

@@ -928,3 +928,33 @@ def test_etilt_offsets_against_geometry(rho0, angle, etilt, tilt):
     assert np.max(np.abs(got - want) / scale) < 1e-6, (got, want)
     L.or_etilt_offsets(rho0, angle, 0.0, tilt, out)
     assert list(out) == [0.0] * 5
+
+
+def test_quadrupole_effective_length_is_drift_quad_drift():
+    """KQUAD LEFFECTIVE (multipole.c:766-777, 1028-1030, 1098-1100): the
+    body has the effective length and strength K1 LEFFECTIVE, and exact
+    drifts of (L - LEFFECTIVE)/2 pad it.  Pinned on the linear map: a small
+    orbit follows drift . thick quad . drift with the textbook cos/cosh
+    blocks (to the integrator's O(h^4) error), and with K1 = 0 the element
+    is an exact drift of L."""
+    L, Le, k = 0.8, 0.5, 1.2
+    lend = (L - Le) / 2
+    x0, xp0 = 1e-6, 2e-7
+    r, n, _ = track([E.KQUAD(L=L, LEFFECTIVE=Le, K1=k, N_KICKS=32)],
+                    rows_of([[x0, xp0, x0, xp0, 0, 0]]))
+    s = math.sqrt(k)
+    D = np.array([[1, lend], [0, 1]])
+    F = np.array([[math.cos(s * Le), math.sin(s * Le) / s], [-s * math.sin(s * Le), math.cos(s * Le)]])
+    Dn = np.array([[math.cosh(s * Le), math.sinh(s * Le) / s], [s * math.sinh(s * Le), math.cosh(s * Le)]])
+    ex = D @ F @ D @ [x0, xp0]
+    ey = D @ Dn @ D @ [x0, xp0]
+    assert abs(r[0, 0] - ex[0]) < 1e-5 * x0 and abs(r[0, 1] - ex[1]) < 1e-5 * x0
+    assert abs(r[0, 2] - ey[0]) < 1e-5 * x0 and abs(r[0, 3] - ey[1]) < 1e-5 * x0
+    # a mistake in the strength or the padding moves the orbit far more
+    wrong = D @ np.array([[math.cos(s * L), math.sin(s * L) / s], [-s * math.sin(s * L), math.cos(s * L)]]) @ D
+    assert abs((wrong @ [x0, xp0])[0] - ex[0]) > 100 * abs(r[0, 0] - ex[0])
+    pts = [[1e-3, 2e-3, -3e-4, 5e-4, 0.1, 1e-3], [-2e-3, -1e-3, 4e-4, 1e-4, 0, -2e-3]]
+    r, n, _ = track([E.KQUAD(L=L, LEFFECTIVE=Le, K1=0.0, N_KICKS=4)], rows_of(pts))
+    for p, q in zip(pts, r):
+        assert abs(q[0] - (p[0] + L * p[1])) < 1e-15 and abs(q[2] - (p[2] + L * p[3])) < 1e-15
+        assert abs(q[4] - (p[4] + L * math.sqrt(1 + p[1] ** 2 + p[3] ** 2))) < 1e-14
