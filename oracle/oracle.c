@@ -145,6 +145,7 @@ void or_set_species(double mass, double charge, double massMV, double radius, do
 static int or_nprops = 8;
 #define OR_NPROPS or_nprops
 void or_set_row_width(long nprops) { or_nprops = nprops < 8 ? 8 : (int)nprops; }
+long or_row_width(void) { return or_nprops; }
 
 /* ---------------------------------------------------------------- drift */
 

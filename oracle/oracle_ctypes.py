@@ -57,6 +57,7 @@ def lib(mode="glibc"):
         L.or_do_match_energy.argtypes = [PP, C.c_long, C.POINTER(C.c_double)]
         L.or_etilt_offsets.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.or_set_row_width.argtypes = [C.c_long]
+        L.or_row_width.restype = C.c_long
         L.or_set_species.argtypes = [C.c_double] * 5
         L.or_set_id_slots_per_bunch.argtypes = [C.c_void_p, C.c_long]
         L.or_real_fft.argtypes = [C.POINTER(C.c_double), C.c_long, C.c_int]
@@ -112,11 +113,19 @@ class Oracle:
         per bunch index (column 8)"""
         self.L.or_set_id_slots_per_bunch(self.ctx, int(n))
 
+    @contextlib.contextmanager
     def _rows(self, rows):
+        """the row pointers, with the library's row width (a global of the
+        checker, as totalPropertiesPerParticle is elegant's) set to this
+        context's for the call and put back to 8 after it, so that direct
+        calls of the library on 8-column rows stay in bounds"""
         if rows.shape[1] < self.row_width:
             raise ValueError("rows narrower than the row width %d" % self.row_width)
         self.L.or_set_row_width(self.row_width)
-        return A.rows_pointer(rows)
+        try:
+            yield A.rows_pointer(rows)
+        finally:
+            self.L.or_set_row_width(8)
 
     def __del__(self):
         try:
@@ -126,10 +135,10 @@ class Oracle:
 
     def do_tracking(self, rows, beamline, Po, n_passes, pass_offset=0):
         """rows: float64 (n, >=8) C-contiguous, modified in place.  Returns nLeft."""
-        ptr, keep = self._rows(rows)
         pc = C.c_double(Po)
-        r = self.L.or_do_tracking(self.ctx, ptr, rows.shape[0], beamline.pointer(), len(beamline), C.byref(pc),
-                                  n_passes, pass_offset)
+        with self._rows(rows) as (ptr, keep):
+            r = self.L.or_do_tracking(self.ctx, ptr, rows.shape[0], beamline.pointer(), len(beamline), C.byref(pc),
+                                      n_passes, pass_offset)
         if r < 0:
             raise RuntimeError("oracle do_tracking failed: %d" % r)
         self.P_central = pc.value  # changed by CHANGE_P0 cavities
@@ -168,15 +177,15 @@ class Oracle:
         return float(self.L.or_get_fiducial(self.ctx, elem_index))
 
     def wake(self, rows, wake, Po, i_pass):
-        ptr, keep = self._rows(rows)
-        r = self.L.or_track_through_wake(self.ctx, ptr, rows.shape[0], C.byref(wake), Po, i_pass)
+        with self._rows(rows) as (ptr, keep):
+            r = self.L.or_track_through_wake(self.ctx, ptr, rows.shape[0], C.byref(wake), Po, i_pass)
         if r < 0:
             raise RuntimeError("oracle wake failed: %d" % r)
         return r
 
     def zlongit(self, rows, zl, Po, i_pass, elem_index=0):
-        ptr, keep = self._rows(rows)
-        r = self.L.or_track_through_zlongit(self.ctx, elem_index, ptr, rows.shape[0], C.byref(zl), Po, i_pass)
+        with self._rows(rows) as (ptr, keep):
+            r = self.L.or_track_through_zlongit(self.ctx, elem_index, ptr, rows.shape[0], C.byref(zl), Po, i_pass)
         if r < 0:
             raise RuntimeError("oracle zlongit failed: %d" % r)
         return r

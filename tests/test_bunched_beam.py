@@ -145,6 +145,22 @@ def test_zlongit_buckets_follow_the_first_buckets_bins():
     assert not np.array_equal(fresh, out[sel1])
 
 
+def test_wide_rows_leave_the_checkers_row_width_at_eight():
+    """the checker's row width is a library global (elegant's
+    totalPropertiesPerParticle): a context with 10-column rows sets it for
+    its own calls only, so later direct calls on 8-column rows stay in
+    bounds"""
+    rows = bunch_train(500, 2, 15e-12, seed=12)
+    table = Lt.resonator_wake()
+    w = E.WAKE(table)
+    o = _oracle(500, 1e-12)
+    o.wake(rows, w.p.wake, PO, 0)
+    assert o.L.or_row_width() == 8
+    with pytest.raises(ValueError):
+        o.wake(np.ascontiguousarray(rows[:, :8]), w.p.wake, PO, 0)
+    assert o.L.or_row_width() == 8
+
+
 # ------------------------------------------------------------------ GPU
 
 def _bunched(tracker, id_slots):
