@@ -108,7 +108,8 @@ class EH_ZLONGIT(C.Structure):
         ["Ra", "Rs", "Q", "freq", "bin_size", "factor"],
         ["broad_band", "n_bins", "max_n_bins", "interpolate", "area_weight", "startOnPass",
          "rampPasses", "reverseTimeOrder", "smoothing", "allowLongBeam", "bunchedBeamMode", "startBunch",
-         "endBunch", "pad"])
+         "endBunch", "pad"]) + [("f", C.POINTER(D)), ("Zreal", C.POINTER(D)), ("Zimag", C.POINTER(D)),
+                                ("n_spect", L)]
 
 
 class EH_MAXAMP(C.Structure):

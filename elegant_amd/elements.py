@@ -176,6 +176,34 @@ class WakeTable:
             raise ValueError("t and W must be 1-D arrays of equal length")
 
 
+class ImpedanceTable:
+    """Owns the f[], Zreal[], Zimag[] arrays a ZLONGIT with BROAD_BAND=0
+    points to (zlongit.c:634-700): n_spect equally spaced frequencies from
+    0 (Hz), n_spect - 1 a power of 2; Zreal or Zimag None for zero."""
+
+    def __init__(self, f, Zreal=None, Zimag=None):
+        self.f = np.ascontiguousarray(f, dtype=np.float64)
+        self.Zreal = None if Zreal is None else np.ascontiguousarray(Zreal, dtype=np.float64)
+        self.Zimag = None if Zimag is None else np.ascontiguousarray(Zimag, dtype=np.float64)
+        for z in (self.Zreal, self.Zimag):
+            if z is not None and z.shape != self.f.shape:
+                raise ValueError("impedance samples and frequencies differ in length")
+
+
+def ZLONGIT_TABLE(table, name="", **kw):
+    """ZLONGIT with an impedance table (BROAD_BAND=0); keep `table` alive
+    while the element is in use (Beamline(..., keep=[table]))"""
+    e = make(A.EH_T_ZLONGIT, name, **kw)
+    z = e.p.zlongit
+    z.broad_band = 0
+    P = C.POINTER(C.c_double)
+    z.f = table.f.ctypes.data_as(P)
+    z.Zreal = table.Zreal.ctypes.data_as(P) if table.Zreal is not None else None
+    z.Zimag = table.Zimag.ctypes.data_as(P) if table.Zimag is not None else None
+    z.n_spect = table.f.size
+    return e
+
+
 def WAKE(table, name="", **kw):
     e = make(A.EH_T_WAKE, name, **kw)
     e.p.wake.t = table.t.ctypes.data_as(C.POINTER(C.c_double))

@@ -105,7 +105,89 @@ static int wake_setup(eh_ctx *c, int slot, const EH_WAKE *w, double *dt, double 
   return EH_OK;
 }
 
-/* set_up_zlongit, broad-band branch (zlongit.c:587-632), done once per element */
+/* the impedance Z[n_bins] and the FFT twiddles cos/sin(2 pi k / n_bins) (the
+ * table the CPU checker builds) to the device slot */
+static int zlongit_upload(eh_ctx *c, int slot, const double *Z, long n_bins, double bin_size) {
+  ehd_dev *d = eh_ctx_dev(c);
+  double *tc = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
+  double *ts = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
+  int rc;
+  if (!tc || !ts) {
+    free(tc);
+    free(ts);
+    return EH_ERR_NOMEM;
+  }
+  for (long k = 0; k <= n_bins / 2; k++) {
+    double a = EH_PIx2 * k / n_bins;
+    tc[k] = cos(a);
+    ts[k] = sin(a);
+  }
+  rc = ehd_put_zlongit(d, slot, Z, tc, ts, (int32_t)n_bins);
+  free(tc);
+  free(ts);
+  if (rc)
+    return msgf(c, rc, ehd_message(d));
+  eh_ctx_zl_nbins(c)[slot] = (int32_t)n_bins;
+  eh_ctx_zl_binsize(c)[slot] = bin_size;
+  eh_ctx_coll_loaded(c)[slot] = 1;
+  return EH_OK;
+}
+
+/* checkPointSpacing (zlongit.c:788-803): spacings equal to within the
+ * tolerance, relative to the whole range */
+static int spacing_ok(const double *x, int64_t n, double tolerance) {
+  double dx0, range;
+  if (n < 3)
+    return 1;
+  if ((range = x[n - 1] - x[0]) <= 0)
+    return 0;
+  dx0 = (x[1] - x[0]) / range;
+  for (int64_t i = 1; i < n - 1; i++)
+    if (fabs((x[i + 1] - x[i]) / range - dx0) > tolerance)
+      return 0;
+  return 1;
+}
+
+/* set_up_zlongit with Z(f) tables (zlongit.c:634-700): the packed spectrum
+ * of the real FFT, the DC and Nyquist terms doubled (as sddsfft scales
+ * them); Z holds 2 (n_spect - 1) values */
+static int zlongit_table(eh_ctx *c, const EH_ZLONGIT *zl, double **Z_out, long *n_bins, double *bin_size) {
+  const int64_t n = zl->n_spect;
+  double *Z;
+  if (!zl->f || (!zl->Zreal && !zl->Zimag))
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: give BROAD_BAND=1, or ZREAL and/or ZIMAG");
+  if (n < 2)
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: too little data in the impedance tables");
+  if (!spacing_ok(zl->f, n, 1e-6))
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: impedance frequencies not equally spaced");
+  if (zl->Zimag && zl->Zimag[0])
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: impedance spectrum has a non-zero imaginary DC term");
+  if (n - 1 < 4 || ((n - 1) & (n - 2)))
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: the number of spectrum points must be 2^n+1, n>1");
+  if (2 * (n - 1) > (1L << 20))
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: more than 2^20 bins");
+  *n_bins = (long)(2 * (n - 1));
+  *bin_size = 1.0 / (*n_bins * (zl->f[1] - zl->f[0]));
+  Z = (double *)malloc(sizeof(double) * *n_bins);
+  if (!Z)
+    return EH_ERR_NOMEM;
+  for (int64_t i = 0; i < n; i++) {
+    const double zr = zl->Zreal ? zl->Zreal[i] : 0, zi = zl->Zimag ? zl->Zimag[i] : 0;
+    if (i == 0)
+      Z[0] = 2 * zr;
+    else if (i == n - 1)
+      Z[2 * i - 1] = 2 * zr; /* Nyquist: N_BINS is even */
+    else {
+      Z[2 * i - 1] = zr;
+      Z[2 * i] = zi;
+    }
+  }
+  *Z_out = Z;
+  return EH_OK;
+}
+
+/* set_up_zlongit (zlongit.c:559-700), done once per element: the broad-band
+ * resonator on bins the optimiser fits to the bunch, or the tables */
 static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po, int32_t use_bunch, int32_t target) {
   ehd_dev *d = eh_ctx_dev(c);
   double tmin, tmax, bin_size, df, Ra;
@@ -114,8 +196,19 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po, i
   int rc;
   if (eh_ctx_coll_loaded(c)[slot])
     return EH_OK;
-  if (!zl->broad_band || zl->area_weight || zl->smoothing || zl->reverseTimeOrder)
-    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: broad-band resonator without AREA_WEIGHT/SMOOTHING/REVERSE only");
+  if (zl->area_weight || zl->smoothing || zl->reverseTimeOrder)
+    return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: AREA_WEIGHT, SMOOTHING and REVERSE_TIME_ORDER are not supported");
+  if (!zl->broad_band) {
+    double *Z = NULL;
+    rc = zlongit_table(c, zl, &Z, &n_bins, &bin_size);
+    if (rc)
+      return rc;
+    rc = zlongit_upload(c, slot, Z, n_bins, bin_size);
+    free(Z);
+    return rc;
+  }
+  if (zl->Zreal || zl->Zimag)
+    return msgf(c, EH_ERR_INVALID, "ZLONGIT: both a broad-band impedance and Z(f) tables");
   if (zl->bin_size <= 0 || (zl->Ra && zl->Rs) || zl->n_bins % 2 != 0)
     return msgf(c, EH_ERR_INVALID, "ZLONGIT: BIN_SIZE, RA/RS or N_BINS invalid");
   rc = ehd_time_span(d, Po, use_bunch, target, &tmin, &tmax, &count);
@@ -135,14 +228,8 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po, i
     return msgf(c, EH_ERR_UNSUPPORTED, "ZLONGIT: a bin count that is not a power of 2");
   {
     double *Z = (double *)malloc(sizeof(double) * n_bins);
-    double *tc = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
-    double *ts = (double *)malloc(sizeof(double) * (n_bins / 2 + 1));
-    if (!Z || !tc || !ts) {
-      free(Z);
-      free(tc);
-      free(ts);
+    if (!Z)
       return EH_ERR_NOMEM;
-    }
     df = 1 / (n_bins * bin_size) / (zl->freq);
     nfreq = n_bins / 2 + 1;
     Z[0] = 0;
@@ -152,23 +239,10 @@ static int zlongit_setup(eh_ctx *c, int slot, const EH_ZLONGIT *zl, double Po, i
       Z[2 * i - 1] = Ra / 2 / (1 + sqr(term));
       Z[2 * i] = Z[2 * i - 1] * term;
     }
-    /* FFT twiddles cos/sin(2 pi k / n), the same table the CPU checker builds */
-    for (long k = 0; k <= n_bins / 2; k++) {
-      double a = EH_PIx2 * k / n_bins;
-      tc[k] = cos(a);
-      ts[k] = sin(a);
-    }
-    rc = ehd_put_zlongit(d, slot, Z, tc, ts, (int32_t)n_bins);
+    rc = zlongit_upload(c, slot, Z, n_bins, bin_size);
     free(Z);
-    free(tc);
-    free(ts);
-    if (rc)
-      return msgf(c, rc, ehd_message(d));
+    return rc;
   }
-  eh_ctx_zl_nbins(c)[slot] = (int32_t)n_bins;
-  eh_ctx_zl_binsize(c)[slot] = bin_size;
-  eh_ctx_coll_loaded(c)[slot] = 1;
-  return EH_OK;
 }
 
 /* START_BUNCH/END_BUNCH (wake.c:84-88, zlongit.c:142-146): bucket b takes no

@@ -312,3 +312,29 @@ def wake_table(path, tcol="t", wcol="W", page=0):
     s = read(path)
     pg = s.pages[page]
     return np.asarray(pg.columns[tcol], dtype=np.float64), np.asarray(pg.columns[wcol], dtype=np.float64)
+
+
+def impedance_table(real=None, imag=None, page=0):
+    """(f, Zreal, Zimag) for a ZLONGIT with BROAD_BAND=0 from elegant's
+    ZREAL/ZIMAG file specifications, each "file=fcol+zcol" (the table
+    reader's form, zlongit.c:640-643); a missing one gives None.  Both
+    files must share their frequencies (count and spacing are checked by
+    the reference, zlongit.c:665-674; here the values must agree)."""
+    out, f = [], None
+    for spec in (real, imag):
+        if spec is None:
+            out.append(None)
+            continue
+        path, _, cols = spec.partition("=")
+        fcol, _, zcol = cols.partition("+")
+        if not fcol or not zcol:
+            raise ValueError("impedance file specification must be file=freqColumn+valueColumn: %r" % spec)
+        pg = read(path).pages[page]
+        fi = np.asarray(pg.columns[fcol], dtype=np.float64)
+        if f is not None and not np.array_equal(f, fi):
+            raise ValueError("ZREAL and ZIMAG tables have different frequencies")
+        f = fi
+        out.append(np.asarray(pg.columns[zcol], dtype=np.float64))
+    if f is None:
+        raise ValueError("give ZREAL and/or ZIMAG")
+    return f, out[0], out[1]

@@ -137,14 +137,22 @@ typedef struct {
   int32_t startBunch, endBunch, pad;
 } EH_WAKE;
 
-/* ZLONGIT (track.h:2836-2865; defaults track_data.c:2251-2283), broad-band
- * resonator form only (zlongit.c:587-632) */
+/* ZLONGIT (track.h:2836-2865; defaults track_data.c:2251-2283): the
+ * broad-band resonator (BROAD_BAND=1, zlongit.c:587-632) or an impedance
+ * given as samples (zlongit.c:634-700) */
 typedef struct {
   double Ra, Rs, Q, freq, bin_size, factor;
   int32_t broad_band, n_bins, max_n_bins, interpolate;
   int32_t area_weight, startOnPass, rampPasses, reverseTimeOrder;
   int32_t smoothing, allowLongBeam, bunchedBeamMode;
   int32_t startBunch, endBunch, pad; /* as in EH_WAKE (zlongit.c:142-146) */
+  /* BROAD_BAND=0: the ZREAL and/or ZIMAG tables, n_spect samples at the
+   * equally spaced frequencies f[] (Hz, from 0; the two files of the
+   * reference must share count and spacing, so one array serves both),
+   * n_spect - 1 a power of 2 of at least 4; Zreal or Zimag NULL: zero.
+   * N_BINS becomes 2 (n_spect - 1) and BIN_SIZE 1 / (N_BINS (f[1] - f[0])). */
+  const double *f, *Zreal, *Zimag;
+  int64_t n_spect;
 } EH_ZLONGIT;
 
 /* MAXAMP (track.h:1955-1961; defaults track_data.c:990-997).  ELLIPTICAL

@@ -1883,6 +1883,60 @@ long or_track_through_zlongit(or_ctx *c, long ei, double **part, long np, const 
   return np;
 }
 
+/* checkPointSpacing (zlongit.c:788-803) */
+static long check_point_spacing(const double *x, long n, double tolerance) {
+  double dx, dx0, range;
+  long i;
+  if (n < 3)
+    return 1;
+  if ((range = x[n - 1] - x[0]) <= 0)
+    return 0;
+  dx0 = (x[1] - x[0]) / range;
+  for (i = 1; i < n - 1; i++) {
+    dx = (x[i + 1] - x[i]) / range;
+    if (fabs(dx - dx0) > tolerance)
+      return 0;
+  }
+  return 1;
+}
+
+/* set_up_zlongit from the ZREAL/ZIMAG tables (zlongit.c:634-700); the two
+ * tables share the frequency array here.  The toolkit's power_of_2 is not
+ * part of the reference: n_spect - 1 is taken as 2^n with n > 1, as the
+ * reference's message states. */
+static int zlongit_table(const EH_ZLONGIT *zl, double **Z_out, long *n_bins, double *bin_size) {
+  long i, n_spect = (long)zl->n_spect;
+  const double *Zr = zl->Zreal, *Zi = zl->Zimag;
+  double df_spect, *Z;
+  if (!zl->f || (!Zr && !Zi))
+    return EH_ERR_INVALID; /* "you must either give broad_band=1, or Zreal and/or Zimag" */
+  if (n_spect < 2 || !check_point_spacing(zl->f, n_spect, 1e-6))
+    return EH_ERR_INVALID;
+  df_spect = zl->f[1] - zl->f[0];
+  if (Zi && Zi[0])
+    return EH_ERR_INVALID; /* non-zero imaginary DC term */
+  if (n_spect - 1 < 4 || ((n_spect - 1) & (n_spect - 2)))
+    return EH_ERR_INVALID;
+  *n_bins = 2 * (n_spect - 1);
+  *bin_size = 1.0 / (*n_bins * df_spect);
+  Z = (double *)malloc(sizeof(double) * 2 * *n_bins);
+  for (i = 0; i < n_spect; i++) {
+    const double zr = Zr ? Zr[i] : 0, zi = Zi ? Zi[i] : 0;
+    if (i == 0)
+      /* DC term (multiply by 2 for consistency with sddsfft) */
+      Z[0] = 2 * zr;
+    else if (i == n_spect - 1 && *n_bins % 2 == 0)
+      /* Nyquist term */
+      Z[2 * i - 1] = 2 * zr;
+    else {
+      Z[2 * i - 1] = zr;
+      Z[2 * i] = zi;
+    }
+  }
+  *Z_out = Z;
+  return EH_OK;
+}
+
 static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_ZLONGIT *zl, double Po,
                            long i_pass) {
   double tmin, tmax, tmean, dt, factor, rampFactor, Ra, bin_size, df;
@@ -1890,7 +1944,7 @@ static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_
   double *time, *Itime, *Vtime, *Z;
   long *pbin;
   int rc;
-  if (!zl->broad_band || zl->area_weight || zl->smoothing || zl->reverseTimeOrder)
+  if (zl->area_weight || zl->smoothing || zl->reverseTimeOrder)
     return EH_ERR_UNSUPPORTED;
   if ((i_pass -= zl->startOnPass) < 0 || zl->factor == 0)
     return np;
@@ -1907,8 +1961,20 @@ static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_
   tmax = -(tmin = DBL_MAX);
   find_min_max(&tmin, &tmax, time, np);
   tmean = compute_average(time, np);
-  /* set_up_zlongit (zlongit.c:559-632), recomputed per call (pure function of the element) */
-  if (zl->bin_size <= 0 || (zl->Ra && zl->Rs) || zl->n_bins % 2 != 0) {
+  /* set_up_zlongit (zlongit.c:559-700), recomputed per call (pure function of the element) */
+  if (!zl->broad_band) {
+    rc = zlongit_table(zl, &Z, &n_bins, &bin_size);
+    if (rc) {
+      free(time);
+      free(pbin);
+      return rc;
+    }
+    nb = n_bins;
+    dt = bin_size;
+    nfreq = n_bins / 2 + 1;
+    goto have_z;
+  }
+  if (zl->bin_size <= 0 || (zl->Ra && zl->Rs) || zl->n_bins % 2 != 0 || zl->Zreal || zl->Zimag) {
     free(time);
     free(pbin);
     return EH_ERR_INVALID;
@@ -1942,6 +2008,7 @@ static long zlongit_bucket(or_ctx *c, long ei, double **part, long np, const EH_
     Z[2 * i - 1] = Ra / 2 / (1 + sqr(term));
     Z[2 * i] = Z[2 * i - 1] * term;
   }
+have_z:
   if ((tmax - tmin) * 2 > nb * dt && !zl->allowLongBeam) {
     free(time);
     free(pbin);

@@ -133,3 +133,35 @@ def test_ring_with_wakes_and_an_elliptical_aperture(tracker):
     assert ng == nc and eg == o.effort()
     assert rows.shape[0] - ng > 10
     assert_rows_bitwise(g, c, "ring with wakes, elliptical aperture")
+
+
+def test_zlongit_from_impedance_tables_bit_exact(tracker):
+    """ZLONGIT with Z(f) tables (BROAD_BAND=0): the resonator's samples as a
+    table give the broad-band element's step on the GPU bit for bit, and a
+    table with a DC term and a Nyquist term (both doubled in the packed
+    spectrum) matches the checker"""
+    from tests.test_oracle import _resonator_table
+    n = 300000
+    rows = wake_bunch(n, 20e-12, seed=44)
+    q = 5e-9 / n
+    table = _resonator_table(8192, 2e-12, 20e9, 1.0, 1e4)
+    bb = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=8192, BIN_SIZE=2e-12)
+    res = []
+    for e in (bb, E.ZLONGIT_TABLE(table)):
+        g = rows.copy()
+        h, v, tmin = tracker.collective_probe(g, e, PO, 0, q)
+        res.append((g, h, v, tmin))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][2], res[1][2])
+    t2 = E.ImpedanceTable(table.f, table.Zreal + 30.0, table.Zimag)  # DC and Nyquist now nonzero
+    t2.Zimag[-1] = 7.0  # ignored: the Nyquist term is real
+    e = E.ZLONGIT_TABLE(t2)
+    g = rows.copy()
+    h, v, tmin = tracker.collective_probe(g, e, PO, 0, q)
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    c = rows.copy()
+    o.zlongit(c, e.p.zlongit, PO, 0)
+    hc, vc, tminc, dt = o.last_hist()
+    assert tmin == tminc and np.array_equal(h, hc) and np.array_equal(v, vc)
+    assert_rows_identical(g, c, "zlongit from tables")
+    assert not np.array_equal(g, res[0][0])

@@ -958,3 +958,100 @@ def test_quadrupole_effective_length_is_drift_quad_drift():
     for p, q in zip(pts, r):
         assert abs(q[0] - (p[0] + L * p[1])) < 1e-15 and abs(q[2] - (p[2] + L * p[3])) < 1e-15
         assert abs(q[4] - (p[4] + L * math.sqrt(1 + p[1] ** 2 + p[3] ** 2))) < 1e-14
+
+
+def _resonator_table(nb, bin_size, freq, Q, Rs):
+    """the broad-band resonator sampled as set_up_zlongit samples it
+    (zlongit.c:610-622), as an impedance table of nb/2 + 1 points; DC and
+    Nyquist zero, as the broad-band spectrum has them"""
+    df_n = 1 / (nb * bin_size) / freq
+    n = nb // 2 + 1
+    zr = np.zeros(n)
+    zi = np.zeros(n)
+    Ra = 2 * Rs
+    for i in range(1, n - 1):
+        term = Q * (1.0 / (i * df_n) - i * df_n)
+        zr[i] = Ra / 2 / (1 + term * term)
+        zi[i] = zr[i] * term
+    f = np.arange(n) * (1 / (nb * bin_size))
+    return E.ImpedanceTable(f, zr, zi)
+
+
+def test_zlongit_table_of_the_resonator_equals_the_broad_band_element():
+    """ZLONGIT from Z(f) tables (zlongit.c:634-700) fed the broad-band
+    resonator's own samples gives the broad-band element's result bit for
+    bit: the packed-spectrum layout (real and imaginary interleaved from
+    index 1, DC and Nyquist doubled) and the bin size 1/(N df) are pinned by
+    an independent construction of the same spectrum"""
+    nb, bs = 8192, 2e-12
+    table = _resonator_table(nb, bs, 20e9, 1.0, 1e4)
+    assert table.f[1] - table.f[0] == 61035156.25  # exact: 1/(N df) rounds back to BIN_SIZE
+    bb = E.ZLONGIT(BROAD_BAND=1, RS=1e4, Q=1.0, FREQ=20e9, N_BINS=nb, BIN_SIZE=bs)
+    tb = E.ZLONGIT_TABLE(table)
+    rows = wake_bunch(20000, 20e-12, seed=41)
+    q = 5e-9 / 20000
+    out = []
+    for e in (bb, tb):
+        r = rows.copy()
+        o = O.Oracle("emath")
+        o.set_macro_charge(q)
+        o.zlongit(r, e.p.zlongit, PO, 0)
+        out.append((r, o.last_hist()))
+    assert np.array_equal(out[0][0], out[1][0]) and not np.array_equal(out[0][0], rows)
+    assert np.array_equal(out[0][1][1], out[1][1][1]) and out[0][1][2] == out[1][1][2]
+    # a spectrum shifted by one index is not the same impedance
+    bad = E.ImpedanceTable(table.f, np.roll(table.Zreal, 1), table.Zimag)
+    r = rows.copy()
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    o.zlongit(r, E.ZLONGIT_TABLE(bad).p.zlongit, PO, 0)
+    assert not np.array_equal(r, out[0][0])
+
+
+def test_zlongit_table_dc_and_nyquist_terms_are_doubled():
+    """a pure DC impedance Z(0) = R: the packed spectrum's DC slot holds 2R
+    (zlongit.c:686-688), so the voltage is flat over the window at 2 R times
+    the mean current, V = 2 R q n / (N dt) for n binned particles of charge q
+    in N bins of dt (undoubled, or with another normalisation, it would be
+    R q n / (N dt) or N times off)"""
+    n = 1025
+    f = np.arange(n) * 1e9
+    zr = np.zeros(n)
+    zr[0] = 50.0
+    t = E.ZLONGIT_TABLE(E.ImpedanceTable(f, zr))
+    rows = wake_bunch(3000, 20e-12, seed=42)
+    q = 1e-12
+    o = O.Oracle("emath")
+    o.set_macro_charge(q)
+    r = rows.copy()
+    o.zlongit(r, t.p.zlongit, PO, 0)
+    h, v, tmin, dt = o.last_hist()
+    nb = 2 * (n - 1)
+    assert h.size == nb and dt == 1.0 / (nb * 1e9)
+    # flat voltage, proportional to the doubled DC term
+    assert np.allclose(v, v[0], rtol=1e-12, atol=0) and v[0] != 0
+    expect = 2 * 50.0 * q / dt * h.sum() / nb
+    assert abs(v[0] / expect - 1) < 1e-12
+
+
+@pytest.mark.parametrize("mutate", ["spacing", "dc_imag", "count", "none"])
+def test_zlongit_table_validation(mutate):
+    n = 65
+    f = np.arange(n) * 1e9
+    zr = np.linspace(0, 10, n)
+    zi = np.linspace(0, 5, n)
+    if mutate == "spacing":
+        f[10] += 0.5e9
+    elif mutate == "dc_imag":
+        zi[0] = 1.0
+    elif mutate == "count":
+        f, zr, zi = f[:64], zr[:64], zi[:64]
+    t = E.ZLONGIT_TABLE(E.ImpedanceTable(f, zr, zi), ALLOW_LONG_BEAM=1)
+    rows = wake_bunch(500, 20e-12, seed=43)
+    o = O.Oracle("emath")
+    o.set_macro_charge(1e-12)
+    if mutate == "none":
+        o.zlongit(rows, t.p.zlongit, PO, 0)
+    else:
+        with pytest.raises(RuntimeError):
+            o.zlongit(rows, t.p.zlongit, PO, 0)

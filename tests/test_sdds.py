@@ -73,3 +73,23 @@ def test_phase_space_and_loss_files(tmp_path):
     pg = sdds.read(str(q)).pages[0]
     assert np.array_equal(pg.columns["Pass"], np.arange(10)) and np.array_equal(pg.columns["s"], lost[:, 4])
     assert np.array_equal(pg.columns["p"], lost[:, 5])
+
+
+def test_impedance_table_from_files(tmp_path):
+    """ZREAL/ZIMAG file specifications ("file=fcol+zcol") read into the
+    arrays a ZLONGIT table element takes"""
+    f = np.arange(17) * 1e9
+    zr = np.linspace(0, 4, 17)
+    zi = np.concatenate([[0.0], np.linspace(1, 2, 16)])
+    p = str(tmp_path / "z.sdds")
+    s = sdds.SDDS()
+    s.columns = [("f", "double", {"units": "Hz"}), ("ZReal", "double", {"units": "Ohm"}),
+                 ("ZImag", "double", {"units": "Ohm"})]
+    s.pages = [sdds.Page(parameters={}, columns={"f": f, "ZReal": zr, "ZImag": zi})]
+    sdds.write(p, s)
+    F, R, I = sdds.impedance_table(p + "=f+ZReal", p + "=f+ZImag")
+    assert np.array_equal(F, f) and np.array_equal(R, zr) and np.array_equal(I, zi)
+    F, R, I = sdds.impedance_table(real=p + "=f+ZReal")
+    assert I is None and np.array_equal(R, zr)
+    with pytest.raises(ValueError):
+        sdds.impedance_table(real=p)
