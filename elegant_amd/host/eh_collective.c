@@ -80,8 +80,8 @@ static double ramp_factor(long i_pass, long rampPasses) {
 
 static int wake_setup(eh_ctx *c, int slot, const EH_WAKE *w, double *dt, double *span) {
   double tmin = DBL_MAX, tmax = -DBL_MAX;
-  if (w->smoothing || w->change_p0)
-    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE with SMOOTHING or CHANGE_P0");
+  if (w->smoothing)
+    return msgf(c, EH_ERR_UNSUPPORTED, "WAKE with SMOOTHING");
   if (w->wakePoints < 2 || !w->t || !w->W)
     return msgf(c, EH_ERR_INVALID, "WAKE: too little wake data");
   for (int64_t i = 0; i < w->wakePoints; i++) {
@@ -279,6 +279,9 @@ static int collective_run(eh_ctx *c, int slot, const eh_element *e, const eh_op 
   cc.ap_xmax = op->ap_xmax;
   cc.ap_ymax = op->ap_ymax;
   cc.flags = (op->flags & EH_F_AP) | flags;
+  /* CHANGE_P0: the checks after the element follow the match step */
+  if (op->flags & EH_F_ALONE)
+    cc.flags |= EHD_COLL_NOCHECKS;
   cc.ap_code = op->ap_code;
   if (is_wake) {
     const EH_WAKE *w = &e->p.wake;
@@ -373,6 +376,8 @@ int eh_collective_probe(eh_ctx *c, double **part, int64_t np, const eh_element *
   eh_program *prog = (eh_program *)eh_ctx_prog(c);
   int rc;
   int64_t nLeft;
+  if (e->type == EH_T_WAKE && e->p.wake.change_p0)
+    return msgf(c, EH_ERR_UNSUPPORTED, "one WAKE step with CHANGE_P0: use eh_do_tracking");
   bl[0] = *e;
   rc = eh_compile(bl, 1, Po, eh_ctx_species(c), prog, eh_ctx_msg(c), 512);
   if (rc)
@@ -414,6 +419,10 @@ int64_t eh_track_through_wake(eh_ctx *c, double **part, int64_t np, const EH_WAK
     return EH_ERR_INVALID;
   if (np <= 0)
     return np;
+  if (wake->change_p0)
+    /* the central momentum it moves has no way back through this signature;
+     * eh_do_tracking runs WAKE with CHANGE_P0 */
+    return msgf(c, EH_ERR_UNSUPPORTED, "eh_track_through_wake: CHANGE_P0 (use eh_do_tracking)");
   eh_element_defaults(&e, EH_T_WAKE);
   e.p.wake = *wake;
   rc = eh_collective_probe(c, part, np, &e, Po, i_pass, macroParticleCharge, NULL, NULL, NULL, NULL);

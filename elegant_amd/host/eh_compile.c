@@ -790,12 +790,13 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       snprintf(msg, msglen, "beamline longer than %d elements", EH_MAX_OPS);
       return EH_ERR_UNSUPPORTED;
     }
-    if (op.op == EH_OP_RFCA && e->p.rfca.change_p0) {
+    if ((op.op == EH_OP_RFCA && e->p.rfca.change_p0) || (op.op == EH_OP_WAKE && e->p.wake.change_p0)) {
       /* CHANGE_P0: do_match_energy runs inside the cavity routine, after its
-       * own removeInvalidParticles and before do_tracking's checks that
-       * follow the element (simple_rfca.c:975-982).  Three records: the
-       * cavity without those checks, the host's match step, and the checks
-       * alone at the cavity's end, all carrying the cavity's element index. */
+       * own removeInvalidParticles, and at the end of track_through_wake,
+       * before do_tracking's checks that follow the element
+       * (simple_rfca.c:975-982, wake.c:288-289).  Three records: the element
+       * without those checks, the host's match step, and the checks alone at
+       * the element's end, all carrying its element index. */
       eh_op step, post;
       memset(&step, 0, sizeof(step));
       step.op = EH_OP_MATCH_P0;
@@ -811,7 +812,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       post.ap_ymax = op.ap_ymax;
       op.flags = (op.flags & ~EH_F_AP) | EH_F_ALONE;
       if (prog->n_p0 >= EH_MAX_RFCA) {
-        snprintf(msg, msglen, "too many RFCA elements with CHANGE_P0");
+        snprintf(msg, msglen, "too many RFCA/WAKE elements with CHANGE_P0");
         return EH_ERR_UNSUPPORTED;
       }
       prog->ops[prog->nops++] = op;

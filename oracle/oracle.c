@@ -1568,7 +1568,23 @@ static int bucket_left_out(long bunched, long start, long end, long b) {
  * pass of the binning, convolution and kick per bucket on its own rows
  * (wake.c:71-117, 285-295), the mean time of a bucket a plain sum in row
  * order */
+static long track_through_wake_buckets(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass);
+
+/* track_through_wake with the central momentum it may move: CHANGE_P0 runs
+ * do_match_energy over the element's rows after every bucket
+ * (wake.c:288-289) */
+long or_track_through_wake_p(or_ctx *c, double **part, long np, const EH_WAKE *w, double *Po, long i_pass) {
+  long r = track_through_wake_buckets(c, part, np, w, *Po, i_pass);
+  if (r >= 0 && w->change_p0)
+    or_do_match_energy(part, np, Po);
+  return r;
+}
+
 long or_track_through_wake(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass) {
+  return or_track_through_wake_p(c, part, np, w, &Po, i_pass);
+}
+
+static long track_through_wake_buckets(or_ctx *c, double **part, long np, const EH_WAKE *w, double Po, long i_pass) {
   long *first, *rows, nbk;
   if (np <= 0 || !(w->bunchedBeamMode && c->id_slots_per_bunch > 0))
     return bucket_left_out(w->bunchedBeamMode, w->startBunch, w->endBunch, 0) ? np
@@ -1605,7 +1621,7 @@ static long wake_bucket(or_ctx *c, double **part, long np, const EH_WAKE *w, dou
   long nb, ib, ip, n_binned;
   double *time, *Itime, *Vtime;
   long *pbin;
-  if (w->smoothing || w->change_p0)
+  if (w->smoothing)
     return EH_ERR_UNSUPPORTED;
   if (w->wakePoints < 2)
     return EH_ERR_INVALID;
@@ -2176,7 +2192,7 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
       case EH_T_WAKE:
         if (!c->charge_seen)
           return EH_ERR_INVALID;
-        rc = or_track_through_wake(c, coord, nToTrack, &e->p.wake, *P_central, i_pass);
+        rc = or_track_through_wake_p(c, coord, nToTrack, &e->p.wake, P_central, i_pass);
         break;
       case EH_T_ZLONGIT:
         if (!c->charge_seen)

@@ -63,6 +63,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
 long or_simple_rf_cavity(or_ctx *c, long elem_index, double **part, long np, const EH_RFCA *rfca, double *P_central,
                          double zEnd);
 long or_track_through_wake(or_ctx *c, double **part, long np, const EH_WAKE *wake, double Po, long i_pass);
+long or_track_through_wake_p(or_ctx *c, double **part, long np, const EH_WAKE *wake, double *Po, long i_pass);
 long or_track_through_zlongit(or_ctx *c, long elem_index, double **part, long np, const EH_ZLONGIT *zl, double Po,
                               long i_pass);
 long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,

@@ -165,3 +165,27 @@ def test_zlongit_from_impedance_tables_bit_exact(tracker):
     assert tmin == tminc and np.array_equal(h, hc) and np.array_equal(v, vc)
     assert_rows_identical(g, c, "zlongit from tables")
     assert not np.array_equal(g, res[0][0])
+
+
+def test_ring_with_wake_change_p0_bit_exact(tracker):
+    """the ring with WAKE CHANGE_P0 (wake.c:288-289): the host's match step
+    after the wake kick, before the checks that follow the element, moves the
+    central momentum every pass and the program is recompiled for it; rows,
+    losses, effort and the final central momentum as the checker's"""
+    bl, Po = Lt.storage_ring_wake(zlongit=False)
+    els = [A.eh_element.from_buffer_copy(e) for e in bl.elements]
+    for e in els:
+        if e.type == A.EH_T_WAKE:
+            e.p.wake.change_p0 = 1
+    ring = E.Beamline(els, keep=bl.keep)
+    rows = Lt.matched_beam(ring, 3000, 1e-9, 1e-11, 1e-3, 20e-12, 33, Po)
+    rows[:300, 0] += 0.0149
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, ring, Po, 4)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, ring, Po, 4)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 10
+    assert tracker.P_central == o.P_central != Po
+    assert_rows_identical(g, c, "ring with wake change_p0")

@@ -1055,3 +1055,28 @@ def test_zlongit_table_validation(mutate):
     else:
         with pytest.raises(RuntimeError):
             o.zlongit(rows, t.p.zlongit, PO, 0)
+
+
+def test_wake_change_p0_matches_the_mean_momentum():
+    """WAKE with CHANGE_P0 (wake.c:288-289): after the wake kick the central
+    momentum becomes the KahanPlus mean momentum of the rows and every delta
+    is re-expressed against it, so each particle's momentum is the one the
+    same wake without CHANGE_P0 gives, and the new central momentum is
+    their mean"""
+    table = Lt.resonator_wake()
+    rows = wake_bunch(20000, 20e-12, seed=45)
+    out = {}
+    for cp in (0, 1):
+        bl = E.Beamline([E.CHARGE("Q", TOTAL=5e-9), E.WAKE(table, "WK", CHANGE_P0=cp)], keep=[table])
+        r = rows.copy()
+        o = O.Oracle("emath")
+        o.do_tracking(r, bl, PO, 1)
+        out[cp] = (r, o.P_central)
+    (ra, Pa), (rb, Pb) = out[0], out[1]
+    assert Pa == PO and Pb < PO  # the wake takes energy: the central momentum drops
+    pa = Pa * (1 + ra[:, 5])
+    pb = Pb * (1 + rb[:, 5])
+    assert np.allclose(pb, pa, rtol=2e-15, atol=0)
+    mean = float(sum(Fraction(x) for x in pa) / len(pa))
+    assert abs(Pb / mean - 1) < 1e-15
+    assert np.array_equal(ra[:, [0, 1, 2, 3, 4]], rb[:, [0, 1, 2, 3, 4]])
