@@ -29,11 +29,12 @@ EH_T_WAKE = 9
 EH_T_ZLONGIT = 10
 EH_T_MAXAMP = 11
 EH_T_CHARGE = 12
+EH_T_MALIGN = 13
 
 TYPE_NAMES = {
     EH_T_MARK: "MARK", EH_T_DRIF: "DRIF", EH_T_EDRIFT: "EDRIFT", EH_T_KQUAD: "KQUAD",
     EH_T_KSEXT: "KSEXT", EH_T_KOCT: "KOCT", EH_T_CSBEND: "CSBEND", EH_T_RFCA: "RFCA",
-    EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE",
+    EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE", EH_T_MALIGN: "MALIGN",
 }
 
 EH_OPEN_PLUS_X, EH_OPEN_PLUS_Y, EH_OPEN_MINUS_X, EH_OPEN_MINUS_Y = 1, 2, 3, 4
@@ -121,10 +122,14 @@ class EH_CHARGE(C.Structure):
     _fields_ = [("charge", D), ("chargePerParticle", D)]
 
 
+class EH_MALIGN(C.Structure):
+    _fields_ = _fields(["dxp", "dyp", "dx", "dy", "dz", "dt", "dp", "de"], ["on_pass", "startPID", "endPID", "pad"])
+
+
 class _P(C.Union):
     _fields_ = [("drift", EH_DRIFT), ("edrift", EH_EDRIFT), ("kquad", EH_KQUAD), ("ksext", EH_KSEXT),
                 ("koct", EH_KOCT), ("csbend", EH_CSBEND), ("rfca", EH_RFCA), ("wake", EH_WAKE),
-                ("zlongit", EH_ZLONGIT), ("maxamp", EH_MAXAMP), ("charge", EH_CHARGE)]
+                ("zlongit", EH_ZLONGIT), ("maxamp", EH_MAXAMP), ("charge", EH_CHARGE), ("malign", EH_MALIGN)]
 
 
 class eh_element(C.Structure):
@@ -153,7 +158,7 @@ PARAM_STRUCT = {
     EH_T_DRIF: ("drift", EH_DRIFT), EH_T_EDRIFT: ("edrift", EH_EDRIFT), EH_T_KQUAD: ("kquad", EH_KQUAD),
     EH_T_KSEXT: ("ksext", EH_KSEXT), EH_T_KOCT: ("koct", EH_KOCT), EH_T_CSBEND: ("csbend", EH_CSBEND),
     EH_T_RFCA: ("rfca", EH_RFCA), EH_T_WAKE: ("wake", EH_WAKE), EH_T_ZLONGIT: ("zlongit", EH_ZLONGIT),
-    EH_T_MAXAMP: ("maxamp", EH_MAXAMP), EH_T_CHARGE: ("charge", EH_CHARGE),
+    EH_T_MAXAMP: ("maxamp", EH_MAXAMP), EH_T_CHARGE: ("charge", EH_CHARGE), EH_T_MALIGN: ("malign", EH_MALIGN),
 }
 
 

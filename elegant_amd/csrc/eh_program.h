@@ -28,6 +28,7 @@
 #define EH_OP_APERTURE 8   /* MAXAMP: only the checks that follow every element */
 #define EH_OP_WAKE 9       /* collective: handled outside the tracking kernel */
 #define EH_OP_ZLONGIT 10   /* collective: handled outside the tracking kernel */
+#define EH_OP_MALIGN 12    /* offset_beam (do_tracking.c:3243-3306) */
 #define EH_OP_MATCH_P0 11  /* CHANGE_P0 after an RFCA: do_match_energy on the host (do_tracking.c:3308-3420);
                               the kernel does nothing here */
 
@@ -108,6 +109,16 @@ typedef struct {
 #define EH_D_DRIFT 56   /* drift / n (nonzero test) */
 #define EH_D_TOTLEN 57  /* sum of all dsh in step order: dzLoss of a completed particle */
 #define EH_D_LEND 58    /* (L - LEFFECTIVE) / 2 */
+
+/* ---- MALIGN: d[0..7] = dxp, dyp, dx, dy, dz, dt, dp, de; nkicks_rf = ON_PASS ---- */
+#define EH_D_MA_DXP 0
+#define EH_D_MA_DYP 1
+#define EH_D_MA_DX 2
+#define EH_D_MA_DY 3
+#define EH_D_MA_DZ 4
+#define EH_D_MA_DT 5
+#define EH_D_MA_DP 6
+#define EH_D_MA_DE 7
 
 /* ---- CSBEND ---- */
 #define EH_D_RHO0 0

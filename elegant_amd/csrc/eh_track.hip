@@ -1022,11 +1022,46 @@ DEV int track_rfca(particle &p, const eh_op &op, const double *z) {
   return 0;
 }
 
+/* offset_beam (do_tracking.c:3243-3306) for one particle, on pass ON_PASS
+ * or, with -1, every pass (do_tracking.c:1818-1822) */
+DEV void track_malign(particle &p, const eh_op &op, int64_t pass) {
+  const int32_t on_pass = op.nkicks_rf;
+  if (!(on_pass == -1 || on_pass == pass))
+    return;
+  const double dz = op.d[EH_D_MA_DZ];
+  const double dt = op.d[EH_D_MA_DT], ddp = op.d[EH_D_MA_DP], de = op.d[EH_D_MA_DE];
+  const double Po = op.Po;
+  double ds = dz ? dz * em_sqrt(1 + sq(p.xp) + sq(p.yp)) : 0;
+  p.x += op.d[EH_D_MA_DX] + dz * p.xp;
+  p.xp += op.d[EH_D_MA_DXP];
+  p.y += op.d[EH_D_MA_DY] + dz * p.yp;
+  p.yp += op.d[EH_D_MA_DYP];
+  p.s += ds;
+  if (dt || ddp || de) {
+    double pc = Po * (1 + p.dp);
+    double gamma = em_sqrt(1 + pc * pc);
+    double beta = pc / gamma;
+    const double t = p.s / (beta * EH_C_MKS) + dt;
+    if (ddp) {
+      p.dp += ddp;
+      pc = Po * (1 + p.dp);
+      beta = pc / em_sqrt(1 + pc * pc);
+    }
+    if (de) {
+      gamma += de * gamma;
+      pc = em_sqrt(gamma * gamma - 1);
+      beta = pc / gamma;
+      p.dp = (pc - Po) / Po;
+    }
+    p.s = t * beta * EH_C_MKS;
+  }
+}
+
 /* One element on one particle.  z points at the element's (z_start, z_end),
  * which only the loss paths read: in the element record itself, or in the
  * position table of a program whose repeated cells share one record. */
 template <bool SH>
-DEV int run_op(particle &p, const eh_op &op, const double *z) {
+DEV int run_op(particle &p, const eh_op &op, const double *z, int64_t pass) {
   int lost = 0;
   switch (op.op) {
   case EH_OP_DRIFT2:
@@ -1049,6 +1084,9 @@ DEV int run_op(particle &p, const eh_op &op, const double *z) {
     break;
   case EH_OP_RFCA:
     lost = track_rfca(p, op, z);
+    break;
+  case EH_OP_MALIGN:
+    track_malign(p, op, pass);
     break;
   case EH_OP_RFCA_DRIFT:
     /* VOLT=0: a drift when the cavity has length, else nothing (simple_rfca.c:389-402) */
@@ -1114,7 +1152,7 @@ __global__ void __launch_bounds__(256, 4) eh_track_kernel(eh_state st, const eh_
       const int32_t f1 = fe < oe ? fe : oe;
       int32_t k = (f - fs) % len;
       for (; f < f1; f++) {
-        lost = run_op<SH>(p, ops[first + k], zpos + 2 * f);
+        lost = run_op<SH>(p, ops[first + k], zpos + 2 * f, pass0 + ip);
         if (lost) {
           lost_op = f;
           lost_pass = pass0 + ip;
@@ -1195,7 +1233,7 @@ __global__ void __launch_bounds__(256) eh_batch_kernel(eh_state st, const eh_op 
         for (int k = 0; k < PPL; k++) {
           if (!live[k])
             continue;
-          int l = run_op<SH>(p[k], op, &op.z_start);
+          int l = run_op<SH>(p[k], op, &op.z_start, pass0 + ip);
           if (l) {
             live[k] = false;
             lost[k] = l;

@@ -62,6 +62,8 @@ _NAMES = {
     A.EH_T_MAXAMP: {"X_MAX": "x_max", "Y_MAX": "y_max", "ELLIPTICAL": "elliptical", "EXPONENT": "exponent",
                     "YEXPONENT": "yExponent", "OPEN_SIDE": "openSide"},
     A.EH_T_CHARGE: {"TOTAL": "charge", "PER_PARTICLE": "chargePerParticle"},
+    A.EH_T_MALIGN: {"DXP": "dxp", "DYP": "dyp", "DX": "dx", "DY": "dy", "DZ": "dz", "DT": "dt", "DP": "dp",
+                    "DE": "de", "ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     A.EH_T_MARK: {},
 }
 
@@ -77,6 +79,7 @@ _DEFAULTS = {
     A.EH_T_WAKE: {"factor": 1.0, "bunchedBeamMode": 1, "startBunch": -1, "endBunch": -1},
     A.EH_T_ZLONGIT: {"n_bins": 128, "factor": 1.0, "bunchedBeamMode": 1, "startBunch": -1, "endBunch": -1},
     A.EH_T_MAXAMP: {"exponent": 2},
+    A.EH_T_MALIGN: {"on_pass": -1, "startPID": -1, "endPID": -1},
 }
 
 
@@ -120,6 +123,10 @@ def make(type_code, name="", **kw):
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])
     return e
+
+
+def MALIGN(name="", **kw):
+    return make(A.EH_T_MALIGN, name, **kw)
 
 
 def DRIF(name="", **kw):

@@ -59,6 +59,9 @@ int eh_element_defaults(eh_element *e, int32_t type) {
   case EH_T_EDRIFT:
   case EH_T_CHARGE:
     break;
+  case EH_T_MALIGN: /* track_data.c:926-940 */
+    e->p.malign.on_pass = e->p.malign.startPID = e->p.malign.endPID = -1;
+    break;
   case EH_T_DRIF:
     e->p.drift.order = 0;
     break;
@@ -138,6 +141,8 @@ int64_t eh_struct_size(int32_t type) {
     return (int64_t)sizeof(EH_MAXAMP);
   case EH_T_CHARGE:
     return (int64_t)sizeof(EH_CHARGE);
+  case EH_T_MALIGN:
+    return (int64_t)sizeof(EH_MALIGN);
   default:
     return -1;
   }
@@ -718,6 +723,26 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
     case EH_T_CHARGE:
       keep = 0; /* NO_APERTURE; the NaN cull after them cannot fire (every element ends with it) */
       break;
+    case EH_T_MALIGN: {
+      /* offset_beam (do_tracking.c:3243-3306), zero length, followed by the
+       * checks of every element (MALIGN is not NO_APERTURE) */
+      const EH_MALIGN *m = &e->p.malign;
+      if (m->startPID != -1 || m->endPID != -1) {
+        snprintf(msg, msglen, "%s: MALIGN with START_PID/END_PID", e->name);
+        return EH_ERR_UNSUPPORTED;
+      }
+      op.op = EH_OP_MALIGN;
+      op.d[EH_D_MA_DXP] = m->dxp;
+      op.d[EH_D_MA_DYP] = m->dyp;
+      op.d[EH_D_MA_DX] = m->dx;
+      op.d[EH_D_MA_DY] = m->dy;
+      op.d[EH_D_MA_DZ] = m->dz;
+      op.d[EH_D_MA_DT] = m->dt;
+      op.d[EH_D_MA_DP] = m->dp;
+      op.d[EH_D_MA_DE] = m->de;
+      op.nkicks_rf = m->on_pass;
+      break;
+    }
     case EH_T_MAXAMP:
       rc = check_maxamp(e, msg, msglen);
       if (rc)

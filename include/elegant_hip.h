@@ -56,6 +56,7 @@ extern "C" {
 #define EH_T_ZLONGIT 10
 #define EH_T_MAXAMP 11
 #define EH_T_CHARGE 12
+#define EH_T_MALIGN 13
 
 /* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
  * (compute_matrices.c:1101-1104, matrix.c:389-429) */
@@ -175,6 +176,16 @@ typedef struct {
   double charge, chargePerParticle;
 } EH_CHARGE;
 
+/* MALIGN (track.h:1902-1905; defaults track_data.c:926-940): offsets of
+ * the beam, applied on pass ON_PASS (-1: every pass) by offset_beam
+ * (do_tracking.c:3243-3306, 1818-1822).  START_PID/END_PID other than -1
+ * (a subset of particle IDs) are not supported here; FORCE_MODIFY_MATRIX and
+ * FLOOR do not touch tracking. */
+typedef struct {
+  double dxp, dyp, dx, dy, dz, dt, dp, de;
+  int32_t on_pass, startPID, endPID, pad;
+} EH_MALIGN;
+
 /* One beamline element; mirrors ELEMENT_LIST's (type, p_elem) pair
  * (track.h:308-344) with the parameters held by value. */
 typedef struct {
@@ -193,6 +204,7 @@ typedef struct {
     EH_ZLONGIT zlongit;
     EH_MAXAMP maxamp;
     EH_CHARGE charge;
+    EH_MALIGN malign;
   } p;
 } eh_element;
 

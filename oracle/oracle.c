@@ -190,6 +190,42 @@ void or_track_particles_drift(double **coord, long np, double length, int order)
   }
 }
 
+/* offset_beam (do_tracking.c:3243-3306) over every particle (no PID range),
+ * the rows from the last to the first as the reference loops */
+void or_offset_beam(double **coord, long nToTrack, const EH_MALIGN *offset, double P_central) {
+  long i_part;
+  double *part, pc, beta, gamma, t, ds;
+  for (i_part = nToTrack - 1; i_part >= 0; i_part--) {
+    part = coord[i_part];
+    if (offset->dz)
+      ds = offset->dz * sqrt(1 + sqr(part[1]) + sqr(part[3]));
+    else
+      ds = 0;
+    part[0] += offset->dx + offset->dz * part[1];
+    part[1] += offset->dxp;
+    part[2] += offset->dy + offset->dz * part[3];
+    part[3] += offset->dyp;
+    part[4] += ds;
+    if (offset->dt || offset->dp || offset->de) {
+      pc = P_central * (1 + part[5]);
+      beta = pc / (gamma = sqrt(1 + pc * pc));
+      t = part[4] / (beta * EH_C_MKS) + offset->dt;
+      if (offset->dp) {
+        part[5] += offset->dp;
+        pc = P_central * (1 + part[5]);
+        beta = pc / sqrt(1 + pc * pc);
+      }
+      if (offset->de) {
+        gamma += offset->de * gamma;
+        pc = sqrt(gamma * gamma - 1);
+        beta = pc / gamma;
+        part[5] = (pc - P_central) / P_central;
+      }
+      part[4] = t * beta * EH_C_MKS;
+    }
+  }
+}
+
 /* exactDrift (csbend.c:3767-3790) */
 void or_exact_drift(double **part, long np, double length) {
   for (long i = 0; i < np; i++) {
@@ -2177,6 +2213,13 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
         break;
       case EH_T_EDRIFT:
         or_exact_drift(coord, nToTrack, e->p.edrift.length);
+        break;
+      case EH_T_MALIGN:
+        /* do_tracking.c:1818-1822 */
+        if (e->p.malign.startPID != -1 || e->p.malign.endPID != -1)
+          return EH_ERR_UNSUPPORTED;
+        if (e->p.malign.on_pass == -1 || e->p.malign.on_pass == i_pass)
+          or_offset_beam(coord, nToTrack, &e->p.malign, *P_central);
         break;
       case EH_T_KQUAD:
       case EH_T_KSEXT:

@@ -1080,3 +1080,44 @@ def test_wake_change_p0_matches_the_mean_momentum():
     mean = float(sum(Fraction(x) for x in pa) / len(pa))
     assert abs(Pb / mean - 1) < 1e-15
     assert np.array_equal(ra[:, [0, 1, 2, 3, 4]], rb[:, [0, 1, 2, 3, 4]])
+
+
+def test_malign_offsets_beam_as_offset_beam():
+    """MALIGN (offset_beam, do_tracking.c:3243-3306): transverse offsets
+    add; DZ is an exact drift; DP and DE change the momentum at constant
+    arrival time, so s scales with the velocity; DT shifts the time; the
+    element applies on ON_PASS only (-1: every pass)"""
+    pts = [[1e-3, 2e-4, -3e-4, 5e-5, 0.2, 1e-3], [-2e-3, -1e-4, 4e-4, 1e-4, -0.1, -2e-3]]
+    rows = rows_of(pts)
+    r, n, _ = track([E.MALIGN(DX=1e-4, DXP=-2e-5, DY=3e-4, DYP=1e-5)], rows)
+    for p, q in zip(pts, r):
+        assert q[0] == p[0] + 1e-4 and q[1] == p[1] + -2e-5 and q[2] == p[2] + 3e-4 and q[3] == p[3] + 1e-5
+        assert q[4] == p[4] and q[5] == p[5]
+    r, n, _ = track([E.MALIGN(DZ=0.3)], rows)
+    for p, q in zip(pts, r):
+        assert q[0] == p[0] + 0.3 * p[1] and q[2] == p[2] + 0.3 * p[3]
+        assert abs(q[4] - (p[4] + 0.3 * math.sqrt(1 + p[1] ** 2 + p[3] ** 2))) < 1e-15
+
+    def beta(P):
+        return P / math.sqrt(P * P + 1)
+
+    for kw in ({"DP": 2e-3}, {"DE": -1e-3}, {"DT": 3e-12}, {"DP": 1e-3, "DT": -2e-12}):
+        r, n, _ = track([E.MALIGN(**kw)], rows)
+        for p, q in zip(pts, r):
+            P0 = PO * (1 + p[5])
+            if "DE" in kw:
+                g1 = math.sqrt(1 + P0 * P0) * (1 + kw["DE"])
+                P1 = math.sqrt(g1 * g1 - 1)
+            else:
+                P1 = PO * (1 + p[5] + kw.get("DP", 0.0))
+            assert abs(PO * (1 + q[5]) / P1 - 1) < 1e-14
+            t = p[4] / (beta(P0) * 2.99792458e8) + kw.get("DT", 0.0)
+            assert abs(q[4] - t * beta(P1) * 2.99792458e8) < 1e-15 * max(1.0, abs(q[4]))
+    bl = [E.MALIGN(DX=1e-3, ON_PASS=1), E.DRIF(L=1.0)]
+    r, n, _ = track(bl, rows, passes=3)
+    expect = rows.copy()
+    for k in range(3):
+        if k == 1:
+            expect[:, 0] += 1e-3
+        expect[:, 0] += 1.0 * expect[:, 1]
+    assert np.allclose(r[:, 0], expect[:, 0], rtol=0, atol=1e-15)
