@@ -293,7 +293,14 @@ DEV int64_t bin_of(double t, double tmin, double dt, int round_half) {
   return (int64_t)v;
 }
 
-__global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, const double *__restrict__ dp,
+/* HB threads per histogram workgroup, one workgroup per CU: 16 waves keep
+ * enough loads in flight per CU, and the merge still costs one global
+ * atomic per bin per CU */
+#define HB 1024
+#define HIST_LDS_WORDS 16384 /* 64 KB of counts */
+#define HIST_U 4 /* particles per lane per step of the histogram sweep (8 measured the same) */
+
+__global__ void __launch_bounds__(HB) histogram(const double *__restrict__ s, const double *__restrict__ dp,
                                                 const int32_t *__restrict__ lp, int64_t n, double Po,
                                                 const coll_plan *plan, double dt, int32_t round_half,
                                                 int32_t lds_bins, unsigned int *hist, const int32_t *__restrict__ bunch,
@@ -303,29 +310,34 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
   const double tmin = plan->tmin;
   if (nb <= 0 || plan->err)
     return;
-  /* counts in LDS, one copy per wave when they fit (a bunch puts most
-   * particles in a few central bins, so fewer lanes share an address), one
-   * per workgroup when only that fits, else straight to memory */
-  const int copies = nb * (CB / 64) <= lds_bins ? CB / 64 : (nb <= lds_bins ? 1 : 0);
-  unsigned int *h = copies ? lh + (copies > 1 ? (threadIdx.x >> 6) * nb : 0) : hist;
+  /* counts in LDS: as many copies (a power of 2, at most one per wave) as
+   * fit, the waves sharing them in turn (a bunch puts most particles in a
+   * few central bins, so fewer lanes share an address); else straight to
+   * memory */
+  int copies = HB / 64;
+  while (copies > 1 && nb * copies > lds_bins)
+    copies >>= 1;
+  if (nb > lds_bins)
+    copies = 0;
+  unsigned int *h = copies ? lh + ((threadIdx.x >> 6) & (copies - 1)) * nb : hist;
   if (copies) {
-    for (int k = threadIdx.x; k < nb * copies; k += CB)
+    for (int k = threadIdx.x; k < nb * copies; k += HB)
       lh[k] = 0;
     __syncthreads();
   }
-  const int64_t stride = (int64_t)gridDim.x * CB;
-  int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
-  for (; i + (SWEEP_U - 1) * stride < n; i += SWEEP_U * stride) {
-    double S[SWEEP_U], D[SWEEP_U];
-    bool live[SWEEP_U];
+  const int64_t stride = (int64_t)gridDim.x * HB;
+  int64_t i = (int64_t)blockIdx.x * HB + threadIdx.x;
+  for (; i + (HIST_U - 1) * stride < n; i += HIST_U * stride) {
+    double S[HIST_U], D[HIST_U];
+    bool live[HIST_U];
 #pragma unroll
-    for (int u = 0; u < SWEEP_U; u++) {
+    for (int u = 0; u < HIST_U; u++) {
       live[u] = in_step(lp, bunch, target, i + u * stride);
       S[u] = s[i + u * stride];
       D[u] = dp[i + u * stride];
     }
 #pragma unroll
-    for (int u = 0; u < SWEEP_U; u++)
+    for (int u = 0; u < HIST_U; u++)
       if (live[u]) {
         int64_t ib = bin_of(arrival_time(S[u], D[u], Po), tmin, dt, round_half);
         if (ib >= 0 && ib <= nb - 1)
@@ -341,7 +353,7 @@ __global__ void __launch_bounds__(CB) histogram(const double *__restrict__ s, co
   }
   if (copies) {
     __syncthreads();
-    for (int k = threadIdx.x; k < nb; k += CB) {
+    for (int k = threadIdx.x; k < nb; k += HB) {
       unsigned int c = 0;
       for (int w = 0; w < copies; w++)
         c += lh[w * nb + k];
@@ -1123,14 +1135,15 @@ static int coll_launch(ehd_dev *d, const ehd_collective *c, double *const crd[6]
       CHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device));
       d->num_cu = cu > 0 ? cu : 256;
     }
-    int hgrid = (int)((n + CB - 1) / CB);
+    int hgrid = (int)((n + HB - 1) / HB);
     if (hgrid < 1)
       hgrid = 1;
     if (hgrid > d->num_cu)
       hgrid = d->num_cu;
-    /* room for one copy of the counts per wave, at most 48 KB of LDS */
-    const int32_t lds_bins = nb_max * (CB / 64) < 12288 ? (int32_t)(nb_max * (CB / 64)) : 12288;
-    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(CB), sizeof(unsigned int) * lds_bins, d->stream, crd[4], crd[5],
+    /* room for one copy of the counts per wave, at most 64 KB of LDS */
+    const int32_t lds_bins =
+        nb_max * (HB / 64) < HIST_LDS_WORDS ? (int32_t)(nb_max * (HB / 64)) : HIST_LDS_WORDS;
+    hipLaunchKernelGGL(histogram, dim3(hgrid), dim3(HB), sizeof(unsigned int) * lds_bins, d->stream, crd[4], crd[5],
                        lp, n, c->Po, plan, c->dt, kind == EH_OP_WAKE ? 1 : 0, lds_bins, hist, bunch, target);
   }
   if (kind == EH_OP_WAKE) {
