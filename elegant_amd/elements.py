@@ -62,6 +62,13 @@ _NAMES = {
     A.EH_T_MAXAMP: {"X_MAX": "x_max", "Y_MAX": "y_max", "ELLIPTICAL": "elliptical", "EXPONENT": "exponent",
                     "YEXPONENT": "yExponent", "OPEN_SIDE": "openSide"},
     A.EH_T_CHARGE: {"TOTAL": "charge", "PER_PARTICLE": "chargePerParticle"},
+    A.EH_T_MONI: {"L": "length", "DX": "dx", "DY": "dy", "WEIGHT": "weight", "TILT": "tilt",
+                  "XCALIBRATION": "xcalibration", "YCALIBRATION": "ycalibration", "XSETPOINT": "xsetpoint",
+                  "YSETPOINT": "ysetpoint", "ORDER": "order"},
+    A.EH_T_HMON: {"L": "length", "DX": "dx", "DY": "dy", "WEIGHT": "weight", "TILT": "tilt",
+                  "CALIBRATION": "xcalibration", "SETPOINT": "xsetpoint", "ORDER": "order"},
+    A.EH_T_VMON: {"L": "length", "DX": "dx", "DY": "dy", "WEIGHT": "weight", "TILT": "tilt",
+                  "CALIBRATION": "ycalibration", "SETPOINT": "ysetpoint", "ORDER": "order"},
     A.EH_T_MALIGN: {"DXP": "dxp", "DYP": "dyp", "DX": "dx", "DY": "dy", "DZ": "dz", "DT": "dt", "DP": "dp",
                     "DE": "de", "ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     A.EH_T_MARK: {},
@@ -80,6 +87,9 @@ _DEFAULTS = {
     A.EH_T_ZLONGIT: {"n_bins": 128, "factor": 1.0, "bunchedBeamMode": 1, "startBunch": -1, "endBunch": -1},
     A.EH_T_MAXAMP: {"exponent": 2},
     A.EH_T_MALIGN: {"on_pass": -1, "startPID": -1, "endPID": -1},
+    A.EH_T_MONI: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
+    A.EH_T_HMON: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
+    A.EH_T_VMON: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
 }
 
 
@@ -123,6 +133,18 @@ def make(type_code, name="", **kw):
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])
     return e
+
+
+def MONI(name="", **kw):
+    return make(A.EH_T_MONI, name, **kw)
+
+
+def HMON(name="", **kw):
+    return make(A.EH_T_HMON, name, **kw)
+
+
+def VMON(name="", **kw):
+    return make(A.EH_T_VMON, name, **kw)
 
 
 def MALIGN(name="", **kw):
@@ -237,7 +259,7 @@ class Beamline:
         total = 0.0
         for e in self.elements:
             if e.type in (A.EH_T_DRIF, A.EH_T_EDRIFT, A.EH_T_KQUAD, A.EH_T_KSEXT, A.EH_T_KOCT,
-                          A.EH_T_CSBEND, A.EH_T_RFCA):
+                          A.EH_T_CSBEND, A.EH_T_RFCA, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON):
                 total += getattr(e.p, A.PARAM_STRUCT[e.type][0]).length
         return total
 

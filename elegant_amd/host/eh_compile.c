@@ -59,6 +59,11 @@ int eh_element_defaults(eh_element *e, int32_t type) {
   case EH_T_EDRIFT:
   case EH_T_CHARGE:
     break;
+  case EH_T_MONI: /* track_data.c:600-647 */
+  case EH_T_HMON:
+  case EH_T_VMON:
+    e->p.moni.weight = e->p.moni.xcalibration = e->p.moni.ycalibration = 1;
+    break;
   case EH_T_MALIGN: /* track_data.c:926-940 */
     e->p.malign.on_pass = e->p.malign.startPID = e->p.malign.endPID = -1;
     break;
@@ -143,6 +148,10 @@ int64_t eh_struct_size(int32_t type) {
     return (int64_t)sizeof(EH_CHARGE);
   case EH_T_MALIGN:
     return (int64_t)sizeof(EH_MALIGN);
+  case EH_T_MONI:
+  case EH_T_HMON:
+  case EH_T_VMON:
+    return (int64_t)sizeof(EH_MONI);
   default:
     return -1;
   }
@@ -164,6 +173,10 @@ double eh_element_length(const eh_element *e) {
     return e->p.csbend.length;
   case EH_T_RFCA:
     return e->p.rfca.length;
+  case EH_T_MONI:
+  case EH_T_HMON:
+  case EH_T_VMON:
+    return e->p.moni.length;
   default:
     return 0;
   }
@@ -171,7 +184,8 @@ double eh_element_length(const eh_element *e) {
 
 int eh_has_length(int32_t type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT ||
-         type == EH_T_KOCT || type == EH_T_CSBEND || type == EH_T_RFCA;
+         type == EH_T_KOCT || type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON ||
+         type == EH_T_VMON;
 }
 
 /* rotateBeamCoordinatesForMisalignment's choice of cos/sin (tilt_matrices.c:161-176) */
@@ -709,7 +723,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
   for (i = 0; i < n_elems; i++) {
     const eh_element *e = &bl[i];
     eh_op op;
-    int rc = EH_OK, keep = 1;
+    int rc = EH_OK, keep = 1, no_aperture = 0;
     memset(&op, 0, sizeof(op));
     last_z = z;
     if (eh_has_length(e->type))
@@ -766,6 +780,20 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       op.op = EH_OP_EXACT_DRIFT;
       op.d[0] = e->p.edrift.length;
       break;
+    case EH_T_MONI:
+    case EH_T_HMON:
+    case EH_T_VMON: {
+      /* the monitor's drift matrix (compute_matrices.c:1317-1328); NO_APERTURE:
+       * only the NaN cull follows it */
+      int order = e->p.moni.order ? e->p.moni.order : 2;
+      op.op = order >= 2 ? EH_OP_DRIFT2 : EH_OP_DRIFT1;
+      op.d[0] = e->p.moni.length;
+      op.d[1] = e->p.moni.length / 2;
+      if (fabs(op.d[0]) >= 0x1.0p-300 && fabs(op.d[0]) <= 0x1.0p+300)
+        op.flags |= EH_F_DFAST;
+      no_aperture = 1;
+      break;
+    }
     case EH_T_KQUAD:
     case EH_T_KSEXT:
     case EH_T_KOCT:
@@ -806,7 +834,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       return rc;
     if (!keep)
       continue;
-    if (post_aperture(maxamp, x_max, y_max, &op.ap_code)) {
+    if (!no_aperture && post_aperture(maxamp, x_max, y_max, &op.ap_code)) {
       op.flags |= EH_F_AP;
       op.ap_xmax = x_max;
       op.ap_ymax = y_max;

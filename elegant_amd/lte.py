@@ -46,7 +46,7 @@ ENTITY_NAMES = (
 SUPPORTED = {"DRIF": A.EH_T_DRIF, "EDRIFT": A.EH_T_EDRIFT, "KQUAD": A.EH_T_KQUAD, "KSEXT": A.EH_T_KSEXT,
              "KOCT": A.EH_T_KOCT, "CSBEND": A.EH_T_CSBEND, "RFCA": A.EH_T_RFCA, "WAKE": A.EH_T_WAKE,
              "ZLONGIT": A.EH_T_ZLONGIT, "MAXAMP": A.EH_T_MAXAMP, "MARK": A.EH_T_MARK, "CHARGE": A.EH_T_CHARGE,
-             "MALIGN": A.EH_T_MALIGN}
+             "MALIGN": A.EH_T_MALIGN, "MONI": A.EH_T_MONI, "HMON": A.EH_T_HMON, "VMON": A.EH_T_VMON}
 
 _RPN_UNARY = {"sqrt": math.sqrt, "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin,
               "acos": math.acos, "atan": math.atan, "exp": math.exp, "ln": math.log, "abs": abs,
@@ -268,6 +268,9 @@ class Lattice:
                 kw["L"] = v
             elif tcode == A.EH_T_WAKE and k in ("INPUTFILE", "TCOLUMN", "WCOLUMN"):
                 continue
+            elif tcode in (A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON) and k in (
+                    "READOUT", "XREADOUT", "YREADOUT", "CO_FITPOINT", "STORE_TURN_BY_TURN"):
+                continue  # readouts only: nothing on the tracked path
             elif tcode == A.EH_T_RFCA and k == "FIDUCIAL":
                 kw[k] = str(v).split(",")[0].lower().replace("+", "")
             else:

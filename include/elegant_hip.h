@@ -57,6 +57,9 @@ extern "C" {
 #define EH_T_MAXAMP 11
 #define EH_T_CHARGE 12
 #define EH_T_MALIGN 13
+#define EH_T_MONI 14
+#define EH_T_HMON 15
+#define EH_T_VMON 16
 
 /* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
  * (compute_matrices.c:1101-1104, matrix.c:389-429) */
@@ -186,6 +189,17 @@ typedef struct {
   int32_t on_pass, startPID, endPID, pad;
 } EH_MALIGN;
 
+/* MONI, HMON, VMON (track.h monitor structs; defaults track_data.c:600-647):
+ * beam-position monitors, tracked as the drift matrix of length L and
+ * ORDER (0: the default order 2; compute_matrices.c:1317-1328) with no
+ * aperture check after them (NO_APERTURE, track_data.c:3928-3930) but the
+ * NaN cull that follows every element.  The other fields only shape
+ * readouts and are kept for the caller. */
+typedef struct {
+  double length, dx, dy, weight, tilt, xcalibration, ycalibration, xsetpoint, ysetpoint;
+  int32_t order, pad;
+} EH_MONI;
+
 /* One beamline element; mirrors ELEMENT_LIST's (type, p_elem) pair
  * (track.h:308-344) with the parameters held by value. */
 typedef struct {
@@ -205,6 +219,7 @@ typedef struct {
     EH_MAXAMP maxamp;
     EH_CHARGE charge;
     EH_MALIGN malign;
+    EH_MONI moni; /* MONI, HMON and VMON */
   } p;
 } eh_element;
 

@@ -2130,7 +2130,7 @@ have_z:
 
 static int has_length(int type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT || type == EH_T_KOCT ||
-         type == EH_T_CSBEND || type == EH_T_RFCA;
+         type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON || type == EH_T_VMON;
 }
 
 static double elem_length(const eh_element *e) {
@@ -2149,6 +2149,10 @@ static double elem_length(const eh_element *e) {
     return e->p.csbend.length;
   case EH_T_RFCA:
     return e->p.rfca.length;
+  case EH_T_MONI:
+  case EH_T_HMON:
+  case EH_T_VMON:
+    return e->p.moni.length;
   default:
     return 0;
   }
@@ -2183,7 +2187,9 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
     for (long ie = 0; ie < n_elems && nToTrack; ie++) {
       const eh_element *e = &bl[ie];
       long rc = 0;
-      int no_aperture = (e->type == EH_T_MARK || e->type == EH_T_CHARGE);
+      /* NO_APERTURE elements (track_data.c:3928-3933) */
+      int no_aperture = (e->type == EH_T_MARK || e->type == EH_T_CHARGE || e->type == EH_T_MONI ||
+                         e->type == EH_T_HMON || e->type == EH_T_VMON);
       last_z = z;
       if (has_length(e->type))
         z += elem_length(e);
@@ -2213,6 +2219,12 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
         break;
       case EH_T_EDRIFT:
         or_exact_drift(coord, nToTrack, e->p.edrift.length);
+        break;
+      case EH_T_MONI:
+      case EH_T_HMON:
+      case EH_T_VMON:
+        /* MATRIX_TRACKING with the drift matrix (compute_matrices.c:1317-1328) */
+        or_track_particles_drift(coord, nToTrack, e->p.moni.length, e->p.moni.order ? e->p.moni.order : 2);
         break;
       case EH_T_MALIGN:
         /* do_tracking.c:1818-1822 */

@@ -183,3 +183,27 @@ def test_example_deck_tracks_on_the_checker():
     ideal = r[r[:, 6] == 1][0]
     assert np.abs(ideal[[0, 1, 2, 3, 5]]).max() < 1e-5
     assert np.abs(r[:, 0]).max() < 5e-3
+
+
+def test_monitors_and_malign_parse():
+    """beam-position monitors (their readout-only parameters ignored) and
+    MALIGN, with elegant's names and defaults"""
+    text = """
+B1: MONI, L=0.1, XCALIBRATION=1.01, CO_FITPOINT=1, STORE_TURN_BY_TURN=1
+B2: HMON, L=0.0, CALIBRATION=0.99, READOUT="x 2 *"
+B3: VMON, ORDER=1
+M1: MALIGN, DX=1e-4, DP=1e-3, ON_PASS=2
+D: DRIF, L=1.0
+R: LINE=(B1, D, B2, M1, D, B3)
+"""
+    lat = lte.Lattice()
+    lat.parse(text)
+    bl = lat.beamline("R")
+    types = [e.type for e in bl.elements]
+    assert types == [A.EH_T_MONI, A.EH_T_DRIF, A.EH_T_HMON, A.EH_T_MALIGN, A.EH_T_DRIF, A.EH_T_VMON]
+    assert bl.elements[0].p.moni.xcalibration == 1.01 and bl.elements[0].p.moni.weight == 1.0
+    assert bl.elements[2].p.moni.xcalibration == 0.99
+    assert bl.elements[5].p.moni.order == 1
+    m = bl.elements[3].p.malign
+    assert (m.dx, m.dp, m.on_pass, m.startPID, m.endPID) == (1e-4, 1e-3, 2, -1, -1)
+    assert abs(bl.length() - 2.1) < 1e-15

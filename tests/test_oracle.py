@@ -1121,3 +1121,25 @@ def test_malign_offsets_beam_as_offset_beam():
             expect[:, 0] += 1e-3
         expect[:, 0] += 1.0 * expect[:, 1]
     assert np.allclose(r[:, 0], expect[:, 0], rtol=0, atol=1e-15)
+
+
+def test_monitors_are_drifts_without_an_aperture_check():
+    """MONI/HMON/VMON: the drift matrix of their length and order
+    (compute_matrices.c:1317-1328) with NO_APERTURE (track_data.c:3928-3930):
+    where a DRIF of the same length loses a particle beyond the MAXAMP
+    aperture, the monitor keeps it; the NaN cull still follows them"""
+    # row 1 starts inside the 20 mm aperture and drifts out of it
+    pts = [[1e-3, 2e-4, -3e-4, 5e-5, 0.2, 1e-3], [0.019, 1e-2, 0, 0, 0, 0]]
+    ap = E.MAXAMP(X_MAX=0.02, Y_MAX=0.02)
+    for mon in (E.MONI(L=0.7), E.HMON(L=0.7), E.VMON(L=0.7)):
+        r, n, _ = track([ap, mon], rows_of(pts))
+        d, nd, _ = track([ap, E.DRIF(L=0.7)], rows_of(pts))
+        assert n == 2 and nd == 1
+        assert np.array_equal(r[0], d[0])  # the same drift
+        assert r[1, 0] == 0.019 + 0.7 * 1e-2 and r[1, 0] > 0.02  # beyond the aperture, kept
+    # an infinite slope makes the drifted x infinite: the NaN cull takes it
+    r, n, _ = track([E.MONI(L=0.7)], rows_of([[0.0, np.inf, 0, 0, 0, 0], pts[0]]))
+    assert n == 1 and r[1, 6] == 1
+    r1, _, _ = track([E.MONI(L=0.7, ORDER=1)], rows_of(pts[:1]))
+    d1, _, _ = track([E.DRIF(L=0.7, ORDER=1)], rows_of(pts[:1]))
+    assert np.array_equal(r1, d1)
