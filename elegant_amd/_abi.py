@@ -33,11 +33,14 @@ EH_T_MALIGN = 13
 EH_T_MONI = 14
 EH_T_HMON = 15
 EH_T_VMON = 16
+EH_T_HKICK = 17
+EH_T_VKICK = 18
 
 TYPE_NAMES = {
     EH_T_MARK: "MARK", EH_T_DRIF: "DRIF", EH_T_EDRIFT: "EDRIFT", EH_T_KQUAD: "KQUAD",
     EH_T_KSEXT: "KSEXT", EH_T_KOCT: "KOCT", EH_T_CSBEND: "CSBEND", EH_T_RFCA: "RFCA",
     EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE", EH_T_MALIGN: "MALIGN", EH_T_MONI: "MONI", EH_T_HMON: "HMON", EH_T_VMON: "VMON",
+    EH_T_HKICK: "HKICK", EH_T_VKICK: "VKICK",
 }
 
 EH_OPEN_PLUS_X, EH_OPEN_PLUS_Y, EH_OPEN_MINUS_X, EH_OPEN_MINUS_Y = 1, 2, 3, 4
@@ -130,6 +133,11 @@ class EH_MONI(C.Structure):
                         "ysetpoint"], ["order", "pad"])
 
 
+class EH_CORRECTOR(C.Structure):
+    _fields_ = _fields(["length", "kick", "tilt", "b2", "calibration", "lEffRad"],
+                       ["edge_effects", "order", "steering", "synchRad", "isr", "pad"])
+
+
 class EH_MALIGN(C.Structure):
     _fields_ = _fields(["dxp", "dyp", "dx", "dy", "dz", "dt", "dp", "de"], ["on_pass", "startPID", "endPID", "pad"])
 
@@ -138,7 +146,7 @@ class _P(C.Union):
     _fields_ = [("drift", EH_DRIFT), ("edrift", EH_EDRIFT), ("kquad", EH_KQUAD), ("ksext", EH_KSEXT),
                 ("koct", EH_KOCT), ("csbend", EH_CSBEND), ("rfca", EH_RFCA), ("wake", EH_WAKE),
                 ("zlongit", EH_ZLONGIT), ("maxamp", EH_MAXAMP), ("charge", EH_CHARGE), ("malign", EH_MALIGN),
-                ("moni", EH_MONI)]
+                ("moni", EH_MONI), ("corrector", EH_CORRECTOR)]
 
 
 class eh_element(C.Structure):
@@ -169,6 +177,7 @@ PARAM_STRUCT = {
     EH_T_RFCA: ("rfca", EH_RFCA), EH_T_WAKE: ("wake", EH_WAKE), EH_T_ZLONGIT: ("zlongit", EH_ZLONGIT),
     EH_T_MAXAMP: ("maxamp", EH_MAXAMP), EH_T_CHARGE: ("charge", EH_CHARGE), EH_T_MALIGN: ("malign", EH_MALIGN),
     EH_T_MONI: ("moni", EH_MONI), EH_T_HMON: ("moni", EH_MONI), EH_T_VMON: ("moni", EH_MONI),
+    EH_T_HKICK: ("corrector", EH_CORRECTOR), EH_T_VKICK: ("corrector", EH_CORRECTOR),
 }
 
 

@@ -69,6 +69,12 @@ _NAMES = {
                   "CALIBRATION": "xcalibration", "SETPOINT": "xsetpoint", "ORDER": "order"},
     A.EH_T_VMON: {"L": "length", "DX": "dx", "DY": "dy", "WEIGHT": "weight", "TILT": "tilt",
                   "CALIBRATION": "ycalibration", "SETPOINT": "ysetpoint", "ORDER": "order"},
+    A.EH_T_HKICK: {"L": "length", "KICK": "kick", "TILT": "tilt", "B2": "b2", "CALIBRATION": "calibration",
+                   "EDGE_EFFECTS": "edge_effects", "ORDER": "order", "STEERING": "steering",
+                   "SYNCH_RAD": "synchRad", "ISR": "isr", "LERAD": "lEffRad"},
+    A.EH_T_VKICK: {"L": "length", "KICK": "kick", "TILT": "tilt", "B2": "b2", "CALIBRATION": "calibration",
+                   "EDGE_EFFECTS": "edge_effects", "ORDER": "order", "STEERING": "steering",
+                   "SYNCH_RAD": "synchRad", "ISR": "isr", "LERAD": "lEffRad"},
     A.EH_T_MALIGN: {"DXP": "dxp", "DYP": "dyp", "DX": "dx", "DY": "dy", "DZ": "dz", "DT": "dt", "DP": "dp",
                     "DE": "de", "ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     A.EH_T_MARK: {},
@@ -88,6 +94,8 @@ _DEFAULTS = {
     A.EH_T_MAXAMP: {"exponent": 2},
     A.EH_T_MALIGN: {"on_pass": -1, "startPID": -1, "endPID": -1},
     A.EH_T_MONI: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
+    A.EH_T_HKICK: {"calibration": 1.0, "steering": 1},
+    A.EH_T_VKICK: {"calibration": 1.0, "steering": 1},
     A.EH_T_HMON: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
     A.EH_T_VMON: {"weight": 1.0, "xcalibration": 1.0, "ycalibration": 1.0},
 }
@@ -133,6 +141,14 @@ def make(type_code, name="", **kw):
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])
     return e
+
+
+def HKICK(name="", **kw):
+    return make(A.EH_T_HKICK, name, **kw)
+
+
+def VKICK(name="", **kw):
+    return make(A.EH_T_VKICK, name, **kw)
 
 
 def MONI(name="", **kw):
@@ -259,7 +275,8 @@ class Beamline:
         total = 0.0
         for e in self.elements:
             if e.type in (A.EH_T_DRIF, A.EH_T_EDRIFT, A.EH_T_KQUAD, A.EH_T_KSEXT, A.EH_T_KOCT,
-                          A.EH_T_CSBEND, A.EH_T_RFCA, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON):
+                          A.EH_T_CSBEND, A.EH_T_RFCA, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON,
+                          A.EH_T_HKICK, A.EH_T_VKICK):
                 total += getattr(e.p, A.PARAM_STRUCT[e.type][0]).length
         return total
 

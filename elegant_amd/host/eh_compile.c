@@ -64,6 +64,11 @@ int eh_element_defaults(eh_element *e, int32_t type) {
   case EH_T_VMON:
     e->p.moni.weight = e->p.moni.xcalibration = e->p.moni.ycalibration = 1;
     break;
+  case EH_T_HKICK: /* track_data.c:544-575 */
+  case EH_T_VKICK:
+    e->p.corrector.calibration = 1;
+    e->p.corrector.steering = 1;
+    break;
   case EH_T_MALIGN: /* track_data.c:926-940 */
     e->p.malign.on_pass = e->p.malign.startPID = e->p.malign.endPID = -1;
     break;
@@ -152,6 +157,9 @@ int64_t eh_struct_size(int32_t type) {
   case EH_T_HMON:
   case EH_T_VMON:
     return (int64_t)sizeof(EH_MONI);
+  case EH_T_HKICK:
+  case EH_T_VKICK:
+    return (int64_t)sizeof(EH_CORRECTOR);
   default:
     return -1;
   }
@@ -177,6 +185,9 @@ double eh_element_length(const eh_element *e) {
   case EH_T_HMON:
   case EH_T_VMON:
     return e->p.moni.length;
+  case EH_T_HKICK:
+  case EH_T_VKICK:
+    return e->p.corrector.length;
   default:
     return 0;
   }
@@ -185,7 +196,7 @@ double eh_element_length(const eh_element *e) {
 int eh_has_length(int32_t type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT ||
          type == EH_T_KOCT || type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON ||
-         type == EH_T_VMON;
+         type == EH_T_VMON || type == EH_T_HKICK || type == EH_T_VKICK;
 }
 
 /* rotateBeamCoordinatesForMisalignment's choice of cos/sin (tilt_matrices.c:161-176) */
@@ -737,6 +748,24 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
     case EH_T_CHARGE:
       keep = 0; /* NO_APERTURE; the NaN cull after them cannot fire (every element ends with it) */
       break;
+    case EH_T_HKICK:
+    case EH_T_VKICK: {
+      /* corrector_matrix with no strength is the drift matrix
+       * (bend_matrix.c:314-317); the checks of every element follow, the
+       * aperture's loss point not extrapolated (do_tracking.c:2463) */
+      const EH_CORRECTOR *k = &e->p.corrector;
+      int order = k->order ? k->order : 2;
+      if (k->kick * k->calibration != 0 || k->synchRad || k->isr) {
+        snprintf(msg, msglen, "%s: steering corrector with a kick or radiation", e->name);
+        return EH_ERR_UNSUPPORTED;
+      }
+      op.op = order >= 2 ? EH_OP_DRIFT2 : EH_OP_DRIFT1;
+      op.d[0] = k->length;
+      op.d[1] = k->length / 2;
+      if (fabs(op.d[0]) >= 0x1.0p-300 && fabs(op.d[0]) <= 0x1.0p+300)
+        op.flags |= EH_F_DFAST;
+      break;
+    }
     case EH_T_MALIGN: {
       /* offset_beam (do_tracking.c:3243-3306), zero length, followed by the
        * checks of every element (MALIGN is not NO_APERTURE) */

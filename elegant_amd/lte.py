@@ -46,7 +46,8 @@ ENTITY_NAMES = (
 SUPPORTED = {"DRIF": A.EH_T_DRIF, "EDRIFT": A.EH_T_EDRIFT, "KQUAD": A.EH_T_KQUAD, "KSEXT": A.EH_T_KSEXT,
              "KOCT": A.EH_T_KOCT, "CSBEND": A.EH_T_CSBEND, "RFCA": A.EH_T_RFCA, "WAKE": A.EH_T_WAKE,
              "ZLONGIT": A.EH_T_ZLONGIT, "MAXAMP": A.EH_T_MAXAMP, "MARK": A.EH_T_MARK, "CHARGE": A.EH_T_CHARGE,
-             "MALIGN": A.EH_T_MALIGN, "MONI": A.EH_T_MONI, "HMON": A.EH_T_HMON, "VMON": A.EH_T_VMON}
+             "MALIGN": A.EH_T_MALIGN, "MONI": A.EH_T_MONI, "HMON": A.EH_T_HMON, "VMON": A.EH_T_VMON,
+             "HKICK": A.EH_T_HKICK, "VKICK": A.EH_T_VKICK}
 
 _RPN_UNARY = {"sqrt": math.sqrt, "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin,
               "acos": math.acos, "atan": math.atan, "exp": math.exp, "ln": math.log, "abs": abs,

@@ -60,6 +60,8 @@ extern "C" {
 #define EH_T_MONI 14
 #define EH_T_HMON 15
 #define EH_T_VMON 16
+#define EH_T_HKICK 17
+#define EH_T_VKICK 18
 
 /* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
  * (compute_matrices.c:1101-1104, matrix.c:389-429) */
@@ -200,6 +202,18 @@ typedef struct {
   int32_t order, pad;
 } EH_MONI;
 
+/* HKICK, VKICK (HCOR/VCOR, track.h; defaults track_data.c:544-575): steering
+ * correctors, tracked with corrector_matrix (bend_matrix.c:300-316) through
+ * track_particles.  Supported here as they are with no strength: KICK x
+ * CALIBRATION == 0 gives the drift matrix of length L and ORDER (0: 2),
+ * followed by the checks of every element, the aperture's loss point not
+ * extrapolated back (only DRIF's is).  A nonzero kick and SYNCH_RAD/ISR are
+ * refused. */
+typedef struct {
+  double length, kick, tilt, b2, calibration, lEffRad;
+  int32_t edge_effects, order, steering, synchRad, isr, pad;
+} EH_CORRECTOR;
+
 /* One beamline element; mirrors ELEMENT_LIST's (type, p_elem) pair
  * (track.h:308-344) with the parameters held by value. */
 typedef struct {
@@ -220,6 +234,7 @@ typedef struct {
     EH_CHARGE charge;
     EH_MALIGN malign;
     EH_MONI moni; /* MONI, HMON and VMON */
+    EH_CORRECTOR corrector; /* HKICK and VKICK */
   } p;
 } eh_element;
 

@@ -2130,7 +2130,8 @@ have_z:
 
 static int has_length(int type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT || type == EH_T_KOCT ||
-         type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON || type == EH_T_VMON;
+         type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON || type == EH_T_VMON ||
+         type == EH_T_HKICK || type == EH_T_VKICK;
 }
 
 static double elem_length(const eh_element *e) {
@@ -2153,6 +2154,9 @@ static double elem_length(const eh_element *e) {
   case EH_T_HMON:
   case EH_T_VMON:
     return e->p.moni.length;
+  case EH_T_HKICK:
+  case EH_T_VKICK:
+    return e->p.corrector.length;
   default:
     return 0;
   }
@@ -2225,6 +2229,15 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
       case EH_T_VMON:
         /* MATRIX_TRACKING with the drift matrix (compute_matrices.c:1317-1328) */
         or_track_particles_drift(coord, nToTrack, e->p.moni.length, e->p.moni.order ? e->p.moni.order : 2);
+        break;
+      case EH_T_HKICK:
+      case EH_T_VKICK:
+        /* track_particles with corrector_matrix, which with no strength is
+         * the drift matrix (bend_matrix.c:314-317, do_tracking.c:2404-2415) */
+        if (e->p.corrector.kick * e->p.corrector.calibration != 0 || e->p.corrector.synchRad || e->p.corrector.isr)
+          return EH_ERR_UNSUPPORTED;
+        or_track_particles_drift(coord, nToTrack, e->p.corrector.length,
+                                 e->p.corrector.order ? e->p.corrector.order : 2);
         break;
       case EH_T_MALIGN:
         /* do_tracking.c:1818-1822 */

@@ -1,6 +1,7 @@
 """MALIGN (offset_beam, do_tracking.c:3243-3306, 1818-1822) and the
 beam-position monitors MONI/HMON/VMON (drifts with no aperture check after
-them) on the GPU against the checker: a lossy FODO ring with beam offsets of
+them) and zero-strength HKICK/VKICK correctors (drifts whose loss point is
+not extrapolated) on the GPU against the checker: a lossy FODO ring with beam offsets of
 every kind, one applied on a single pass, and monitors in every cell,
 through do_tracking and through a batch."""
 import numpy as np
@@ -20,6 +21,7 @@ def ring_with_maligns():
         els += [E.DRIF("D1", L=2.0), E.KQUAD("QF", L=0.5, K1=1.2, N_KICKS=4), E.KSEXT("SF", L=0.2, K2=20.0),
                 E.DRIF("D2", L=2.0), E.KQUAD("QD", L=0.5, K1=-1.2, N_KICKS=4), E.KSEXT("SD", L=0.2, K2=-35.0)]
         els.append([E.MONI("BPM", L=0.1), E.HMON("HM", L=0.05, ORDER=1), E.VMON("VM")][cell % 3])
+        els.append([E.HKICK("HC", L=0.1), E.VKICK("VC", L=0.1, ORDER=1)][cell % 2])
         if cell == 2:
             els.append(E.MALIGN("M1", DX=2e-4, DXP=-3e-5, DY=-1e-4, DYP=2e-5, DZ=0.05))
         if cell == 5:

@@ -1143,3 +1143,22 @@ def test_monitors_are_drifts_without_an_aperture_check():
     r1, _, _ = track([E.MONI(L=0.7, ORDER=1)], rows_of(pts[:1]))
     d1, _, _ = track([E.DRIF(L=0.7, ORDER=1)], rows_of(pts[:1]))
     assert np.array_equal(r1, d1)
+
+
+def test_zero_strength_correctors_are_drifts_without_extrapolation():
+    """HKICK/VKICK with no kick: corrector_matrix returns the drift matrix
+    (bend_matrix.c:314-317), and the aperture check after them does not
+    extrapolate the loss point back as it does after a DRIF
+    (do_tracking.c:2463); a kick is refused"""
+    pts = [[1e-3, 2e-4, -3e-4, 5e-5, 0.2, 1e-3], [0.019, 1e-2, 0, 0, 0, 0]]
+    ap = E.MAXAMP(X_MAX=0.02, Y_MAX=0.02)
+    for k in (E.HKICK(L=0.7, TILT=0.1), E.VKICK(L=0.7, KICK=1e-3, CALIBRATION=0.0)):
+        r, n, _ = track([ap, k], rows_of(pts))
+        d, nd, _ = track([ap, E.DRIF(L=0.7)], rows_of(pts))
+        assert n == nd == 1
+        assert np.array_equal(r[0], d[0])
+        # the lost row: drifted to the end, s at the element's end, not extrapolated back
+        assert r[1, 0] == 0.019 + 0.7 * 1e-2 and r[1, 4] == 0.7
+        assert d[1, 4] < 0.7
+    with pytest.raises(RuntimeError):
+        track([E.HKICK(L=0.1, KICK=1e-4)], rows_of(pts))
