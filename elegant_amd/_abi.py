@@ -35,12 +35,13 @@ EH_T_HMON = 15
 EH_T_VMON = 16
 EH_T_HKICK = 17
 EH_T_VKICK = 18
+EH_T_RECIRC = 19
 
 TYPE_NAMES = {
     EH_T_MARK: "MARK", EH_T_DRIF: "DRIF", EH_T_EDRIFT: "EDRIFT", EH_T_KQUAD: "KQUAD",
     EH_T_KSEXT: "KSEXT", EH_T_KOCT: "KOCT", EH_T_CSBEND: "CSBEND", EH_T_RFCA: "RFCA",
     EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE", EH_T_MALIGN: "MALIGN", EH_T_MONI: "MONI", EH_T_HMON: "HMON", EH_T_VMON: "VMON",
-    EH_T_HKICK: "HKICK", EH_T_VKICK: "VKICK",
+    EH_T_HKICK: "HKICK", EH_T_VKICK: "VKICK", EH_T_RECIRC: "RECIRC",
 }
 
 EH_OPEN_PLUS_X, EH_OPEN_PLUS_Y, EH_OPEN_MINUS_X, EH_OPEN_MINUS_Y = 1, 2, 3, 4

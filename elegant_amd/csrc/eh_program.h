@@ -61,6 +61,7 @@
 #define EH_F_CHANGE_T (1 << 25) /* RFCA CHANGE_T: EH_D_TOFF whole RF periods taken out of t and s */
 #define EH_F_CHT_SELF (1 << 26) /* CHANGE_T with each particle as its own beam: the offset from its own time */
 #define EH_F_LEND (1 << 27)    /* KQUAD LEFFECTIVE: exact drifts of EH_D_LEND around the body */
+#define EH_F_FIRSTPASS (1 << 28) /* before the RECIRC point: tracked on pass 0 only (do_tracking.c:459-465) */
 
 /* Aperture shapes other than the closed rectangle, packed in one int32:
  * bit 0 (super-)ellipse, bits 1-3 the open side (EH_OPEN_*), bits 8-15 the

@@ -1063,6 +1063,9 @@ DEV void track_malign(particle &p, const eh_op &op, int64_t pass) {
 template <bool SH>
 DEV int run_op(particle &p, const eh_op &op, const double *z, int64_t pass) {
   int lost = 0;
+  /* elements before the recirculation point act on pass 0 only */
+  if ((op.flags & EH_F_FIRSTPASS) && pass != 0)
+    return 0;
   switch (op.op) {
   case EH_OP_DRIFT2:
     if ((op.flags & EH_F_DFAST) && ALL_LANES(drift2_ok(p)))

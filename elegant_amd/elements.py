@@ -78,6 +78,7 @@ _NAMES = {
     A.EH_T_MALIGN: {"DXP": "dxp", "DYP": "dyp", "DX": "dx", "DY": "dy", "DZ": "dz", "DT": "dt", "DP": "dp",
                     "DE": "de", "ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     A.EH_T_MARK: {},
+    A.EH_T_RECIRC: {},
 }
 
 # defaults that differ from zero (track_data.c; IS_LONG/IS_SHORT defaults come
@@ -141,6 +142,10 @@ def make(type_code, name="", **kw):
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])
     return e
+
+
+def RECIRC(name=""):
+    return make(A.EH_T_RECIRC, name)
 
 
 def HKICK(name="", **kw):

@@ -731,6 +731,12 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
   prog->n_coll = 0;
   prog->n_p0 = 0;
   prog->Po = Po;
+  /* the recirculation point: passes after pass 0 start at the first RECIRC
+   * (do_tracking.c:459-465), so the elements before it act on pass 0 only */
+  int64_t i_recirc = -1;
+  for (i = 0; i < n_elems && i_recirc < 0; i++)
+    if (bl[i].type == EH_T_RECIRC)
+      i_recirc = i;
   for (i = 0; i < n_elems; i++) {
     const eh_element *e = &bl[i];
     eh_op op;
@@ -746,6 +752,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
     switch (e->type) {
     case EH_T_MARK:
     case EH_T_CHARGE:
+    case EH_T_RECIRC:
       keep = 0; /* NO_APERTURE; the NaN cull after them cannot fire (every element ends with it) */
       break;
     case EH_T_HKICK:
@@ -871,6 +878,14 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
     if (prog->nops + 3 > EH_MAX_OPS) {
       snprintf(msg, msglen, "beamline longer than %d elements", EH_MAX_OPS);
       return EH_ERR_UNSUPPORTED;
+    }
+    if (i_recirc >= 0 && i < i_recirc) {
+      if (op.op == EH_OP_WAKE || op.op == EH_OP_ZLONGIT ||
+          (op.op == EH_OP_RFCA && (e->p.rfca.change_p0 || e->p.rfca.change_t))) {
+        snprintf(msg, msglen, "%s: WAKE, ZLONGIT or RFCA with CHANGE_P0/CHANGE_T before the RECIRC point", e->name);
+        return EH_ERR_UNSUPPORTED;
+      }
+      op.flags |= EH_F_FIRSTPASS;
     }
     if ((op.op == EH_OP_RFCA && e->p.rfca.change_p0) || (op.op == EH_OP_WAKE && e->p.wake.change_p0)) {
       /* CHANGE_P0: do_match_energy runs inside the cavity routine, after its

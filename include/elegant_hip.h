@@ -62,6 +62,7 @@ extern "C" {
 #define EH_T_VMON 16
 #define EH_T_HKICK 17
 #define EH_T_VKICK 18
+#define EH_T_RECIRC 19 /* recirculation point: passes after pass 0 start here (do_tracking.c:459-465) */
 
 /* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
  * (compute_matrices.c:1101-1104, matrix.c:389-429) */

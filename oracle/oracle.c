@@ -2181,25 +2181,40 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
   for (long ip = nLeft; ip < nToTrack; ip++)
     coord[ip][7] = 0;
   nToTrack = nLeft;
+  /* the recirculation point (do_tracking.c:459-465, 1382-1387): passes
+   * other than pass 0 start at the first RECIRC, at its z */
+  long i_recirc = 0;
+  double z_recirc = 0;
+  for (long ie = 0; ie < n_elems; ie++) {
+    if (bl[ie].type == EH_T_RECIRC) {
+      i_recirc = ie;
+      break;
+    }
+    if (has_length(bl[ie].type))
+      z_recirc += elem_length(&bl[ie]);
+    if (ie == n_elems - 1)
+      z_recirc = 0; /* no RECIRC */
+  }
   for (i_pass = passOffset; i_pass < n_passes + passOffset; i_pass++) {
-    double z = 0, last_z = 0;
+    double z = i_pass ? z_recirc : 0, last_z = z;
     /* stop_tracking_particle_limit (do_tracking.c:500-520) */
     if (c->particle_limit > 0 && nToTrack < c->particle_limit)
       break;
     if (!nToTrack)
       break;
-    for (long ie = 0; ie < n_elems && nToTrack; ie++) {
+    for (long ie = i_pass ? i_recirc : 0; ie < n_elems && nToTrack; ie++) {
       const eh_element *e = &bl[ie];
       long rc = 0;
       /* NO_APERTURE elements (track_data.c:3928-3933) */
       int no_aperture = (e->type == EH_T_MARK || e->type == EH_T_CHARGE || e->type == EH_T_MONI ||
-                         e->type == EH_T_HMON || e->type == EH_T_VMON);
+                         e->type == EH_T_HMON || e->type == EH_T_VMON || e->type == EH_T_RECIRC);
       last_z = z;
       if (has_length(e->type))
         z += elem_length(e);
       nLeft = nToTrack;
       switch (e->type) {
       case EH_T_MARK:
+      case EH_T_RECIRC:
         break;
       case EH_T_CHARGE:
         if (i_pass == passOffset) {

@@ -59,3 +59,26 @@ def test_malign_in_a_batch_bit_exact(tracker):
     nc = O.Oracle("emath").do_tracking(c, bl, Po, 5)
     assert ng == nc and rows.shape[0] - ng > 50
     assert_rows_identical(g, c, "batch with MALIGN")
+
+
+def test_recirc_bit_exact(tracker):
+    """an injection offset and a monitor before the recirculation point act
+    on pass 0 only (do_tracking.c:459-465); the rest of the ring every pass"""
+    ring = ring_with_maligns()
+    els = [E.MALIGN("INJ", DX=0.003, DXP=-1e-4), E.MONI("B0", L=0.2), E.RECIRC("RC")] + list(ring.elements)
+    bl = E.Beamline(els)
+    _, Po, rows = lossy_fodo(1200, seed=53)
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 5)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 5)
+    assert ng == nc and eg == o.effort()
+    assert_rows_identical(g, c, "ring with RECIRC")
+    # continuing from pass 5 starts at the recirculation point
+    g2, c2 = g[:ng].copy(), c[:nc].copy()
+    ng2, _ = tracker.do_tracking(g2, bl, Po, 2, pass_offset=5)
+    nc2 = O.Oracle("emath").do_tracking(c2, bl, Po, 2, pass_offset=5)
+    assert ng2 == nc2
+    assert_rows_identical(g2, c2, "ring with RECIRC, later passes")
+

@@ -1162,3 +1162,22 @@ def test_zero_strength_correctors_are_drifts_without_extrapolation():
         assert d[1, 4] < 0.7
     with pytest.raises(RuntimeError):
         track([E.HKICK(L=0.1, KICK=1e-4)], rows_of(pts))
+
+
+def test_elements_before_recirc_act_on_the_first_pass_only():
+    """RECIRC (do_tracking.c:459-465, 1382-1387): passes after pass 0 start
+    at the recirculation point, so an offset placed before it acts once"""
+    pts = [[1e-4, 2e-5, -3e-5, 1e-5, 0.0, 0.0]]
+    r, n, _ = track([E.MALIGN(DX=1e-3), E.RECIRC(), E.DRIF(L=1.5)], rows_of(pts), passes=3)
+    x = pts[0][0] + 1e-3
+    for _ in range(3):
+        x = x + 1.5 * pts[0][1]
+    assert abs(r[0, 0] - x) < 1e-18 and abs(r[0, 4] - 3 * (1.5 + 0.75 * (2e-5 ** 2 + 1e-5 ** 2))) < 1e-15
+    r2, _, _ = track([E.MALIGN(DX=1e-3), E.DRIF(L=1.5)], rows_of(pts), passes=3)
+    assert abs(r2[0, 0] - (x + 2e-3)) < 1e-17  # without RECIRC it acts every pass
+    # a later call (pass offset > 0) starts at the recirculation point at once
+    bl = E.Beamline([E.MALIGN(DX=1e-3), E.RECIRC(), E.DRIF(L=1.5)])
+    o = O.Oracle("glibc")
+    r3 = rows_of(pts)
+    o.do_tracking(r3, bl, PO, 2, pass_offset=5)
+    assert abs(r3[0, 0] - (pts[0][0] + 3.0 * pts[0][1])) < 1e-18
