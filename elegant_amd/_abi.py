@@ -36,12 +36,13 @@ EH_T_VMON = 16
 EH_T_HKICK = 17
 EH_T_VKICK = 18
 EH_T_RECIRC = 19
+EH_T_RCOL = 20
 
 TYPE_NAMES = {
     EH_T_MARK: "MARK", EH_T_DRIF: "DRIF", EH_T_EDRIFT: "EDRIFT", EH_T_KQUAD: "KQUAD",
     EH_T_KSEXT: "KSEXT", EH_T_KOCT: "KOCT", EH_T_CSBEND: "CSBEND", EH_T_RFCA: "RFCA",
     EH_T_WAKE: "WAKE", EH_T_ZLONGIT: "ZLONGIT", EH_T_MAXAMP: "MAXAMP", EH_T_CHARGE: "CHARGE", EH_T_MALIGN: "MALIGN", EH_T_MONI: "MONI", EH_T_HMON: "HMON", EH_T_VMON: "VMON",
-    EH_T_HKICK: "HKICK", EH_T_VKICK: "VKICK", EH_T_RECIRC: "RECIRC",
+    EH_T_HKICK: "HKICK", EH_T_VKICK: "VKICK", EH_T_RECIRC: "RECIRC", EH_T_RCOL: "RCOL",
 }
 
 EH_OPEN_PLUS_X, EH_OPEN_PLUS_Y, EH_OPEN_MINUS_X, EH_OPEN_MINUS_Y = 1, 2, 3, 4
@@ -139,6 +140,10 @@ class EH_CORRECTOR(C.Structure):
                        ["edge_effects", "order", "steering", "synchRad", "isr", "pad"])
 
 
+class EH_RCOL(C.Structure):
+    _fields_ = _fields(["length", "x_max", "y_max", "dx", "dy"], ["openSide", "invert"])
+
+
 class EH_MALIGN(C.Structure):
     _fields_ = _fields(["dxp", "dyp", "dx", "dy", "dz", "dt", "dp", "de"], ["on_pass", "startPID", "endPID", "pad"])
 
@@ -147,7 +152,7 @@ class _P(C.Union):
     _fields_ = [("drift", EH_DRIFT), ("edrift", EH_EDRIFT), ("kquad", EH_KQUAD), ("ksext", EH_KSEXT),
                 ("koct", EH_KOCT), ("csbend", EH_CSBEND), ("rfca", EH_RFCA), ("wake", EH_WAKE),
                 ("zlongit", EH_ZLONGIT), ("maxamp", EH_MAXAMP), ("charge", EH_CHARGE), ("malign", EH_MALIGN),
-                ("moni", EH_MONI), ("corrector", EH_CORRECTOR)]
+                ("moni", EH_MONI), ("corrector", EH_CORRECTOR), ("rcol", EH_RCOL)]
 
 
 class eh_element(C.Structure):
@@ -179,6 +184,7 @@ PARAM_STRUCT = {
     EH_T_MAXAMP: ("maxamp", EH_MAXAMP), EH_T_CHARGE: ("charge", EH_CHARGE), EH_T_MALIGN: ("malign", EH_MALIGN),
     EH_T_MONI: ("moni", EH_MONI), EH_T_HMON: ("moni", EH_MONI), EH_T_VMON: ("moni", EH_MONI),
     EH_T_HKICK: ("corrector", EH_CORRECTOR), EH_T_VKICK: ("corrector", EH_CORRECTOR),
+    EH_T_RCOL: ("rcol", EH_RCOL),
 }
 
 

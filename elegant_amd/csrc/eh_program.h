@@ -29,6 +29,8 @@
 #define EH_OP_WAKE 9       /* collective: handled outside the tracking kernel */
 #define EH_OP_ZLONGIT 10   /* collective: handled outside the tracking kernel */
 #define EH_OP_MALIGN 12    /* offset_beam (do_tracking.c:3243-3306) */
+#define EH_OP_RCOL_IN 13   /* rectangular_collimator's entrance check (limit_amplitudes.c:70-110) */
+#define EH_OP_RCOL_BODY 14 /* its drift and exit check (limit_amplitudes.c:111-186) */
 #define EH_OP_MATCH_P0 11  /* CHANGE_P0 after an RFCA: do_match_energy on the host (do_tracking.c:3308-3420);
                               the kernel does nothing here */
 
@@ -110,6 +112,13 @@ typedef struct {
 #define EH_D_DRIFT 56   /* drift / n (nonzero test) */
 #define EH_D_TOTLEN 57  /* sum of all dsh in step order: dzLoss of a completed particle */
 #define EH_D_LEND 58    /* (L - LEFFECTIVE) / 2 */
+
+/* ---- RCOL: d[0..4] = length, x_max, y_max, dx, dy; order0 = INVERT, order1 = open-side code ---- */
+#define EH_D_RC_LEN 0
+#define EH_D_RC_XMAX 1
+#define EH_D_RC_YMAX 2
+#define EH_D_RC_DX 3
+#define EH_D_RC_DY 4
 
 /* ---- MALIGN: d[0..7] = dxp, dyp, dx, dy, dz, dt, dp, de; nkicks_rf = ON_PASS ---- */
 #define EH_D_MA_DXP 0

@@ -1057,6 +1057,75 @@ DEV void track_malign(particle &p, const eh_op &op, int64_t pass) {
   }
 }
 
+/* rectangular_collimator (limit_amplitudes.c:32-187), the entrance check:
+ * lost outside the half-widths (an open side may spare it) or with a
+ * non-finite x or y, INVERT swapping the verdict; the loss at the
+ * element's start */
+DEV int track_rcol_in(particle &p, const eh_op &op, const double *z) {
+  const double xsize = op.d[EH_D_RC_XMAX], ysize = op.d[EH_D_RC_YMAX];
+  const double dx = p.x - op.d[EH_D_RC_DX], dy = p.y - op.d[EH_D_RC_DY];
+  const int open = op.order1;
+  bool lost = false;
+  if ((xsize > 0 && __builtin_fabs(dx) > xsize) || (ysize > 0 && __builtin_fabs(dy) > ysize))
+    lost = open ? ap_open_lost(open, dx, dy, xsize, ysize) : true;
+  else if (__builtin_isinf(p.x) || __builtin_isinf(p.y) || __builtin_isnan(p.x) || __builtin_isnan(p.y))
+    lost = true;
+  if (op.order0)
+    lost = !lost;
+  if (lost) {
+    p.s = opc(&z[0]);
+    p.dp = op.Po * (1 + p.dp);
+    return EH_STAGE_ELEMENT + 1;
+  }
+  return 0;
+}
+
+/* the collimator's drift and exit check: a particle outside at the exit
+ * is lost where its straight orbit met the jaw (no open side), a
+ * non-finite exit position at the exit; survivors take the drift */
+DEV int track_rcol_body(particle &p, const eh_op &op, const double *z) {
+  const double length = op.d[EH_D_RC_LEN];
+  const double xsize = op.d[EH_D_RC_XMAX], ysize = op.d[EH_D_RC_YMAX];
+  const double xc = op.d[EH_D_RC_DX], yc = op.d[EH_D_RC_DY];
+  const int open = op.order1;
+  const double x1 = p.x + length * p.xp, y1 = p.y + length * p.yp;
+  const double dx = x1 - xc, dy = y1 - yc;
+  const double z0 = opc(&z[0]);
+  int is_out = 0;
+  if (xsize > 0 && __builtin_fabs(dx) > xsize)
+    is_out += 1 * (open ? (ap_open_lost(open, dx, 0, xsize, ysize) ? 1 : 0) : 1);
+  if (ysize > 0 && __builtin_fabs(dy) > ysize)
+    is_out += 2 * (open ? (ap_open_lost(open, 0, dy, xsize, ysize) ? 1 : 0) : 1);
+  if (__builtin_isinf(x1) || __builtin_isinf(y1) || __builtin_isnan(x1) || __builtin_isnan(y1))
+    is_out += 4;
+  if (is_out & 4) {
+    p.s = z0 + length;
+    p.x = x1;
+    p.y = y1;
+    p.dp = op.Po * (1 + p.dp);
+    return EH_STAGE_ELEMENT + 1;
+  }
+  if ((is_out && !op.order0) || (!is_out && op.order0)) {
+    if (!open) {
+      double zx = 1.7976931348623157e308, zy = 1.7976931348623157e308;
+      if ((is_out & 1) && p.xp != 0)
+        zx = ((p.xp > 0 ? 1 : -1) * xsize - (p.x - xc)) / p.xp;
+      if ((is_out & 2) && p.yp != 0)
+        zy = ((p.yp > 0 ? 1 : -1) * ysize - (p.y - yc)) / p.yp;
+      const double zz = zx < zy ? zx : zy;
+      p.x += p.xp * zz;
+      p.y += p.yp * zz;
+      p.s = z0 + zz;
+    }
+    p.dp = op.Po * (1 + p.dp);
+    return EH_STAGE_ELEMENT + 1;
+  }
+  p.s += length * em_sqrt(1 + sq(p.xp) + sq(p.yp));
+  p.x = x1;
+  p.y = y1;
+  return 0;
+}
+
 /* One element on one particle.  z points at the element's (z_start, z_end),
  * which only the loss paths read: in the element record itself, or in the
  * position table of a program whose repeated cells share one record. */
@@ -1090,6 +1159,12 @@ DEV int run_op(particle &p, const eh_op &op, const double *z, int64_t pass) {
     break;
   case EH_OP_MALIGN:
     track_malign(p, op, pass);
+    break;
+  case EH_OP_RCOL_IN:
+    lost = track_rcol_in(p, op, z);
+    break;
+  case EH_OP_RCOL_BODY:
+    lost = track_rcol_body(p, op, z);
     break;
   case EH_OP_RFCA_DRIFT:
     /* VOLT=0: a drift when the cavity has length, else nothing (simple_rfca.c:389-402) */

@@ -75,6 +75,8 @@ _NAMES = {
     A.EH_T_VKICK: {"L": "length", "KICK": "kick", "TILT": "tilt", "B2": "b2", "CALIBRATION": "calibration",
                    "EDGE_EFFECTS": "edge_effects", "ORDER": "order", "STEERING": "steering",
                    "SYNCH_RAD": "synchRad", "ISR": "isr", "LERAD": "lEffRad"},
+    A.EH_T_RCOL: {"L": "length", "X_MAX": "x_max", "Y_MAX": "y_max", "DX": "dx", "DY": "dy",
+                  "OPEN_SIDE": "openSide", "INVERT": "invert"},
     A.EH_T_MALIGN: {"DXP": "dxp", "DYP": "dyp", "DX": "dx", "DY": "dy", "DZ": "dz", "DT": "dt", "DP": "dp",
                     "DE": "de", "ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     A.EH_T_MARK: {},
@@ -142,6 +144,10 @@ def make(type_code, name="", **kw):
     elif kw:
         raise KeyError("%s takes no parameters" % A.TYPE_NAMES[type_code])
     return e
+
+
+def RCOL(name="", **kw):
+    return make(A.EH_T_RCOL, name, **kw)
 
 
 def RECIRC(name=""):
@@ -281,7 +287,7 @@ class Beamline:
         for e in self.elements:
             if e.type in (A.EH_T_DRIF, A.EH_T_EDRIFT, A.EH_T_KQUAD, A.EH_T_KSEXT, A.EH_T_KOCT,
                           A.EH_T_CSBEND, A.EH_T_RFCA, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON,
-                          A.EH_T_HKICK, A.EH_T_VKICK):
+                          A.EH_T_HKICK, A.EH_T_VKICK, A.EH_T_RCOL):
                 total += getattr(e.p, A.PARAM_STRUCT[e.type][0]).length
         return total
 

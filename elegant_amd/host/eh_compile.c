@@ -58,6 +58,8 @@ int eh_element_defaults(eh_element *e, int32_t type) {
   case EH_T_MARK:
   case EH_T_EDRIFT:
   case EH_T_CHARGE:
+  case EH_T_RECIRC: /* track_data.c:1018-1020 */
+  case EH_T_RCOL:   /* track_data.c:651-659: every parameter defaults to zero */
     break;
   case EH_T_MONI: /* track_data.c:600-647 */
   case EH_T_HMON:
@@ -157,6 +159,8 @@ int64_t eh_struct_size(int32_t type) {
   case EH_T_HMON:
   case EH_T_VMON:
     return (int64_t)sizeof(EH_MONI);
+  case EH_T_RCOL:
+    return (int64_t)sizeof(EH_RCOL);
   case EH_T_HKICK:
   case EH_T_VKICK:
     return (int64_t)sizeof(EH_CORRECTOR);
@@ -188,6 +192,8 @@ double eh_element_length(const eh_element *e) {
   case EH_T_HKICK:
   case EH_T_VKICK:
     return e->p.corrector.length;
+  case EH_T_RCOL:
+    return e->p.rcol.length;
   default:
     return 0;
   }
@@ -196,7 +202,7 @@ double eh_element_length(const eh_element *e) {
 int eh_has_length(int32_t type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT ||
          type == EH_T_KOCT || type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON ||
-         type == EH_T_VMON || type == EH_T_HKICK || type == EH_T_VKICK;
+         type == EH_T_VMON || type == EH_T_HKICK || type == EH_T_VKICK || type == EH_T_RCOL;
 }
 
 /* rotateBeamCoordinatesForMisalignment's choice of cos/sin (tilt_matrices.c:161-176) */
@@ -740,7 +746,7 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
   for (i = 0; i < n_elems; i++) {
     const eh_element *e = &bl[i];
     eh_op op;
-    int rc = EH_OK, keep = 1, no_aperture = 0;
+    int rc = EH_OK, keep = 1, no_aperture = 0, rcol_body = 0;
     memset(&op, 0, sizeof(op));
     last_z = z;
     if (eh_has_length(e->type))
@@ -771,6 +777,31 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       op.d[1] = k->length / 2;
       if (fabs(op.d[0]) >= 0x1.0p-300 && fabs(op.d[0]) <= 0x1.0p+300)
         op.flags |= EH_F_DFAST;
+      break;
+    }
+    case EH_T_RCOL: {
+      /* rectangular_collimator (limit_amplitudes.c:32-185): the entrance
+       * check, and with a length the drift and exit check as a second
+       * record, each compacting its own losses as the two loops do */
+      const EH_RCOL *r = &e->p.rcol;
+      if (r->openSide < 0 || r->openSide > 4) {
+        snprintf(msg, msglen, "%s: RCOL open side code out of range", e->name);
+        return EH_ERR_INVALID;
+      }
+      if (r->x_max <= 0 && r->y_max <= 0) {
+        op.op = EH_OP_EXACT_DRIFT; /* no jaws: exactDrift (limit_amplitudes.c:66-69) */
+        op.d[0] = r->length;
+        break;
+      }
+      op.op = EH_OP_RCOL_IN;
+      op.d[EH_D_RC_LEN] = r->length;
+      op.d[EH_D_RC_XMAX] = r->x_max;
+      op.d[EH_D_RC_YMAX] = r->y_max;
+      op.d[EH_D_RC_DX] = r->dx;
+      op.d[EH_D_RC_DY] = r->dy;
+      op.order0 = r->invert != 0;
+      op.order1 = r->openSide;
+      rcol_body = r->length > 0;
       break;
     }
     case EH_T_MALIGN: {
@@ -917,6 +948,13 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
       prog->ops[prog->nops++] = step;
       prog->ops[prog->nops++] = post;
       continue;
+    }
+    if (rcol_body) {
+      /* the entrance check alone, then the body with the element's checks */
+      eh_op in = op;
+      in.flags = (op.flags & ~EH_F_AP) | EH_F_ALONE;
+      prog->ops[prog->nops++] = in;
+      op.op = EH_OP_RCOL_BODY;
     }
     prog->ops[prog->nops++] = op;
   }

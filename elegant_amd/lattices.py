@@ -261,7 +261,7 @@ def flops_per_particle_turn(beamline):
     total = 0
     for e in beamline.elements:
         t = e.type
-        if t in (A.EH_T_DRIF, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON, A.EH_T_HKICK, A.EH_T_VKICK):
+        if t in (A.EH_T_DRIF, A.EH_T_MONI, A.EH_T_HMON, A.EH_T_VMON, A.EH_T_HKICK, A.EH_T_VKICK, A.EH_T_RCOL):
             total += 10
         elif t == A.EH_T_EDRIFT:
             total += 11

@@ -47,7 +47,8 @@ SUPPORTED = {"DRIF": A.EH_T_DRIF, "EDRIFT": A.EH_T_EDRIFT, "KQUAD": A.EH_T_KQUAD
              "KOCT": A.EH_T_KOCT, "CSBEND": A.EH_T_CSBEND, "RFCA": A.EH_T_RFCA, "WAKE": A.EH_T_WAKE,
              "ZLONGIT": A.EH_T_ZLONGIT, "MAXAMP": A.EH_T_MAXAMP, "MARK": A.EH_T_MARK, "CHARGE": A.EH_T_CHARGE,
              "MALIGN": A.EH_T_MALIGN, "MONI": A.EH_T_MONI, "HMON": A.EH_T_HMON, "VMON": A.EH_T_VMON,
-             "HKICK": A.EH_T_HKICK, "VKICK": A.EH_T_VKICK, "RECIRC": A.EH_T_RECIRC}
+             "HKICK": A.EH_T_HKICK, "VKICK": A.EH_T_VKICK, "RECIRC": A.EH_T_RECIRC,
+             "RCOL": A.EH_T_RCOL}
 
 _RPN_UNARY = {"sqrt": math.sqrt, "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin,
               "acos": math.acos, "atan": math.atan, "exp": math.exp, "ln": math.log, "abs": abs,
@@ -329,7 +330,7 @@ def dumps(beamline, line_name="RING"):
                 if key in ("N_KICKS",) and e.type == A.EH_T_CSBEND:
                     continue   # alias of N_SLICES for CSBEND
                 v = getattr(sub, attr)
-                if key == "OPEN_SIDE" and e.type == A.EH_T_MAXAMP:
+                if key == "OPEN_SIDE" and e.type in (A.EH_T_MAXAMP, A.EH_T_RCOL):
                     if v:
                         parts.append('OPEN_SIDE="%s"' % {c: t for t, c in E.OPEN_SIDE_CODES.items()}[v])
                     continue

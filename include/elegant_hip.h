@@ -63,6 +63,7 @@ extern "C" {
 #define EH_T_HKICK 17
 #define EH_T_VKICK 18
 #define EH_T_RECIRC 19 /* recirculation point: passes after pass 0 start here (do_tracking.c:459-465) */
+#define EH_T_RCOL 20
 
 /* DRIF (track.h:1310-1316); tracked with the order-2 drift matrix
  * (compute_matrices.c:1101-1104, matrix.c:389-429) */
@@ -215,6 +216,17 @@ typedef struct {
   int32_t edge_effects, order, steering, synchRad, isr, pad;
 } EH_CORRECTOR;
 
+/* RCOL (track.h; defaults track_data.c:651-659): the rectangular
+ * collimator of rectangular_collimator (limit_amplitudes.c:32-187): an
+ * entrance check at the element's start, then for a positive length the
+ * drift with an exit check whose losses are placed where the orbit met the
+ * jaw.  OPEN_SIDE is held as determineOpenSideCode returns it (as in
+ * EH_MAXAMP); INVERT swaps lost and transmitted. */
+typedef struct {
+  double length, x_max, y_max, dx, dy;
+  int32_t openSide, invert;
+} EH_RCOL;
+
 /* One beamline element; mirrors ELEMENT_LIST's (type, p_elem) pair
  * (track.h:308-344) with the parameters held by value. */
 typedef struct {
@@ -236,6 +248,7 @@ typedef struct {
     EH_MALIGN malign;
     EH_MONI moni; /* MONI, HMON and VMON */
     EH_CORRECTOR corrector; /* HKICK and VKICK */
+    EH_RCOL rcol;
   } p;
 } eh_element;
 

@@ -226,6 +226,96 @@ void or_offset_beam(double **coord, long nToTrack, const EH_MALIGN *offset, doub
   }
 }
 
+/* rectangular_collimator (limit_amplitudes.c:32-185): the entrance loop
+ * and, for a positive length, the drift loop with the exit check, each
+ * compacting its losses by swaps */
+static long or_open_side_lost(long code, double dx, double dy, double xsize, double ysize);
+void or_exact_drift(double **part, long np, double length);
+long or_rectangular_collimator(double **initial, const EH_RCOL *rcol, long np, double z, double Po) {
+  double length, *ini, xsize = rcol->x_max, ysize = rcol->y_max, x_center = rcol->dx, y_center = rcol->dy;
+  double x1, y1, zx, zy, dx, dy;
+  long ip, itop, is_out, lost, openCode = rcol->openSide;
+  if (xsize <= 0 && ysize <= 0) {
+    or_exact_drift(initial, np, rcol->length);
+    return np;
+  }
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    ini = initial[ip];
+    dx = ini[0] - x_center;
+    dy = ini[2] - y_center;
+    lost = 0;
+    if ((xsize > 0 && fabs(dx) > xsize) || (ysize > 0 && fabs(dy) > ysize))
+      lost = openCode ? or_open_side_lost(openCode, dx, dy, xsize, ysize) : 1;
+    else if (isinf(ini[0]) || isinf(ini[2]) || isnan(ini[0]) || isnan(ini[2]))
+      lost = 1;
+    if (rcol->invert)
+      lost = !lost;
+    if (lost) {
+      swap_rows(initial[ip], initial[itop], OR_NPROPS);
+      initial[itop][4] = z;
+      initial[itop][5] = Po * (1 + initial[itop][5]);
+      --itop;
+      --ip;
+      --np;
+    }
+  }
+  if (np == 0 || (length = rcol->length) <= 0)
+    return np;
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    ini = initial[ip];
+    x1 = ini[0] + length * ini[1];
+    y1 = ini[2] + length * ini[3];
+    dx = x1 - x_center;
+    dy = y1 - y_center;
+    is_out = 0;
+    if (xsize > 0 && fabs(dx) > xsize)
+      is_out += 1 * (openCode ? or_open_side_lost(openCode, dx, 0, xsize, ysize) : 1);
+    if (ysize > 0 && fabs(dy) > ysize)
+      is_out += 2 * (openCode ? or_open_side_lost(openCode, 0, dy, xsize, ysize) : 1);
+    if (isinf(x1) || isinf(y1) || isnan(x1) || isnan(y1))
+      is_out += 4;
+    if (is_out & 4) {
+      ini[4] = z + length;
+      ini[0] = x1;
+      ini[2] = y1;
+      ini[5] = Po * (1 + ini[5]);
+      swap_rows(initial[ip], initial[itop], OR_NPROPS);
+      --itop;
+      --ip;
+      --np;
+    } else if ((is_out && !rcol->invert) || (!is_out && rcol->invert)) {
+      if (!openCode) {
+        zx = zy = DBL_MAX;
+        if (is_out & 1 && ini[1] != 0)
+          zx = ((ini[1] > 0 ? 1 : -1) * xsize - (ini[0] - x_center)) / ini[1];
+        if (is_out & 2 && ini[3] != 0)
+          zy = ((ini[3] > 0 ? 1 : -1) * ysize - (ini[2] - y_center)) / ini[3];
+        if (zx < zy) {
+          ini[0] += ini[1] * zx;
+          ini[2] += ini[3] * zx;
+          ini[4] = z + zx;
+        } else {
+          ini[0] += ini[1] * zy;
+          ini[2] += ini[3] * zy;
+          ini[4] = z + zy;
+        }
+      }
+      ini[5] = Po * (1 + ini[5]);
+      swap_rows(initial[ip], initial[itop], OR_NPROPS);
+      --itop;
+      --ip;
+      --np;
+    } else {
+      ini[4] += length * sqrt(1 + sqr(ini[1]) + sqr(ini[3]));
+      ini[0] = x1;
+      ini[2] = y1;
+    }
+  }
+  return np;
+}
+
 /* exactDrift (csbend.c:3767-3790) */
 void or_exact_drift(double **part, long np, double length) {
   for (long i = 0; i < np; i++) {
@@ -2131,7 +2221,7 @@ have_z:
 static int has_length(int type) {
   return type == EH_T_DRIF || type == EH_T_EDRIFT || type == EH_T_KQUAD || type == EH_T_KSEXT || type == EH_T_KOCT ||
          type == EH_T_CSBEND || type == EH_T_RFCA || type == EH_T_MONI || type == EH_T_HMON || type == EH_T_VMON ||
-         type == EH_T_HKICK || type == EH_T_VKICK;
+         type == EH_T_HKICK || type == EH_T_VKICK || type == EH_T_RCOL;
 }
 
 static double elem_length(const eh_element *e) {
@@ -2157,6 +2247,8 @@ static double elem_length(const eh_element *e) {
   case EH_T_HKICK:
   case EH_T_VKICK:
     return e->p.corrector.length;
+  case EH_T_RCOL:
+    return e->p.rcol.length;
   default:
     return 0;
   }
@@ -2245,6 +2337,9 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
         /* MATRIX_TRACKING with the drift matrix (compute_matrices.c:1317-1328) */
         or_track_particles_drift(coord, nToTrack, e->p.moni.length, e->p.moni.order ? e->p.moni.order : 2);
         break;
+      case EH_T_RCOL:
+        rc = or_rectangular_collimator(coord, &e->p.rcol, nToTrack, last_z, *P_central);
+        break;
       case EH_T_HKICK:
       case EH_T_VKICK:
         /* track_particles with corrector_matrix, which with no strength is
@@ -2288,7 +2383,7 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
       if (rc < 0)
         return rc;
       if (e->type == EH_T_KQUAD || e->type == EH_T_KSEXT || e->type == EH_T_KOCT || e->type == EH_T_CSBEND ||
-          e->type == EH_T_RFCA)
+          e->type == EH_T_RFCA || e->type == EH_T_RCOL)
         nLeft = rc;
       if (!no_aperture && (x_max || y_max)) {
         nLeft = or_maxamp_check(coord, maxamp, x_max, y_max, nLeft, z, *P_central, e->type == EH_T_DRIF);

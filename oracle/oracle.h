@@ -78,6 +78,7 @@ void or_etilt_offsets(double rho0, double angle, double etilt, double tilt, doub
 void or_set_row_width(long nprops);
 long or_row_width(void);
 /* elegant's species globals (change_particle, elegant.c:2933-2991) */
+long or_rectangular_collimator(double **initial, const EH_RCOL *rcol, long np, double z, double Po);
 void or_offset_beam(double **coord, long nToTrack, const EH_MALIGN *offset, double P_central);
 void or_set_species(double mass, double charge, double massMV, double radius, double relSign);
 void or_set_id_slots_per_bunch(or_ctx *c, long n);

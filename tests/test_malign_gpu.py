@@ -82,3 +82,51 @@ def test_recirc_bit_exact(tracker):
     assert ng2 == nc2
     assert_rows_identical(g2, c2, "ring with RECIRC, later passes")
 
+
+
+def test_collimators_bit_exact(tracker):
+    """RCOL (rectangular_collimator): thick with interpolated jaw hits, an
+    open side, a thin inverted one (a stop in the middle of the beam) and
+    one without jaws (an exact drift) in the ring"""
+    ring = ring_with_maligns()
+    els = list(ring.elements)
+    els.insert(5, E.RCOL("C1", L=0.3, X_MAX=0.02, Y_MAX=0.015, DX=5e-4))
+    els.insert(15, E.RCOL("C2", L=0.2, X_MAX=0.016, OPEN_SIDE="-x"))
+    els.insert(25, E.RCOL("C3", L=0.0, X_MAX=5e-5, Y_MAX=5e-5, INVERT=1))
+    els.insert(35, E.RCOL("C4", L=0.4))
+    bl = E.Beamline(els)
+    _, Po, rows = lossy_fodo(1500, seed=54)
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 4)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 4)
+    assert ng == nc and eg == o.effort()
+    assert 100 < rows.shape[0] - ng < rows.shape[0] - 100
+    assert_rows_identical(g, c, "ring with collimators")
+
+
+def test_example_deck_with_its_malign_recirc_and_collimator(tracker):
+    """the reference's parTrack.lte with only its WATCH left out: the
+    MALIGN before the recirculation point, RECIRC and the RCOL of 0.2 m
+    half-width, 30 turns of a beam with a halo, bit-exact to the checker"""
+    import os
+    from elegant_amd import lattices as Lt
+    from elegant_amd import lte
+    from tests.test_elements_gpu import beam
+    lat = lte.load(os.path.join(os.path.dirname(__file__), "golden", "parTrack.lte"))
+    deck = lat.beamline("PAR", transmute={"QUAD": "KQUAD", "SEXT": "KSEXT"}, ignore=("WATCH",))
+    types = [e.type for e in deck.elements]
+    from elegant_amd import _abi as A
+    assert types[0] == A.EH_T_MALIGN and types[1] == A.EH_T_RECIRC and A.EH_T_RCOL in types
+    Po = Lt.momentum(450.0)
+    bl = E.Beamline([E.MAXAMP("AP", X_MAX=0.01, Y_MAX=0.005)] + list(deck.elements), keep=deck.keep)
+    rows = beam(2000, 22, halo=0.3, xshift=0.006)
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 30)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 30)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 10
+    assert_rows_identical(g, c, "parTrack deck, whole")

@@ -1181,3 +1181,35 @@ def test_elements_before_recirc_act_on_the_first_pass_only():
     r3 = rows_of(pts)
     o.do_tracking(r3, bl, PO, 2, pass_offset=5)
     assert abs(r3[0, 0] - (pts[0][0] + 3.0 * pts[0][1])) < 1e-18
+
+
+def test_rectangular_collimator_losses():
+    """RCOL (rectangular_collimator, limit_amplitudes.c:32-187): a particle
+    outside the jaws at the entrance is lost at the element's start; one
+    that leaves through a jaw inside the collimator is lost where its
+    straight orbit meets the jaw; an open side spares particles leaving
+    through it (without placing them); INVERT swaps lost and kept; without
+    jaws the element is an exact drift"""
+    L, xm = 0.5, 0.01
+    pts = [[0.02, 0.0, 0, 0, 0, 0],        # outside at the entrance
+           [0.009, 0.004, 0, 0, 0, 0],     # hits the +x jaw at z = (0.01-0.009)/0.004 = 0.25
+           [-0.009, -0.01, 0, 0, 0, 0],    # hits the -x jaw at z = 0.1
+           [0.001, 1e-4, 2e-3, 1e-4, 0.3, 0]]  # passes
+    # the same particles one metre of drift upstream, so that they reach the
+    # collimator (at z = 1) as listed
+    up = [[p[0] - 1.0 * p[1], p[1], p[2] - 1.0 * p[3], p[3], p[4], p[5]] for p in pts]
+    r, n, _ = track([E.DRIF(L=1.0), E.RCOL(L=L, X_MAX=xm, Y_MAX=0.02)], rows_of(up))
+    assert n == 1 and r[0, 6] == 4
+    lost = {int(row[6]): row for row in r[1:]}
+    assert lost[1][4] == 1.0
+    assert abs(lost[2][4] - 1.25) < 1e-15 and abs(lost[2][0] - 0.01) < 1e-15
+    assert abs(lost[3][4] - 1.1) < 1e-15 and abs(lost[3][0] + 0.01) < 1e-15
+    assert abs(r[0, 4] - (0.3 + 1.0 + (1e-4 ** 2 + 1e-4 ** 2) / 2 + L * math.sqrt(1 + 2e-8))) < 1e-15
+    r, n, _ = track([E.RCOL(L=L, X_MAX=xm, OPEN_SIDE="+x")], rows_of(pts))
+    kept = set(r[:n, 6].astype(int))
+    assert kept == {1, 2, 4}  # the +x side is open: rows 1 and 2 leave through it
+    r, n, _ = track([E.RCOL(L=0.0, X_MAX=xm, INVERT=1)], rows_of(pts))
+    assert set(r[:n, 6].astype(int)) == {1}
+    r, n, _ = track([E.RCOL(L=L)], rows_of(pts))
+    e, _, _ = track([E.EDRIFT(L=L)], rows_of(pts))
+    assert n == 4 and np.array_equal(r, e)
