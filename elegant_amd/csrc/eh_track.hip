@@ -1057,7 +1057,7 @@ DEV void track_malign(particle &p, const eh_op &op, int64_t pass) {
   }
 }
 
-/* rectangular_collimator (limit_amplitudes.c:32-187), the entrance check:
+/* rectangular_collimator (limit_amplitudes.c:32-185), the entrance check:
  * lost outside the half-widths (an open side may spare it) or with a
  * non-finite x or y, INVERT swapping the verdict; the loss at the
  * element's start */

@@ -1184,7 +1184,7 @@ def test_elements_before_recirc_act_on_the_first_pass_only():
 
 
 def test_rectangular_collimator_losses():
-    """RCOL (rectangular_collimator, limit_amplitudes.c:32-187): a particle
+    """RCOL (rectangular_collimator, limit_amplitudes.c:32-185): a particle
     outside the jaws at the entrance is lost at the element's start; one
     that leaves through a jaw inside the collimator is lost where its
     straight orbit meets the jaw; an open side spares particles leaving
