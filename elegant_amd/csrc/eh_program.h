@@ -31,6 +31,7 @@
 #define EH_OP_MALIGN 12    /* offset_beam (do_tracking.c:3243-3306) */
 #define EH_OP_RCOL_IN 13   /* rectangular_collimator's entrance check (limit_amplitudes.c:70-110) */
 #define EH_OP_RCOL_BODY 14 /* its drift and exit check (limit_amplitudes.c:111-186) */
+#define EH_OP_MATRIX1 15   /* track_particles with a first-order matrix (matrix.c:430-458) */
 #define EH_OP_MATCH_P0 11  /* CHANGE_P0 after an RFCA: do_match_energy on the host (do_tracking.c:3308-3420);
                               the kernel does nothing here */
 
@@ -121,6 +122,9 @@ typedef struct {
 #define EH_D_RC_DY 4
 
 /* ---- MALIGN: d[0..7] = dxp, dyp, dx, dy, dz, dt, dp, de; nkicks_rf = ON_PASS ---- */
+/* EH_OP_MATRIX1: C[6] then R[6][6] row by row */
+#define EH_D_M1_C 0
+#define EH_D_M1_R 6
 #define EH_D_MA_DXP 0
 #define EH_D_MA_DYP 1
 #define EH_D_MA_DX 2

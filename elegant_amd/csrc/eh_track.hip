@@ -205,6 +205,27 @@ DEV void drift1(particle &p, double L) {
   p.x = 0.0 + 0.0 * dp0 + 0.0 * s0 + 0.0 * yp0 + 0.0 * y0 + L * xp0 + 1.0 * x0;
 }
 
+/* track_particles with a first-order matrix (matrix.c:430-458): each
+ * coordinate is C[i] plus R[i][j] x[j] added from j = 5 down to 0 */
+DEV void matrix1(particle &p, const double *m) {
+  const double x[6] = {p.x, p.xp, p.y, p.yp, p.s, p.dp};
+  double o[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    double sum = m[EH_D_M1_C + i];
+#pragma unroll
+    for (int j = 5; j >= 0; j--)
+      sum += m[EH_D_M1_R + 6 * i + j] * x[j];
+    o[i] = sum;
+  }
+  p.x = o[0];
+  p.xp = o[1];
+  p.y = o[2];
+  p.yp = o[3];
+  p.s = o[4];
+  p.dp = o[5];
+}
+
 DEV void exact_drift(particle &p, double L) {
   p.x += p.xp * L;
   p.y += p.yp * L;
@@ -1147,6 +1168,9 @@ DEV int run_op(particle &p, const eh_op &op, const double *z, int64_t pass) {
     break;
   case EH_OP_EXACT_DRIFT:
     exact_drift(p, op.d[0]);
+    break;
+  case EH_OP_MATRIX1:
+    matrix1(p, op.d);
     break;
   case EH_OP_MULT:
     lost = track_multipole<SH>(p, op, z);

@@ -222,6 +222,67 @@ static void rotation_cs(double angle, double *c, double *s) {
   }
 }
 
+/* concat_matrices for two first-order matrices (concat_mat.c:62-97):
+ * M2 = M1 M0, sums started at zero and taken over j = 0..5 */
+static void concat1(double C2[6], double R2[6][6], const double C1[6], const double R1[6][6],
+                    const double C0[6], const double R0[6][6]) {
+  for (int i = 0; i < 6; i++) {
+    double sum = 0;
+    for (int j = 0; j < 6; j++)
+      sum += R1[i][j] * C0[j];
+    C2[i] = sum + C1[i];
+  }
+  for (int i = 0; i < 6; i++)
+    for (int m = 0; m < 6; m++) {
+      double sum = 0;
+      for (int j = 0; j < 6; j++)
+        sum += R1[i][j] * R0[j][m];
+      R2[i][m] = sum;
+    }
+}
+
+/* corrector_matrix of a zero-length corrector with a kick (bend_matrix.c:
+ * 300-330): C[1] = kick and R the identity with R[1][5] = -sin(kick); then
+ * tilt_matrices (tilt_matrices.c:18-73), M <- IRot (M Rot), with its exact
+ * sine and cosine at +-pi/2 and +-pi.  Out: C[6] then R row by row */
+static void thin_corrector_matrix(double kick, double tilt, double *out) {
+  double C[6] = {0}, R[6][6] = {{0}};
+  for (int i = 0; i < 6; i++)
+    R[i][i] = 1;
+  C[1] = kick;
+  R[1][5] = -sin(kick);
+  if (tilt) {
+    double s, c, Cr[6] = {0}, Rot[6][6] = {{0}}, IRot[6][6] = {{0}}, Cm[6], Rm[6][6];
+    if (fabs(tilt - EH_PIo2) < 1e-12) {
+      s = 1;
+      c = 0;
+    } else if (fabs(tilt + EH_PIo2) < 1e-12) {
+      s = -1;
+      c = 0;
+    } else if (fabs(tilt - EH_PI) < 1e-12 || fabs(tilt + EH_PI) < 1e-12) {
+      s = 0;
+      c = -1;
+    } else {
+      s = sin(tilt);
+      c = cos(tilt);
+    }
+    for (int k = 0; k < 2; k++) { /* (x, y) at k = 0, (x', y') at k = 1 */
+      IRot[k][k] = Rot[k][k] = c;
+      IRot[k][k + 2] = -(Rot[k][k + 2] = s);
+      IRot[k + 2][k] = -(Rot[k + 2][k] = -s);
+      IRot[k + 2][k + 2] = Rot[k + 2][k + 2] = c;
+    }
+    IRot[4][4] = IRot[5][5] = Rot[4][4] = Rot[5][5] = 1;
+    concat1(Cm, Rm, C, R, Cr, Rot);   /* Mr = M Rot */
+    concat1(C, R, Cr, IRot, Cm, Rm);  /* M = IRot Mr */
+  }
+  for (int i = 0; i < 6; i++) {
+    out[EH_D_M1_C + i] = C[i];
+    for (int j = 0; j < 6; j++)
+      out[EH_D_M1_R + 6 * i + j] = R[i][j];
+  }
+}
+
 static int rotation_active(double angle) { return angle && !(fabs(fabs(angle) - EH_PIx2) < 1e-12); }
 
 /* x and y exponents of an elliptical MAXAMP (YEXPONENT 0 = EXPONENT) */
@@ -768,9 +829,21 @@ int eh_compile(const eh_element *bl, int64_t n_elems, double Po, const eh_specie
        * aperture's loss point not extrapolated (do_tracking.c:2463) */
       const EH_CORRECTOR *k = &e->p.corrector;
       int order = k->order ? k->order : 2;
-      if (k->kick * k->calibration != 0 || k->synchRad || k->isr) {
-        snprintf(msg, msglen, "%s: steering corrector with a kick or radiation", e->name);
+      double kick = k->kick * k->calibration;
+      if (k->synchRad || k->isr) {
+        snprintf(msg, msglen, "%s: steering corrector radiation", e->name);
         return EH_ERR_UNSUPPORTED;
+      }
+      if (kick != 0) {
+        /* a zero-length corrector's kick matrix (bend_matrix.c:323-330);
+         * VKICK turns it by pi/2 (compute_matrices.c:1341-1346) */
+        if (k->length != 0) {
+          snprintf(msg, msglen, "%s: steering corrector with a length and a kick", e->name);
+          return EH_ERR_UNSUPPORTED;
+        }
+        op.op = EH_OP_MATRIX1;
+        thin_corrector_matrix(kick, e->type == EH_T_VKICK ? k->tilt + EH_PIo2 : k->tilt, op.d);
+        break;
       }
       op.op = order >= 2 ? EH_OP_DRIFT2 : EH_OP_DRIFT1;
       op.d[0] = k->length;

@@ -152,6 +152,64 @@ long or_row_width(void) { return or_nprops; }
 /* track_particles, order-2 branch (matrix.c:389-429), with the drift matrix
  * of compute_matrices.c:1101-1104: C[4]=L, R=identity with R[0][1]=R[2][3]=L,
  * T[4][1][1]=T[4][3][3]=L/2.  The sparse loop is executed literally. */
+/* track_particles for a zero-length corrector with a kick: corrector_matrix
+ * (bend_matrix.c:300-330) turned by tilt_matrices (tilt_matrices.c:18-73,
+ * IRot (M Rot) by concat_matrices, concat_mat.c:62-97), applied as the
+ * first-order case of track_particles (matrix.c:430-458) */
+static void or_mat1_mul(double C2[6], double R2[6][6], const double C1[6], const double R1[6][6],
+                        const double C0[6], const double R0[6][6]) {
+  long i, j, m;
+  double sum;
+  for (i = 0; i < 6; i++) {
+    for (j = 0, sum = 0; j < 6; j++)
+      sum += R1[i][j] * C0[j];
+    C2[i] = sum + C1[i];
+  }
+  for (i = 0; i < 6; i++)
+    for (m = 0; m < 6; m++) {
+      for (j = 0, sum = 0; j < 6; j++)
+        sum += R1[i][j] * R0[j][m];
+      R2[i][m] = sum;
+    }
+}
+
+void or_thin_corrector(double **coord, long np, double kick, double tilt) {
+  double C[6] = {0, kick, 0, 0, 0, 0}, R[6][6] = {{0}}, sum, tmp[6];
+  long i, j, ip;
+  for (i = 0; i < 6; i++)
+    R[i][i] = 1;
+  R[1][5] = -sin(kick);
+  if (tilt) {
+    double st = sin(tilt), ct = cos(tilt), Z[6] = {0}, Rot[6][6] = {{0}}, IRot[6][6] = {{0}}, Cr[6], Rr[6][6];
+    if (fabs(tilt - EH_PIo2) < 1e-12)
+      st = 1, ct = 0;
+    else if (fabs(tilt + EH_PIo2) < 1e-12)
+      st = -1, ct = 0;
+    else if (fabs(tilt - EH_PI) < 1e-12 || fabs(tilt + EH_PI) < 1e-12)
+      st = 0, ct = -1;
+    Rot[0][0] = Rot[2][2] = Rot[1][1] = Rot[3][3] = ct;
+    IRot[0][0] = IRot[2][2] = IRot[1][1] = IRot[3][3] = ct;
+    Rot[0][2] = Rot[1][3] = st;
+    IRot[0][2] = IRot[1][3] = -st;
+    Rot[2][0] = Rot[3][1] = -st;
+    IRot[2][0] = IRot[3][1] = -(-st);
+    Rot[4][4] = Rot[5][5] = IRot[4][4] = IRot[5][5] = 1;
+    or_mat1_mul(Cr, Rr, C, R, Z, Rot);
+    or_mat1_mul(C, R, Z, IRot, Cr, Rr);
+  }
+  for (ip = np - 1; ip >= 0; ip--) {
+    double *x = coord[ip];
+    for (i = 5; i >= 0; i--) {
+      sum = C[i];
+      for (j = 5; j >= 0; j--)
+        sum += R[i][j] * x[j];
+      tmp[i] = sum;
+    }
+    for (i = 5; i >= 0; i--)
+      x[i] = tmp[i];
+  }
+}
+
 void or_track_particles_drift(double **coord, long np, double length, int order) {
   double C[6] = {0, 0, 0, 0, 0, 0};
   double R[6][6];
@@ -2344,8 +2402,15 @@ long or_do_tracking(or_ctx *c, double **coord, long nOriginal, const eh_element 
       case EH_T_VKICK:
         /* track_particles with corrector_matrix, which with no strength is
          * the drift matrix (bend_matrix.c:314-317, do_tracking.c:2404-2415) */
-        if (e->p.corrector.kick * e->p.corrector.calibration != 0 || e->p.corrector.synchRad || e->p.corrector.isr)
+        if (e->p.corrector.synchRad || e->p.corrector.isr)
           return EH_ERR_UNSUPPORTED;
+        if (e->p.corrector.kick * e->p.corrector.calibration != 0) {
+          if (e->p.corrector.length != 0)
+            return EH_ERR_UNSUPPORTED;
+          or_thin_corrector(coord, nToTrack, e->p.corrector.kick * e->p.corrector.calibration,
+                            e->type == EH_T_VKICK ? e->p.corrector.tilt + EH_PIo2 : e->p.corrector.tilt);
+          break;
+        }
         or_track_particles_drift(coord, nToTrack, e->p.corrector.length,
                                  e->p.corrector.order ? e->p.corrector.order : 2);
         break;

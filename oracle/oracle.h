@@ -55,6 +55,7 @@ void or_math_eval(int fn, const double *in, const double *in2, double *out, long
 
 /* element routines (reference names, or_ prefix) */
 void or_track_particles_drift(double **coord, long np, double length, int order);
+void or_thin_corrector(double **coord, long np, double kick, double tilt);
 void or_exact_drift(double **part, long np, double length);
 long or_multipole_tracking2(double **particle, long n_part, const eh_element *elem, double Po, double z_start,
                             const EH_MAXAMP *maxamp);

@@ -130,3 +130,30 @@ def test_example_deck_with_its_malign_recirc_and_collimator(tracker):
     assert ng == nc and eg == o.effort()
     assert rows.shape[0] - ng > 10
     assert_rows_identical(g, c, "parTrack deck, whole")
+
+
+def test_thin_steering_correctors_bit_exact(tracker):
+    """zero-length HKICK/VKICK with kicks, tilts and calibrations (the
+    first-order corrector matrix) in the ring: rows, losses and effort as
+    the checker's; a thick corrector with a kick is refused"""
+    ring = ring_with_maligns()
+    els = list(ring.elements)
+    for i, e in [(3, E.HKICK("H1", KICK=2e-4)), (9, E.VKICK("V1", KICK=-1.5e-4, CALIBRATION=1.1)),
+                 (17, E.HKICK("H2", KICK=1e-4, TILT=0.3)), (29, E.VKICK("V2", KICK=3e-4, TILT=-0.2)),
+                 (41, E.HKICK("H3", KICK=-2e-4, TILT=3.141592653589793))]:
+        els.insert(i, e)
+    bl = E.Beamline(els)
+    _, Po, rows = lossy_fodo(1500, seed=55)
+    g = rows.copy()
+    ng, eg = tracker.do_tracking(g, bl, Po, 4)
+    c = rows.copy()
+    o = O.Oracle("emath")
+    nc = o.do_tracking(c, bl, Po, 4)
+    assert ng == nc and eg == o.effort()
+    assert rows.shape[0] - ng > 100
+    assert_rows_identical(g, c, "ring with thin correctors")
+    base = rows.copy()
+    O.Oracle("emath").do_tracking(base, E.Beamline(list(ring.elements)), Po, 4)
+    assert not np.array_equal(base, c)
+    with pytest.raises(native.EHError):
+        tracker.do_tracking(rows.copy(), E.Beamline([E.HKICK("H", L=0.2, KICK=1e-4)] + list(ring.elements)), Po, 1)

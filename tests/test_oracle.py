@@ -1164,6 +1164,36 @@ def test_zero_strength_correctors_are_drifts_without_extrapolation():
         track([E.HKICK(L=0.1, KICK=1e-4)], rows_of(pts))
 
 
+def test_thin_corrector_kicks():
+    """a zero-length HKICK/VKICK with a kick: corrector_matrix's C[1] = kick,
+    R[1][5] = -sin(kick) (bend_matrix.c:323-330) applied by track_particles
+    (matrix.c:430-458): x' -> (kick - sin(kick) dp) + x' exactly, VKICK the
+    same in y through tilt_matrices' exact quarter turn; a tilt splits the
+    kick by cos and sin (a wrong rotation sign fails the y half); calibration
+    scales it; a thick corrector with a kick stays refused"""
+    pts = [[1e-3, 2e-4, -3e-4, 5e-5, 0.2, 1e-3], [-2e-3, -1e-4, 4e-4, 0.0, -0.1, -2.5e-3]]
+    rows = rows_of(pts)
+    k = 3e-4
+    r, n, _ = track([E.HKICK(KICK=k)], rows)
+    assert n == 2
+    for a, b in zip(r, rows):
+        assert a[1] == (k - math.sin(k) * b[5]) + b[1]
+        assert np.array_equal(a[[0, 2, 3, 4, 5]], b[[0, 2, 3, 4, 5]])
+    r, n, _ = track([E.VKICK(KICK=k / 2, CALIBRATION=2.0)], rows)
+    for a, b in zip(r, rows):
+        assert a[3] == (k - math.sin(k) * b[5]) + b[3]
+        assert np.array_equal(a[[0, 1, 2, 4, 5]], b[[0, 1, 2, 4, 5]])
+    th = 0.4
+    r, n, _ = track([E.HKICK(KICK=k, TILT=th)], rows)
+    for a, b in zip(r, rows):
+        g = k - math.sin(k) * b[5]
+        assert abs((a[1] - b[1]) - math.cos(th) * g) < 1e-18
+        assert abs((a[3] - b[3]) - math.sin(th) * g) < 1e-18
+        assert np.allclose(a[[0, 2, 4, 5]], b[[0, 2, 4, 5]], rtol=1e-15, atol=1e-20)
+    with pytest.raises(RuntimeError):
+        track([E.VKICK(L=0.2, KICK=k)], rows)
+
+
 def test_elements_before_recirc_act_on_the_first_pass_only():
     """RECIRC (do_tracking.c:459-465, 1382-1387): passes after pass 0 start
     at the recirculation point, so an offset placed before it acts once"""
